@@ -1,0 +1,185 @@
+/* quarry_abi.h - C ABI of libquarry_hip.so, the MI355X (gfx950) BVH build and
+ * query library.
+ *
+ * Plain C: opaque handles, POD structs, raw pointers and sizes; no HIP or torch
+ * types.  Every entry point returns a qr_status; on failure qr_last_error()
+ * (thread-local) holds the message.  The C++ API in <gpu/gpu_bvh.h> and the
+ * Python module `quarry` are thin layers over these functions.
+ *
+ * Each declaration names the reference interface it replaces, as
+ * src/gpu/gpu_bvh.h:<line> of ycl-berserk/quarry-gpu.
+ *
+ * Semantics shared by every call (reference: docs/api.md:146, :203):
+ *   - calls are synchronous: results are ready on return;
+ *   - returned milliseconds are device time measured with events;
+ *   - work is issued on the stream set with qr_set_stream (default: null stream).
+ *
+ * Wire layouts (byte-identical to the reference):
+ *   triangle  9 x f32 (a, b, c)                          36 B
+ *   point     3 x f32                                    12 B
+ *   Gaussian  mean 3 f32, quat xyzw 4 f32, scale 3 f32   40 B
+ *   ray       origin 3 f32, dir 3 f32, tMin, tMax        32 B
+ *   node      lo 3 f32, hi 3 f32, left i32, right i32    32 B (internal 0..n-2,
+ *             leaf n-1+slot with left = -(slot+1), right = 1)
+ *   closest/kNN result  {u32 dist2 bits, i32 index}       8 B (index -1: none)
+ *   ray result u64 (hit << 32) | t bits, 0 for a miss
+ *   radius CSR offsets i32[n+1], keys u64 (dist2 bits << 32 | index), ascending
+ */
+#ifndef QUARRY_ABI_H
+#define QUARRY_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QR_API __attribute__((visibility("default")))
+
+typedef struct qr_bvh qr_bvh;
+typedef struct qr_rays qr_rays;
+typedef struct qr_points qr_points;
+typedef struct qr_csr qr_csr; /* radius-query CSR result */
+
+typedef enum {
+    QR_OK = 0,
+    QR_ERR_INVALID = 1, /* bad argument (std::runtime_error in the reference API) */
+    QR_ERR_DEVICE = 2,  /* HIP failure */
+    QR_ERR_OOM = 3,     /* device allocation failed */
+    QR_ERR_EDIT = 4,    /* invalid topology edit (std::invalid_argument) */
+    QR_ERR_NO_DEVICE = 5
+} qr_status;
+
+typedef enum { QR_PRIM_TRIANGLE = 0, QR_PRIM_POINT = 1, QR_PRIM_GAUSSIAN = 2 } qr_prim;
+typedef enum { QR_BUILDER_LBVH = 0, QR_BUILDER_SAH = 1 } qr_builder;
+/* RayOrder / QueryOrder (src/gpu/gpu_bvh.h:224, :274) */
+typedef enum { QR_ORDER_INPUT = 0, QR_ORDER_MORTON = 1, QR_ORDER_MORTON_DIRECTION = 2 } qr_order;
+
+/* GpuBuildTimings (src/gpu/gpu_bvh.h:79-89) */
+typedef struct {
+    double morton_ms, sort_ms, tree_ms, scatter_ms, refit_ms;
+    double sah_bin_ms, sah_split_ms, sah_part_ms, total_ms;
+} qr_build_timings;
+
+/* GpuRefitTimings (src/gpu/gpu_bvh.h:129-138) */
+typedef struct {
+    double upload_ms, gather_ms, tree_ms, mark_ms, merge_ms, wide_ms, total_ms;
+    int updated;
+} qr_refit_timings;
+
+/* GpuTraversalStats / GpuQueryStats (src/gpu/gpu_bvh.h:240-245, :264-269) */
+typedef struct {
+    uint64_t node_tests, leaf_tests, stack_overflows;
+    double sort_ms;
+} qr_traversal_stats;
+
+/* RayStats (src/core/fingerprint.h:16-43) */
+typedef struct {
+    uint64_t rays, hits, hash;
+    double sum_t;
+} qr_ray_stats;
+
+/* Public fields of GpuBVH (src/gpu/gpu_bvh.h:26-33) plus the device arrays. */
+typedef struct {
+    int prim_count, node_count, depth, wide_count, prim_kind, builder;
+    void* d_nodes;      /* node[node_count] */
+    void* d_items;      /* primitives in leaf order */
+    void* d_prim_ids;   /* i32[prim_count], leaf slot -> scene index */
+    void* d_wide_nodes; /* 128 B 4-wide nodes, or NULL */
+} qr_bvh_info;
+
+QR_API const char* qr_last_error(void);
+
+/* gpuDescribe (src/gpu/gpu_bvh.h:100): device name into buf; QR_ERR_NO_DEVICE if none. */
+QR_API int qr_describe(char* buf, int len);
+/* warmupBuildGPU / warmupQueryGPU (src/gpu/gpu_bvh.h:122, :343) */
+QR_API int qr_warmup(void);
+/* Stream for subsequent calls on this host thread (a hipStream_t; NULL = default). */
+QR_API int qr_set_stream(void* hip_stream);
+
+/* ---- build (src/gpu/gpu_bvh.h:103-118) ----------------------------------------
+ * buildBVHGPU / buildPointBVHGPU / buildGaussianBVHGPU (builder LBVH) and
+ * buildBVHSahGPU / buildPointBVHSahGPU / buildGaussianBVHSahGPU (builder SAH).
+ * items_host: n primitives of kind `prim`, scene order.  n >= 2. */
+QR_API int qr_bvh_build(int prim, int builder, const void* items_host, int n,
+                        qr_build_timings* timings, qr_bvh** out);
+/* Same with the primitives already in device memory (not retained). */
+QR_API int qr_bvh_build_device(int prim, int builder, const void* items_dev, int n,
+                               qr_build_timings* timings, qr_bvh** out);
+/* destroyBVHGPU (src/gpu/gpu_bvh.h:215) */
+QR_API int qr_bvh_destroy(qr_bvh* bvh);
+QR_API int qr_bvh_get_info(const qr_bvh* bvh, qr_bvh_info* out);
+/* buildWideNodesGPU (src/gpu/gpu_bvh.h:229) */
+QR_API int qr_bvh_build_wide(qr_bvh* bvh, double* ms);
+
+/* refit{,Point,Gaussian}BVHGPU (src/gpu/gpu_bvh.h:141-155).  items_host holds all n
+ * primitives in scene order (topology unchanged).  dirty == NULL: full refit;
+ * otherwise only those scene indices are re-uploaded and only their ancestors
+ * are merged.  Wide nodes, if built, are re-emitted. */
+QR_API int qr_bvh_refit(qr_bvh* bvh, const void* items_host, int n, const int* dirty, int n_dirty,
+                        qr_refit_timings* timings, double* ms);
+/* Full refit from device-resident scene-order primitives. */
+QR_API int qr_bvh_refit_device(qr_bvh* bvh, const void* items_dev, int n,
+                               qr_refit_timings* timings, double* ms);
+
+/* validate{,Point,Gaussian}BVHGPU (src/gpu/gpu_bvh.h:347-349): why receives "" when
+ * every structural invariant holds. */
+QR_API int qr_bvh_validate(qr_bvh* bvh, const void* items_host, int n, char* why, int len);
+/* treeHashGPU (src/gpu/gpu_bvh.h:351-352): FNV-1a over the node array bytes. */
+QR_API int qr_bvh_tree_hash(const qr_bvh* bvh, uint64_t* out);
+/* Copies the node array (node_count x 32 B) and/or prim ids (prim_count) to the host. */
+QR_API int qr_bvh_read(const qr_bvh* bvh, void* nodes_host, int* prim_ids_host);
+
+/* ---- rays (src/gpu/gpu_bvh.h:217-246) ------------------------------------------ */
+/* uploadRaysGPU */
+QR_API int qr_rays_upload(const void* rays_host, int n, qr_rays** out);
+/* destroyRayBufferGPU */
+QR_API int qr_rays_destroy(qr_rays* rays);
+/* raycastGPU: closest hit per ray into the buffer's device results. */
+QR_API int qr_raycast(const qr_bvh* bvh, qr_rays* rays, int collect_stats, int order, int wide,
+                      double* ms);
+/* summarizeGPU: fingerprint of the per-ray results; per_ray (optional) gets the u64s. */
+QR_API int qr_rays_summarize(const qr_rays* rays, uint64_t* per_ray, qr_ray_stats* out);
+/* lastTraversalStatsGPU */
+QR_API int qr_last_traversal_stats(qr_traversal_stats* out);
+/* Device-pointer variant: rays_dev[n] -> results_dev[n] (u64). */
+QR_API int qr_raycast_device(const qr_bvh* bvh, const void* rays_dev, int n, uint64_t* results_dev,
+                             int order, int wide, double* ms);
+
+/* ---- point queries (src/gpu/gpu_bvh.h:276-342) ---------------------------------- */
+/* uploadPointsGPU: results_per_point = result slots per query (closest 1, kNN k). */
+QR_API int qr_points_upload(const float* points_host, int n, int results_per_point,
+                            qr_points** out);
+/* destroyPointBufferGPU */
+QR_API int qr_points_destroy(qr_points* pts);
+/* closestPointGPU / closestVertexGPU / closestGaussianGPU (leaf kind from the tree) */
+QR_API int qr_closest(const qr_bvh* bvh, qr_points* pts, int collect_stats, int order, int wide,
+                      double* ms);
+/* kNearestGPU / kNearestVertexGPU / kNearestGaussianGPU; 1 <= k <= 32. */
+QR_API int qr_knn(const qr_bvh* bvh, qr_points* pts, int k, int collect_stats, int order, int wide,
+                  double* ms);
+/* radiusQueryGPU / radiusQueryVertexGPU / radiusQueryGaussianGPU; *out is a new CSR. */
+QR_API int qr_radius(const qr_bvh* bvh, qr_points* pts, float radius, int collect_stats, int order,
+                     int wide, qr_csr** out, double* ms);
+/* readQueryResultsGPU: count * stride records. */
+QR_API int qr_points_read(const qr_points* pts, void* results_host);
+QR_API int qr_points_info(const qr_points* pts, int* count, int* stride);
+/* readRadiusResultsGPU: offsets_host[count + 1], keys_host[total]. */
+QR_API int qr_radius_info(const qr_csr* r, int* count, int* total);
+QR_API int qr_radius_read(const qr_csr* r, int* offsets_host, uint64_t* keys_host);
+/* destroyRadiusBufferGPU */
+QR_API int qr_radius_destroy(qr_csr* r);
+/* lastQueryStatsGPU */
+QR_API int qr_last_query_stats(qr_traversal_stats* out);
+/* Device-pointer variants: points_dev[n] (3 f32) -> results_dev[n * k] (8 B each). */
+QR_API int qr_closest_device(const qr_bvh* bvh, const float* points_dev, int n, void* results_dev,
+                             int order, int wide, double* ms);
+QR_API int qr_knn_device(const qr_bvh* bvh, const float* points_dev, int n, int k,
+                         void* results_dev, int order, int wide, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QUARRY_ABI_H */

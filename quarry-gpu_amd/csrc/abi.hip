@@ -1,0 +1,768 @@
+// C ABI of libquarry_hip.so (declared in include/quarry/quarry_abi.h): handle
+// management, host<->device transfers, timing and error mapping around the HIP
+// kernels.  No kernel math lives here.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/quarry/quarry_abi.h"
+#include "qr_bvh.h"
+#include "qr_traverse.h"
+
+using namespace qr;
+
+struct qr_bvh {
+    BvhState s;
+};
+
+struct qr_rays {
+    DevBuf<Ray> rays;
+    DevBuf<uint64_t> results;
+    int count = 0;
+    // reordering scratch, allocated on first use
+    DevBuf<uint32_t> keys, keysSorted, order, bb;
+    SortWorkspace ws;
+};
+
+struct qr_points {
+    DevBuf<V3> pts;
+    DevBuf<ClosestResult> results;
+    int count = 0;
+    int stride = 1;
+    DevBuf<uint32_t> keys, keysSorted, order, bb;
+    SortWorkspace ws;
+};
+
+struct qr_csr {
+    DevBuf<int> offsets;
+    DevBuf<uint64_t> keys;
+    int count = 0;
+    int total = 0;
+};
+
+namespace qr {
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (e == hipErrorOutOfMemory) throw OomError(std::string("HIP ") + what + ": " + hipGetErrorString(e));
+        throw DeviceError(std::string("HIP ") + what + ": " + hipGetErrorString(e));
+    }
+}
+
+namespace {
+thread_local hipStream_t tStream = nullptr;
+}
+hipStream_t currentStream() { return tStream; }
+
+}  // namespace qr
+
+namespace {
+
+thread_local std::string tLastError;
+thread_local qr_traversal_stats tRayStats{};
+thread_local qr_traversal_stats tQueryStats{};
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        return QR_OK;
+    } catch (const ArgError& e) {
+        tLastError = e.what();
+        return QR_ERR_INVALID;
+    } catch (const EditError& e) {
+        tLastError = e.what();
+        return QR_ERR_EDIT;
+    } catch (const OomError& e) {
+        tLastError = e.what();
+        return QR_ERR_OOM;
+    } catch (const std::exception& e) {
+        tLastError = e.what();
+        return QR_ERR_DEVICE;
+    }
+}
+
+void need(bool ok, const char* msg) {
+    if (!ok) throw ArgError(msg);
+}
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s;
+    explicit Timer(hipStream_t st) : s(st) {
+        hipCheck(hipEventCreate(&a), "event");
+        hipCheck(hipEventCreate(&b), "event");
+        hipCheck(hipEventRecord(a, s), "event");
+    }
+    ~Timer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    double stop() {
+        hipCheck(hipEventRecord(b, s), "event");
+        hipCheck(hipEventSynchronize(b), "sync");
+        float ms = 0.0f;
+        hipCheck(hipEventElapsedTime(&ms, a, b), "elapsed");
+        return ms;
+    }
+};
+
+struct Stats {
+    DevBuf<unsigned long long> d;
+    Stats() : d(4) {}
+};
+
+void readStats(const unsigned long long* dev, qr_traversal_stats* out) {
+    unsigned long long h[4] = {0, 0, 0, 0};
+    hipCheck(hipMemcpy(h, dev, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "stats");
+    out->node_tests = h[0];
+    out->leaf_tests = h[1];
+    out->stack_overflows = h[2];
+}
+
+TraverseArgs argsFor(const BvhState& b, bool wide) {
+    TraverseArgs a;
+    a.kind = b.kind;
+    a.wide = wide;
+    a.nodes = b.nodes.p;
+    a.wideNodes = b.wide.p;
+    a.items = b.items.p;
+    a.primIds = b.primIds.p;
+    return a;
+}
+
+void checkTree(const qr_bvh* bvh, bool wide) {
+    need(bvh != nullptr, "null bvh handle");
+    need(bvh->s.n >= 2, "bvh is empty");
+    if (wide && !bvh->s.wideValid) throw ArgError("wide nodes not built");
+}
+
+// Morton visiting order for a buffer of elements `stride` floats apart.
+template <typename Buf>
+const uint32_t* reorder(Buf& buf, const float* base, int stride, int count, int fixedBox,
+                        hipStream_t s) {
+    buf.keys.reserve(static_cast<size_t>(count));
+    buf.keysSorted.reserve(static_cast<size_t>(count));
+    buf.order.reserve(static_cast<size_t>(count));
+    buf.bb.reserve(8);
+    mortonOrder(base, stride, count, fixedBox, buf.ws, buf.keys.p, buf.keysSorted.p, buf.order.p,
+                buf.bb.p, s);
+    return buf.order.p;
+}
+
+void buildInto(BvhState& b, int prim, int builder, const void* dItems, int n, qr_build_timings* t) {
+    need(prim >= 0 && prim <= 2, "unknown primitive kind");
+    need(builder == QR_BUILDER_LBVH || builder == QR_BUILDER_SAH, "unknown builder");
+    need(n >= 2, "at least 2 primitives are required");
+    BuildTimingsC bt{};
+    if (builder == QR_BUILDER_LBVH) {
+        buildLbvhDevice(b, prim, dItems, n, currentStream(), &bt);
+    } else {
+        buildSahDevice(b, prim, dItems, n, currentStream(), &bt);
+    }
+    if (t) {
+        t->morton_ms = bt.mortonMs;
+        t->sort_ms = bt.sortMs;
+        t->tree_ms = bt.treeMs;
+        t->scatter_ms = bt.scatterMs;
+        t->refit_ms = bt.refitMs;
+        t->sah_bin_ms = bt.sahBinMs;
+        t->sah_split_ms = bt.sahSplitMs;
+        t->sah_part_ms = bt.sahPartMs;
+        t->total_ms = bt.totalMs;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* qr_last_error(void) { return tLastError.c_str(); }
+
+int qr_describe(char* buf, int len) {
+    int devices = 0;
+    const hipError_t e = hipGetDeviceCount(&devices);
+    if (e != hipSuccess || devices == 0) {
+        (void)hipGetLastError();
+        tLastError = e != hipSuccess ? hipGetErrorString(e) : "no HIP device";
+        if (buf && len > 0) std::snprintf(buf, static_cast<size_t>(len), "%s", tLastError.c_str());
+        return QR_ERR_NO_DEVICE;
+    }
+    return guard([&] {
+        int dev = 0;
+        hipCheck(hipGetDevice(&dev), "get device");
+        hipDeviceProp_t prop{};
+        hipCheck(hipGetDeviceProperties(&prop, dev), "device properties");
+        if (buf && len > 0) {
+            std::snprintf(buf, static_cast<size_t>(len), "%s %s (%d CU, %.1f GiB)", prop.name,
+                          prop.gcnArchName, prop.multiProcessorCount,
+                          static_cast<double>(prop.totalGlobalMem) / (1024.0 * 1024.0 * 1024.0));
+        }
+    });
+}
+
+int qr_warmup(void) {
+    return guard([&] {
+        DevBuf<unsigned long long> d(4);
+        zeroStats(d.p, currentStream());
+        hipCheck(hipStreamSynchronize(currentStream()), "warmup");
+    });
+}
+
+int qr_set_stream(void* s) {
+    tStream = static_cast<hipStream_t>(s);
+    return QR_OK;
+}
+
+int qr_bvh_build(int prim, int builder, const void* items, int n, qr_build_timings* t,
+                 qr_bvh** out) {
+    return guard([&] {
+        need(out != nullptr, "null output handle");
+        need(items != nullptr || n == 0, "null primitive array");
+        need(prim >= 0 && prim <= 2, "unknown primitive kind");
+        need(n >= 2, "at least 2 primitives are required");
+        const size_t bytes = static_cast<size_t>(n) * itemSize(prim);
+        DevBuf<unsigned char> src(bytes);
+        hipCheck(hipMemcpy(src.p, items, bytes, hipMemcpyHostToDevice), "H2D primitives");
+        auto* h = new qr_bvh();
+        try {
+            buildInto(h->s, prim, builder, src.p, n, t);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int qr_bvh_build_device(int prim, int builder, const void* items, int n, qr_build_timings* t,
+                        qr_bvh** out) {
+    return guard([&] {
+        need(out != nullptr, "null output handle");
+        need(items != nullptr, "null primitive array");
+        auto* h = new qr_bvh();
+        try {
+            buildInto(h->s, prim, builder, items, n, t);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int qr_bvh_destroy(qr_bvh* bvh) {
+    return guard([&] { delete bvh; });
+}
+
+int qr_bvh_get_info(const qr_bvh* bvh, qr_bvh_info* out) {
+    return guard([&] {
+        need(bvh && out, "null argument");
+        const BvhState& b = bvh->s;
+        out->prim_count = b.n;
+        out->node_count = b.nodeCount;
+        out->depth = b.depth;
+        out->wide_count = b.wideValid ? b.wideCount : 0;
+        out->prim_kind = b.kind;
+        out->builder = b.builder;
+        out->d_nodes = b.nodes.p;
+        out->d_items = b.items.p;
+        out->d_prim_ids = b.primIds.p;
+        out->d_wide_nodes = b.wideValid ? b.wide.p : nullptr;
+    });
+}
+
+int qr_bvh_build_wide(qr_bvh* bvh, double* ms) {
+    return guard([&] {
+        need(bvh != nullptr, "null bvh handle");
+        need(bvh->s.n >= 2, "bvh is empty");
+        hipStream_t s = currentStream();
+        // The depth table is part of the collapse cost (the reference keeps it from
+        // the build); measure both.
+        Timer tm(s);
+        buildWide(bvh->s, s);
+        const double t = tm.stop();
+        if (ms) *ms = t;
+    });
+}
+
+int qr_bvh_refit(qr_bvh* bvh, const void* items, int n, const int* dirty, int nDirty,
+                 qr_refit_timings* t, double* ms) {
+    return guard([&] {
+        need(bvh != nullptr, "null bvh handle");
+        BvhState& b = bvh->s;
+        need(n == b.n, "refit: primitive count differs from the tree");
+        need(items != nullptr, "null primitive array");
+        hipStream_t s = currentStream();
+        const size_t isz = itemSize(b.kind);
+        qr_refit_timings tt{};
+        hipEvent_t ev[5];
+        for (auto& e : ev) hipCheck(hipEventCreate(&e), "event");
+        if (dirty == nullptr) {
+            b.stage.reserve(static_cast<size_t>(n) * isz);
+            hipCheck(hipEventRecord(ev[0], s), "event");
+            hipCheck(hipMemcpyAsync(b.stage.p, items, static_cast<size_t>(n) * isz,
+                                    hipMemcpyHostToDevice, s), "H2D refit");
+            hipCheck(hipEventRecord(ev[1], s), "event");
+            refitFull(b, b.stage.p, s);
+            hipCheck(hipEventRecord(ev[2], s), "event");
+            hipCheck(hipEventRecord(ev[3], s), "event");
+            tt.updated = n;
+        } else {
+            std::vector<int> ids(dirty, dirty + nDirty);
+            std::sort(ids.begin(), ids.end());
+            ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+            for (int id : ids) need(id >= 0 && id < n, "refit: dirty index out of range");
+            const int m = static_cast<int>(ids.size());
+            std::vector<unsigned char> staged(static_cast<size_t>(m) * isz);
+            const auto* src = static_cast<const unsigned char*>(items);
+            for (int i = 0; i < m; ++i)
+                std::memcpy(&staged[static_cast<size_t>(i) * isz], src + static_cast<size_t>(ids[i]) * isz, isz);
+            b.stage.reserve(staged.size() + 1);
+            b.dirtyIds.reserve(static_cast<size_t>(m) + 1);
+            b.dirtyLeaves.reserve(static_cast<size_t>(m) + 1);
+            hipCheck(hipEventRecord(ev[0], s), "event");
+            if (m > 0) {
+                hipCheck(hipMemcpyAsync(b.stage.p, staged.data(), staged.size(), hipMemcpyHostToDevice, s), "H2D dirty");
+                hipCheck(hipMemcpyAsync(b.dirtyIds.p, ids.data(), static_cast<size_t>(m) * sizeof(int),
+                                        hipMemcpyHostToDevice, s), "H2D dirty ids");
+            }
+            hipCheck(hipEventRecord(ev[1], s), "event");
+            refitDirty(b, b.stage.p, b.dirtyIds.p, m, b.dirtyLeaves.p, s, ev[2]);
+            if (m <= 0) hipCheck(hipEventRecord(ev[2], s), "event");
+            hipCheck(hipEventRecord(ev[3], s), "event");
+            tt.updated = m;
+        }
+        reemitWide(b, s);
+        hipCheck(hipEventRecord(ev[4], s), "event");
+        hipCheck(hipEventSynchronize(ev[4]), "refit sync");
+        float v[4];
+        for (int i = 0; i < 4; ++i) hipCheck(hipEventElapsedTime(&v[i], ev[i], ev[i + 1]), "elapsed");
+        float total = 0.0f;
+        hipCheck(hipEventElapsedTime(&total, ev[0], ev[4]), "elapsed");
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        tt.upload_ms = v[0];
+        if (dirty == nullptr) {
+            tt.gather_ms = 0.0;
+            tt.merge_ms = v[1];
+            tt.mark_ms = 0.0;
+        } else {
+            tt.mark_ms = v[1];
+            tt.merge_ms = v[2];
+        }
+        tt.tree_ms = tt.mark_ms + tt.merge_ms;
+        tt.wide_ms = v[3];
+        tt.total_ms = total;
+        if (t) *t = tt;
+        if (ms) *ms = total;
+    });
+}
+
+int qr_bvh_refit_device(qr_bvh* bvh, const void* items, int n, qr_refit_timings* t, double* ms) {
+    return guard([&] {
+        need(bvh != nullptr, "null bvh handle");
+        BvhState& b = bvh->s;
+        need(n == b.n, "refit: primitive count differs from the tree");
+        hipStream_t s = currentStream();
+        Timer tm(s);
+        refitFull(b, items, s);
+        reemitWide(b, s);
+        const double total = tm.stop();
+        if (t) {
+            *t = qr_refit_timings{};
+            t->merge_ms = total;
+            t->tree_ms = total;
+            t->total_ms = total;
+            t->updated = n;
+        }
+        if (ms) *ms = total;
+    });
+}
+
+int qr_bvh_validate(qr_bvh* bvh, const void* items, int n, char* why, int len) {
+    return guard([&] {
+        need(bvh != nullptr, "null bvh handle");
+        BvhState& b = bvh->s;
+        std::string msg;
+        if (n != b.n) {
+            msg = "prim count mismatch";
+        } else {
+            const size_t isz = itemSize(b.kind);
+            DevBuf<unsigned char> src(static_cast<size_t>(n) * isz);
+            hipCheck(hipMemcpy(src.p, items, static_cast<size_t>(n) * isz, hipMemcpyHostToDevice), "H2D validate");
+            int errors[6] = {0, 0, 0, 0, 0, 0};
+            validateTree(b, src.p, errors, currentStream());
+            // Root box must equal the scene bounds (union of primitive boxes).
+            Node2 root{};
+            hipCheck(hipMemcpy(&root, b.nodes.p, sizeof(Node2), hipMemcpyDeviceToHost), "root D2H");
+            Box sb = emptyBox();
+            const auto* raw = static_cast<const unsigned char*>(items);
+            for (int i = 0; i < n; ++i) {
+                const void* it = raw + static_cast<size_t>(i) * isz;
+                Box pb;
+                if (b.kind == kPrimTri) pb = triBounds(*static_cast<const Tri*>(it));
+                else if (b.kind == kPrimPoint) pb = PointLeaf::bounds(*static_cast<const V3*>(it));
+                else pb = gaussBounds(*static_cast<const Gaussian*>(it));
+                growBox(sb, pb);
+            }
+            const bool badRoot = root.lo[0] != sb.lo.x || root.lo[1] != sb.lo.y || root.lo[2] != sb.lo.z ||
+                                 root.hi[0] != sb.hi.x || root.hi[1] != sb.hi.y || root.hi[2] != sb.hi.z;
+            const char* names[6] = {"bad_children_or_box", "bad_leaf_box", "bad_refcount",
+                                    "bad_leaf_id", "bad_prim_map", "bad_depth"};
+            char buf[96];
+            for (int i = 0; i < 6; ++i) {
+                if (errors[i]) {
+                    std::snprintf(buf, sizeof(buf), "%s%s=%d", msg.empty() ? "" : " ", names[i], errors[i]);
+                    msg += buf;
+                }
+            }
+            if (badRoot) msg += msg.empty() ? "root_box_mismatch=1" : " root_box_mismatch=1";
+        }
+        if (why && len > 0) std::snprintf(why, static_cast<size_t>(len), "%s", msg.c_str());
+    });
+}
+
+int qr_bvh_tree_hash(const qr_bvh* bvh, uint64_t* out) {
+    return guard([&] {
+        need(bvh && out, "null argument");
+        const BvhState& b = bvh->s;
+        std::vector<unsigned char> bytes(static_cast<size_t>(b.nodeCount) * sizeof(Node2));
+        if (!bytes.empty())
+            hipCheck(hipMemcpy(bytes.data(), b.nodes.p, bytes.size(), hipMemcpyDeviceToHost), "tree D2H");
+        uint64_t h = 0xcbf29ce484222325ull;
+        for (unsigned char c : bytes) {
+            h ^= static_cast<uint64_t>(c);
+            h *= 0x100000001b3ull;
+        }
+        *out = h;
+    });
+}
+
+int qr_bvh_read(const qr_bvh* bvh, void* nodes, int* primIds) {
+    return guard([&] {
+        need(bvh != nullptr, "null bvh handle");
+        const BvhState& b = bvh->s;
+        if (nodes)
+            hipCheck(hipMemcpy(nodes, b.nodes.p, static_cast<size_t>(b.nodeCount) * sizeof(Node2),
+                               hipMemcpyDeviceToHost), "nodes D2H");
+        if (primIds)
+            hipCheck(hipMemcpy(primIds, b.primIds.p, static_cast<size_t>(b.n) * sizeof(int),
+                               hipMemcpyDeviceToHost), "prim ids D2H");
+    });
+}
+
+// ---- rays ---------------------------------------------------------------------------
+
+int qr_rays_upload(const void* rays, int n, qr_rays** out) {
+    return guard([&] {
+        need(out != nullptr, "null output handle");
+        need(n >= 0, "negative ray count");
+        need(rays != nullptr || n == 0, "null ray array");
+        auto* r = new qr_rays();
+        try {
+            r->count = n;
+            r->rays.alloc(static_cast<size_t>(n));
+            r->results.alloc(static_cast<size_t>(n));
+            if (n > 0)
+                hipCheck(hipMemcpy(r->rays.p, rays, static_cast<size_t>(n) * sizeof(Ray), hipMemcpyHostToDevice), "H2D rays");
+        } catch (...) {
+            delete r;
+            throw;
+        }
+        *out = r;
+    });
+}
+
+int qr_rays_destroy(qr_rays* r) {
+    return guard([&] { delete r; });
+}
+
+namespace {
+double raycastImpl(const qr_bvh* bvh, const Ray* rays, int n, uint64_t* out, qr_rays* scratch,
+                   int collectStats, int order, int wide) {
+    checkTree(bvh, wide != 0);
+    need(bvh->s.kind == kPrimTri, "raycast needs a triangle BVH");
+    need(order >= 0 && order <= 2, "unknown ray order");
+    tRayStats = qr_traversal_stats{};
+    if (n <= 0) return 0.0;
+    hipStream_t s = currentStream();
+    Stats st;
+    if (collectStats) zeroStats(st.d.p, s);
+    hipEvent_t evs[3];
+    for (auto& e : evs) hipCheck(hipEventCreate(&e), "event");
+    hipCheck(hipEventRecord(evs[0], s), "event");
+    TraverseArgs a = argsFor(bvh->s, wide != 0);
+    a.stats = collectStats ? st.d.p : nullptr;
+    if (order != QR_ORDER_INPUT) {
+        const float* base = reinterpret_cast<const float*>(rays) + (order == QR_ORDER_MORTON ? 0 : 3);
+        a.order = reorder(*scratch, base, 8, n, order == QR_ORDER_MORTON_DIRECTION ? 1 : 0, s);
+    }
+    hipCheck(hipEventRecord(evs[1], s), "event");
+    launchRays(a, rays, n, out, s);
+    hipCheck(hipEventRecord(evs[2], s), "event");
+    hipCheck(hipEventSynchronize(evs[2]), "raycast sync");
+    float total = 0.0f, sortMs = 0.0f;
+    hipCheck(hipEventElapsedTime(&total, evs[0], evs[2]), "elapsed");
+    hipCheck(hipEventElapsedTime(&sortMs, evs[0], evs[1]), "elapsed");
+    for (auto& e : evs) (void)hipEventDestroy(e);
+    if (collectStats) readStats(st.d.p, &tRayStats);
+    tRayStats.sort_ms = order != QR_ORDER_INPUT ? sortMs : 0.0;
+    return total;
+}
+}  // namespace
+
+int qr_raycast(const qr_bvh* bvh, qr_rays* r, int collectStats, int order, int wide, double* ms) {
+    return guard([&] {
+        need(r != nullptr, "null ray buffer");
+        const double t = raycastImpl(bvh, r->rays.p, r->count, r->results.p, r, collectStats, order, wide);
+        if (ms) *ms = t;
+    });
+}
+
+int qr_raycast_device(const qr_bvh* bvh, const void* rays, int n, uint64_t* results, int order,
+                      int wide, double* ms) {
+    return guard([&] {
+        need(n >= 0, "negative ray count");
+        qr_rays scratch;
+        const double t = raycastImpl(bvh, static_cast<const Ray*>(rays), n, results, &scratch, 0, order, wide);
+        if (ms) *ms = t;
+    });
+}
+
+int qr_rays_summarize(const qr_rays* r, uint64_t* perRay, qr_ray_stats* out) {
+    return guard([&] {
+        need(r && out, "null argument");
+        std::vector<uint64_t> host(static_cast<size_t>(r->count));
+        if (r->count > 0)
+            hipCheck(hipMemcpy(host.data(), r->results.p, host.size() * sizeof(uint64_t), hipMemcpyDeviceToHost), "D2H results");
+        qr_ray_stats st{0, 0, 0xcbf29ce484222325ull, 0.0};
+        for (uint64_t v : host) {
+            ++st.rays;
+            if (v >> 32) {
+                ++st.hits;
+                st.sum_t += static_cast<double>(bitsFloat(static_cast<uint32_t>(v)));
+            }
+            st.hash ^= v;
+            st.hash *= 0x100000001b3ull;
+        }
+        if (perRay && !host.empty()) std::memcpy(perRay, host.data(), host.size() * sizeof(uint64_t));
+        *out = st;
+    });
+}
+
+int qr_last_traversal_stats(qr_traversal_stats* out) {
+    if (!out) return QR_ERR_INVALID;
+    *out = tRayStats;
+    return QR_OK;
+}
+
+// ---- point queries ------------------------------------------------------------------------
+
+int qr_points_upload(const float* pts, int n, int rpp, qr_points** out) {
+    return guard([&] {
+        need(out != nullptr, "null output handle");
+        need(n >= 0, "negative point count");
+        need(pts != nullptr || n == 0, "null point array");
+        auto* p = new qr_points();
+        try {
+            p->count = n;
+            p->stride = rpp > 0 ? rpp : 1;
+            p->pts.alloc(static_cast<size_t>(n));
+            p->results.alloc(static_cast<size_t>(n) * p->stride);
+            if (n > 0)
+                hipCheck(hipMemcpy(p->pts.p, pts, static_cast<size_t>(n) * sizeof(V3), hipMemcpyHostToDevice), "H2D points");
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+int qr_points_destroy(qr_points* p) {
+    return guard([&] { delete p; });
+}
+
+int qr_points_info(const qr_points* p, int* count, int* stride) {
+    if (!p) return QR_ERR_INVALID;
+    if (count) *count = p->count;
+    if (stride) *stride = p->stride;
+    return QR_OK;
+}
+
+namespace {
+
+// Shared body of closest and kNN (k = 0 selects closest).
+double pointQuery(const qr_bvh* bvh, const V3* pts, int n, int k, int outStride, ClosestResult* out,
+                  qr_points* scratch, int collectStats, int order, int wide) {
+    checkTree(bvh, wide != 0);
+    need(order == QR_ORDER_INPUT || order == QR_ORDER_MORTON, "query order must be input or morton");
+    tQueryStats = qr_traversal_stats{};
+    if (n <= 0) return 0.0;
+    hipStream_t s = currentStream();
+    Stats st;
+    if (collectStats) zeroStats(st.d.p, s);
+    hipEvent_t evs[3];
+    for (auto& e : evs) hipCheck(hipEventCreate(&e), "event");
+    hipCheck(hipEventRecord(evs[0], s), "event");
+    TraverseArgs a = argsFor(bvh->s, wide != 0);
+    a.stats = collectStats ? st.d.p : nullptr;
+    if (order == QR_ORDER_MORTON) a.order = reorder(*scratch, reinterpret_cast<const float*>(pts), 3, n, 0, s);
+    hipCheck(hipEventRecord(evs[1], s), "event");
+    if (k == 0) {
+        launchClosest(a, pts, n, out, s);
+    } else {
+        launchKnn(a, pts, n, k, outStride, out, s);
+    }
+    hipCheck(hipEventRecord(evs[2], s), "event");
+    hipCheck(hipEventSynchronize(evs[2]), "query sync");
+    float total = 0.0f, sortMs = 0.0f;
+    hipCheck(hipEventElapsedTime(&total, evs[0], evs[2]), "elapsed");
+    hipCheck(hipEventElapsedTime(&sortMs, evs[0], evs[1]), "elapsed");
+    for (auto& e : evs) (void)hipEventDestroy(e);
+    if (collectStats) readStats(st.d.p, &tQueryStats);
+    tQueryStats.sort_ms = order == QR_ORDER_MORTON ? sortMs : 0.0;
+    return total;
+}
+
+}  // namespace
+
+int qr_closest(const qr_bvh* bvh, qr_points* p, int collectStats, int order, int wide, double* ms) {
+    return guard([&] {
+        need(p != nullptr, "null point buffer");
+        const double t = pointQuery(bvh, p->pts.p, p->count, 0, p->stride, p->results.p, p, collectStats, order, wide);
+        if (ms) *ms = t;
+    });
+}
+
+int qr_knn(const qr_bvh* bvh, qr_points* p, int k, int collectStats, int order, int wide, double* ms) {
+    return guard([&] {
+        need(p != nullptr, "null point buffer");
+        need(k >= 1 && k <= 32, "kNearestGPU: k must be in [1, 32]");
+        need(p->stride >= k, "kNearestGPU: result buffer was uploaded with a smaller stride");
+        const double t = pointQuery(bvh, p->pts.p, p->count, k, p->stride, p->results.p, p, collectStats, order, wide);
+        if (ms) *ms = t;
+    });
+}
+
+int qr_closest_device(const qr_bvh* bvh, const float* pts, int n, void* results, int order, int wide,
+                      double* ms) {
+    return guard([&] {
+        qr_points scratch;
+        const double t = pointQuery(bvh, reinterpret_cast<const V3*>(pts), n, 0, 1,
+                                    static_cast<ClosestResult*>(results), &scratch, 0, order, wide);
+        if (ms) *ms = t;
+    });
+}
+
+int qr_knn_device(const qr_bvh* bvh, const float* pts, int n, int k, void* results, int order,
+                  int wide, double* ms) {
+    return guard([&] {
+        need(k >= 1 && k <= 32, "kNearestGPU: k must be in [1, 32]");
+        qr_points scratch;
+        const double t = pointQuery(bvh, reinterpret_cast<const V3*>(pts), n, k, k,
+                                    static_cast<ClosestResult*>(results), &scratch, 0, order, wide);
+        if (ms) *ms = t;
+    });
+}
+
+int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, int order, int wide,
+              qr_csr** out, double* ms) {
+    return guard([&] {
+        need(p != nullptr && out != nullptr, "null argument");
+        checkTree(bvh, wide != 0);
+        need(order == QR_ORDER_INPUT || order == QR_ORDER_MORTON, "query order must be input or morton");
+        tQueryStats = qr_traversal_stats{};
+        auto* res = new qr_csr();
+        try {
+            const int n = p->count;
+            res->count = n;
+            res->offsets.alloc(static_cast<size_t>(n) + 1);
+            if (n <= 0 || radius < 0.0f) {
+                // Empty CSR: every segment is empty (the reference returns no hits for r < 0).
+                std::vector<int> zeros(static_cast<size_t>(n) + 1, 0);
+                hipCheck(hipMemcpy(res->offsets.p, zeros.data(), zeros.size() * sizeof(int), hipMemcpyHostToDevice), "H2D offsets");
+                *out = res;
+                if (ms) *ms = 0.0;
+                return;
+            }
+            hipStream_t s = currentStream();
+            Stats st;
+            if (collectStats) zeroStats(st.d.p, s);
+            const float r2 = radius * radius;
+            hipEvent_t evs[3];
+            for (auto& e : evs) hipCheck(hipEventCreate(&e), "event");
+            hipCheck(hipEventRecord(evs[0], s), "event");
+            TraverseArgs a = argsFor(bvh->s, wide != 0);
+            a.stats = collectStats ? st.d.p : nullptr;
+            if (order == QR_ORDER_MORTON) a.order = reorder(*p, reinterpret_cast<const float*>(p->pts.p), 3, n, 0, s);
+            hipCheck(hipEventRecord(evs[1], s), "event");
+            DevBuf<int> counts(static_cast<size_t>(n));
+            DevBuf<int> scratch(static_cast<size_t>(std::max(scanScratchInts(n), 4 + 2 * n)));
+            launchRadiusCount(a, p->pts.p, n, r2, counts.p, s);
+            scanExclusive(counts.p, res->offsets.p, n, res->offsets.p + n, scratch.p, s);
+            int total = 0;
+            hipCheck(hipMemcpyAsync(&total, res->offsets.p + n, sizeof(int), hipMemcpyDeviceToHost, s), "D2H total");
+            hipCheck(hipStreamSynchronize(s), "radius total sync");
+            res->total = total;
+            res->keys.alloc(static_cast<size_t>(total > 0 ? total : 1));
+            if (total > 0) {
+                a.stats = nullptr;
+                launchRadiusFill(a, p->pts.p, n, r2, res->offsets.p, res->keys.p, s);
+                segmentSort(res->offsets.p, res->keys.p, n, 1 << 30, scratch.p, s);
+            }
+            hipCheck(hipEventRecord(evs[2], s), "event");
+            hipCheck(hipEventSynchronize(evs[2]), "radius sync");
+            float total_ms = 0.0f, sortMs = 0.0f;
+            hipCheck(hipEventElapsedTime(&total_ms, evs[0], evs[2]), "elapsed");
+            hipCheck(hipEventElapsedTime(&sortMs, evs[0], evs[1]), "elapsed");
+            for (auto& e : evs) (void)hipEventDestroy(e);
+            if (collectStats) readStats(st.d.p, &tQueryStats);
+            tQueryStats.sort_ms = order == QR_ORDER_MORTON ? sortMs : 0.0;
+            if (ms) *ms = total_ms;
+        } catch (...) {
+            delete res;
+            throw;
+        }
+        *out = res;
+    });
+}
+
+int qr_points_read(const qr_points* p, void* host) {
+    return guard([&] {
+        need(p != nullptr && host != nullptr, "null argument");
+        const size_t bytes = static_cast<size_t>(p->count) * p->stride * sizeof(ClosestResult);
+        if (bytes) hipCheck(hipMemcpy(host, p->results.p, bytes, hipMemcpyDeviceToHost), "D2H results");
+    });
+}
+
+int qr_radius_info(const qr_csr* r, int* count, int* total) {
+    if (!r) return QR_ERR_INVALID;
+    if (count) *count = r->count;
+    if (total) *total = r->total;
+    return QR_OK;
+}
+
+int qr_radius_read(const qr_csr* r, int* offsets, uint64_t* keys) {
+    return guard([&] {
+        need(r != nullptr, "null radius buffer");
+        if (offsets)
+            hipCheck(hipMemcpy(offsets, r->offsets.p, (static_cast<size_t>(r->count) + 1) * sizeof(int), hipMemcpyDeviceToHost), "D2H offsets");
+        if (keys && r->total > 0)
+            hipCheck(hipMemcpy(keys, r->keys.p, static_cast<size_t>(r->total) * sizeof(uint64_t), hipMemcpyDeviceToHost), "D2H keys");
+    });
+}
+
+int qr_radius_destroy(qr_csr* r) {
+    return guard([&] { delete r; });
+}
+
+int qr_last_query_stats(qr_traversal_stats* out) {
+    if (!out) return QR_ERR_INVALID;
+    *out = tQueryStats;
+    return QR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,456 @@
+// LBVH build for gfx950: centroid bounds -> 30-bit Morton codes (fused with the
+// radix histogram) -> onesweep sort -> Karras 2012 hierarchy emission ->
+// fused leaf scatter + bottom-up box propagation with per-node arrival counters.
+//
+// Topology and node bytes follow the reference's LBVH exactly
+// (src/gpu/lbvh.cu:277-315 Morton, :366-414 Karras, :345-354 leaf boxes,
+// :472-514 parent box merge), so treeHash equals the reference's for the same
+// input.  What differs is the schedule: the reference sorts 64-bit keys with 8
+// digit passes, computes depths by walking parent chains, depth-sorts the nodes
+// and launches one merge kernel per level; here the sort runs 4 passes and the
+// boxes are merged in a single bottom-up pass where the second child to arrive
+// at a node merges it (no depth table is needed to build).
+#include "qr_bvh.h"
+
+namespace qr {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---- coherent (agent-scope) accessors for data produced inside one kernel ------
+// Boxes written by one workgroup are read by another that may sit on a different
+// XCD with its own L2.  sc1 loads/stores keep those accesses coherent; the
+// explicit vmcnt wait before the arrival atomic orders the box stores ahead of it.
+__device__ __forceinline__ void stAgent(float* p, float v) {
+    __hip_atomic_store(reinterpret_cast<uint32_t*>(p), floatBits(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ldAgent(const float* p) {
+    return bitsFloat(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void stAgentI(int* p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ldAgentI(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void storesDone() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+__device__ __forceinline__ void writeBoxAgent(Node2* nd, const Box& b) {
+    stAgent(&nd->lo[0], b.lo.x);
+    stAgent(&nd->lo[1], b.lo.y);
+    stAgent(&nd->lo[2], b.lo.z);
+    stAgent(&nd->hi[0], b.hi.x);
+    stAgent(&nd->hi[1], b.hi.y);
+    stAgent(&nd->hi[2], b.hi.z);
+}
+__device__ __forceinline__ Box readBoxAgent(const Node2* nd) {
+    return {{ldAgent(&nd->lo[0]), ldAgent(&nd->lo[1]), ldAgent(&nd->lo[2])},
+            {ldAgent(&nd->hi[0]), ldAgent(&nd->hi[1]), ldAgent(&nd->hi[2])}};
+}
+
+// ---- centroid bounds -----------------------------------------------------------
+// cb[0..2] = ordered(min), cb[3..5] = ordered(max) (src/gpu/lbvh.cu:233-275).
+
+__global__ void buildPrepKernel(uint32_t* __restrict__ cb, uint32_t* __restrict__ hist,
+                                uint32_t* __restrict__ desc, long long descWords,
+                                uint32_t* __restrict__ tickets) {
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long j = i; j < descWords; j += stride) desc[j] = 0u;
+    if (i < 4 * 256) hist[i] = 0u;
+    if (i < 4) tickets[i] = 0u;
+    if (i < 3) cb[i] = orderedBits(kInf);
+    if (i >= 3 && i < 6) cb[i] = orderedBits(-kInf);
+}
+
+template <typename L>
+__global__ __launch_bounds__(kBlock) void centroidBoundsKernel(
+    const typename L::Item* __restrict__ items, int n, uint32_t* __restrict__ cb) {
+    float lo0 = kInf, lo1 = kInf, lo2 = kInf;
+    float hi0 = -kInf, hi1 = -kInf, hi2 = -kInf;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kBlock * gridDim.x) {
+        const V3 c = L::centroid(items[i]);
+        lo0 = fminf(lo0, c.x);
+        lo1 = fminf(lo1, c.y);
+        lo2 = fminf(lo2, c.z);
+        hi0 = fmaxf(hi0, c.x);
+        hi1 = fmaxf(hi1, c.y);
+        hi2 = fmaxf(hi2, c.z);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lo0 = fminf(lo0, __shfl_xor(lo0, off, 64));
+        lo1 = fminf(lo1, __shfl_xor(lo1, off, 64));
+        lo2 = fminf(lo2, __shfl_xor(lo2, off, 64));
+        hi0 = fmaxf(hi0, __shfl_xor(hi0, off, 64));
+        hi1 = fmaxf(hi1, __shfl_xor(hi1, off, 64));
+        hi2 = fmaxf(hi2, __shfl_xor(hi2, off, 64));
+    }
+    __shared__ float s[6][kBlock / 64];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s[0][w] = lo0;
+        s[1][w] = lo1;
+        s[2][w] = lo2;
+        s[3][w] = hi0;
+        s[4][w] = hi1;
+        s[5][w] = hi2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int c = threadIdx.x;
+        float v = s[c][0];
+        for (int k = 1; k < kBlock / 64; ++k) v = c < 3 ? fminf(v, s[c][k]) : fmaxf(v, s[c][k]);
+        if (c < 3) {
+            atomicMin(&cb[c], orderedBits(v));
+        } else {
+            atomicMax(&cb[c], orderedBits(v));
+        }
+    }
+}
+
+// ---- Morton codes + radix histogram ----------------------------------------------
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// quantize (src/gpu/lbvh.cu:289-294): (v - lo) / extent * 1023 + 0.5, clamped, truncated.
+__device__ __forceinline__ uint32_t quantize1023(float v, float lo, float extent) {
+    if (extent <= 0.0f) return 0u;
+    const float q = (v - lo) / extent * 1023.0f + 0.5f;
+    return static_cast<uint32_t>(static_cast<int>(fminf(1023.0f, fmaxf(0.0f, q))));
+}
+
+template <typename L>
+__global__ __launch_bounds__(kBlock) void mortonHistKernel(const typename L::Item* __restrict__ items,
+                                                           int n, const uint32_t* __restrict__ cb,
+                                                           uint32_t* __restrict__ codes,
+                                                           uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[4][256];
+    for (int i = threadIdx.x; i < 4 * 256; i += kBlock) (&h[0][0])[i] = 0u;
+    const float lo0 = fromOrderedBits(cb[0]), lo1 = fromOrderedBits(cb[1]),
+                lo2 = fromOrderedBits(cb[2]);
+    const float hi0 = fromOrderedBits(cb[3]), hi1 = fromOrderedBits(cb[4]),
+                hi2 = fromOrderedBits(cb[5]);
+    const float ex0 = hi0 - lo0, ex1 = hi1 - lo1, ex2 = hi2 - lo2;
+    __syncthreads();
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kBlock * gridDim.x) {
+        const V3 c = L::centroid(items[i]);
+        const uint32_t code = spread10(quantize1023(c.x, lo0, ex0)) |
+                              (spread10(quantize1023(c.y, lo1, ex1)) << 1) |
+                              (spread10(quantize1023(c.z, lo2, ex2)) << 2);
+        codes[i] = code;
+        atomicAdd(&h[0][code & 255u], 1u);
+        atomicAdd(&h[1][(code >> 8) & 255u], 1u);
+        atomicAdd(&h[2][(code >> 16) & 255u], 1u);
+        atomicAdd(&h[3][code >> 24], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 4 * 256; i += kBlock) {
+        const uint32_t v = (&h[0][0])[i];
+        if (v) atomicAdd(&hist[i], v);
+    }
+}
+
+// ---- Karras 2012 hierarchy ---------------------------------------------------------
+// Common-prefix length of the 64-bit keys (code << 32 | index) at sorted
+// positions a and b; -1 outside [0, n) (src/gpu/lbvh.cu:373-377).
+__device__ __forceinline__ int prefixLen(const uint32_t* __restrict__ code,
+                                         const uint32_t* __restrict__ idx, int n, int a, int b) {
+    if (b < 0 || b >= n) return -1;
+    const uint32_t ca = code[a], cb = code[b];
+    if (ca != cb) return __clz(ca ^ cb);
+    const uint32_t x = idx[a] ^ idx[b];
+    return x == 0u ? 64 : 32 + __clz(x);
+}
+
+__global__ __launch_bounds__(kBlock) void karrasKernel(const uint32_t* __restrict__ code,
+                                                       const uint32_t* __restrict__ idx, int n,
+                                                       Node2* __restrict__ nodes,
+                                                       int* __restrict__ parent,
+                                                       int* __restrict__ arrive) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    arrive[i] = 0;
+    if (i == 0) parent[0] = -1;
+
+    const int dir = prefixLen(code, idx, n, i, i + 1) > prefixLen(code, idx, n, i, i - 1) ? 1 : -1;
+    const int dmin = prefixLen(code, idx, n, i, i - dir);
+
+    // Exponential search for an upper bound on the range length, then binary
+    // search for the far end j.
+    int span = 2;
+    while (prefixLen(code, idx, n, i, i + span * dir) > dmin) span <<= 1;
+    int len = 0;
+    for (int step = span >> 1; step > 0; step >>= 1) {
+        if (prefixLen(code, idx, n, i, i + (len + step) * dir) > dmin) len += step;
+    }
+    const int j = i + len * dir;
+    const int first = i < j ? i : j;
+    const int last = i < j ? j : i;
+
+    // Split: the last position whose key still shares more than the node's common
+    // prefix with the first key of the range.
+    const int dnode = prefixLen(code, idx, n, i, j);
+    int lo = first, hi = last;  // invariant: prefixLen(first, lo) > dnode
+    while (hi - lo > 1) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (prefixLen(code, idx, n, first, mid) > dnode) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    const int split = lo;
+
+    const int left = (split == first) ? (n - 1 + split) : split;
+    const int right = (split + 1 == last) ? (n - 1 + last) : split + 1;
+    nodes[i].left = left;
+    nodes[i].right = right;
+    parent[left] = i;
+    parent[right] = i;
+}
+
+// ---- leaf scatter + bottom-up propagation -------------------------------------------
+// One thread per leaf slot k: gather the item into leaf order, write leaf node
+// n-1+k, then climb.  At each parent the arrival counter decides: the first child
+// to arrive stops, the second merges both child boxes (left operand = left child,
+// as the reference's refit does) and continues.  Height (edges to the deepest
+// leaf) rides along, so the root's height is the tree depth.
+template <typename L, bool kGather>
+__global__ __launch_bounds__(kBlock) void leafUpKernel(
+    const typename L::Item* __restrict__ src, const uint32_t* __restrict__ order, int n,
+    typename L::Item* __restrict__ sorted, int* __restrict__ primIds, Node2* __restrict__ nodes,
+    const int* __restrict__ parent, int* __restrict__ arrive, int* __restrict__ height) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+
+    typename L::Item item;
+    if (kGather) {
+        const int sceneIdx = static_cast<int>(order[k]);
+        item = src[sceneIdx];
+        sorted[k] = item;
+        primIds[k] = sceneIdx;
+    } else {
+        item = sorted[k];
+    }
+    Box box = L::bounds(item);
+    Node2* leaf = &nodes[n - 1 + k];
+    writeBoxAgent(leaf, box);
+    leaf->left = -(k + 1);
+    leaf->right = 1;
+
+    int node = n - 1 + k;
+    int h = 0;
+    while (true) {
+        const int p = parent[node];
+        if (p < 0) break;
+        storesDone();
+        const int prior = __hip_atomic_fetch_add(&arrive[p], 1, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+        if (prior == 0) break;
+        const int l = nodes[p].left;
+        const int r = nodes[p].right;
+        const int sib = (l == node) ? r : l;
+        const Box sb = readBoxAgent(&nodes[sib]);
+        const int sh = sib >= n - 1 ? 0 : ldAgentI(&height[sib]);
+        box = (l == node) ? mergeChildren(box, sb) : mergeChildren(sb, box);
+        h = (h > sh ? h : sh) + 1;
+        writeBoxAgent(&nodes[p], box);
+        stAgentI(&height[p], h);
+        node = p;
+    }
+}
+
+}  // namespace
+
+template <typename L>
+static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStream_t s,
+                       BuildTimingsC* t) {
+    using Item = typename L::Item;
+    b.n = n;
+    b.nodeCount = 2 * n - 1;
+    b.nodes.alloc(static_cast<size_t>(b.nodeCount));
+    b.items.alloc(static_cast<size_t>(n) * sizeof(Item));
+    b.primIds.alloc(static_cast<size_t>(n));
+    b.parent.alloc(static_cast<size_t>(b.nodeCount));
+    b.arrive.alloc(static_cast<size_t>(n - 1));
+    b.height.alloc(static_cast<size_t>(n - 1));
+    b.sortWs.reserve(static_cast<size_t>(n));
+    b.counters.reserve(16);
+
+    DevBuf<uint32_t> codes(static_cast<size_t>(n));
+    DevBuf<uint32_t> codesSorted(static_cast<size_t>(n));
+    DevBuf<uint32_t> order(static_cast<size_t>(n));
+    uint32_t* cb = reinterpret_cast<uint32_t*>(b.counters.p);
+
+    hipEvent_t ev[6];
+    for (auto& e : ev) hipCheck(hipEventCreate(&e), "event create");
+    hipCheck(hipEventRecord(ev[0], s), "event");
+
+    const int tiles = sortTiles(static_cast<size_t>(n));
+    const long long descWords = 4LL * tiles * 256;
+    int prepGrid = blocksFor(descWords > 1024 ? descWords : 1024, kBlock);
+    if (prepGrid > 1024) prepGrid = 1024;
+    buildPrepKernel<<<prepGrid, kBlock, 0, s>>>(cb, b.sortWs.hist.p, b.sortWs.desc.p, descWords,
+                                                b.sortWs.tickets.p);
+    int grid = blocksFor(n, kBlock * 4);
+    if (grid > 2048) grid = 2048;
+    centroidBoundsKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb);
+    mortonHistKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb, codes.p, b.sortWs.hist.p);
+    hipCheck(hipGetLastError(), "morton");
+    hipCheck(hipEventRecord(ev[1], s), "event");
+
+    sortPairs(b.sortWs, codes.p, nullptr, codesSorted.p, order.p, static_cast<size_t>(n), 4, s);
+    hipCheck(hipEventRecord(ev[2], s), "event");
+
+    karrasKernel<<<blocksFor(n - 1, kBlock), kBlock, 0, s>>>(codesSorted.p, order.p, n, b.nodes.p,
+                                                            b.parent.p, b.arrive.p);
+    hipCheck(hipGetLastError(), "karras");
+    hipCheck(hipEventRecord(ev[3], s), "event");
+
+    leafUpKernel<L, true><<<blocksFor(n, kBlock), kBlock, 0, s>>>(
+        dSrc, order.p, n, reinterpret_cast<Item*>(b.items.p), b.primIds.p, b.nodes.p, b.parent.p,
+        b.arrive.p, b.height.p);
+    hipCheck(hipGetLastError(), "leaf up");
+    hipCheck(hipEventRecord(ev[4], s), "event");
+
+    int depth = 0;
+    hipCheck(hipMemcpyAsync(&depth, b.height.p, sizeof(int), hipMemcpyDeviceToHost, s), "depth");
+    hipCheck(hipEventRecord(ev[5], s), "event");
+    hipCheck(hipEventSynchronize(ev[5]), "build sync");
+    b.depth = depth;
+
+    if (t) {
+        float ms[5];
+        for (int i = 0; i < 5; ++i) hipCheck(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]), "elapsed");
+        t->mortonMs = ms[0];
+        t->sortMs = ms[1];
+        t->treeMs = ms[2];
+        t->scatterMs = 0.0;
+        t->refitMs = ms[3];
+        t->sahBinMs = t->sahSplitMs = t->sahPartMs = 0.0;
+        float total = 0;
+        hipCheck(hipEventElapsedTime(&total, ev[0], ev[4]), "elapsed");
+        t->totalMs = total;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    b.primSlotValid = false;
+    b.depthValid = false;
+    b.wideValid = false;
+    b.wideCount = 0;
+}
+
+void buildLbvhDevice(BvhState& b, int kind, const void* dItems, int n, hipStream_t s,
+                     BuildTimingsC* t) {
+    if (n < 2) throw ArgError("build: at least 2 primitives required");
+    b.kind = kind;
+    b.builder = kBuildLbvh;
+    switch (kind) {
+        case kPrimTri:
+            buildLbvhT<TriLeaf>(b, static_cast<const Tri*>(dItems), n, s, t);
+            break;
+        case kPrimPoint:
+            buildLbvhT<PointLeaf>(b, static_cast<const V3*>(dItems), n, s, t);
+            break;
+        case kPrimGauss:
+            buildLbvhT<GaussLeaf>(b, static_cast<const Gaussian*>(dItems), n, s, t);
+            break;
+        default:
+            throw ArgError("build: unknown primitive kind");
+    }
+}
+
+// Refit of every leaf from the leaf-ordered items already in b.items: reset the
+// arrival counters, then the same bottom-up pass without the gather.
+namespace {
+__global__ void resetArriveKernel(int* __restrict__ arrive, int count) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < count) arrive[i] = 0;
+}
+}  // namespace
+
+void propagateAll(BvhState& b, hipStream_t s) {
+    const int n = b.n;
+    resetArriveKernel<<<blocksFor(n - 1, kBlock), kBlock, 0, s>>>(b.arrive.p, n - 1);
+    const int grid = blocksFor(n, kBlock);
+    switch (b.kind) {
+        case kPrimTri:
+            leafUpKernel<TriLeaf, false><<<grid, kBlock, 0, s>>>(
+                nullptr, nullptr, n, reinterpret_cast<Tri*>(b.items.p), nullptr, b.nodes.p,
+                b.parent.p, b.arrive.p, b.height.p);
+            break;
+        case kPrimPoint:
+            leafUpKernel<PointLeaf, false><<<grid, kBlock, 0, s>>>(
+                nullptr, nullptr, n, reinterpret_cast<V3*>(b.items.p), nullptr, b.nodes.p,
+                b.parent.p, b.arrive.p, b.height.p);
+            break;
+        default:
+            leafUpKernel<GaussLeaf, false><<<grid, kBlock, 0, s>>>(
+                nullptr, nullptr, n, reinterpret_cast<Gaussian*>(b.items.p), nullptr, b.nodes.p,
+                b.parent.p, b.arrive.p, b.height.p);
+            break;
+    }
+    hipCheck(hipGetLastError(), "propagate");
+}
+
+// ---- depth by pointer jumping --------------------------------------------------------
+namespace {
+// Each node walks towards the root by jumping through ancestors' (anc, dist)
+// pairs; pairs are published as one 64-bit word so a reader always sees a
+// consistent jump.  Every step strictly shortens the remaining path, so the
+// loop ends within depth iterations (typically ~log2(depth)).
+__global__ void depthJumpKernel(const int* __restrict__ parent, int nodeCount,
+                                unsigned long long* __restrict__ jump, int* __restrict__ depth) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nodeCount) return;
+    int anc = parent[i];
+    int dist = anc < 0 ? 0 : 1;
+    while (anc > 0) {
+        const unsigned long long pj = __hip_atomic_load(&jump[anc], __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+        const int a2 = static_cast<int>(pj & 0xffffffffu);
+        const int d2 = static_cast<int>(pj >> 32);
+        dist += d2;
+        anc = a2;
+        const unsigned long long mine = (static_cast<unsigned long long>(static_cast<uint32_t>(dist)) << 32) |
+                                        static_cast<uint32_t>(anc);
+        __hip_atomic_store(&jump[i], mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    depth[i] = dist;
+}
+
+__global__ void depthJumpInitKernel(const int* __restrict__ parent, int nodeCount,
+                                    unsigned long long* __restrict__ jump) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nodeCount) return;
+    const int p = parent[i];
+    // Root: (anc 0, dist 0) so anything that reaches it stops there.
+    const int anc = p < 0 ? 0 : p;
+    const int dist = p < 0 ? 0 : 1;
+    jump[i] = (static_cast<unsigned long long>(static_cast<uint32_t>(dist)) << 32) |
+              static_cast<uint32_t>(anc);
+}
+}  // namespace
+
+void computeDepth(BvhState& b, hipStream_t s) {
+    if (b.depthValid) return;
+    const int nc = b.nodeCount;
+    b.depthArr.reserve(static_cast<size_t>(nc));
+    DevBuf<unsigned long long> jump(static_cast<size_t>(nc));
+    depthJumpInitKernel<<<blocksFor(nc, kBlock), kBlock, 0, s>>>(b.parent.p, nc, jump.p);
+    depthJumpKernel<<<blocksFor(nc, kBlock), kBlock, 0, s>>>(b.parent.p, nc, jump.p, b.depthArr.p);
+    hipCheck(hipGetLastError(), "depth");
+    hipCheck(hipStreamSynchronize(s), "depth sync");
+    b.depthValid = true;
+}
+
+}  // namespace qr

@@ -1,0 +1,383 @@
+// Tree maintenance: 4-wide collapse, refit (full and dirty-only), structural
+// validation and the node-array hash.
+#include "qr_bvh.h"
+
+namespace qr {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ void stAgent(float* p, float v) {
+    __hip_atomic_store(reinterpret_cast<uint32_t*>(p), floatBits(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ldAgent(const float* p) {
+    return bitsFloat(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void storesDone() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ __forceinline__ void writeBoxAgent(Node2* nd, const Box& b) {
+    stAgent(&nd->lo[0], b.lo.x);
+    stAgent(&nd->lo[1], b.lo.y);
+    stAgent(&nd->lo[2], b.lo.z);
+    stAgent(&nd->hi[0], b.hi.x);
+    stAgent(&nd->hi[1], b.hi.y);
+    stAgent(&nd->hi[2], b.hi.z);
+}
+__device__ __forceinline__ Box readBoxAgent(const Node2* nd) {
+    return {{ldAgent(&nd->lo[0]), ldAgent(&nd->lo[1]), ldAgent(&nd->lo[2])},
+            {ldAgent(&nd->hi[0]), ldAgent(&nd->hi[1]), ldAgent(&nd->hi[2])}};
+}
+
+// ---- 4-wide collapse ----------------------------------------------------------------
+// Even-depth internal nodes become 4-wide nodes holding their children, or, for
+// internal children, their grandchildren (src/gpu/lbvh.cu:731-807).  Leaves sit
+// at odd depth (direct children) or even depth (grandchildren), so every leaf is
+// referenced exactly once.
+
+__global__ void wideFlagKernel(const int* __restrict__ depth, int internalCount,
+                               int* __restrict__ flags) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < internalCount) flags[i] = (depth[i] & 1) == 0 ? 1 : 0;
+}
+
+__device__ __forceinline__ void setEntry(WideEntry& e, const Node2& box, int child) {
+    e.lo[0] = box.lo[0];
+    e.lo[1] = box.lo[1];
+    e.lo[2] = box.lo[2];
+    e.hi[0] = box.hi[0];
+    e.hi[1] = box.hi[1];
+    e.hi[2] = box.hi[2];
+    e.child = child;
+    e.pad = 0;
+}
+
+__device__ __forceinline__ void clearEntry(WideEntry& e) {
+    e.lo[0] = e.lo[1] = e.lo[2] = kInf;
+    e.hi[0] = e.hi[1] = e.hi[2] = -kInf;
+    e.child = kWideEmpty;
+    e.pad = 0;
+}
+
+__global__ void wideEmitKernel(const Node2* __restrict__ nodes, int n,
+                               const int* __restrict__ depth, const int* __restrict__ wideId,
+                               Wide4* __restrict__ wide) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1 || (depth[i] & 1) != 0) return;
+    Wide4 out;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) clearEntry(out.c[s]);
+    const Node2 nd = nodes[i];
+    const int kids[2] = {nd.left, nd.right};
+    int slot = 0;
+    for (int k = 0; k < 2; ++k) {
+        const int c = kids[k];
+        const Node2 cn = nodes[c];
+        if (c >= n - 1) {
+            setEntry(out.c[slot++], cn, cn.left);
+            continue;
+        }
+        const int g[2] = {cn.left, cn.right};
+        for (int j = 0; j < 2; ++j) {
+            const Node2 gn = nodes[g[j]];
+            setEntry(out.c[slot++], gn, g[j] >= n - 1 ? gn.left : wideId[g[j]]);
+        }
+    }
+    wide[wideId[i]] = out;
+}
+
+// ---- refit --------------------------------------------------------------------------
+
+// Full refit: gather scene-order items into leaf order, then the shared
+// bottom-up pass.  primSlot (scene index -> slot) is the inverse of primIds.
+__global__ void invertPermKernel(const int* __restrict__ primIds, int n, int* __restrict__ primSlot) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < n) primSlot[primIds[k]] = k;
+}
+
+template <typename L>
+__global__ void gatherKernel(const typename L::Item* __restrict__ src, const int* __restrict__ primIds,
+                             int n, typename L::Item* __restrict__ dst) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < n) dst[k] = src[primIds[k]];
+}
+
+// Dirty refit, pass 1: write the new items and leaf boxes, then mark the paths to
+// the root.  need[p] ends as the number of p's children that lie on a dirty path
+// (1 or 2): the first visitor of p continues upward, the second stops there.
+template <typename L>
+__global__ void dirtyMarkKernel(const typename L::Item* __restrict__ staged,
+                                const int* __restrict__ sceneIds, int count,
+                                const int* __restrict__ primSlot, int n,
+                                typename L::Item* __restrict__ items, Node2* __restrict__ nodes,
+                                const int* __restrict__ parent, int* __restrict__ need,
+                                int* __restrict__ leafOut) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    const int slot = primSlot[sceneIds[i]];
+    const typename L::Item it = staged[i];
+    items[slot] = it;
+    const Box b = L::bounds(it);
+    Node2* leaf = &nodes[n - 1 + slot];
+    leaf->lo[0] = b.lo.x;
+    leaf->lo[1] = b.lo.y;
+    leaf->lo[2] = b.lo.z;
+    leaf->hi[0] = b.hi.x;
+    leaf->hi[1] = b.hi.y;
+    leaf->hi[2] = b.hi.z;
+    leafOut[i] = n - 1 + slot;
+    int p = parent[n - 1 + slot];
+    while (p >= 0) {
+        if (atomicAdd(&need[p], 1) != 0) break;
+        p = parent[p];
+    }
+}
+
+// Dirty refit, pass 2: climb from every dirty leaf; at a node the last of its
+// dirty children to arrive merges it and resets the node's counters, so the
+// counters are all zero again when the pass ends.
+__global__ void dirtyClimbKernel(const int* __restrict__ leaves, int count, Node2* __restrict__ nodes,
+                                 const int* __restrict__ parent, int* __restrict__ need,
+                                 int* __restrict__ arrive) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    int node = leaves[i];
+    while (true) {
+        const int p = parent[node];
+        if (p < 0) break;
+        storesDone();
+        const int prior = __hip_atomic_fetch_add(&arrive[p], 1, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+        const int want = __hip_atomic_load(&need[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (prior + 1 < want) break;
+        const int l = nodes[p].left;
+        const int r = nodes[p].right;
+        const Box lb = readBoxAgent(&nodes[l]);
+        const Box rb = readBoxAgent(&nodes[r]);
+        writeBoxAgent(&nodes[p], mergeChildren(lb, rb));
+        __hip_atomic_store(&arrive[p], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&need[p], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        node = p;
+    }
+}
+
+__global__ void zeroIntKernel(int* __restrict__ p, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ---- validation (src/gpu/lbvh.cu:613-729) --------------------------------------------
+// errors: 0 bad children or parent box, 1 bad leaf box, 2 bad reference count,
+// 3 bad leaf id, 4 bad slot -> scene map or payload, 5 bad depth.
+
+template <typename L>
+__global__ void checkNodesKernel(const Node2* __restrict__ nodes,
+                                 const typename L::Item* __restrict__ sorted, int n, int nodeCount,
+                                 int* __restrict__ visited, int* __restrict__ errors) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nodeCount) return;
+    const Node2 nd = nodes[i];
+    if (i < n - 1) {
+        if (nd.left < 0 || nd.left >= nodeCount || nd.right < 0 || nd.right >= nodeCount) {
+            atomicAdd(&errors[0], 1);
+            return;
+        }
+        atomicAdd(&visited[nd.left], 1);
+        atomicAdd(&visited[nd.right], 1);
+        const Box mine = nodeBox(nd);
+        Box uni = emptyBox();
+        growBox(uni, nodeBox(nodes[nd.left]));
+        growBox(uni, nodeBox(nodes[nd.right]));
+        if (mine.lo.x != uni.lo.x || mine.lo.y != uni.lo.y || mine.lo.z != uni.lo.z ||
+            mine.hi.x != uni.hi.x || mine.hi.y != uni.hi.y || mine.hi.z != uni.hi.z) {
+            atomicAdd(&errors[0], 1);
+        }
+    } else {
+        const int slot = -nd.left - 1;
+        if (slot < 0 || slot >= n) {
+            atomicAdd(&errors[0], 1);
+            return;
+        }
+        if (i != n - 1 + slot || nd.right != 1) atomicAdd(&errors[3], 1);
+        const Box mine = nodeBox(nd);
+        const Box prim = L::bounds(sorted[slot]);
+        if (mine.lo.x != prim.lo.x || mine.lo.y != prim.lo.y || mine.lo.z != prim.lo.z ||
+            mine.hi.x != prim.hi.x || mine.hi.y != prim.hi.y || mine.hi.z != prim.hi.z) {
+            atomicAdd(&errors[1], 1);
+        }
+    }
+}
+
+__global__ void checkRefsKernel(const int* __restrict__ visited, int nodeCount,
+                                int* __restrict__ errors) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nodeCount) return;
+    if (visited[i] != (i == 0 ? 0 : 1)) atomicAdd(&errors[2], 1);
+}
+
+__global__ void checkPermKernel(const int* __restrict__ primIds, int n, int* __restrict__ hist,
+                                int* __restrict__ errors) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int id = primIds[i];
+    if (id < 0 || id >= n) {
+        atomicAdd(&errors[4], 1);
+        return;
+    }
+    atomicAdd(&hist[id], 1);
+}
+
+__global__ void checkPermCountKernel(const int* __restrict__ hist, int n, int* __restrict__ errors) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n && hist[i] != 1) atomicAdd(&errors[4], 1);
+}
+
+template <typename L>
+__global__ void checkPayloadKernel(const typename L::Item* __restrict__ sorted,
+                                   const typename L::Item* __restrict__ items,
+                                   const int* __restrict__ primIds, int n, int* __restrict__ errors) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const int id = primIds[s];
+    if (id < 0 || id >= n) return;  // counted by checkPermKernel
+    const uint32_t* a = reinterpret_cast<const uint32_t*>(&sorted[s]);
+    const uint32_t* b = reinterpret_cast<const uint32_t*>(&items[id]);
+    uint32_t diff = 0;
+    for (size_t k = 0; k < sizeof(typename L::Item) / 4; ++k) diff |= a[k] ^ b[k];
+    if (diff) atomicAdd(&errors[4], 1);
+}
+
+__global__ void checkDepthKernel(const Node2* __restrict__ nodes, const int* __restrict__ depth,
+                                 int n, int nodeCount, int* __restrict__ errors) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nodeCount) return;
+    if (i == 0 && depth[0] != 0) atomicAdd(&errors[5], 1);
+    if (i < n - 1) {
+        const int l = nodes[i].left;
+        const int r = nodes[i].right;
+        if (l >= 0 && l < nodeCount && depth[l] != depth[i] + 1) atomicAdd(&errors[5], 1);
+        if (r >= 0 && r < nodeCount && depth[r] != depth[i] + 1) atomicAdd(&errors[5], 1);
+    }
+    if (depth[i] >= 63) atomicAdd(&errors[5], 1);
+}
+
+}  // namespace
+
+void buildWide(BvhState& b, hipStream_t s) {
+    if (b.n < 2) throw ArgError("buildWideNodes: tree is empty");
+    computeDepth(b, s);
+    const int internal = b.n - 1;
+    b.wideId.reserve(static_cast<size_t>(internal));
+    DevBuf<int> flags(static_cast<size_t>(internal));
+    DevBuf<int> scratch(static_cast<size_t>(scanScratchInts(internal)));
+    DevBuf<int> total(1);
+    wideFlagKernel<<<blocksFor(internal, kBlock), kBlock, 0, s>>>(b.depthArr.p, internal, flags.p);
+    scanExclusive(flags.p, b.wideId.p, internal, total.p, scratch.p, s);
+    int count = 0;
+    hipCheck(hipMemcpyAsync(&count, total.p, sizeof(int), hipMemcpyDeviceToHost, s), "wide count");
+    hipCheck(hipStreamSynchronize(s), "wide count sync");
+    if (count <= 0) throw DeviceError("buildWideNodes: no wide nodes");
+    b.wide.reserve(static_cast<size_t>(count));
+    b.wideCount = count;
+    wideEmitKernel<<<blocksFor(internal, kBlock), kBlock, 0, s>>>(b.nodes.p, b.n, b.depthArr.p,
+                                                                 b.wideId.p, b.wide.p);
+    hipCheck(hipGetLastError(), "wide emit");
+    b.wideValid = true;
+}
+
+// Re-emits every wide node from the current binary boxes (after a refit).
+void reemitWide(BvhState& b, hipStream_t s) {
+    if (!b.wideValid) return;
+    const int internal = b.n - 1;
+    wideEmitKernel<<<blocksFor(internal, kBlock), kBlock, 0, s>>>(b.nodes.p, b.n, b.depthArr.p,
+                                                                 b.wideId.p, b.wide.p);
+    hipCheck(hipGetLastError(), "wide re-emit");
+}
+
+void ensurePrimSlot(BvhState& b, hipStream_t s) {
+    if (b.primSlotValid) return;
+    b.primSlot.reserve(static_cast<size_t>(b.n));
+    invertPermKernel<<<blocksFor(b.n, kBlock), kBlock, 0, s>>>(b.primIds.p, b.n, b.primSlot.p);
+    hipCheck(hipGetLastError(), "invert perm");
+    b.primSlotValid = true;
+}
+
+template <typename L>
+static void refitFullT(BvhState& b, const void* dSceneItems, hipStream_t s) {
+    gatherKernel<L><<<blocksFor(b.n, kBlock), kBlock, 0, s>>>(
+        static_cast<const typename L::Item*>(dSceneItems), b.primIds.p, b.n,
+        reinterpret_cast<typename L::Item*>(b.items.p));
+    hipCheck(hipGetLastError(), "refit gather");
+    propagateAll(b, s);
+}
+
+void refitFull(BvhState& b, const void* dSceneItems, hipStream_t s) {
+    switch (b.kind) {
+        case kPrimTri: refitFullT<TriLeaf>(b, dSceneItems, s); break;
+        case kPrimPoint: refitFullT<PointLeaf>(b, dSceneItems, s); break;
+        default: refitFullT<GaussLeaf>(b, dSceneItems, s); break;
+    }
+}
+
+template <typename L>
+static void refitDirtyT(BvhState& b, const void* dStaged, const int* dSceneIds, int count,
+                        int* dLeaves, hipStream_t s, hipEvent_t mid) {
+    dirtyMarkKernel<L><<<blocksFor(count, kBlock), kBlock, 0, s>>>(
+        static_cast<const typename L::Item*>(dStaged), dSceneIds, count, b.primSlot.p, b.n,
+        reinterpret_cast<typename L::Item*>(b.items.p), b.nodes.p, b.parent.p, b.need.p, dLeaves);
+    hipCheck(hipGetLastError(), "dirty mark");
+    if (mid) hipCheck(hipEventRecord(mid, s), "event");
+    dirtyClimbKernel<<<blocksFor(count, kBlock), kBlock, 0, s>>>(dLeaves, count, b.nodes.p,
+                                                                b.parent.p, b.need.p, b.arrive.p);
+    hipCheck(hipGetLastError(), "dirty climb");
+}
+
+void refitDirty(BvhState& b, const void* dStaged, const int* dSceneIds, int count, int* dLeaves,
+                hipStream_t s, hipEvent_t mid) {
+    if (count <= 0) return;
+    ensurePrimSlot(b, s);
+    if (!b.need.p || b.need.n < static_cast<size_t>(b.n - 1)) {
+        b.need.alloc(static_cast<size_t>(b.n - 1));
+        zeroIntKernel<<<blocksFor(b.n - 1, kBlock), kBlock, 0, s>>>(b.need.p, b.n - 1);
+    }
+    switch (b.kind) {
+        case kPrimTri: refitDirtyT<TriLeaf>(b, dStaged, dSceneIds, count, dLeaves, s, mid); break;
+        case kPrimPoint: refitDirtyT<PointLeaf>(b, dStaged, dSceneIds, count, dLeaves, s, mid); break;
+        default: refitDirtyT<GaussLeaf>(b, dStaged, dSceneIds, count, dLeaves, s, mid); break;
+    }
+}
+
+template <typename L>
+static void validateT(const BvhState& b, const void* dSceneItems, int* errors, hipStream_t s) {
+    const int n = b.n;
+    const int nc = b.nodeCount;
+    DevBuf<int> visited(static_cast<size_t>(nc));
+    DevBuf<int> hist(static_cast<size_t>(n));
+    zeroIntKernel<<<blocksFor(nc, kBlock), kBlock, 0, s>>>(visited.p, nc);
+    zeroIntKernel<<<blocksFor(n, kBlock), kBlock, 0, s>>>(hist.p, n);
+    const auto* sorted = reinterpret_cast<const typename L::Item*>(b.items.p);
+    checkNodesKernel<L><<<blocksFor(nc, kBlock), kBlock, 0, s>>>(b.nodes.p, sorted, n, nc, visited.p, errors);
+    checkRefsKernel<<<blocksFor(nc, kBlock), kBlock, 0, s>>>(visited.p, nc, errors);
+    checkPermKernel<<<blocksFor(n, kBlock), kBlock, 0, s>>>(b.primIds.p, n, hist.p, errors);
+    checkPermCountKernel<<<blocksFor(n, kBlock), kBlock, 0, s>>>(hist.p, n, errors);
+    checkPayloadKernel<L><<<blocksFor(n, kBlock), kBlock, 0, s>>>(
+        sorted, static_cast<const typename L::Item*>(dSceneItems), b.primIds.p, n, errors);
+    checkDepthKernel<<<blocksFor(nc, kBlock), kBlock, 0, s>>>(b.nodes.p, b.depthArr.p, n, nc, errors);
+    hipCheck(hipGetLastError(), "validate");
+    hipCheck(hipStreamSynchronize(s), "validate sync");
+}
+
+void validateTree(BvhState& b, const void* dSceneItems, int* hostErrors, hipStream_t s) {
+    computeDepth(b, s);
+    DevBuf<int> errors(8);
+    zeroIntKernel<<<1, kBlock, 0, s>>>(errors.p, 8);
+    switch (b.kind) {
+        case kPrimTri: validateT<TriLeaf>(b, dSceneItems, errors.p, s); break;
+        case kPrimPoint: validateT<PointLeaf>(b, dSceneItems, errors.p, s); break;
+        default: validateT<GaussLeaf>(b, dSceneItems, errors.p, s); break;
+    }
+    hipCheck(hipMemcpy(hostErrors, errors.p, 6 * sizeof(int), hipMemcpyDeviceToHost), "errors D2H");
+}
+
+}  // namespace qr

@@ -1,0 +1,85 @@
+// The state behind an opaque qr_bvh handle and the build entry points.
+#pragma once
+
+#include "qr_internal.h"
+
+namespace qr {
+
+enum PrimKind : int { kPrimTri = 0, kPrimPoint = 1, kPrimGauss = 2 };
+enum Builder : int { kBuildLbvh = 0, kBuildSah = 1 };
+
+inline size_t itemSize(int kind) {
+    return kind == kPrimTri ? sizeof(Tri) : (kind == kPrimPoint ? sizeof(V3) : sizeof(Gaussian));
+}
+
+struct BuildTimingsC {
+    double mortonMs, sortMs, treeMs, scatterMs, refitMs, sahBinMs, sahSplitMs, sahPartMs, totalMs;
+};
+
+struct BvhState {
+    int kind = kPrimTri;
+    int builder = kBuildLbvh;
+    int n = 0;
+    int nodeCount = 0;
+    int depth = 0;
+
+    DevBuf<Node2> nodes;            // 2n-1
+    DevBuf<unsigned char> items;    // leaf order, n * itemSize(kind)
+    DevBuf<int> primIds;            // slot -> scene index
+    DevBuf<int> parent;             // per node, root -1
+    DevBuf<int> arrive;             // per internal node: bottom-up arrival counters
+    DevBuf<int> height;             // per internal node: subtree height (leaves are 0)
+
+    // scene index -> leaf slot; rebuilt lazily after a build or topology update
+    DevBuf<int> primSlot;
+    bool primSlotValid = false;
+
+    // per node depth (root 0); computed on demand for wide collapse and validate
+    DevBuf<int> depthArr;
+    bool depthValid = false;
+
+    // 4-wide collapse
+    DevBuf<Wide4> wide;
+    DevBuf<int> wideId;             // per internal node: wide index or -1
+    int wideCount = 0;
+    bool wideValid = false;
+
+    // refit scratch (grows, reused across calls)
+    DevBuf<int> need;               // per internal node: dirty children (dirty refit)
+    DevBuf<unsigned char> stage;    // uploaded items (full or dirty)
+    DevBuf<int> dirtyIds;
+    DevBuf<int> dirtyLeaves;
+    DevBuf<int> counters;           // small device counters
+
+    // scene-order primitives kept for validate / refit-by-device-pointer paths
+    SortWorkspace sortWs;
+};
+
+// Builds into an empty state from scene-order items already on the device.
+void buildLbvhDevice(BvhState& b, int kind, const void* dItems, int n, hipStream_t s,
+                     BuildTimingsC* t);
+void buildSahDevice(BvhState& b, int kind, const void* dItems, int n, hipStream_t s,
+                    BuildTimingsC* t);
+
+// Per-node depth by pointer jumping over the parent array (root 0).
+void computeDepth(BvhState& b, hipStream_t s);
+
+// Leaf init + bottom-up box propagation for all leaves, given items already in
+// leaf order (refit path shares it).
+void propagateAll(BvhState& b, hipStream_t s);
+
+void buildWide(BvhState& b, hipStream_t s);
+void reemitWide(BvhState& b, hipStream_t s);
+void ensurePrimSlot(BvhState& b, hipStream_t s);
+// Full refit from scene-order items on the device.
+void refitFull(BvhState& b, const void* dSceneItems, hipStream_t s);
+// Dirty refit: dStaged[i] is the new item of scene index dSceneIds[i] (unique ids).
+// dLeaves is scratch of count ints.  mid (optional) is recorded between the mark
+// and merge passes.
+void refitDirty(BvhState& b, const void* dStaged, const int* dSceneIds, int count, int* dLeaves,
+                hipStream_t s, hipEvent_t mid);
+// errors[6] as the reference's validate (bad children/box, leaf box, refcount,
+// leaf id, prim map, depth).
+void validateTree(BvhState& b, const void* dSceneItems, int* hostErrors, hipStream_t s);
+
+}  // namespace qr

@@ -1,0 +1,164 @@
+// Internal host/device declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "qr_geom.h"
+
+namespace qr {
+
+// ---- device node layouts ------------------------------------------------------
+// Binary node, 32 B, byte-identical to the reference's GpuBVHNode
+// (src/gpu/gpu_node.h:9-35): internal nodes 0..n-2 hold child ids, leaf n-1+slot
+// holds left = -(slot+1), right = 1.
+struct Node2 {
+    float lo[3];
+    float hi[3];
+    int left;
+    int right;
+};
+static_assert(sizeof(Node2) == 32, "Node2 layout");
+
+// 4-wide node, 128 B = one cache line (src/gpu/gpu_node.h:38-64): child >= 0 is a
+// wide index, child < 0 a leaf reference -(slot+1), kWideEmpty an unused slot.
+constexpr int kWideEmpty = INT_MIN;
+struct WideEntry {
+    float lo[3];
+    float hi[3];
+    int child;
+    int pad;
+};
+struct alignas(128) Wide4 {
+    WideEntry c[4];
+};
+static_assert(sizeof(Wide4) == 128, "Wide4 layout");
+
+QR_HD Box nodeBox(const Node2& n) {
+    return {{n.lo[0], n.lo[1], n.lo[2]}, {n.hi[0], n.hi[1], n.hi[2]}};
+}
+QR_HD Box entryBox(const WideEntry& e) {
+    return {{e.lo[0], e.lo[1], e.lo[2]}, {e.hi[0], e.hi[1], e.hi[2]}};
+}
+
+// Result record for closest / kNN (src/gpu/gpu_bvh.h:250-253).
+struct ClosestResult {
+    uint32_t dist2Bits;
+    int32_t prim;
+};
+
+// ---- errors -------------------------------------------------------------------
+// Device failures -> DeviceError (runtime_error in the C++ API); bad arguments ->
+// ArgError; invalid topology edits -> EditError (invalid_argument in the C++ API).
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct EditError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct OomError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hipCheck(hipError_t e, const char* what);
+
+inline int blocksFor(long long n, int block) { return static_cast<int>((n + block - 1) / block); }
+
+// Plain device allocation owned by a scope.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            throw OomError(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        }
+        n = count;
+    }
+    // Grows to at least count elements; contents are not preserved.
+    void reserve(size_t count) {
+        if (count > n || p == nullptr) alloc(count);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T* release_ownership() {
+        T* q = p;
+        p = nullptr;
+        n = 0;
+        return q;
+    }
+};
+
+// Stream the library issues work on (per host thread; null = legacy default).
+hipStream_t currentStream();
+
+// ---- sort ---------------------------------------------------------------------
+// Workspace for the onesweep LSD radix sort of 32-bit keys with 32-bit values.
+struct SortWorkspace {
+    DevBuf<uint32_t> keysAlt, valsAlt;   // ping-pong buffers
+    DevBuf<uint32_t> hist;               // [passes][256] global digit counts
+    DevBuf<uint32_t> desc;               // [passes][tiles][256] look-back descriptors
+    DevBuf<uint32_t> tickets;            // [passes] tile tickets
+    size_t capacity = 0;
+    void reserve(size_t n);
+};
+
+constexpr int kSortTile = 4096;  // keys per onesweep tile (256 threads x 16)
+
+int sortTiles(size_t n);
+
+// Clears the histogram, descriptors and tickets for `passes` passes over n keys.
+void sortClear(SortWorkspace& ws, size_t n, int passes, hipStream_t s);
+
+// Sorts (keys, vals) stably by bits [0, 8*passes) of the key.  If vals == nullptr
+// the pass-0 values are the element indices.  The histogram for every pass must
+// already be in ws.hist (sortHistogram or a fused producer).  Results land in
+// keysOut/valsOut.  keys is not modified; buffers must not alias.
+void sortPairs(SortWorkspace& ws, const uint32_t* keys, const uint32_t* vals, uint32_t* keysOut,
+               uint32_t* valsOut, size_t n, int passes, hipStream_t s);
+
+// Standalone histogram of all passes (when the producer did not fuse it).
+void sortHistogram(SortWorkspace& ws, const uint32_t* keys, size_t n, int passes, hipStream_t s);
+
+// Device-side exclusive scan of n ints (reduce / scan block sums / apply).  in and
+// out may alias.  scratch must hold scanScratchInts(n) ints; *total_dev (optional)
+// receives the sum of all n inputs.
+int scanScratchInts(int n);
+void scanExclusive(const int* in, int* out, int n, int* total_dev, int* scratch, hipStream_t s);
+
+}  // namespace qr

@@ -1,0 +1,44 @@
+// Launchers for the traversal kernels (rays, closest, kNN, radius) and the
+// Morton reordering of queries.
+#pragma once
+
+#include "qr_internal.h"
+
+namespace qr {
+
+struct TraverseArgs {
+    int kind = 0;                       // 0 tri, 1 point, 2 Gaussian
+    bool wide = false;
+    const Node2* nodes = nullptr;
+    const Wide4* wideNodes = nullptr;
+    const void* items = nullptr;        // leaf-ordered primitives
+    const int* primIds = nullptr;       // leaf slot -> scene index
+    const uint32_t* order = nullptr;    // optional visiting order (lane i -> query order[i])
+    unsigned long long* stats = nullptr;  // optional [node tests, leaf tests, overflows]
+};
+
+void zeroStats(unsigned long long* stats, hipStream_t s);
+
+// Computes a visiting order by 30-bit Morton code of the first three floats of
+// each element (elements are `stride` floats apart).  fixedBox = 1 quantises over
+// [-1,1]^3 instead of the measured bounds.  bb: 6 device words of scratch.
+void mortonOrder(const float* base, int stride, int count, int fixedBox, SortWorkspace& ws,
+                 uint32_t* keys, uint32_t* keysSorted, uint32_t* orderOut, uint32_t* bb,
+                 hipStream_t s);
+
+void launchRays(const TraverseArgs& a, const Ray* rays, int count, uint64_t* out, hipStream_t s);
+void launchClosest(const TraverseArgs& a, const V3* pts, int count, ClosestResult* out,
+                   hipStream_t s);
+// Result j of query q lands at out[q * outStride + j] (outStride >= k).
+void launchKnn(const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
+               ClosestResult* out, hipStream_t s);
+void launchRadiusCount(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts,
+                       hipStream_t s);
+// offsets has count + 1 entries (CSR, in query order).
+void launchRadiusFill(const TraverseArgs& a, const V3* pts, int count, float r2,
+                      const int* offsets, uint64_t* keys, hipStream_t s);
+// scratchInts must hold 4 + 2 * count ints.
+void segmentSort(const int* offsets, uint64_t* keys, int count, int maxSeg, int* scratchInts,
+                 hipStream_t s);
+
+}  // namespace qr

@@ -1,0 +1,294 @@
+"""ctypes binding of libquarry_hip.so (include/quarry/quarry_abi.h).
+
+This is the only place Python touches the native library.  Every call checks
+the returned status and raises.  The reference maps std::runtime_error (bad
+query parameters, device failures) to RuntimeError and std::invalid_argument
+(bad topology edits) to ValueError (docs/api.md:390-392), so the mapping here is
+
+    QR_ERR_INVALID, QR_ERR_DEVICE, QR_ERR_OOM, QR_ERR_NO_DEVICE -> QuarryError
+        (a RuntimeError subclass)
+    QR_ERR_EDIT -> ValueError
+
+Argument validation that the reference's Python layer performs up front
+(shapes, k, radius) raises ValueError in quarry/__init__.py before any call.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libquarry_hip.so")
+
+QR_OK, QR_ERR_INVALID, QR_ERR_DEVICE, QR_ERR_OOM, QR_ERR_EDIT, QR_ERR_NO_DEVICE = range(6)
+PRIM_TRI, PRIM_POINT, PRIM_GAUSS = 0, 1, 2
+BUILDER_LBVH, BUILDER_SAH = 0, 1
+ORDER_INPUT, ORDER_MORTON, ORDER_MORTON_DIRECTION = 0, 1, 2
+ITEM_FLOATS = {PRIM_TRI: 9, PRIM_POINT: 3, PRIM_GAUSS: 10}
+
+
+class QuarryError(RuntimeError):
+    pass
+
+
+class BuildTimings(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "morton_ms", "sort_ms", "tree_ms", "scatter_ms", "refit_ms",
+        "sah_bin_ms", "sah_split_ms", "sah_part_ms", "total_ms")]
+
+
+class RefitTimings(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "upload_ms", "gather_ms", "tree_ms", "mark_ms", "merge_ms", "wide_ms", "total_ms")] + [
+        ("updated", ctypes.c_int)]
+
+
+class TraversalStats(ctypes.Structure):
+    _fields_ = [("node_tests", ctypes.c_uint64), ("leaf_tests", ctypes.c_uint64),
+                ("stack_overflows", ctypes.c_uint64), ("sort_ms", ctypes.c_double)]
+
+
+class RayStats(ctypes.Structure):
+    _fields_ = [("rays", ctypes.c_uint64), ("hits", ctypes.c_uint64), ("hash", ctypes.c_uint64),
+                ("sum_t", ctypes.c_double)]
+
+
+class BvhInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "prim_count", "node_count", "depth", "wide_count", "prim_kind", "builder")] + [
+        ("d_nodes", ctypes.c_void_p), ("d_items", ctypes.c_void_p),
+        ("d_prim_ids", ctypes.c_void_p), ("d_wide_nodes", ctypes.c_void_p)]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_ip = ctypes.c_int
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise QuarryError(
+                f"{LIB_PATH} is missing: build it with `python quarry-gpu_amd/build.py`")
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "qr_last_error": ([], ctypes.c_char_p),
+            "qr_describe": ([ctypes.c_char_p, _ip], _ip),
+            "qr_warmup": ([], _ip),
+            "qr_set_stream": ([_vp], _ip),
+            "qr_bvh_build": ([_ip, _ip, _vp, _ip, _vp, _vp], _ip),
+            "qr_bvh_build_device": ([_ip, _ip, _vp, _ip, _vp, _vp], _ip),
+            "qr_bvh_destroy": ([_vp], _ip),
+            "qr_bvh_get_info": ([_vp, _vp], _ip),
+            "qr_bvh_build_wide": ([_vp, _dp], _ip),
+            "qr_bvh_refit": ([_vp, _vp, _ip, _vp, _ip, _vp, _dp], _ip),
+            "qr_bvh_refit_device": ([_vp, _vp, _ip, _vp, _dp], _ip),
+            "qr_bvh_validate": ([_vp, _vp, _ip, ctypes.c_char_p, _ip], _ip),
+            "qr_bvh_tree_hash": ([_vp, ctypes.POINTER(ctypes.c_uint64)], _ip),
+            "qr_bvh_read": ([_vp, _vp, _vp], _ip),
+            "qr_rays_upload": ([_vp, _ip, _vp], _ip),
+            "qr_rays_destroy": ([_vp], _ip),
+            "qr_raycast": ([_vp, _vp, _ip, _ip, _ip, _dp], _ip),
+            "qr_rays_summarize": ([_vp, _vp, _vp], _ip),
+            "qr_last_traversal_stats": ([_vp], _ip),
+            "qr_raycast_device": ([_vp, _vp, _ip, _vp, _ip, _ip, _dp], _ip),
+            "qr_points_upload": ([_vp, _ip, _ip, _vp], _ip),
+            "qr_points_destroy": ([_vp], _ip),
+            "qr_closest": ([_vp, _vp, _ip, _ip, _ip, _dp], _ip),
+            "qr_knn": ([_vp, _vp, _ip, _ip, _ip, _ip, _dp], _ip),
+            "qr_radius": ([_vp, _vp, ctypes.c_float, _ip, _ip, _ip, _vp, _dp], _ip),
+            "qr_points_read": ([_vp, _vp], _ip),
+            "qr_points_info": ([_vp, _vp, _vp], _ip),
+            "qr_radius_info": ([_vp, _vp, _vp], _ip),
+            "qr_radius_read": ([_vp, _vp, _vp], _ip),
+            "qr_radius_destroy": ([_vp], _ip),
+            "qr_last_query_stats": ([_vp], _ip),
+            "qr_closest_device": ([_vp, _vp, _ip, _vp, _ip, _ip, _dp], _ip),
+            "qr_knn_device": ([_vp, _vp, _ip, _ip, _vp, _ip, _ip, _dp], _ip),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return [n for n in dir(lib()) if n.startswith("qr_")]
+
+
+def check(status: int) -> None:
+    if status == QR_OK:
+        return
+    msg = lib().qr_last_error().decode(errors="replace")
+    if status == QR_ERR_EDIT:
+        raise ValueError(msg)
+    raise QuarryError(msg)
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def describe() -> str | None:
+    buf = ctypes.create_string_buffer(256)
+    st = lib().qr_describe(buf, 256)
+    return buf.value.decode() if st == QR_OK else None
+
+
+class Bvh:
+    """Owning wrapper of a qr_bvh handle."""
+
+    def __init__(self, kind: int, items: np.ndarray, builder: int = BUILDER_LBVH):
+        items = np.ascontiguousarray(items, dtype=np.float32).reshape(-1, ITEM_FLOATS[kind])
+        self.kind = kind
+        self.h = ctypes.c_void_p()
+        self.timings = BuildTimings()
+        check(lib().qr_bvh_build(kind, builder, ptr(items), items.shape[0],
+                                 ctypes.byref(self.timings), ctypes.byref(self.h)))
+
+    @classmethod
+    def from_device(cls, kind: int, dev_ptr: int, n: int, builder: int = BUILDER_LBVH) -> "Bvh":
+        self = cls.__new__(cls)
+        self.kind = kind
+        self.h = ctypes.c_void_p()
+        self.timings = BuildTimings()
+        check(lib().qr_bvh_build_device(kind, builder, ctypes.c_void_p(dev_ptr), n,
+                                        ctypes.byref(self.timings), ctypes.byref(self.h)))
+        return self
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            check(lib().qr_bvh_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> BvhInfo:
+        out = BvhInfo()
+        check(lib().qr_bvh_get_info(self.h, ctypes.byref(out)))
+        return out
+
+    def build_wide(self) -> float:
+        ms = ctypes.c_double()
+        check(lib().qr_bvh_build_wide(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def refit(self, items: np.ndarray, dirty: np.ndarray | None = None) -> RefitTimings:
+        items = np.ascontiguousarray(items, dtype=np.float32).reshape(-1, ITEM_FLOATS[self.kind])
+        t = RefitTimings()
+        ms = ctypes.c_double()
+        if dirty is None:
+            check(lib().qr_bvh_refit(self.h, ptr(items), items.shape[0], None, 0, ctypes.byref(t), ctypes.byref(ms)))
+        else:
+            d = np.ascontiguousarray(dirty, dtype=np.int32)
+            check(lib().qr_bvh_refit(self.h, ptr(items), items.shape[0], ptr(d), d.shape[0], ctypes.byref(t), ctypes.byref(ms)))
+        return t
+
+    def validate(self, items: np.ndarray) -> str:
+        items = np.ascontiguousarray(items, dtype=np.float32).reshape(-1, ITEM_FLOATS[self.kind])
+        buf = ctypes.create_string_buffer(512)
+        check(lib().qr_bvh_validate(self.h, ptr(items), items.shape[0], buf, 512))
+        return buf.value.decode()
+
+    def tree_hash(self) -> int:
+        out = ctypes.c_uint64()
+        check(lib().qr_bvh_tree_hash(self.h, ctypes.byref(out)))
+        return int(out.value)
+
+    def read(self):
+        inf = self.info()
+        nodes = np.zeros(inf.node_count * 32, dtype=np.uint8)
+        ids = np.zeros(inf.prim_count, dtype=np.int32)
+        check(lib().qr_bvh_read(self.h, ptr(nodes), ptr(ids)))
+        return nodes, ids
+
+    # ---- queries -------------------------------------------------------------------
+    def raycast(self, rays: np.ndarray, order: int = ORDER_INPUT, wide: bool = False,
+                stats: bool = False):
+        """Returns (u64 results, ms, stats or None)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        rb = ctypes.c_void_p()
+        check(lib().qr_rays_upload(ptr(rays), rays.shape[0], ctypes.byref(rb)))
+        try:
+            ms = ctypes.c_double()
+            check(lib().qr_raycast(self.h, rb, int(stats), order, int(wide), ctypes.byref(ms)))
+            out = np.zeros(rays.shape[0], dtype=np.uint64)
+            rs = RayStats()
+            check(lib().qr_rays_summarize(rb, ptr(out), ctypes.byref(rs)))
+            st = None
+            if stats:
+                st = TraversalStats()
+                check(lib().qr_last_traversal_stats(ctypes.byref(st)))
+            return out, ms.value, st
+        finally:
+            lib().qr_rays_destroy(rb)
+
+    def _points(self, pts: np.ndarray, stride: int):
+        pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+        pb = ctypes.c_void_p()
+        check(lib().qr_points_upload(ptr(pts), pts.shape[0], stride, ctypes.byref(pb)))
+        return pts, pb
+
+    def closest(self, pts: np.ndarray, order: int = ORDER_INPUT, wide: bool = False,
+                stats: bool = False):
+        """Returns (records uint32 (n, 1, 2), ms, stats or None)."""
+        pts, pb = self._points(pts, 1)
+        try:
+            ms = ctypes.c_double()
+            check(lib().qr_closest(self.h, pb, int(stats), order, int(wide), ctypes.byref(ms)))
+            out = np.zeros((pts.shape[0], 1, 2), dtype=np.uint32)
+            check(lib().qr_points_read(pb, ptr(out)))
+            return out, ms.value, self._qstats(stats)
+        finally:
+            lib().qr_points_destroy(pb)
+
+    def knn(self, pts: np.ndarray, k: int, order: int = ORDER_INPUT, wide: bool = False,
+            stats: bool = False):
+        pts, pb = self._points(pts, k)
+        try:
+            ms = ctypes.c_double()
+            check(lib().qr_knn(self.h, pb, k, int(stats), order, int(wide), ctypes.byref(ms)))
+            out = np.zeros((pts.shape[0], k, 2), dtype=np.uint32)
+            check(lib().qr_points_read(pb, ptr(out)))
+            return out, ms.value, self._qstats(stats)
+        finally:
+            lib().qr_points_destroy(pb)
+
+    def radius(self, pts: np.ndarray, r: float, order: int = ORDER_INPUT, wide: bool = False,
+               stats: bool = False):
+        """Returns (offsets int32 (n+1), keys uint64, ms, stats or None)."""
+        pts, pb = self._points(pts, 1)
+        csr = ctypes.c_void_p()
+        try:
+            ms = ctypes.c_double()
+            check(lib().qr_radius(self.h, pb, float(r), int(stats), order, int(wide),
+                                  ctypes.byref(csr), ctypes.byref(ms)))
+            cnt, tot = ctypes.c_int(), ctypes.c_int()
+            check(lib().qr_radius_info(csr, ctypes.byref(cnt), ctypes.byref(tot)))
+            offs = np.zeros(cnt.value + 1, dtype=np.int32)
+            keys = np.zeros(max(tot.value, 1), dtype=np.uint64)
+            check(lib().qr_radius_read(csr, ptr(offs), ptr(keys)))
+            return offs, keys[: tot.value], ms.value, self._qstats(stats)
+        finally:
+            if csr.value:
+                lib().qr_radius_destroy(csr)
+            lib().qr_points_destroy(pb)
+
+    @staticmethod
+    def _qstats(want: bool):
+        if not want:
+            return None
+        st = TraversalStats()
+        check(lib().qr_last_query_stats(ctypes.byref(st)))
+        return st
