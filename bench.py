@@ -1,0 +1,327 @@
+#!/usr/bin/env python3
+"""Benchmark of the BVH query path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): a 1024 x 1024-cell
+z-up terrain (2,097,152 FP32 triangles) with 100,000 query points uniform in the
+scene box grown by 4 on every side.  One step is one serving pass over the
+device-resident index: closest point, k-nearest (k = 16) and radius (r = 2.0)
+for all 100,000 queries, Morton-ordered, on 4-wide nodes.  value = queries
+answered per second over the K timed steps (300,000 per step), in millions.
+
+Around the timed region the same run also measures and checks:
+  * the LBVH build of the 2M-triangle scene (median of several builds) with its
+    HBM roofline, and the 4-wide collapse;
+  * a dirty refit after moving 1% of the grid vertices by N(0, 0.05);
+  * closest-hit rays: 1,000,000 down rays on the 256 x 256-cell terrain;
+  * bit-exact parity of every result set against the CPU oracle;
+  * the reference CPU path (or the oracle restatement when the reference
+    build is absent) timed on a bounded sample of the same queries.
+Progress goes to stderr; stdout carries one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "quarry-gpu_amd"))
+
+HBM_PEAK_GBS = 8000.0
+TRI_BYTES = 36
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=1024)
+    ap.add_argument("--queries", type=int, default=100_000)
+    ap.add_argument("--k", type=int, default=16)
+    ap.add_argument("--radius", type=float, default=2.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity legs")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--profile", action="store_true", help="short run for rocprof: no CPU legs, no extras")
+    return ap.parse_args()
+
+
+def main() -> int:
+    import faulthandler
+
+    faulthandler.enable()
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus above 1 is not "
+                                   "supported (sharding is described in DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    from quarry import _abi as A
+    from quarry import scenes as S
+
+    dev = A.describe()
+    if dev is None:
+        log("no HIP device visible")
+        return 1
+    log("device:", dev)
+    A.check(A.lib().qr_warmup())
+
+    # ---- workload ---------------------------------------------------------------
+    t0 = time.time()
+    tris, queries = S.workload_c2(seed=1, cells=args.cells, queries=args.queries)
+    n = tris.shape[0]
+    log(f"scene: {n} triangles, {queries.shape[0]} queries ({time.time() - t0:.1f}s to generate)")
+
+    dtris = A.DeviceArray(tris)
+    builds = []
+    bvh = None
+    for i in range(5 if not args.profile else 2):
+        if bvh is not None:
+            bvh.close()
+        bvh = A.Bvh.from_device(A.PRIM_TRI, dtris.ptr.value, n)
+        builds.append(bvh.timings)
+    build_ms = statistics.median(t.total_ms for t in builds)
+    bt = sorted(builds, key=lambda t: t.total_ms)[len(builds) // 2]
+    wide_ms = bvh.build_wide()
+    info = bvh.info()
+    log(f"build: {build_ms:.3f} ms (morton {bt.morton_ms:.3f}, sort {bt.sort_ms:.3f}, "
+        f"tree {bt.tree_ms:.3f}, leaves+boxes {bt.refit_ms:.3f}); depth {info.depth}; "
+        f"wide collapse {wide_ms:.3f} ms ({info.wide_count} nodes)")
+
+    pb1 = A.PointBuffer(queries, 1)
+    pbk = A.PointBuffer(queries, args.k)
+
+    def step():
+        a = pb1.closest(bvh)
+        sa = A.last_query_stats().kernel_ms
+        b = pbk.knn(bvh, args.k)
+        sb = A.last_query_stats().kernel_ms
+        c = pb1.radius(bvh, args.radius)
+        sc = A.last_query_stats().kernel_ms
+        return (a, b, c), (sa, sb, sc)
+
+    for _ in range(args.warmup):
+        step()
+    A.check(A.lib().qr_dev_synchronize())
+    call_ms = []
+    kern_ms = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        c, k = step()
+        call_ms.append(c)
+        kern_ms.append(k)
+    A.check(A.lib().qr_dev_synchronize())
+    wall = time.perf_counter() - t_start
+    ms_per_step = 1000.0 * wall / args.steps
+    q_per_step = 3 * queries.shape[0]
+    value = q_per_step / (wall / args.steps) / 1e6
+    per_op = {name: statistics.median(c[i] for c in call_ms) for i, name in
+              enumerate(("closest", "knn", "radius"))}
+    per_kernel = {name: statistics.mean(k[i] for k in kern_ms) for i, name in
+                  enumerate(("closest", "knn", "radius"))}
+    log(f"step: {ms_per_step:.3f} ms wall; per call (event ms) {per_op}; traversal kernels {per_kernel}")
+
+    # ---- roofline of the dominant kernel ------------------------------------------
+    dom = max(per_kernel, key=per_kernel.get)
+    nq = queries.shape[0]
+    radius_total = None
+    _, offs, keys = pb1.radius(bvh, args.radius, keep=True)
+    radius_total = int(offs[-1])
+    # Algorithmic bytes per launch: every query point read once, every result
+    # written once (traversal of the resident index is cache/latency-bound work,
+    # not streaming, so these are the bytes that must cross HBM).
+    alg = {
+        "closest": nq * (12 + 8),
+        "knn": nq * (12 + 8 * args.k),
+        # count + fill passes: points read twice, counts and offsets, keys written
+        "radius": nq * (12 + 4) + nq * (12 + 4) + radius_total * 8,
+    }[dom]
+    achieved = alg / (per_kernel[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                "avg_kernel_ms": round(per_kernel[dom], 4)}
+
+    # Build roofline: 5S + 232 bytes per primitive (S = 36 for triangles).
+    build_alg = n * (5 * TRI_BYTES + 232)
+    build_gbs = build_alg / (build_ms * 1e-3) / 1e9
+    build = {"ms": round(build_ms, 4), "morton_ms": round(bt.morton_ms, 4),
+             "sort_ms": round(bt.sort_ms, 4), "tree_ms": round(bt.tree_ms, 4),
+             "leaves_boxes_ms": round(bt.refit_ms, 4), "wide_ms": round(wide_ms, 4),
+             "depth": info.depth,
+             "roofline": {"bound": "hbm", "achieved": round(build_gbs, 2), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
+                          "alg_bytes": build_alg}}
+
+    extra = {}
+    parity = {}
+    cpu_baseline = None
+    if not args.profile:
+        # ---- parity of the query pass against the CPU oracle -----------------------
+        import oracle as O
+
+        threads = min(16, os.cpu_count() or 1)
+        log("parity: building the CPU oracle BVH")
+        t0 = time.time()
+        cb = O.CpuBvh(0, tris)
+        res_c = pb1.read()
+        pb1.closest(bvh)
+        res_c = pb1.read()
+        pbk.knn(bvh, args.k)
+        res_k = pbk.read()
+        exp_c = cb.knn(queries, 0, threads)
+        exp_k = cb.knn(queries, args.k, threads)
+        exp_o, exp_keys = cb.radius(queries, args.radius, threads)
+        parity["closest"] = bool(np.array_equal(res_c, exp_c))
+        parity["knn"] = bool(np.array_equal(res_k, exp_k))
+        parity["radius"] = bool(np.array_equal(offs, exp_o) and np.array_equal(keys, exp_keys))
+        fp = {"closest": hex(O.closest_hash(res_c)[0]), "knn": hex(O.knn_hash(res_k, args.k)[0]),
+              "radius": hex(O.csr_hash(offs, keys)[0])}
+        log(f"parity {parity} ({time.time() - t0:.1f}s); fingerprints {fp}")
+        extra["fingerprints"] = fp
+        del cb
+
+        # ---- dirty refit after moving 1% of the vertices ---------------------------
+        moved, dirty = S.displace_vertices(args.cells, tris, 0.01, 0.05, seed=4)
+        rt = bvh.refit(moved, dirty)
+        why = bvh.validate(moved)
+        pb1.closest(bvh)
+        res_rc = pb1.read()
+        cb2 = O.CpuBvh(0, moved)
+        parity["refit_closest"] = bool(np.array_equal(res_rc, cb2.knn(queries, 0, threads)))
+        parity["refit_validate"] = why == ""
+        del cb2
+        extra["refit"] = {"dirty_tris": int(rt.updated), "upload_ms": round(rt.upload_ms, 4),
+                          "mark_ms": round(rt.mark_ms, 4), "merge_ms": round(rt.merge_ms, 4),
+                          "wide_ms": round(rt.wide_ms, 4), "total_ms": round(rt.total_ms, 4)}
+        log(f"refit: {extra['refit']} validate={why!r}")
+        bvh.refit(tris)
+        log("refit back to the original scene done")
+
+        # ---- rays: 1M down rays on the 256 x 256 terrain ----------------------------
+        tr1, rays = S.workload_c1(seed=1)
+        b1 = A.Bvh(A.PRIM_TRI, tr1)
+        b1.build_wide()
+        log("rays: tree built")
+        rb = A.RayBuffer(rays)
+        for _ in range(3):
+            rb.cast(b1)
+        rms = [rb.cast(b1) for _ in range(10)]
+        got, rs = rb.summarize()
+        cbr = O.CpuBvh(0, tr1)
+        exp_r = cbr.ray(rays, threads)
+        parity["rays"] = bool(np.array_equal(got, exp_r))
+        ray_ms = statistics.median(rms)
+        extra["rays_c1"] = {"mrays_per_s": round(rays.shape[0] / (ray_ms * 1e-3) / 1e6, 2),
+                            "ms": round(ray_ms, 4), "hits": int(rs.hits), "hash": hex(rs.hash),
+                            "build_ms": round(b1.timings.total_ms, 4)}
+        log(f"rays: {extra['rays_c1']} parity={parity['rays']}")
+        del cbr, rb, b1
+
+        # ---- CPU baseline on a bounded sample ------------------------------------------
+        if not args.no_cpu:
+            cpu_baseline = cpu_leg(O, tris, queries, args, threads)
+            log(f"cpu baseline: {cpu_baseline}")
+
+    out = {
+        "metric": "Mqueries/s (closest + kNN16 + radius r=2, bit-exact), 2M-triangle terrain",
+        "value": round(value, 3),
+        "unit": "Mqueries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded z-up sinusoid terrain, uniform queries)",
+        "config": {"workload": "terrain 1024x1024 cells (2,097,152 tris), 100k queries, closest + "
+                               f"kNN{args.k} + radius {args.radius}, Morton order, 4-wide nodes",
+                   "triangles": n, "queries": nq, "k": args.k, "radius": args.radius,
+                   "radius_results": radius_total},
+        "per_op_ms": {k: round(v, 4) for k, v in per_op.items()},
+        "per_op_mqps": {k: round(nq / (v * 1e-3) / 1e6, 2) for k, v in per_op.items()},
+        "roofline": roofline,
+        "build": build,
+        "parity": parity,
+        "cpu_baseline": cpu_baseline,
+        "device": dev,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+    return 0 if all(parity.values()) else 3
+
+
+def cpu_leg(O, tris, queries, args, threads):
+    """Times the reference CPU path (oracle/_ref) -- or the oracle restatement when
+    the reference build is absent -- on a bounded sample of the same queries."""
+    ref = O.ref() if O.ref_available() else None
+    kind = "reference" if ref is not None else "port"
+    p = np.ascontiguousarray(queries, dtype=np.float32)
+    if ref is not None:
+        f32p, u32p, i32p = O._f32p, O._u32p, O._i32p
+        t0 = time.time()
+        h = ref.ref_bvh_new(np.ascontiguousarray(tris).ctypes.data_as(f32p), tris.shape[0])
+        build_s = time.time() - t0
+
+        def run(m, thr):
+            q = p[:m]
+            out1 = np.zeros((m, 1, 2), np.uint32)
+            outk = np.zeros((m, args.k, 2), np.uint32)
+            cnt = np.zeros(m, np.int32)
+            ref.ref_bvh_knn(h, q.ctypes.data_as(f32p), m, 0, out1.ctypes.data_as(u32p), thr)
+            ref.ref_bvh_knn(h, q.ctypes.data_as(f32p), m, args.k, outk.ctypes.data_as(u32p), thr)
+            ref.ref_bvh_radius(h, q.ctypes.data_as(f32p), m, args.radius, cnt.ctypes.data_as(i32p), None, None, thr)
+    else:
+        t0 = time.time()
+        cb = O.CpuBvh(0, tris)
+        build_s = time.time() - t0
+
+        def run(m, thr):
+            q = p[:m]
+            cb.knn(q, 0, thr)
+            cb.knn(q, args.k, thr)
+            cb.radius(q, args.radius, thr)
+
+    # calibrate the single-thread sample to the time budget
+    m = 500
+    t0 = time.time()
+    run(m, 1)
+    dt = max(time.time() - t0, 1e-3)
+    m1 = int(min(p.shape[0], max(500, m * (0.5 * args.cpu_seconds) / dt)))
+    t0 = time.time()
+    run(m1, 1)
+    single = 3 * m1 / (time.time() - t0) / 1e6
+    m2 = int(min(p.shape[0], m1 * threads))
+    t0 = time.time()
+    run(m2, threads)
+    multi = 3 * m2 / (time.time() - t0) / 1e6
+    if ref is not None:
+        ref.ref_bvh_free(h)
+    try:
+        import subprocess
+        model = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        model = next((ln.split(":", 1)[1].strip() for ln in model.splitlines()
+                      if ln.startswith("Model name")), "unknown")
+    except Exception:
+        model = "unknown"
+    return {"value": round(multi, 4), "unit": "Mqueries/s", "cores": threads, "kind": kind,
+            "single_thread_value": round(single, 4),
+            "sample": f"closest + kNN{args.k} + radius {args.radius} over the first {m1} queries "
+                      f"(1 thread) and {m2} queries ({threads} threads) of the same workload; "
+                      f"CPU BVH build {build_s:.2f}s not counted",
+            "cpu_model": model, "nproc": os.cpu_count()}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -72,6 +72,7 @@ typedef struct {
 typedef struct {
     uint64_t node_tests, leaf_tests, stack_overflows;
     double sort_ms;
+    double kernel_ms; /* traversal kernel(s) alone (radius: count + fill passes) */
 } qr_traversal_stats;
 
 /* RayStats (src/core/fingerprint.h:16-43) */
@@ -97,6 +98,15 @@ QR_API int qr_describe(char* buf, int len);
 QR_API int qr_warmup(void);
 /* Stream for subsequent calls on this host thread (a hipStream_t; NULL = default). */
 QR_API int qr_set_stream(void* hip_stream);
+
+/* Device memory helpers for callers without their own HIP runtime use
+ * (benchmarks, C consumers of the *_device entry points).  No reference
+ * counterpart: the reference API only takes host arrays. */
+QR_API int qr_dev_alloc(size_t bytes, void** out);
+QR_API int qr_dev_free(void* ptr);
+QR_API int qr_dev_upload(void* dst, const void* src_host, size_t bytes);
+QR_API int qr_dev_download(void* dst_host, const void* src, size_t bytes);
+QR_API int qr_dev_synchronize(void);
 
 /* ---- build (src/gpu/gpu_bvh.h:103-118) ----------------------------------------
  * buildBVHGPU / buildPointBVHGPU / buildGaussianBVHGPU (builder LBVH) and

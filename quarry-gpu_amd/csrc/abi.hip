@@ -89,23 +89,13 @@ void need(bool ok, const char* msg) {
 }
 
 struct Timer {
-    hipEvent_t a = nullptr, b = nullptr;
+    EventSet<2> ev;
     hipStream_t s;
-    explicit Timer(hipStream_t st) : s(st) {
-        hipCheck(hipEventCreate(&a), "event");
-        hipCheck(hipEventCreate(&b), "event");
-        hipCheck(hipEventRecord(a, s), "event");
-    }
-    ~Timer() {
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
-    }
+    explicit Timer(hipStream_t st) : s(st) { ev.record(0, s); }
     double stop() {
-        hipCheck(hipEventRecord(b, s), "event");
-        hipCheck(hipEventSynchronize(b), "sync");
-        float ms = 0.0f;
-        hipCheck(hipEventElapsedTime(&ms, a, b), "elapsed");
-        return ms;
+        ev.record(1, s);
+        ev.sync(1);
+        return ev.ms(0, 1);
     }
 };
 
@@ -216,6 +206,36 @@ int qr_set_stream(void* s) {
     return QR_OK;
 }
 
+int qr_dev_alloc(size_t bytes, void** out) {
+    return guard([&] {
+        need(out != nullptr, "null output pointer");
+        *out = nullptr;
+        hipCheck(hipMalloc(out, bytes > 0 ? bytes : 1), "hipMalloc");
+    });
+}
+
+int qr_dev_free(void* ptr) {
+    return guard([&] {
+        if (ptr) hipCheck(hipFree(ptr), "hipFree");
+    });
+}
+
+int qr_dev_upload(void* dst, const void* src, size_t bytes) {
+    return guard([&] {
+        if (bytes) hipCheck(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "H2D");
+    });
+}
+
+int qr_dev_download(void* dst, const void* src, size_t bytes) {
+    return guard([&] {
+        if (bytes) hipCheck(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "D2H");
+    });
+}
+
+int qr_dev_synchronize(void) {
+    return guard([&] { hipCheck(hipDeviceSynchronize(), "synchronize"); });
+}
+
 int qr_bvh_build(int prim, int builder, const void* items, int n, qr_build_timings* t,
                  qr_bvh** out) {
     return guard([&] {
@@ -298,17 +318,16 @@ int qr_bvh_refit(qr_bvh* bvh, const void* items, int n, const int* dirty, int nD
         hipStream_t s = currentStream();
         const size_t isz = itemSize(b.kind);
         qr_refit_timings tt{};
-        hipEvent_t ev[5];
-        for (auto& e : ev) hipCheck(hipEventCreate(&e), "event");
+        EventSet<5> ev;
         if (dirty == nullptr) {
             b.stage.reserve(static_cast<size_t>(n) * isz);
-            hipCheck(hipEventRecord(ev[0], s), "event");
+            ev.record(0, s);
             hipCheck(hipMemcpyAsync(b.stage.p, items, static_cast<size_t>(n) * isz,
                                     hipMemcpyHostToDevice, s), "H2D refit");
-            hipCheck(hipEventRecord(ev[1], s), "event");
+            ev.record(1, s);
             refitFull(b, b.stage.p, s);
-            hipCheck(hipEventRecord(ev[2], s), "event");
-            hipCheck(hipEventRecord(ev[3], s), "event");
+            ev.record(2, s);
+            ev.record(3, s);
             tt.updated = n;
         } else {
             std::vector<int> ids(dirty, dirty + nDirty);
@@ -323,26 +342,24 @@ int qr_bvh_refit(qr_bvh* bvh, const void* items, int n, const int* dirty, int nD
             b.stage.reserve(staged.size() + 1);
             b.dirtyIds.reserve(static_cast<size_t>(m) + 1);
             b.dirtyLeaves.reserve(static_cast<size_t>(m) + 1);
-            hipCheck(hipEventRecord(ev[0], s), "event");
+            ev.record(0, s);
             if (m > 0) {
                 hipCheck(hipMemcpyAsync(b.stage.p, staged.data(), staged.size(), hipMemcpyHostToDevice, s), "H2D dirty");
                 hipCheck(hipMemcpyAsync(b.dirtyIds.p, ids.data(), static_cast<size_t>(m) * sizeof(int),
                                         hipMemcpyHostToDevice, s), "H2D dirty ids");
             }
-            hipCheck(hipEventRecord(ev[1], s), "event");
-            refitDirty(b, b.stage.p, b.dirtyIds.p, m, b.dirtyLeaves.p, s, ev[2]);
-            if (m <= 0) hipCheck(hipEventRecord(ev[2], s), "event");
-            hipCheck(hipEventRecord(ev[3], s), "event");
+            ev.record(1, s);
+            refitDirty(b, b.stage.p, b.dirtyIds.p, m, b.dirtyLeaves.p, s, ev.e[2]);
+            if (m <= 0) ev.record(2, s);
+            ev.record(3, s);
             tt.updated = m;
         }
         reemitWide(b, s);
-        hipCheck(hipEventRecord(ev[4], s), "event");
-        hipCheck(hipEventSynchronize(ev[4]), "refit sync");
+        ev.record(4, s);
+        ev.sync(4);
         float v[4];
-        for (int i = 0; i < 4; ++i) hipCheck(hipEventElapsedTime(&v[i], ev[i], ev[i + 1]), "elapsed");
-        float total = 0.0f;
-        hipCheck(hipEventElapsedTime(&total, ev[0], ev[4]), "elapsed");
-        for (auto& e : ev) (void)hipEventDestroy(e);
+        for (int i = 0; i < 4; ++i) v[i] = ev.ms(i, i + 1);
+        const float total = ev.ms(0, 4);
         tt.upload_ms = v[0];
         if (dirty == nullptr) {
             tt.gather_ms = 0.0;
@@ -490,25 +507,22 @@ double raycastImpl(const qr_bvh* bvh, const Ray* rays, int n, uint64_t* out, qr_
     hipStream_t s = currentStream();
     Stats st;
     if (collectStats) zeroStats(st.d.p, s);
-    hipEvent_t evs[3];
-    for (auto& e : evs) hipCheck(hipEventCreate(&e), "event");
-    hipCheck(hipEventRecord(evs[0], s), "event");
+    EventSet<3> ev;
+    ev.record(0, s);
     TraverseArgs a = argsFor(bvh->s, wide != 0);
     a.stats = collectStats ? st.d.p : nullptr;
     if (order != QR_ORDER_INPUT) {
         const float* base = reinterpret_cast<const float*>(rays) + (order == QR_ORDER_MORTON ? 0 : 3);
         a.order = reorder(*scratch, base, 8, n, order == QR_ORDER_MORTON_DIRECTION ? 1 : 0, s);
     }
-    hipCheck(hipEventRecord(evs[1], s), "event");
+    ev.record(1, s);
     launchRays(a, rays, n, out, s);
-    hipCheck(hipEventRecord(evs[2], s), "event");
-    hipCheck(hipEventSynchronize(evs[2]), "raycast sync");
-    float total = 0.0f, sortMs = 0.0f;
-    hipCheck(hipEventElapsedTime(&total, evs[0], evs[2]), "elapsed");
-    hipCheck(hipEventElapsedTime(&sortMs, evs[0], evs[1]), "elapsed");
-    for (auto& e : evs) (void)hipEventDestroy(e);
+    ev.record(2, s);
+    ev.sync(2);
+    const double total = ev.ms(0, 2);
     if (collectStats) readStats(st.d.p, &tRayStats);
-    tRayStats.sort_ms = order != QR_ORDER_INPUT ? sortMs : 0.0;
+    tRayStats.sort_ms = order != QR_ORDER_INPUT ? ev.ms(0, 1) : 0.0;
+    tRayStats.kernel_ms = ev.ms(1, 2);
     return total;
 }
 }  // namespace
@@ -604,26 +618,23 @@ double pointQuery(const qr_bvh* bvh, const V3* pts, int n, int k, int outStride,
     hipStream_t s = currentStream();
     Stats st;
     if (collectStats) zeroStats(st.d.p, s);
-    hipEvent_t evs[3];
-    for (auto& e : evs) hipCheck(hipEventCreate(&e), "event");
-    hipCheck(hipEventRecord(evs[0], s), "event");
+    EventSet<3> ev;
+    ev.record(0, s);
     TraverseArgs a = argsFor(bvh->s, wide != 0);
     a.stats = collectStats ? st.d.p : nullptr;
     if (order == QR_ORDER_MORTON) a.order = reorder(*scratch, reinterpret_cast<const float*>(pts), 3, n, 0, s);
-    hipCheck(hipEventRecord(evs[1], s), "event");
+    ev.record(1, s);
     if (k == 0) {
         launchClosest(a, pts, n, out, s);
     } else {
         launchKnn(a, pts, n, k, outStride, out, s);
     }
-    hipCheck(hipEventRecord(evs[2], s), "event");
-    hipCheck(hipEventSynchronize(evs[2]), "query sync");
-    float total = 0.0f, sortMs = 0.0f;
-    hipCheck(hipEventElapsedTime(&total, evs[0], evs[2]), "elapsed");
-    hipCheck(hipEventElapsedTime(&sortMs, evs[0], evs[1]), "elapsed");
-    for (auto& e : evs) (void)hipEventDestroy(e);
+    ev.record(2, s);
+    ev.sync(2);
+    const double total = ev.ms(0, 2);
     if (collectStats) readStats(st.d.p, &tQueryStats);
-    tQueryStats.sort_ms = order == QR_ORDER_MORTON ? sortMs : 0.0;
+    tQueryStats.sort_ms = order == QR_ORDER_MORTON ? ev.ms(0, 1) : 0.0;
+    tQueryStats.kernel_ms = ev.ms(1, 2);
     return total;
 }
 
@@ -692,35 +703,37 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
             Stats st;
             if (collectStats) zeroStats(st.d.p, s);
             const float r2 = radius * radius;
-            hipEvent_t evs[3];
-            for (auto& e : evs) hipCheck(hipEventCreate(&e), "event");
-            hipCheck(hipEventRecord(evs[0], s), "event");
+            EventSet<3> ev;
+            EventSet<4> kev;
+            ev.record(0, s);
             TraverseArgs a = argsFor(bvh->s, wide != 0);
             a.stats = collectStats ? st.d.p : nullptr;
             if (order == QR_ORDER_MORTON) a.order = reorder(*p, reinterpret_cast<const float*>(p->pts.p), 3, n, 0, s);
-            hipCheck(hipEventRecord(evs[1], s), "event");
+            ev.record(1, s);
             DevBuf<int> counts(static_cast<size_t>(n));
             DevBuf<int> scratch(static_cast<size_t>(std::max(scanScratchInts(n), 4 + 2 * n)));
+            kev.record(0, s);
             launchRadiusCount(a, p->pts.p, n, r2, counts.p, s);
+            kev.record(1, s);
             scanExclusive(counts.p, res->offsets.p, n, res->offsets.p + n, scratch.p, s);
             int total = 0;
             hipCheck(hipMemcpyAsync(&total, res->offsets.p + n, sizeof(int), hipMemcpyDeviceToHost, s), "D2H total");
             hipCheck(hipStreamSynchronize(s), "radius total sync");
             res->total = total;
             res->keys.alloc(static_cast<size_t>(total > 0 ? total : 1));
+            kev.record(2, s);
             if (total > 0) {
                 a.stats = nullptr;
                 launchRadiusFill(a, p->pts.p, n, r2, res->offsets.p, res->keys.p, s);
-                segmentSort(res->offsets.p, res->keys.p, n, 1 << 30, scratch.p, s);
             }
-            hipCheck(hipEventRecord(evs[2], s), "event");
-            hipCheck(hipEventSynchronize(evs[2]), "radius sync");
-            float total_ms = 0.0f, sortMs = 0.0f;
-            hipCheck(hipEventElapsedTime(&total_ms, evs[0], evs[2]), "elapsed");
-            hipCheck(hipEventElapsedTime(&sortMs, evs[0], evs[1]), "elapsed");
-            for (auto& e : evs) (void)hipEventDestroy(e);
+            kev.record(3, s);
+            if (total > 0) segmentSort(res->offsets.p, res->keys.p, n, 1 << 30, scratch.p, s);
+            ev.record(2, s);
+            ev.sync(2);
+            const double total_ms = ev.ms(0, 2);
             if (collectStats) readStats(st.d.p, &tQueryStats);
-            tQueryStats.sort_ms = order == QR_ORDER_MORTON ? sortMs : 0.0;
+            tQueryStats.sort_ms = order == QR_ORDER_MORTON ? ev.ms(0, 1) : 0.0;
+            tQueryStats.kernel_ms = kev.ms(0, 1) + kev.ms(2, 3);
             if (ms) *ms = total_ms;
         } catch (...) {
             delete res;

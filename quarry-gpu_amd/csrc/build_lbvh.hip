@@ -223,8 +223,9 @@ __global__ __launch_bounds__(kBlock) void karrasKernel(const uint32_t* __restric
 // One thread per leaf slot k: gather the item into leaf order, write leaf node
 // n-1+k, then climb.  At each parent the arrival counter decides: the first child
 // to arrive stops, the second merges both child boxes (left operand = left child,
-// as the reference's refit does) and continues.  Height (edges to the deepest
-// leaf) rides along, so the root's height is the tree depth.
+// as the reference's refit does), resets the counter and continues.  Height
+// (edges to the deepest leaf) rides along, so the root's height is the tree
+// depth.  Counters start at zero (karrasKernel) and end at zero.
 template <typename L, bool kGather>
 __global__ __launch_bounds__(kBlock) void leafUpKernel(
     const typename L::Item* __restrict__ src, const uint32_t* __restrict__ order, int n,
@@ -266,6 +267,9 @@ __global__ __launch_bounds__(kBlock) void leafUpKernel(
         h = (h > sh ? h : sh) + 1;
         writeBoxAgent(&nodes[p], box);
         stAgentI(&height[p], h);
+        // Both children have arrived: return the counter to zero so every arrival
+        // counter is zero between kernels (the dirty refit relies on it).
+        stAgentI(&arrive[p], 0);
         node = p;
     }
 }
@@ -292,9 +296,8 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     DevBuf<uint32_t> order(static_cast<size_t>(n));
     uint32_t* cb = reinterpret_cast<uint32_t*>(b.counters.p);
 
-    hipEvent_t ev[6];
-    for (auto& e : ev) hipCheck(hipEventCreate(&e), "event create");
-    hipCheck(hipEventRecord(ev[0], s), "event");
+    EventSet<6> ev;
+    ev.record(0, s);
 
     const int tiles = sortTiles(static_cast<size_t>(n));
     const long long descWords = 4LL * tiles * 256;
@@ -307,42 +310,39 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     centroidBoundsKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb);
     mortonHistKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb, codes.p, b.sortWs.hist.p);
     hipCheck(hipGetLastError(), "morton");
-    hipCheck(hipEventRecord(ev[1], s), "event");
+    ev.record(1, s);
 
     sortPairs(b.sortWs, codes.p, nullptr, codesSorted.p, order.p, static_cast<size_t>(n), 4, s);
-    hipCheck(hipEventRecord(ev[2], s), "event");
+    ev.record(2, s);
 
     karrasKernel<<<blocksFor(n - 1, kBlock), kBlock, 0, s>>>(codesSorted.p, order.p, n, b.nodes.p,
                                                             b.parent.p, b.arrive.p);
     hipCheck(hipGetLastError(), "karras");
-    hipCheck(hipEventRecord(ev[3], s), "event");
+    ev.record(3, s);
 
     leafUpKernel<L, true><<<blocksFor(n, kBlock), kBlock, 0, s>>>(
         dSrc, order.p, n, reinterpret_cast<Item*>(b.items.p), b.primIds.p, b.nodes.p, b.parent.p,
         b.arrive.p, b.height.p);
     hipCheck(hipGetLastError(), "leaf up");
-    hipCheck(hipEventRecord(ev[4], s), "event");
+    ev.record(4, s);
 
     int depth = 0;
     hipCheck(hipMemcpyAsync(&depth, b.height.p, sizeof(int), hipMemcpyDeviceToHost, s), "depth");
-    hipCheck(hipEventRecord(ev[5], s), "event");
-    hipCheck(hipEventSynchronize(ev[5]), "build sync");
+    ev.record(5, s);
+    ev.sync(5);
     b.depth = depth;
 
     if (t) {
         float ms[5];
-        for (int i = 0; i < 5; ++i) hipCheck(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]), "elapsed");
+        for (int i = 0; i < 5; ++i) ms[i] = ev.ms(i, i + 1);
         t->mortonMs = ms[0];
         t->sortMs = ms[1];
         t->treeMs = ms[2];
         t->scatterMs = 0.0;
         t->refitMs = ms[3];
         t->sahBinMs = t->sahSplitMs = t->sahPartMs = 0.0;
-        float total = 0;
-        hipCheck(hipEventElapsedTime(&total, ev[0], ev[4]), "elapsed");
-        t->totalMs = total;
+        t->totalMs = ev.ms(0, 4);
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
     b.primSlotValid = false;
     b.depthValid = false;
     b.wideValid = false;
@@ -369,18 +369,10 @@ void buildLbvhDevice(BvhState& b, int kind, const void* dItems, int n, hipStream
     }
 }
 
-// Refit of every leaf from the leaf-ordered items already in b.items: reset the
-// arrival counters, then the same bottom-up pass without the gather.
-namespace {
-__global__ void resetArriveKernel(int* __restrict__ arrive, int count) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < count) arrive[i] = 0;
-}
-}  // namespace
-
+// Refit of every leaf from the leaf-ordered items already in b.items: the same
+// bottom-up pass without the gather (arrival counters are zero between kernels).
 void propagateAll(BvhState& b, hipStream_t s) {
     const int n = b.n;
-    resetArriveKernel<<<blocksFor(n - 1, kBlock), kBlock, 0, s>>>(b.arrive.p, n - 1);
     const int grid = blocksFor(n, kBlock);
     switch (b.kind) {
         case kPrimTri:

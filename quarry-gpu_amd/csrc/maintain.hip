@@ -146,10 +146,13 @@ __global__ void dirtyClimbKernel(const int* __restrict__ leaves, int count, Node
     while (true) {
         const int p = parent[node];
         if (p < 0) break;
+        // need[p] is read before this arrival is counted: the last arrival resets it,
+        // so a read after the increment could see the reset value.  The vmcnt wait
+        // also orders this thread's box stores ahead of the increment.
+        const int want = __hip_atomic_load(&need[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         storesDone();
         const int prior = __hip_atomic_fetch_add(&arrive[p], 1, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT);
-        const int want = __hip_atomic_load(&need[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (prior + 1 < want) break;
         const int l = nodes[p].left;
         const int r = nodes[p].right;

@@ -127,6 +127,29 @@ struct DevBuf {
 // Stream the library issues work on (per host thread; null = legacy default).
 hipStream_t currentStream();
 
+// A fixed set of timing events, destroyed with the scope (so a query of the
+// elapsed time can never touch a destroyed event).
+template <int N>
+struct EventSet {
+    hipEvent_t e[N] = {};
+    EventSet() {
+        for (auto& x : e) hipCheck(hipEventCreate(&x), "event create");
+    }
+    ~EventSet() {
+        for (auto& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    EventSet(const EventSet&) = delete;
+    EventSet& operator=(const EventSet&) = delete;
+    void record(int i, hipStream_t s) { hipCheck(hipEventRecord(e[i], s), "event record"); }
+    void sync(int i) { hipCheck(hipEventSynchronize(e[i]), "event sync"); }
+    float ms(int a, int b) const {
+        float v = 0.0f;
+        hipCheck(hipEventElapsedTime(&v, e[a], e[b]), "event elapsed");
+        return v;
+    }
+};
+
 // ---- sort ---------------------------------------------------------------------
 // Workspace for the onesweep LSD radix sort of 32-bit keys with 32-bit values.
 struct SortWorkspace {

@@ -47,7 +47,8 @@ class RefitTimings(ctypes.Structure):
 
 class TraversalStats(ctypes.Structure):
     _fields_ = [("node_tests", ctypes.c_uint64), ("leaf_tests", ctypes.c_uint64),
-                ("stack_overflows", ctypes.c_uint64), ("sort_ms", ctypes.c_double)]
+                ("stack_overflows", ctypes.c_uint64), ("sort_ms", ctypes.c_double),
+                ("kernel_ms", ctypes.c_double)]
 
 
 class RayStats(ctypes.Structure):
@@ -81,6 +82,11 @@ def lib():
             "qr_describe": ([ctypes.c_char_p, _ip], _ip),
             "qr_warmup": ([], _ip),
             "qr_set_stream": ([_vp], _ip),
+            "qr_dev_alloc": ([ctypes.c_size_t, _vp], _ip),
+            "qr_dev_free": ([_vp], _ip),
+            "qr_dev_upload": ([_vp, _vp, ctypes.c_size_t], _ip),
+            "qr_dev_download": ([_vp, _vp, ctypes.c_size_t], _ip),
+            "qr_dev_synchronize": ([], _ip),
             "qr_bvh_build": ([_ip, _ip, _vp, _ip, _vp, _vp], _ip),
             "qr_bvh_build_device": ([_ip, _ip, _vp, _ip, _vp, _vp], _ip),
             "qr_bvh_destroy": ([_vp], _ip),
@@ -226,10 +232,8 @@ class Bvh:
             out = np.zeros(rays.shape[0], dtype=np.uint64)
             rs = RayStats()
             check(lib().qr_rays_summarize(rb, ptr(out), ctypes.byref(rs)))
-            st = None
-            if stats:
-                st = TraversalStats()
-                check(lib().qr_last_traversal_stats(ctypes.byref(st)))
+            st = TraversalStats()
+            check(lib().qr_last_traversal_stats(ctypes.byref(st)))
             return out, ms.value, st
         finally:
             lib().qr_rays_destroy(rb)
@@ -287,8 +291,135 @@ class Bvh:
 
     @staticmethod
     def _qstats(want: bool):
-        if not want:
-            return None
         st = TraversalStats()
         check(lib().qr_last_query_stats(ctypes.byref(st)))
         return st
+
+
+class PointBuffer:
+    """Device-resident query points + result slots (uploadPointsGPU)."""
+
+    def __init__(self, pts: np.ndarray, stride: int = 1):
+        pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+        self.count = pts.shape[0]
+        self.stride = stride
+        self.h = ctypes.c_void_p()
+        check(lib().qr_points_upload(ptr(pts), self.count, stride, ctypes.byref(self.h)))
+
+    def close(self) -> None:
+        if self.h.value:
+            lib().qr_points_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def closest(self, bvh: Bvh, order: int = ORDER_MORTON, wide: bool = True, stats: bool = False) -> float:
+        ms = ctypes.c_double()
+        check(lib().qr_closest(bvh.h, self.h, int(stats), order, int(wide), ctypes.byref(ms)))
+        return ms.value
+
+    def knn(self, bvh: Bvh, k: int, order: int = ORDER_MORTON, wide: bool = True, stats: bool = False) -> float:
+        ms = ctypes.c_double()
+        check(lib().qr_knn(bvh.h, self.h, k, int(stats), order, int(wide), ctypes.byref(ms)))
+        return ms.value
+
+    def radius(self, bvh: Bvh, r: float, order: int = ORDER_MORTON, wide: bool = True,
+               stats: bool = False, keep: bool = False):
+        """Runs a radius query; returns ms (and the CSR arrays when keep=True)."""
+        csr = ctypes.c_void_p()
+        ms = ctypes.c_double()
+        check(lib().qr_radius(bvh.h, self.h, float(r), int(stats), order, int(wide),
+                              ctypes.byref(csr), ctypes.byref(ms)))
+        try:
+            if not keep:
+                return ms.value
+            cnt, tot = ctypes.c_int(), ctypes.c_int()
+            check(lib().qr_radius_info(csr, ctypes.byref(cnt), ctypes.byref(tot)))
+            offs = np.zeros(cnt.value + 1, dtype=np.int32)
+            keys = np.zeros(max(tot.value, 1), dtype=np.uint64)
+            check(lib().qr_radius_read(csr, ptr(offs), ptr(keys)))
+            return ms.value, offs, keys[: tot.value]
+        finally:
+            lib().qr_radius_destroy(csr)
+
+    def read(self) -> np.ndarray:
+        out = np.zeros((self.count, self.stride, 2), dtype=np.uint32)
+        check(lib().qr_points_read(self.h, ptr(out)))
+        return out
+
+
+class RayBuffer:
+    """Device-resident rays + per-ray results (uploadRaysGPU)."""
+
+    def __init__(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        self.count = rays.shape[0]
+        self.h = ctypes.c_void_p()
+        check(lib().qr_rays_upload(ptr(rays), self.count, ctypes.byref(self.h)))
+
+    def close(self) -> None:
+        if self.h.value:
+            lib().qr_rays_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cast(self, bvh: Bvh, order: int = ORDER_MORTON, wide: bool = True, stats: bool = False) -> float:
+        ms = ctypes.c_double()
+        check(lib().qr_raycast(bvh.h, self.h, int(stats), order, int(wide), ctypes.byref(ms)))
+        return ms.value
+
+    def summarize(self):
+        out = np.zeros(self.count, dtype=np.uint64)
+        rs = RayStats()
+        check(lib().qr_rays_summarize(self.h, ptr(out), ctypes.byref(rs)))
+        return out, rs
+
+
+def last_query_stats() -> TraversalStats:
+    st = TraversalStats()
+    check(lib().qr_last_query_stats(ctypes.byref(st)))
+    return st
+
+
+def last_ray_stats() -> TraversalStats:
+    st = TraversalStats()
+    check(lib().qr_last_traversal_stats(ctypes.byref(st)))
+    return st
+
+
+class DeviceArray:
+    """A device allocation holding a copy of a host array (for *_device calls)."""
+
+    def __init__(self, host: np.ndarray):
+        host = np.ascontiguousarray(host)
+        self.nbytes = host.nbytes
+        self.shape = host.shape
+        self.dtype = host.dtype
+        self.ptr = ctypes.c_void_p()
+        check(lib().qr_dev_alloc(self.nbytes, ctypes.byref(self.ptr)))
+        check(lib().qr_dev_upload(self.ptr, ptr(host), self.nbytes))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib().qr_dev_download(ptr(out), self.ptr, self.nbytes))
+        return out
+
+    def close(self) -> None:
+        if self.ptr.value:
+            lib().qr_dev_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,254 @@
+"""Bit-exact parity of the HIP path with the CPU oracle (through the C ABI).
+
+Every result set is compared whole: ray results as (hit << 32 | t bits), closest /
+kNN records as (dist2 bits, index), radius CSR offsets and keys.  The oracle is
+brute force for small inputs and the median-split CPU BVH (itself equal to brute
+force, tests/test_oracle_*.py) at benchmark sizes.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def A(qabi):
+    if qabi.describe() is None:
+        pytest.fail("no HIP device visible to the GPU tests")
+    return qabi
+
+
+def eq(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def random_tris(rng, n, spread=30.0, size=1.5):
+    a = rng.uniform(-spread, spread, (n, 3))
+    return np.concatenate([a, a + rng.normal(0, size, (n, 3)), a + rng.normal(0, size, (n, 3))], 1).astype(np.float32)
+
+
+def random_gaussians(rng, n, spread=30.0):
+    return np.concatenate([rng.uniform(-spread, spread, (n, 3)), rng.normal(size=(n, 4)),
+                           np.exp(rng.uniform(np.log(0.05), np.log(1.5), (n, 3)))], 1).astype(np.float32)
+
+
+# ---- build --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("kind,n", [(0, 2), (0, 3), (0, 5000), (1, 7000), (2, 3000)])
+def test_node_bytes_match_cpu_lbvh(A, orc, scenes, kind, n):
+    log(f"lbvh kind={kind} n={n}")
+    rng = np.random.default_rng(n + kind)
+    items = {0: lambda: random_tris(rng, n), 1: lambda: rng.uniform(-9, 9, (n, 3)).astype(np.float32),
+             2: lambda: random_gaussians(rng, n)}[kind]()
+    b = A.Bvh(kind, items)
+    nodes, ids = b.read()
+    cn, cids, depth = orc.lbvh_nodes(kind, items)
+    assert eq(ids, cids)
+    assert eq(nodes, cn)
+    assert b.tree_hash() == orc.fnv_bytes(cn)
+    assert b.info().depth == depth
+    assert b.validate(items) == ""
+
+
+def test_terrain_build_deterministic(A, orc, scenes):
+    tris = scenes.terrain(256, 1)
+    h = {A.Bvh(0, tris).tree_hash() for _ in range(3)}
+    assert len(h) == 1
+    cn, _, _ = orc.lbvh_nodes(0, tris)
+    assert h.pop() == orc.fnv_bytes(cn)
+
+
+def test_duplicate_centroids(A, orc):
+    pts = np.zeros((4096, 3), np.float32)  # one Morton cell: order by index only
+    pts[::3] = 1.0
+    b = A.Bvh(1, pts)
+    cn, _, _ = orc.lbvh_nodes(1, pts)
+    assert eq(b.read()[0], cn)
+    q = np.array([[0.5, 0.5, 0.5], [1, 1, 1], [0, 0, 0]], np.float32)
+    assert eq(b.knn(q, 32)[0], orc.knn_brute(1, pts, q, 32))
+
+
+# ---- rays ---------------------------------------------------------------------------
+
+def test_rays_small_vs_brute(A, orc, scenes):
+    rng = np.random.default_rng(7)
+    tris = np.concatenate([scenes.terrain(32, 2), random_tris(rng, 500, 16, 2.0) + np.float32(16)])
+    b = A.Bvh(0, tris)
+    b.build_wide()
+    down = scenes.down_rays(tris, 4000, 3)
+    o = rng.uniform(-40, 80, (4000, 3))
+    d = rng.normal(size=(4000, 3))
+    d[:50, :2] = 0.0  # exercise the |d| < 1e-20 axes
+    rand = np.concatenate([o, d, np.full((4000, 1), 1e-6), np.full((4000, 1), np.inf)], 1).astype(np.float32)
+    rand[100:200, 7] = 5.0  # finite tMax
+    for rays in (down, rand):
+        want = orc.ray_brute(tris, rays, threads=THREADS)
+        for wide in (False, True):
+            for order in (A.ORDER_INPUT, A.ORDER_MORTON, A.ORDER_MORTON_DIRECTION):
+                got, _, st = b.raycast(rays, order=order, wide=wide, stats=True)
+                assert eq(got, want), (wide, order)
+                assert st.stack_overflows == 0
+
+
+def test_rays_c1_full_size(A, orc, scenes):
+    tris, rays = scenes.workload_c1(seed=1)
+    b = A.Bvh(0, tris)
+    b.build_wide()
+    want = orc.CpuBvh(0, tris).ray(rays, threads=THREADS)
+    for wide in (False, True):
+        got, ms, _ = b.raycast(rays, order=A.ORDER_MORTON, wide=wide)
+        assert eq(got, want), wide
+    log(f"C1 rays: hash {hex(orc.ray_hash(want)[0])}")
+
+
+# ---- point queries --------------------------------------------------------------------
+
+def check_queries(A, orc, kind, items, q, radius, ks=(1, 4, 8, 16, 32), brute=True):
+    b = A.Bvh(kind, items)
+    b.build_wide()
+    ref = orc.CpuBvh(kind, items)
+    want_c = orc.knn_brute(kind, items, q, 0, threads=THREADS) if brute else ref.knn(q, 0, THREADS)
+    want_r = orc.radius_brute(kind, items, q, radius, threads=THREADS) if brute else ref.radius(q, radius, THREADS)
+    for wide in (False, True):
+        for order in (A.ORDER_INPUT, A.ORDER_MORTON):
+            assert eq(b.closest(q, order=order, wide=wide)[0], want_c), ("closest", wide, order)
+            o, k, _, _ = b.radius(q, radius, order=order, wide=wide)
+            assert eq(o, want_r[0]) and eq(k, want_r[1]), ("radius", wide, order)
+    for k in ks:
+        want_k = orc.knn_brute(kind, items, q, k, threads=THREADS) if brute else ref.knn(q, k, THREADS)
+        for wide in (False, True):
+            assert eq(b.knn(q, k, order=A.ORDER_MORTON, wide=wide)[0], want_k), ("knn", k, wide)
+    return b
+
+
+def test_triangle_queries_vs_brute(A, orc, scenes):
+    rng = np.random.default_rng(3)
+    tris = np.concatenate([scenes.terrain(24, 5), random_tris(rng, 800, 12)])
+    q = rng.uniform(-16, 30, (1500, 3)).astype(np.float32)
+    check_queries(A, orc, 0, tris, q, 2.0)
+
+
+def test_point_queries_vs_brute(A, orc, scenes):
+    pts = scenes.point_cloud(6000, 2)
+    pts[100:140] = pts[7]  # exact ties broken by index
+    q = np.random.default_rng(4).uniform(-4, 68, (1200, 3)).astype(np.float32)
+    check_queries(A, orc, 1, pts, q, 3.0)
+
+
+def test_gaussian_queries_vs_brute(A, orc, scenes):
+    g = scenes.gaussians(4000, 3)
+    g[:, 3:7] *= np.float32(2.5)  # unnormalised quaternions
+    q = np.random.default_rng(5).uniform(-4, 68, (800, 3)).astype(np.float32)
+    check_queries(A, orc, 2, g, q, 1.5, ks=(1, 8, 32))
+
+
+def test_c2_full_size(A, orc, scenes):
+    tris, q = scenes.workload_c2(seed=1)
+    log("C2: 2M triangles")
+    b = A.Bvh(0, tris)
+    b.build_wide()
+    ref = orc.CpuBvh(0, tris)
+    assert eq(b.closest(q, order=A.ORDER_MORTON, wide=True)[0], ref.knn(q, 0, THREADS))
+    assert eq(b.knn(q, 16, order=A.ORDER_MORTON, wide=True)[0], ref.knn(q, 16, THREADS))
+    o, k, _, _ = b.radius(q, 2.0, order=A.ORDER_MORTON, wide=True)
+    wo, wk = ref.radius(q, 2.0, THREADS)
+    assert eq(o, wo) and eq(k, wk)
+
+
+def test_golden_fixtures(A):
+    """GPU results equal the reference CPU code's stored outputs."""
+    g = np.load(os.path.join(GOLD, "tri_mixed.npz"), allow_pickle=False)
+    b = A.Bvh(0, g["tris"])
+    b.build_wide()
+    for wide in (False, True):
+        assert eq(b.raycast(g["rays_down"], wide=wide)[0], g["ray_down"])
+        assert eq(b.raycast(g["rays_random"], wide=wide, order=A.ORDER_MORTON)[0], g["ray_random"])
+        assert eq(b.closest(g["points"], wide=wide)[0], g["closest"])
+        assert eq(b.knn(g["points"], 6, wide=wide)[0], g["knn6"])
+        o, k, _, _ = b.radius(g["points"], 3.0, wide=wide)
+        assert eq(o, g["radius3_offsets"]) and eq(k, g["radius3_keys"])
+    p = np.load(os.path.join(GOLD, "points.npz"), allow_pickle=False)
+    b = A.Bvh(1, p["cloud"])
+    assert eq(b.closest(p["points"])[0], p["closest"])
+    assert eq(b.knn(p["points"], 8)[0], p["knn8"])
+    o, k, _, _ = b.radius(p["points"], 2.5)
+    assert eq(o, p["radius_offsets"]) and eq(k, p["radius_keys"])
+    gg = np.load(os.path.join(GOLD, "gaussians.npz"), allow_pickle=False)
+    b = A.Bvh(2, gg["gaussians"])
+    assert eq(b.closest(gg["points"])[0], gg["closest"])
+    assert eq(b.knn(gg["points"], 8)[0], gg["knn8"])
+    o, k, _, _ = b.radius(gg["points"], 1.0)
+    assert eq(o, gg["radius_offsets"]) and eq(k, gg["radius_keys"])
+
+
+# ---- refit ------------------------------------------------------------------------------
+
+def test_refit_dirty_rounds(A, orc, scenes):
+    cells = 128
+    tris = scenes.terrain(cells, 1)
+    b = A.Bvh(0, tris)
+    b.build_wide()
+    q = scenes.query_points(np.array([-4, -4, -12], np.float32), np.array([132, 132, 12], np.float32), 3000, 9)
+    cur = tris
+    for rnd in range(4):
+        log(f"refit round {rnd}")
+        moved, dirty = scenes.displace_vertices(cells, cur, 0.01 * (rnd + 1), 0.05, seed=20 + rnd)
+        t = b.refit(moved, dirty)
+        assert t.updated == len(np.unique(dirty))
+        assert b.validate(moved) == ""
+        cur = moved
+        cn_boxes_ok = b.validate(cur) == ""
+        assert cn_boxes_ok
+        assert eq(b.closest(q, wide=True)[0], orc.knn_brute(0, cur, q, 0, threads=THREADS))
+    # a full refit after dirty rounds restores the original boxes exactly
+    b.refit(tris)
+    cn, _, _ = orc.lbvh_nodes(0, tris)
+    assert eq(b.read()[0], cn)
+    # dirty refit with duplicate ids and an empty list
+    b.refit(tris, np.array([5, 5, 5], np.int32))
+    b.refit(tris, np.array([], np.int32))
+    assert eq(b.read()[0], cn)
+
+
+def test_refit_points_and_gaussians(A, orc, scenes):
+    rng = np.random.default_rng(8)
+    for kind, items in ((1, scenes.point_cloud(5000, 4)), (2, scenes.gaussians(3000, 4))):
+        b = A.Bvh(kind, items)
+        moved = items.copy()
+        idx = rng.choice(items.shape[0], 200, replace=False)
+        moved[idx, 0:3] += rng.normal(0, 0.5, (200, 3)).astype(np.float32)
+        b.refit(moved, idx)
+        assert b.validate(moved) == ""
+        q = rng.uniform(0, 64, (500, 3)).astype(np.float32)
+        assert eq(b.knn(q, 8)[0], orc.knn_brute(kind, moved, q, 8, threads=THREADS))
+
+
+# ---- error contract -----------------------------------------------------------------------
+
+def test_errors(A, scenes):
+    tris = scenes.terrain(8, 1)
+    b = A.Bvh(0, tris)
+    q = np.zeros((4, 3), np.float32)
+    with pytest.raises(A.QuarryError):
+        b.knn(q, 0)
+    with pytest.raises(A.QuarryError):
+        b.knn(q, 33)
+    with pytest.raises(A.QuarryError):
+        b.closest(q, wide=True)  # wide nodes not built
+    with pytest.raises(A.QuarryError):
+        A.Bvh(0, tris[:1])
+    with pytest.raises(A.QuarryError):
+        b.refit(tris[:-1])
+    o, k, _, _ = b.radius(q, -1.0)
+    assert o.tolist() == [0] * 5 and k.size == 0
+    # empty query sets are fine
+    assert b.closest(np.zeros((0, 3), np.float32))[0].shape == (0, 1, 2)
