@@ -131,6 +131,19 @@ def main() -> int:
                   enumerate(("closest", "knn", "radius"))}
     log(f"step: {ms_per_step:.3f} ms wall; per call (event ms) {per_op}; traversal kernels {per_kernel}")
 
+    # traversal work per query (one extra counted pass, outside the timed region)
+    work = {}
+    pb1.closest(bvh, stats=True)
+    st = A.last_query_stats()
+    work["closest"] = (st.node_tests / queries.shape[0], st.leaf_tests / queries.shape[0])
+    pbk.knn(bvh, args.k, stats=True)
+    st = A.last_query_stats()
+    work["knn"] = (st.node_tests / queries.shape[0], st.leaf_tests / queries.shape[0])
+    pb1.radius(bvh, args.radius, stats=True)
+    st = A.last_query_stats()
+    work["radius"] = (st.node_tests / queries.shape[0], st.leaf_tests / queries.shape[0])
+    log("node/leaf tests per query: " + ", ".join(f"{k} {v[0]:.1f}/{v[1]:.1f}" for k, v in work.items()))
+
     # ---- roofline of the dominant kernel ------------------------------------------
     dom = max(per_kernel, key=per_kernel.get)
     nq = queries.shape[0]
@@ -251,6 +264,8 @@ def main() -> int:
                    "radius_results": radius_total},
         "per_op_ms": {k: round(v, 4) for k, v in per_op.items()},
         "per_op_mqps": {k: round(nq / (v * 1e-3) / 1e6, 2) for k, v in per_op.items()},
+        "per_op_kernel_ms": {k: round(v, 4) for k, v in per_kernel.items()},
+        "tests_per_query": {k: {"node": round(v[0], 2), "leaf": round(v[1], 2)} for k, v in work.items()},
         "roofline": roofline,
         "build": build,
         "parity": parity,

@@ -2,6 +2,7 @@
 // management, host<->device transfers, timing and error mapping around the HIP
 // kernels.  No kernel math lives here.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -20,8 +21,11 @@ struct qr_rays {
     DevBuf<Ray> rays;
     DevBuf<uint64_t> results;
     int count = 0;
-    // reordering scratch, allocated on first use
+    // reordering scratch, allocated on first use; bb holds the origin bounds
+    // measured at upload (the reference computes them on the host as well,
+    // src/gpu/raycast.cu:315-328)
     DevBuf<uint32_t> keys, keysSorted, order, bb;
+    bool bbValid = false;
     SortWorkspace ws;
 };
 
@@ -30,7 +34,9 @@ struct qr_points {
     DevBuf<ClosestResult> results;
     int count = 0;
     int stride = 1;
+    // bb: query bounds measured at upload (src/gpu/query.cu:822-847)
     DevBuf<uint32_t> keys, keysSorted, order, bb;
+    bool bbValid = false;
     SortWorkspace ws;
 };
 
@@ -131,15 +137,32 @@ void checkTree(const qr_bvh* bvh, bool wide) {
 
 // Morton visiting order for a buffer of elements `stride` floats apart.
 template <typename Buf>
-const uint32_t* reorder(Buf& buf, const float* base, int stride, int count, int fixedBox,
+const uint32_t* reorder(Buf& buf, const float* base, int stride, int count, int boxMode,
                         hipStream_t s) {
     buf.keys.reserve(static_cast<size_t>(count));
     buf.keysSorted.reserve(static_cast<size_t>(count));
     buf.order.reserve(static_cast<size_t>(count));
     buf.bb.reserve(8);
-    mortonOrder(base, stride, count, fixedBox, buf.ws, buf.keys.p, buf.keysSorted.p, buf.order.p,
+    mortonOrder(base, stride, count, boxMode, buf.ws, buf.keys.p, buf.keysSorted.p, buf.order.p,
                 buf.bb.p, s);
     return buf.order.p;
+}
+
+// Bounds of the first three floats of n elements `stride` floats apart, as the
+// ordered-bit words the Morton kernels read (lo xyz, hi xyz).
+void hostBounds(const float* base, int stride, int n, uint32_t out[6]) {
+    float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
+    for (int i = 0; i < n; ++i) {
+        const float* p = base + static_cast<size_t>(i) * stride;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::fmin(lo[a], p[a]);
+            hi[a] = std::fmax(hi[a], p[a]);
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        out[a] = orderedBits(lo[a]);
+        out[3 + a] = orderedBits(hi[a]);
+    }
 }
 
 void buildInto(BvhState& b, int prim, int builder, const void* dItems, int n, qr_build_timings* t) {
@@ -482,8 +505,14 @@ int qr_rays_upload(const void* rays, int n, qr_rays** out) {
             r->count = n;
             r->rays.alloc(static_cast<size_t>(n));
             r->results.alloc(static_cast<size_t>(n));
-            if (n > 0)
+            if (n > 0) {
                 hipCheck(hipMemcpy(r->rays.p, rays, static_cast<size_t>(n) * sizeof(Ray), hipMemcpyHostToDevice), "H2D rays");
+                uint32_t bb[6];
+                hostBounds(static_cast<const float*>(rays), 8, n, bb);
+                r->bb.reserve(8);
+                hipCheck(hipMemcpy(r->bb.p, bb, sizeof(bb), hipMemcpyHostToDevice), "H2D ray bounds");
+                r->bbValid = true;
+            }
         } catch (...) {
             delete r;
             throw;
@@ -513,7 +542,9 @@ double raycastImpl(const qr_bvh* bvh, const Ray* rays, int n, uint64_t* out, qr_
     a.stats = collectStats ? st.d.p : nullptr;
     if (order != QR_ORDER_INPUT) {
         const float* base = reinterpret_cast<const float*>(rays) + (order == QR_ORDER_MORTON ? 0 : 3);
-        a.order = reorder(*scratch, base, 8, n, order == QR_ORDER_MORTON_DIRECTION ? 1 : 0, s);
+        const int mode = order == QR_ORDER_MORTON_DIRECTION ? kBoxUnit
+                                                            : (scratch->bbValid ? kBoxGiven : kBoxMeasure);
+        a.order = reorder(*scratch, base, 8, n, mode, s);
     }
     ev.record(1, s);
     launchRays(a, rays, n, out, s);
@@ -585,8 +616,14 @@ int qr_points_upload(const float* pts, int n, int rpp, qr_points** out) {
             p->stride = rpp > 0 ? rpp : 1;
             p->pts.alloc(static_cast<size_t>(n));
             p->results.alloc(static_cast<size_t>(n) * p->stride);
-            if (n > 0)
+            if (n > 0) {
                 hipCheck(hipMemcpy(p->pts.p, pts, static_cast<size_t>(n) * sizeof(V3), hipMemcpyHostToDevice), "H2D points");
+                uint32_t bb[6];
+                hostBounds(pts, 3, n, bb);
+                p->bb.reserve(8);
+                hipCheck(hipMemcpy(p->bb.p, bb, sizeof(bb), hipMemcpyHostToDevice), "H2D point bounds");
+                p->bbValid = true;
+            }
         } catch (...) {
             delete p;
             throw;
@@ -622,7 +659,9 @@ double pointQuery(const qr_bvh* bvh, const V3* pts, int n, int k, int outStride,
     ev.record(0, s);
     TraverseArgs a = argsFor(bvh->s, wide != 0);
     a.stats = collectStats ? st.d.p : nullptr;
-    if (order == QR_ORDER_MORTON) a.order = reorder(*scratch, reinterpret_cast<const float*>(pts), 3, n, 0, s);
+    if (order == QR_ORDER_MORTON)
+        a.order = reorder(*scratch, reinterpret_cast<const float*>(pts), 3, n,
+                          scratch->bbValid ? kBoxGiven : kBoxMeasure, s);
     ev.record(1, s);
     if (k == 0) {
         launchClosest(a, pts, n, out, s);
@@ -708,7 +747,9 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
             ev.record(0, s);
             TraverseArgs a = argsFor(bvh->s, wide != 0);
             a.stats = collectStats ? st.d.p : nullptr;
-            if (order == QR_ORDER_MORTON) a.order = reorder(*p, reinterpret_cast<const float*>(p->pts.p), 3, n, 0, s);
+            if (order == QR_ORDER_MORTON)
+                a.order = reorder(*p, reinterpret_cast<const float*>(p->pts.p), 3, n,
+                                  p->bbValid ? kBoxGiven : kBoxMeasure, s);
             ev.record(1, s);
             DevBuf<int> counts(static_cast<size_t>(n));
             DevBuf<int> scratch(static_cast<size_t>(std::max(scanScratchInts(n), 4 + 2 * n)));

@@ -19,10 +19,17 @@ struct TraverseArgs {
 
 void zeroStats(unsigned long long* stats, hipStream_t s);
 
+// Quantisation domain of a Morton reordering.
+enum MortonBox : int {
+    kBoxMeasure = 0,  // measure the bounds of the elements on the device (into bb)
+    kBoxUnit = 1,     // [-1, 1]^3 (ray directions)
+    kBoxGiven = 2,    // bb already holds the bounds (ordered bits: lo xyz, hi xyz)
+};
+
 // Computes a visiting order by 30-bit Morton code of the first three floats of
-// each element (elements are `stride` floats apart).  fixedBox = 1 quantises over
-// [-1,1]^3 instead of the measured bounds.  bb: 6 device words of scratch.
-void mortonOrder(const float* base, int stride, int count, int fixedBox, SortWorkspace& ws,
+// each element (elements are `stride` floats apart) quantised over the domain
+// given by boxMode.  bb: 6 device words.
+void mortonOrder(const float* base, int stride, int count, int boxMode, SortWorkspace& ws,
                  uint32_t* keys, uint32_t* keysSorted, uint32_t* orderOut, uint32_t* bb,
                  hipStream_t s);
 

@@ -1,15 +1,24 @@
 // Traversal kernels: closest-hit rays, closest / kNN / radius point queries, over
 // binary (32 B) or 4-wide (128 B) nodes, for triangle, point and Gaussian leaves.
 //
-// One lane per ray or query, depth-first with an explicit per-lane stack.  The
-// pruning predicates are the reference's (src/gpu/raycast.cu:75-245,
-// src/gpu/query.cu:52-637): padded slab test against the current best t for
-// rays, box lower bound <= best for closest/kNN (so equal-distance primitives
-// with a smaller index are still reached), box lower bound <= r^2 for radius.
-// Because every predicate only ever prunes subtrees that cannot hold a better
-// (dist2, index) or a smaller t, the result is the brute-force answer whatever
-// the tree, node width or visiting order - which is what makes binary, wide,
-// LBVH, SAH and reordered runs bit-identical.
+// One lane per ray or query, depth-first.  The pruning predicates are the
+// reference's (src/gpu/raycast.cu:75-245, src/gpu/query.cu:52-637): padded slab
+// test against the current best t for rays, box lower bound <= bound for
+// closest / kNN (so equal-distance primitives with a smaller index are still
+// reached), box lower bound <= r^2 for radius.  Every predicate only prunes
+// subtrees that cannot hold a better (dist2, index) or a smaller t, so the result
+// is the brute-force answer whatever the tree, node width or visiting order -
+// which is what makes binary, wide, LBVH, SAH and reordered runs bit-identical.
+//
+// Layout for gfx950:
+//   * the per-lane DFS stack lives in LDS ([depth][lane], 32 entries = 32 KB per
+//     256-lane workgroup); deeper entries spill to scratch, which only very deep
+//     or very wide traversals touch;
+//   * a node's children are ordered with a fixed compare-exchange network in
+//     registers (no dynamically indexed private arrays);
+//   * leaf children are tested as soon as their parent is opened, nearest first,
+//     against the live bound, and only interior children go on the stack;
+//   * closest is kNN with k = 1 (identical record and tie rule).
 //
 // Queries may be visited in a permuted order (Morton reordering): lane i works
 // on query order[i] and writes its result straight to slot order[i], so no
@@ -21,8 +30,8 @@ namespace qr {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kStack = 72;       // binary: one net push per level, depth <= 64 for LBVH
-constexpr int kStackWide = 144;  // 4-wide: at most 3 net pushes per wide level
+constexpr int kLdsDepth = 32;
+constexpr int kSpill = 112;  // scratch entries beyond the LDS part (never reached by LBVH trees)
 
 // ---- stats -------------------------------------------------------------------------
 // stats[0] node tests, [1] leaf tests, [2] stack overflows.
@@ -46,253 +55,87 @@ __device__ __forceinline__ void flushStats(unsigned long long* stats, int nodeTe
     }
 }
 
-// ---- rays --------------------------------------------------------------------------
+// ---- per-lane stack: LDS first, scratch beyond ---------------------------------------
+// The stack pointer and LDS column are plain registers; only the spill area (the
+// part beyond the LDS depth) lives in scratch.  Both areas are accessed through
+// volatile pointers: that keeps every LDS access a ds_read/ds_write (which one
+// wave executes in order) and stops the compiler from merging the two branches
+// of a pop into one generic (flat) load, which is not ordered behind the
+// preceding ds_write pushes.
+typedef __attribute__((address_space(3))) volatile int LdsInt;      // workgroup LDS
+typedef __attribute__((address_space(5))) volatile int PrivateInt;  // per-lane scratch
 
-template <bool kStats>
-__global__ __launch_bounds__(kBlock) void rayBinaryKernel(const Node2* __restrict__ nodes,
-                                                          const Tri* __restrict__ tris,
-                                                          const Ray* __restrict__ rays,
-                                                          const uint32_t* __restrict__ order,
-                                                          int count, uint64_t* __restrict__ out,
-                                                          unsigned long long* __restrict__ stats) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    int nodeTests = 0, leafTests = 0, overflow = 0;
-    if (i < count) {
-        const int q = order ? static_cast<int>(order[i]) : i;
-        const Ray ray = rays[q];
-        float best = ray.tMax;
-        bool any = false;
-        int stack[kStack];
-        int sp = 0;
-        float tEnter;
-        if (rayBox(ray, nodeBox(nodes[0]), best, tEnter)) {
-            ++nodeTests;
-            stack[sp++] = 0;
-            while (sp > 0) {
-                const int id = stack[--sp];
-                const Node2 nd = nodes[id];
-                if (nd.left < 0) {
-                    ++leafTests;
-                    float t;
-                    if (rayTri(ray, tris[-nd.left - 1], t) && t < best) {
-                        best = t;
-                        any = true;
-                    }
-                    continue;
-                }
-                float tl, tr;
-                const bool hl = rayBox(ray, nodeBox(nodes[nd.left]), best, tl);
-                const bool hr = rayBox(ray, nodeBox(nodes[nd.right]), best, tr);
-                nodeTests += static_cast<int>(hl) + static_cast<int>(hr);
-                if (sp + 2 > kStack) {
-                    overflow = 1;
-                    break;
-                }
-                if (hl && hr) {
-                    const bool leftFirst = tl <= tr;
-                    stack[sp++] = leftFirst ? nd.right : nd.left;
-                    stack[sp++] = leftFirst ? nd.left : nd.right;
-                } else if (hl) {
-                    stack[sp++] = nd.left;
-                } else if (hr) {
-                    stack[sp++] = nd.right;
-                }
-            }
-        }
-        out[q] = any ? ((uint64_t{1} << 32) | floatBits(best)) : 0ull;
+struct LaneStack {
+    LdsInt* base;  // this lane's column of the workgroup's LDS stack
+    int sp;
+    __device__ __forceinline__ void init(int* lds) {
+        base = (LdsInt*)(lds + threadIdx.x);
+        sp = 0;
     }
-    flushStats<kStats>(stats, nodeTests, leafTests, overflow);
+    __device__ __forceinline__ bool empty() const { return sp == 0; }
+    __device__ __forceinline__ bool room(int n) const { return sp + n <= kLdsDepth + kSpill; }
+};
+struct SpillArea {
+    int v[kSpill];
+    __device__ __forceinline__ PrivateInt* at(int i) {
+        return (PrivateInt*)(&v[i]);
+    }
+};
+
+#define QR_PUSH(st, spill, val)                               \
+    do {                                                      \
+        const int qrv_ = (val);                               \
+        if ((st).sp < kLdsDepth) {                            \
+            (st).base[(st).sp * kBlock] = qrv_;               \
+        } else {                                              \
+            *(spill).at((st).sp - kLdsDepth) = qrv_;          \
+        }                                                     \
+        ++(st).sp;                                            \
+    } while (0)
+
+#define QR_POP(st, spill, out)                                \
+    do {                                                      \
+        --(st).sp;                                            \
+        if ((st).sp < kLdsDepth) {                            \
+            (out) = (st).base[(st).sp * kBlock];              \
+        } else {                                              \
+            (out) = *(spill).at((st).sp - kLdsDepth);         \
+        }                                                     \
+    } while (0)
+
+// Compare-exchange on (key, payload) pairs: ascending by key.
+__device__ __forceinline__ void cx(float& ka, int& va, float& kb, int& vb) {
+    const bool sw = kb < ka;
+    const float tk = ka;
+    const int tv = va;
+    ka = sw ? kb : ka;
+    va = sw ? vb : va;
+    kb = sw ? tk : kb;
+    vb = sw ? tv : vb;
 }
 
-template <bool kStats>
-__global__ __launch_bounds__(kBlock) void rayWideKernel(const Wide4* __restrict__ wide,
-                                                        const Tri* __restrict__ tris,
-                                                        const Ray* __restrict__ rays,
-                                                        const uint32_t* __restrict__ order,
-                                                        int count, uint64_t* __restrict__ out,
-                                                        unsigned long long* __restrict__ stats) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    int nodeTests = 0, leafTests = 0, overflow = 0;
-    if (i < count) {
-        const int q = order ? static_cast<int>(order[i]) : i;
-        const Ray ray = rays[q];
-        float best = ray.tMax;
-        bool any = false;
-        int stack[kStackWide];
-        int sp = 0;
-        stack[sp++] = 0;
-        while (sp > 0) {
-            const int id = stack[--sp];
-            if (id < 0) {
-                ++leafTests;
-                float t;
-                if (rayTri(ray, tris[-id - 1], t) && t < best) {
-                    best = t;
-                    any = true;
-                }
-                continue;
-            }
-            const Wide4& wn = wide[id];
-            int child[4];
-            float key[4];
-            int hits = 0;
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const WideEntry e = wn.c[s];
-                if (e.child == kWideEmpty) continue;
-                float t;
-                if (rayBox(ray, entryBox(e), best, t)) {
-                    ++nodeTests;
-                    int j = hits++;
-                    while (j > 0 && key[j - 1] > t) {
-                        key[j] = key[j - 1];
-                        child[j] = child[j - 1];
-                        --j;
-                    }
-                    key[j] = t;
-                    child[j] = e.child;
-                }
-            }
-            if (sp + hits > kStackWide) {
-                overflow = 1;
-                break;
-            }
-            for (int j = hits - 1; j >= 0; --j) stack[sp++] = child[j];
-        }
-        out[q] = any ? ((uint64_t{1} << 32) | floatBits(best)) : 0ull;
-    }
-    flushStats<kStats>(stats, nodeTests, leafTests, overflow);
+// Sorts four (key, child) pairs ascending (5 compare-exchanges).
+__device__ __forceinline__ void sort4(float (&k)[4], int (&c)[4]) {
+    cx(k[0], c[0], k[1], c[1]);
+    cx(k[2], c[2], k[3], c[3]);
+    cx(k[0], c[0], k[2], c[2]);
+    cx(k[1], c[1], k[3], c[3]);
+    cx(k[1], c[1], k[2], c[2]);
 }
-
-// ---- closest -------------------------------------------------------------------------
 
 __device__ __forceinline__ bool closer(float d, int id, float bd, int bid) {
     return d < bd || (d == bd && id < bid);
 }
 
-template <typename L, bool kStats>
-__global__ __launch_bounds__(kBlock) void closestBinaryKernel(
-    const Node2* __restrict__ nodes, const typename L::Item* __restrict__ prims,
-    const int* __restrict__ primIds, const V3* __restrict__ points,
-    const uint32_t* __restrict__ order, int count, ClosestResult* __restrict__ out,
-    unsigned long long* __restrict__ stats) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    int nodeTests = 0, leafTests = 0, overflow = 0;
-    if (i < count) {
-        const int q = order ? static_cast<int>(order[i]) : i;
-        const V3 p = points[q];
-        float best2 = kInf;
-        int bestPrim = -1;
-        int stack[kStack];
-        int sp = 0;
-        stack[sp++] = 0;
-        while (sp > 0) {
-            const int id = stack[--sp];
-            const Node2 nd = nodes[id];
-            ++nodeTests;
-            if (nd.left < 0) {
-                const int slot = -nd.left - 1;
-                ++leafTests;
-                const float d2 = L::dist2(p, prims[slot]);
-                const int prim = primIds[slot];
-                if (bestPrim < 0 || closer(d2, prim, best2, bestPrim)) {
-                    best2 = d2;
-                    bestPrim = prim;
-                }
-                continue;
-            }
-            const float dl = boxDist2(p, nodeBox(nodes[nd.left]));
-            const float dr = boxDist2(p, nodeBox(nodes[nd.right]));
-            const bool okL = bestPrim < 0 || dl <= best2;
-            const bool okR = bestPrim < 0 || dr <= best2;
-            if (sp + 2 > kStack) {
-                overflow = 1;
-                break;
-            }
-            if (okL && okR) {
-                const bool leftFirst = dl <= dr;
-                stack[sp++] = leftFirst ? nd.right : nd.left;
-                stack[sp++] = leftFirst ? nd.left : nd.right;
-            } else if (okL) {
-                stack[sp++] = nd.left;
-            } else if (okR) {
-                stack[sp++] = nd.right;
-            }
-        }
-        ClosestResult r{0u, -1};
-        if (bestPrim >= 0) r = {floatBits(best2), bestPrim};
-        out[q] = r;
-    }
-    flushStats<kStats>(stats, nodeTests, leafTests, overflow);
-}
-
-template <typename L, bool kStats>
-__global__ __launch_bounds__(kBlock) void closestWideKernel(
-    const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
-    const int* __restrict__ primIds, const V3* __restrict__ points,
-    const uint32_t* __restrict__ order, int count, ClosestResult* __restrict__ out,
-    unsigned long long* __restrict__ stats) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    int nodeTests = 0, leafTests = 0, overflow = 0;
-    if (i < count) {
-        const int q = order ? static_cast<int>(order[i]) : i;
-        const V3 p = points[q];
-        float best2 = kInf;
-        int bestPrim = -1;
-        int stack[kStackWide];
-        int sp = 0;
-        stack[sp++] = 0;
-        while (sp > 0) {
-            const int id = stack[--sp];
-            if (id < 0) {
-                const int slot = -id - 1;
-                ++leafTests;
-                const float d2 = L::dist2(p, prims[slot]);
-                const int prim = primIds[slot];
-                if (bestPrim < 0 || closer(d2, prim, best2, bestPrim)) {
-                    best2 = d2;
-                    bestPrim = prim;
-                }
-                continue;
-            }
-            const Wide4& wn = wide[id];
-            int child[4];
-            float key[4];
-            int hits = 0;
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const WideEntry e = wn.c[s];
-                if (e.child == kWideEmpty) continue;
-                const float d = boxDist2(p, entryBox(e));
-                if (bestPrim >= 0 && d > best2) continue;
-                ++nodeTests;
-                int j = hits++;
-                while (j > 0 && key[j - 1] > d) {
-                    key[j] = key[j - 1];
-                    child[j] = child[j - 1];
-                    --j;
-                }
-                key[j] = d;
-                child[j] = e.child;
-            }
-            if (sp + hits > kStackWide) {
-                overflow = 1;
-                break;
-            }
-            for (int j = hits - 1; j >= 0; --j) stack[sp++] = child[j];
-        }
-        ClosestResult r{0u, -1};
-        if (bestPrim >= 0) r = {floatBits(best2), bestPrim};
-        out[q] = r;
-    }
-    flushStats<kStats>(stats, nodeTests, leafTests, overflow);
-}
-
-// ---- kNN --------------------------------------------------------------------------
-// The k best so far live in registers, sorted by (dist2, index).  A candidate is
-// inserted by one ascending compare-exchange sweep with static indices (no
-// dynamically indexed private array); the displaced worst drops off the end.
-// Semantics equal the reference's sorted table (src/gpu/query.cu:150-158).
+// ---- k best so far ------------------------------------------------------------------
+// Capacity-K list sorted by (dist2, index) in registers, for any k <= K.  It keeps
+// the K best seen so far and prunes with the K-th: the first k entries are then
+// the exact k best (a subtree beyond the K-th best cannot hold one of the k best),
+// and every index into the list is static, so it never leaves registers.  A
+// candidate is inserted by one ascending compare-exchange sweep; the displaced
+// worst drops off the end.  Semantics equal the reference's sorted table
+// (src/gpu/query.cu:150-158) and, for k = 1, its closest-point rule
+// (src/gpu/query.cu:83-90).
 template <int K>
 struct KnnList {
     float d[K];
@@ -306,47 +149,232 @@ struct KnnList {
             id[s] = 0x7fffffff;
         }
     }
-    // Worst kept entry when full (the pruning bound).
-    __device__ __forceinline__ bool full(int k) const { return n >= k; }
-    __device__ __forceinline__ float worstD(int k) const {
-        float w = d[0];
-#pragma unroll
-        for (int s = 1; s < K; ++s)
-            if (s == k - 1) w = d[s];
-        return w;
-    }
-    __device__ __forceinline__ int worstId(int k) const {
-        int w = id[0];
-#pragma unroll
-        for (int s = 1; s < K; ++s)
-            if (s == k - 1) w = id[s];
-        return w;
-    }
-    __device__ __forceinline__ void insert(float nd, int nid, int k) {
+    __device__ __forceinline__ bool full() const { return n >= K; }
+    // Pruning test for a subtree or primitive whose lower bound is lb.
+    __device__ __forceinline__ bool admits(float lb) const { return !full() || lb <= d[K - 1]; }
+    __device__ __forceinline__ void offer(float nd, int nid) {
+        if (full() && !closer(nd, nid, d[K - 1], id[K - 1])) return;
         float cd = nd;
         int cid = nid;
 #pragma unroll
         for (int s = 0; s < K; ++s) {
-            if (s < k) {
-                const bool sw = closer(cd, cid, d[s], id[s]);
-                const float td = d[s];
-                const int tid = id[s];
-                d[s] = sw ? cd : td;
-                id[s] = sw ? cid : tid;
-                cd = sw ? td : cd;
-                cid = sw ? tid : cid;
-            }
+            const bool sw = closer(cd, cid, d[s], id[s]);
+            const float td = d[s];
+            const int tid = id[s];
+            d[s] = sw ? cd : td;
+            id[s] = sw ? cid : tid;
+            cd = sw ? td : cd;
+            cid = sw ? tid : cid;
         }
-        n = n < k ? n + 1 : k;
+        n = n < K ? n + 1 : K;
     }
 };
 
+// ---- rays --------------------------------------------------------------------------
+// Each iteration opens one interior node: its children are tested, leaves are
+// intersected nearest first, and the nearest surviving interior child becomes the
+// next node without a stack round trip; the other survivors are pushed far first.
+
+template <bool kStats>
+__global__ __launch_bounds__(kBlock) void rayBinaryKernel(const Node2* __restrict__ nodes,
+                                                          const Tri* __restrict__ tris,
+                                                          const Ray* __restrict__ rays,
+                                                          const uint32_t* __restrict__ order,
+                                                          int count, uint64_t* __restrict__ out,
+                                                          unsigned long long* __restrict__ stats) {
+    __shared__ int lds[kLdsDepth * kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    int nodeTests = 0, leafTests = 0, overflow = 0;
+    if (i < count) {
+        const int q = order ? static_cast<int>(order[i]) : i;
+        const Ray ray = rays[q];
+        float best = ray.tMax;
+        bool any = false;
+        LaneStack st;
+        SpillArea spill;
+        st.init(lds);
+        float tEnter;
+        const Node2 root = nodes[0];
+        int id = -1;
+        if (rayBox(ray, nodeBox(root), best, tEnter)) {
+            ++nodeTests;
+            id = 0;
+        }
+        while (id >= 0) {
+            const Node2 nd = nodes[id];
+            const int kid[2] = {nd.left, nd.right};
+            float t[2];
+            int c[2];
+            bool leaf[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const Node2 cn = nodes[kid[s]];
+                leaf[s] = cn.left < 0;
+                c[s] = leaf[s] ? -cn.left - 1 : kid[s];
+                t[s] = kInf;
+                if (rayBox(ray, nodeBox(cn), best, t[s])) {
+                    ++nodeTests;
+                } else {
+                    c[s] = kWideEmpty;
+                    t[s] = kInf;
+                }
+            }
+            const bool swap = t[1] < t[0];
+            const int c0 = swap ? c[1] : c[0], c1 = swap ? c[0] : c[1];
+            const bool l0 = swap ? leaf[1] : leaf[0], l1 = swap ? leaf[0] : leaf[1];
+            float tt;
+            if (c0 != kWideEmpty && l0) {
+                ++leafTests;
+                if (rayTri(ray, tris[c0], tt) && tt < best) {
+                    best = tt;
+                    any = true;
+                }
+            }
+            if (c1 != kWideEmpty && l1) {
+                ++leafTests;
+                if (rayTri(ray, tris[c1], tt) && tt < best) {
+                    best = tt;
+                    any = true;
+                }
+            }
+            const bool in0 = c0 != kWideEmpty && !l0;
+            const bool in1 = c1 != kWideEmpty && !l1;
+            if (in0) {
+                if (in1) {
+                    if (!st.room(1)) {
+                        overflow = 1;
+                        break;
+                    }
+                    QR_PUSH(st, spill, c1);
+                }
+                id = c0;
+            } else if (in1) {
+                id = c1;
+            } else if (!st.empty()) {
+                QR_POP(st, spill, id);
+            } else {
+                id = -1;
+            }
+        }
+        out[q] = any ? ((uint64_t{1} << 32) | floatBits(best)) : 0ull;
+    }
+    flushStats<kStats>(stats, nodeTests, leafTests, overflow);
+}
+
+__device__ __forceinline__ void loadWide(const Wide4* __restrict__ wide, int id, WideEntry (&e)[4]) {
+    const Wide4& wn = wide[id];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) e[s] = wn.c[s];
+}
+
+// After the four children are sorted: the first interior survivor continues the
+// descent, the rest are pushed far first.  Returns the next node, or pops one
+// (-1 when the traversal is complete).  ok[s] says whether slot s survives.
+__device__ __forceinline__ int descend(const int (&c)[4], const bool (&ok)[4], LaneStack& st,
+                                       SpillArea& spill, int& overflow) {
+    int next = -1;
+    int first = -1;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (ok[s] && c[s] >= 0 && first < 0) first = s;
+    }
+    if (!st.room(3)) {
+        overflow = 1;
+        return -1;
+    }
+#pragma unroll
+    for (int s = 3; s >= 0; --s) {
+        if (ok[s] && c[s] >= 0 && s != first) QR_PUSH(st, spill, c[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        if (s == first) next = c[s];
+    if (first < 0) {
+        if (!st.empty()) {
+            QR_POP(st, spill, next);
+        }
+    }
+    return next;
+}
+
+template <bool kStats>
+__global__ __launch_bounds__(kBlock) void rayWideKernel(const Wide4* __restrict__ wide,
+                                                        const Tri* __restrict__ tris,
+                                                        const Ray* __restrict__ rays,
+                                                        const uint32_t* __restrict__ order,
+                                                        int count, uint64_t* __restrict__ out,
+                                                        unsigned long long* __restrict__ stats) {
+    __shared__ int lds[kLdsDepth * kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    int nodeTests = 0, leafTests = 0, overflow = 0;
+    if (i < count) {
+        const int q = order ? static_cast<int>(order[i]) : i;
+        const Ray ray = rays[q];
+        float best = ray.tMax;
+        bool any = false;
+        LaneStack st;
+        SpillArea spill;
+        st.init(lds);
+        int id = 0;
+        while (id >= 0) {
+            WideEntry e[4];
+            loadWide(wide, id, e);
+            float key[4];
+            int c[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                c[s] = e[s].child;
+                key[s] = kInf;
+                float t;
+                if (c[s] != kWideEmpty && rayBox(ray, entryBox(e[s]), best, t)) {
+                    key[s] = t;
+                    ++nodeTests;
+                } else {
+                    c[s] = kWideEmpty;
+                }
+            }
+            sort4(key, c);
+            bool ok[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                ok[s] = c[s] != kWideEmpty;
+                if (ok[s] && c[s] < 0) {
+                    ++leafTests;
+                    float t;
+                    if (rayTri(ray, tris[-c[s] - 1], t) && t < best) {
+                        best = t;
+                        any = true;
+                    }
+                }
+            }
+            id = descend(c, ok, st, spill, overflow);
+        }
+        out[q] = any ? ((uint64_t{1} << 32) | floatBits(best)) : 0ull;
+    }
+    flushStats<kStats>(stats, nodeTests, leafTests, overflow);
+}
+
+// ---- closest / kNN ------------------------------------------------------------------
+
+template <typename L, int K>
+__device__ __forceinline__ void writeList(const KnnList<K>& list, int k, ClosestResult* dst) {
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        if (s < k) {
+            ClosestResult r{0u, -1};
+            if (s < list.n) r = {floatBits(list.d[s]), list.id[s]};
+            dst[s] = r;
+        }
+    }
+}
+
 template <typename L, int K, bool kStats>
-__global__ __launch_bounds__(kBlock) void knnBinaryKernel(
+__global__ __launch_bounds__(kBlock) void nearestBinaryKernel(
     const Node2* __restrict__ nodes, const typename L::Item* __restrict__ prims,
     const int* __restrict__ primIds, const V3* __restrict__ points,
     const uint32_t* __restrict__ order, int count, int k, int outStride,
     ClosestResult* __restrict__ out, unsigned long long* __restrict__ stats) {
+    __shared__ int lds[kLdsDepth * kBlock];
     const int i = blockIdx.x * kBlock + threadIdx.x;
     int nodeTests = 0, leafTests = 0, overflow = 0;
     if (i < count) {
@@ -354,61 +382,67 @@ __global__ __launch_bounds__(kBlock) void knnBinaryKernel(
         const V3 p = points[q];
         KnnList<K> list;
         list.init();
-        int stack[kStack];
-        int sp = 0;
-        stack[sp++] = 0;
-        while (sp > 0) {
-            const int id = stack[--sp];
+        LaneStack st;
+        SpillArea spill;
+        st.init(lds);
+        int id = 0;
+        while (id >= 0) {
             const Node2 nd = nodes[id];
             ++nodeTests;
-            if (nd.left < 0) {
-                const int slot = -nd.left - 1;
-                ++leafTests;
-                const float d2 = L::dist2(p, prims[slot]);
-                const int prim = primIds[slot];
-                if (list.full(k) && !closer(d2, prim, list.worstD(k), list.worstId(k))) continue;
-                list.insert(d2, prim, k);
-                continue;
-            }
-            const float dl = boxDist2(p, nodeBox(nodes[nd.left]));
-            const float dr = boxDist2(p, nodeBox(nodes[nd.right]));
-            const bool isFull = list.full(k);
-            const float bound = list.worstD(k);
-            const bool okL = !isFull || dl <= bound;
-            const bool okR = !isFull || dr <= bound;
-            if (sp + 2 > kStack) {
-                overflow = 1;
-                break;
-            }
-            if (okL && okR) {
-                const bool leftFirst = dl <= dr;
-                stack[sp++] = leftFirst ? nd.right : nd.left;
-                stack[sp++] = leftFirst ? nd.left : nd.right;
-            } else if (okL) {
-                stack[sp++] = nd.left;
-            } else if (okR) {
-                stack[sp++] = nd.right;
-            }
-        }
-        ClosestResult* dst = out + static_cast<size_t>(q) * outStride;
+            const int kid[2] = {nd.left, nd.right};
+            float key[2];
+            int c[2];
+            bool leaf[2];
 #pragma unroll
-        for (int s = 0; s < K; ++s) {
-            if (s < k) {
-                ClosestResult r{0u, -1};
-                if (s < list.n) r = {floatBits(list.d[s]), list.id[s]};
-                dst[s] = r;
+            for (int s = 0; s < 2; ++s) {
+                const Node2 cn = nodes[kid[s]];
+                leaf[s] = cn.left < 0;
+                c[s] = leaf[s] ? -cn.left - 1 : kid[s];
+                key[s] = boxDist2(p, nodeBox(cn));
+            }
+            const bool swap = key[1] < key[0];
+            const int c0 = swap ? c[1] : c[0], c1 = swap ? c[0] : c[1];
+            const bool l0 = swap ? leaf[1] : leaf[0], l1 = swap ? leaf[0] : leaf[1];
+            const float k0 = swap ? key[1] : key[0], k1 = swap ? key[0] : key[1];
+            if (l0 && list.admits(k0)) {
+                ++leafTests;
+                list.offer(L::dist2(p, prims[c0]), primIds[c0]);
+            }
+            if (l1 && list.admits(k1)) {
+                ++leafTests;
+                list.offer(L::dist2(p, prims[c1]), primIds[c1]);
+            }
+            const bool in0 = !l0 && list.admits(k0);
+            const bool in1 = !l1 && list.admits(k1);
+            if (in0) {
+                if (in1) {
+                    if (!st.room(1)) {
+                        overflow = 1;
+                        break;
+                    }
+                    QR_PUSH(st, spill, c1);
+                }
+                id = c0;
+            } else if (in1) {
+                id = c1;
+            } else if (!st.empty()) {
+                QR_POP(st, spill, id);
+            } else {
+                id = -1;
             }
         }
+        writeList<L, K>(list, k, out + static_cast<size_t>(q) * outStride);
     }
     flushStats<kStats>(stats, nodeTests, leafTests, overflow);
 }
 
 template <typename L, int K, bool kStats>
-__global__ __launch_bounds__(kBlock) void knnWideKernel(
+__global__ __launch_bounds__(kBlock) void nearestWideKernel(
     const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
     const int* __restrict__ primIds, const V3* __restrict__ points,
     const uint32_t* __restrict__ order, int count, int k, int outStride,
     ClosestResult* __restrict__ out, unsigned long long* __restrict__ stats) {
+    __shared__ int lds[kLdsDepth * kBlock];
     const int i = blockIdx.x * kBlock + threadIdx.x;
     int nodeTests = 0, leafTests = 0, overflow = 0;
     if (i < count) {
@@ -416,65 +450,53 @@ __global__ __launch_bounds__(kBlock) void knnWideKernel(
         const V3 p = points[q];
         KnnList<K> list;
         list.init();
-        int stack[kStackWide];
-        int sp = 0;
-        stack[sp++] = 0;
-        while (sp > 0) {
-            const int id = stack[--sp];
-            if (id < 0) {
-                const int slot = -id - 1;
-                ++leafTests;
-                const float d2 = L::dist2(p, prims[slot]);
-                const int prim = primIds[slot];
-                if (list.full(k) && !closer(d2, prim, list.worstD(k), list.worstId(k))) continue;
-                list.insert(d2, prim, k);
-                continue;
-            }
-            const Wide4& wn = wide[id];
-            int child[4];
+        LaneStack st;
+        SpillArea spill;
+        st.init(lds);
+        int id = 0;
+        while (id >= 0) {
+            WideEntry e[4];
+            loadWide(wide, id, e);
             float key[4];
-            int hits = 0;
-            const bool isFull = list.full(k);
-            const float bound = list.worstD(k);
+            int c[4];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const WideEntry e = wn.c[s];
-                if (e.child == kWideEmpty) continue;
-                const float d = boxDist2(p, entryBox(e));
-                if (isFull && d > bound) continue;
-                ++nodeTests;
-                int j = hits++;
-                while (j > 0 && key[j - 1] > d) {
-                    key[j] = key[j - 1];
-                    child[j] = child[j - 1];
-                    --j;
+                c[s] = e[s].child;
+                key[s] = kInf;
+                if (c[s] != kWideEmpty) {
+                    const float d = boxDist2(p, entryBox(e[s]));
+                    if (list.admits(d)) {
+                        key[s] = d;
+                        ++nodeTests;
+                    } else {
+                        c[s] = kWideEmpty;
+                    }
                 }
-                key[j] = d;
-                child[j] = e.child;
             }
-            if (sp + hits > kStackWide) {
-                overflow = 1;
-                break;
-            }
-            for (int j = hits - 1; j >= 0; --j) stack[sp++] = child[j];
-        }
-        ClosestResult* dst = out + static_cast<size_t>(q) * outStride;
+            sort4(key, c);
+            // leaves now, nearest first, against the live bound
 #pragma unroll
-        for (int s = 0; s < K; ++s) {
-            if (s < k) {
-                ClosestResult r{0u, -1};
-                if (s < list.n) r = {floatBits(list.d[s]), list.id[s]};
-                dst[s] = r;
+            for (int s = 0; s < 4; ++s) {
+                if (c[s] != kWideEmpty && c[s] < 0 && list.admits(key[s])) {
+                    const int slot = -c[s] - 1;
+                    ++leafTests;
+                    list.offer(L::dist2(p, prims[slot]), primIds[slot]);
+                }
             }
+            bool ok[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) ok[s] = c[s] != kWideEmpty && list.admits(key[s]);
+            id = descend(c, ok, st, spill, overflow);
         }
+        writeList<L, K>(list, k, out + static_cast<size_t>(q) * outStride);
     }
     flushStats<kStats>(stats, nodeTests, leafTests, overflow);
 }
 
 // ---- radius: count and fill passes ---------------------------------------------------
 // kFill = false counts matches per query; kFill = true writes the 64-bit keys
-// (dist2 bits << 32 | index) at the query's CSR offset.  Both passes visit the same
-// nodes in the same order, so the fill never writes past its segment.
+// (dist2 bits << 32 | index) into the query's CSR segment (bounded by the next
+// offset, so a fill can never write past its segment).
 template <typename L, bool kWide, bool kFill, bool kStats>
 __global__ __launch_bounds__(kBlock) void radiusKernel(
     const Node2* __restrict__ nodes, const Wide4* __restrict__ wide,
@@ -482,6 +504,7 @@ __global__ __launch_bounds__(kBlock) void radiusKernel(
     const V3* __restrict__ points, const uint32_t* __restrict__ order, int count, float r2,
     int* __restrict__ counts, const int* __restrict__ offsets, uint64_t* __restrict__ keys,
     unsigned long long* __restrict__ stats) {
+    __shared__ int lds[kLdsDepth * kBlock];
     const int i = blockIdx.x * kBlock + threadIdx.x;
     int nodeTests = 0, leafTests = 0, overflow = 0;
     if (i < count) {
@@ -490,72 +513,69 @@ __global__ __launch_bounds__(kBlock) void radiusKernel(
         int found = 0;
         int write = kFill ? offsets[q] : 0;
         const int end = kFill ? offsets[q + 1] : 0;
-        int stack[kWide ? kStackWide : kStack];
-        constexpr int cap = kWide ? kStackWide : kStack;
-        int sp = 0;
-        stack[sp++] = 0;
-        while (sp > 0) {
-            const int id = stack[--sp];
+        auto leafTest = [&](int slot) {
+            ++leafTests;
+            const float d2 = L::dist2(p, prims[slot]);
+            if (d2 <= r2) {
+                if (kFill) {
+                    if (write < end)
+                        keys[write++] = (static_cast<uint64_t>(floatBits(d2)) << 32) |
+                                        static_cast<uint32_t>(primIds[slot]);
+                } else {
+                    ++found;
+                }
+            }
+        };
+        LaneStack st;
+        SpillArea spill;
+        st.init(lds);
+        int id = -1;
+        if (!kWide) {
+            const Node2 root = nodes[0];
+            if (boxDist2(p, nodeBox(root)) <= r2) {
+                ++nodeTests;
+                id = 0;
+            }
+        } else {
+            id = 0;
+        }
+        while (id >= 0) {
+            int c[4] = {kWideEmpty, kWideEmpty, kWideEmpty, kWideEmpty};
+            bool ok[4] = {false, false, false, false};
             if (!kWide) {
                 const Node2 nd = nodes[id];
-                if (boxDist2(p, nodeBox(nd)) > r2) continue;
-                ++nodeTests;
-                if (nd.left < 0) {
-                    const int slot = -nd.left - 1;
-                    ++leafTests;
-                    const float d2 = L::dist2(p, prims[slot]);
-                    if (d2 <= r2) {
-                        if (kFill) {
-                            if (write < end)
-                                keys[write++] = (static_cast<uint64_t>(floatBits(d2)) << 32) |
-                                                static_cast<uint32_t>(primIds[slot]);
-                        } else {
-                            ++found;
-                        }
+                const int kid[2] = {nd.left, nd.right};
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const Node2 cn = nodes[kid[s]];
+                    if (boxDist2(p, nodeBox(cn)) > r2) continue;
+                    ++nodeTests;
+                    if (cn.left < 0) {
+                        leafTest(-cn.left - 1);
+                    } else {
+                        c[s] = kid[s];
+                        ok[s] = true;
                     }
-                    continue;
                 }
-                if (sp + 2 > cap) {
-                    overflow = 1;
-                    break;
-                }
-                stack[sp++] = nd.right;
-                stack[sp++] = nd.left;
             } else {
-                if (id < 0) {
-                    const int slot = -id - 1;
-                    ++leafTests;
-                    const float d2 = L::dist2(p, prims[slot]);
-                    if (d2 <= r2) {
-                        if (kFill) {
-                            if (write < end)
-                                keys[write++] = (static_cast<uint64_t>(floatBits(d2)) << 32) |
-                                                static_cast<uint32_t>(primIds[slot]);
-                        } else {
-                            ++found;
-                        }
-                    }
-                    continue;
-                }
-                const Wide4& wn = wide[id];
-                int push[4];
-                int hits = 0;
+                WideEntry e[4];
+                loadWide(wide, id, e);
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
-                    const WideEntry e = wn.c[s];
-                    if (e.child == kWideEmpty) continue;
-                    if (boxDist2(p, entryBox(e)) > r2) continue;
+                    const int ch = e[s].child;
+                    if (ch == kWideEmpty || boxDist2(p, entryBox(e[s])) > r2) continue;
                     ++nodeTests;
-                    push[hits++] = e.child;
+                    if (ch < 0) {
+                        leafTest(-ch - 1);
+                    } else {
+                        c[s] = ch;
+                        ok[s] = true;
+                    }
                 }
-                if (sp + hits > cap) {
-                    overflow = 1;
-                    break;
-                }
-                for (int j = hits - 1; j >= 0; --j) stack[sp++] = push[j];
             }
+            id = descend(c, ok, st, spill, overflow);
         }
-        if (!kFill) counts[q] = overflow ? 0 : found;
+        if (!kFill) counts[q] = found;
     }
     flushStats<kStats>(stats, nodeTests, leafTests, overflow);
 }
@@ -738,7 +758,7 @@ void zeroStats(unsigned long long* stats, hipStream_t s) {
     zeroU64Kernel<<<1, kBlock, 0, s>>>(stats, 4);
 }
 
-void mortonOrder(const float* base, int stride, int count, int fixedBox, SortWorkspace& ws,
+void mortonOrder(const float* base, int stride, int count, int boxMode, SortWorkspace& ws,
                  uint32_t* keys, uint32_t* keysSorted, uint32_t* orderOut, uint32_t* bb,
                  hipStream_t s) {
     ws.reserve(static_cast<size_t>(count));
@@ -746,11 +766,12 @@ void mortonOrder(const float* base, int stride, int count, int fixedBox, SortWor
     int grid = blocksFor(count, kBlock * 4);
     if (grid < 1) grid = 1;
     if (grid > 1024) grid = 1024;
-    if (!fixedBox) {
+    if (boxMode == kBoxMeasure) {
         pointBoundsInitKernel<<<1, 64, 0, s>>>(bb);
         pointBoundsKernel<<<grid, kBlock, 0, s>>>(base, stride, count, bb);
     }
-    mortonKeysKernel<<<grid, kBlock, 0, s>>>(base, stride, count, bb, fixedBox, keys, ws.hist.p);
+    mortonKeysKernel<<<grid, kBlock, 0, s>>>(base, stride, count, bb, boxMode == kBoxUnit ? 1 : 0,
+                                             keys, ws.hist.p);
     hipCheck(hipGetLastError(), "morton keys");
     sortPairs(ws, keys, nullptr, keysSorted, orderOut, static_cast<size_t>(count), 4, s);
 }
@@ -760,91 +781,72 @@ void launchRays(const TraverseArgs& a, const Ray* rays, int count, uint64_t* out
     if (grid == 0) return;
     const Tri* tris = static_cast<const Tri*>(a.items);
     if (a.wide) {
-        if (a.stats) {
+        if (a.stats)
             rayWideKernel<true><<<grid, kBlock, 0, s>>>(a.wideNodes, tris, rays, a.order, count, out, a.stats);
-        } else {
+        else
             rayWideKernel<false><<<grid, kBlock, 0, s>>>(a.wideNodes, tris, rays, a.order, count, out, nullptr);
-        }
     } else {
-        if (a.stats) {
+        if (a.stats)
             rayBinaryKernel<true><<<grid, kBlock, 0, s>>>(a.nodes, tris, rays, a.order, count, out, a.stats);
-        } else {
+        else
             rayBinaryKernel<false><<<grid, kBlock, 0, s>>>(a.nodes, tris, rays, a.order, count, out, nullptr);
-        }
     }
     hipCheck(hipGetLastError(), "raycast");
 }
 
-template <typename L>
-static void launchClosestT(const TraverseArgs& a, const V3* pts, int count, ClosestResult* out,
-                           hipStream_t s) {
+template <typename L, int K>
+static void launchNearestTK(const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
+                            ClosestResult* out, hipStream_t s) {
     const int grid = blocksFor(count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
     if (a.wide) {
         if (a.stats)
-            closestWideKernel<L, true><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, out, a.stats);
+            nearestWideKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats);
         else
-            closestWideKernel<L, false><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, out, nullptr);
+            nearestWideKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr);
     } else {
         if (a.stats)
-            closestBinaryKernel<L, true><<<grid, kBlock, 0, s>>>(a.nodes, prims, a.primIds, pts, a.order, count, out, a.stats);
+            nearestBinaryKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.nodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats);
         else
-            closestBinaryKernel<L, false><<<grid, kBlock, 0, s>>>(a.nodes, prims, a.primIds, pts, a.order, count, out, nullptr);
+            nearestBinaryKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.nodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr);
     }
+}
+
+template <typename L>
+static void launchNearestT(const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
+                           ClosestResult* out, hipStream_t s) {
+    if (k == 1) {
+        launchNearestTK<L, 1>(a, pts, count, k, outStride, out, s);
+    } else if (k <= 4) {
+        launchNearestTK<L, 4>(a, pts, count, k, outStride, out, s);
+    } else if (k <= 8) {
+        launchNearestTK<L, 8>(a, pts, count, k, outStride, out, s);
+    } else if (k <= 16) {
+        launchNearestTK<L, 16>(a, pts, count, k, outStride, out, s);
+    } else {
+        launchNearestTK<L, 32>(a, pts, count, k, outStride, out, s);
+    }
+}
+
+static void launchNearest(const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
+                          ClosestResult* out, hipStream_t s) {
+    if (count <= 0) return;
+    switch (a.kind) {
+        case 0: launchNearestT<TriLeaf>(a, pts, count, k, outStride, out, s); break;
+        case 1: launchNearestT<PointLeaf>(a, pts, count, k, outStride, out, s); break;
+        default: launchNearestT<GaussLeaf>(a, pts, count, k, outStride, out, s); break;
+    }
+    hipCheck(hipGetLastError(), k == 1 ? "closest" : "knn");
 }
 
 void launchClosest(const TraverseArgs& a, const V3* pts, int count, ClosestResult* out,
                    hipStream_t s) {
-    if (count <= 0) return;
-    switch (a.kind) {
-        case 0: launchClosestT<TriLeaf>(a, pts, count, out, s); break;
-        case 1: launchClosestT<PointLeaf>(a, pts, count, out, s); break;
-        default: launchClosestT<GaussLeaf>(a, pts, count, out, s); break;
-    }
-    hipCheck(hipGetLastError(), "closest");
-}
-
-template <typename L, int K>
-static void launchKnnTK(const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
-                        ClosestResult* out, hipStream_t s) {
-    const int grid = blocksFor(count, kBlock);
-    const auto* prims = static_cast<const typename L::Item*>(a.items);
-    if (a.wide) {
-        if (a.stats)
-            knnWideKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats);
-        else
-            knnWideKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr);
-    } else {
-        if (a.stats)
-            knnBinaryKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.nodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats);
-        else
-            knnBinaryKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.nodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr);
-    }
-}
-
-template <typename L>
-static void launchKnnT(const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
-                       ClosestResult* out, hipStream_t s) {
-    if (k <= 4) {
-        launchKnnTK<L, 4>(a, pts, count, k, outStride, out, s);
-    } else if (k <= 8) {
-        launchKnnTK<L, 8>(a, pts, count, k, outStride, out, s);
-    } else if (k <= 16) {
-        launchKnnTK<L, 16>(a, pts, count, k, outStride, out, s);
-    } else {
-        launchKnnTK<L, 32>(a, pts, count, k, outStride, out, s);
-    }
+    launchNearest(a, pts, count, 1, 1, out, s);
 }
 
 void launchKnn(const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
                ClosestResult* out, hipStream_t s) {
-    if (count <= 0) return;
-    switch (a.kind) {
-        case 0: launchKnnT<TriLeaf>(a, pts, count, k, outStride, out, s); break;
-        case 1: launchKnnT<PointLeaf>(a, pts, count, k, outStride, out, s); break;
-        default: launchKnnT<GaussLeaf>(a, pts, count, k, outStride, out, s); break;
-    }
-    hipCheck(hipGetLastError(), "knn");
+    launchNearest(a, pts, count, k, outStride, out, s);
 }
 
 template <typename L, bool kFill>
