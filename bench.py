@@ -75,6 +75,8 @@ def main() -> int:
         return 1
     log("device:", dev)
     A.check(A.lib().qr_warmup())
+    if args.profile:
+        calibration_copy(A)
 
     # ---- workload ---------------------------------------------------------------
     t0 = time.time()
@@ -133,36 +135,50 @@ def main() -> int:
 
     # traversal work per query (one extra counted pass, outside the timed region)
     work = {}
-    pb1.closest(bvh, stats=True)
-    st = A.last_query_stats()
-    work["closest"] = (st.node_tests / queries.shape[0], st.leaf_tests / queries.shape[0])
-    pbk.knn(bvh, args.k, stats=True)
-    st = A.last_query_stats()
-    work["knn"] = (st.node_tests / queries.shape[0], st.leaf_tests / queries.shape[0])
-    pb1.radius(bvh, args.radius, stats=True)
-    st = A.last_query_stats()
-    work["radius"] = (st.node_tests / queries.shape[0], st.leaf_tests / queries.shape[0])
-    log("node/leaf tests per query: " + ", ".join(f"{k} {v[0]:.1f}/{v[1]:.1f}" for k, v in work.items()))
+    nq = queries.shape[0]
+    for name, fn in (("closest", lambda: pb1.closest(bvh, stats=True)),
+                     ("knn", lambda: pbk.knn(bvh, args.k, stats=True)),
+                     ("radius", lambda: pb1.radius(bvh, args.radius, stats=True))):
+        fn()
+        st = A.last_query_stats()
+        work[name] = (st.node_tests / nq, st.leaf_tests / nq, st.nodes_opened / nq)
+    log("node/leaf tests, nodes opened per query: " +
+        ", ".join(f"{k} {v[0]:.1f}/{v[1]:.1f}/{v[2]:.1f}" for k, v in work.items()))
+
+    # the same three queries on binary nodes (reported beside the wide pass)
+    binary = {}
+    for name, fn in (("closest", lambda: pb1.closest(bvh, wide=False)),
+                     ("knn", lambda: pbk.knn(bvh, args.k, wide=False)),
+                     ("radius", lambda: pb1.radius(bvh, args.radius, wide=False))):
+        fn()
+        ks = []
+        for _ in range(5):
+            fn()
+            ks.append(A.last_query_stats().kernel_ms)
+        binary[name] = round(statistics.median(ks), 4)
+    log(f"binary-node traversal kernels (ms): {binary}")
 
     # ---- roofline of the dominant kernel ------------------------------------------
     dom = max(per_kernel, key=per_kernel.get)
-    nq = queries.shape[0]
-    radius_total = None
     _, offs, keys = pb1.radius(bvh, args.radius, keep=True)
     radius_total = int(offs[-1])
-    # Algorithmic bytes per launch: every query point read once, every result
-    # written once (traversal of the resident index is cache/latency-bound work,
-    # not streaming, so these are the bytes that must cross HBM).
+    # Algorithmic bytes per launch: the bytes the traversal has to read -- each
+    # opened 4-wide node (128 B), each tested leaf (36 B triangle + 4 B prim id),
+    # the query (12 B) and its order slot (4 B) -- plus the results it writes.
+    # Node and leaf counts come from the kernels' own counters above.
+    opened, leaves = work[dom][2] * nq, work[dom][1] * nq
+    touched = opened * 128 + leaves * (TRI_BYTES + 4) + nq * (12 + 4)
     alg = {
-        "closest": nq * (12 + 8),
-        "knn": nq * (12 + 8 * args.k),
-        # count + fill passes: points read twice, counts and offsets, keys written
-        "radius": nq * (12 + 4) + nq * (12 + 4) + radius_total * 8,
+        "closest": touched + nq * 8,
+        "knn": touched + nq * 8 * args.k,
+        # count + fill passes walk the same nodes; counts, offsets and keys written
+        "radius": 2 * touched + nq * 8 + radius_total * 8,
     }[dom]
     achieved = alg / (per_kernel[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                "avg_kernel_ms": round(per_kernel[dom], 4)}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": traffic_for(dom, args),
+                "alg_bytes_per_launch": int(alg), "avg_kernel_ms": round(per_kernel[dom], 4)}
 
     # Build roofline: 5S + 232 bytes per primitive (S = 36 for triangles).
     build_alg = n * (5 * TRI_BYTES + 232)
@@ -265,7 +281,9 @@ def main() -> int:
         "per_op_ms": {k: round(v, 4) for k, v in per_op.items()},
         "per_op_mqps": {k: round(nq / (v * 1e-3) / 1e6, 2) for k, v in per_op.items()},
         "per_op_kernel_ms": {k: round(v, 4) for k, v in per_kernel.items()},
-        "tests_per_query": {k: {"node": round(v[0], 2), "leaf": round(v[1], 2)} for k, v in work.items()},
+        "tests_per_query": {k: {"node": round(v[0], 2), "leaf": round(v[1], 2),
+                                "opened": round(v[2], 2)} for k, v in work.items()},
+        "binary_kernel_ms": binary,
         "roofline": roofline,
         "build": build,
         "parity": parity,
@@ -275,6 +293,41 @@ def main() -> int:
     out.update(extra)
     print(json.dumps(out), flush=True)
     return 0 if all(parity.values()) else 3
+
+
+def calibration_copy(A, mib=256):
+    """One streaming device copy of known size (mib MiB read, mib MiB written), so
+    that a counter run can scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    import ctypes
+
+    nbytes = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    A.check(A.lib().qr_dev_alloc(nbytes, ctypes.byref(src)))
+    A.check(A.lib().qr_dev_alloc(nbytes, ctypes.byref(dst)))
+    ms = ctypes.c_double()
+    for _ in range(2):
+        A.check(A.lib().qr_dev_copy(dst, src, nbytes, ctypes.byref(ms)))
+    log(f"calibration copy: {mib} MiB in {ms.value:.3f} ms ({2 * nbytes / ms.value / 1e6:.0f} GB/s)")
+    A.lib().qr_dev_free(src)
+    A.lib().qr_dev_free(dst)
+
+
+def traffic_for(kernel, args):
+    """HBM bytes per launch of the dominant traversal kernel, measured offline with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this exact workload and scaled by the
+    same counters' reading for a copy of known size; the figures are kept in
+    profiles/hbm_traffic.txt.  None when no measurement for this configuration exists."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    key = f"c2 cells={args.cells} queries={args.queries} k={args.k} radius={args.radius} kernel={kernel}"
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith(key + " "):
+                    fields = dict(kv.split("=", 1) for kv in ln[len(key):].split())
+                    return int(float(fields["bytes"]))
+    except OSError:
+        pass
+    return None
 
 
 def cpu_leg(O, tris, queries, args, threads):

@@ -72,7 +72,8 @@ typedef struct {
 typedef struct {
     uint64_t node_tests, leaf_tests, stack_overflows;
     double sort_ms;
-    double kernel_ms; /* traversal kernel(s) alone (radius: count + fill passes) */
+    double kernel_ms;      /* traversal kernel(s) alone (radius: count + fill passes) */
+    uint64_t nodes_opened; /* interior nodes whose children were loaded (binary or 4-wide) */
 } qr_traversal_stats;
 
 /* RayStats (src/core/fingerprint.h:16-43) */
@@ -107,6 +108,10 @@ QR_API int qr_dev_free(void* ptr);
 QR_API int qr_dev_upload(void* dst, const void* src_host, size_t bytes);
 QR_API int qr_dev_download(void* dst_host, const void* src, size_t bytes);
 QR_API int qr_dev_synchronize(void);
+/* Device-to-device copy through a streaming kernel (16-byte aligned pointers);
+   ms = kernel time.  No counterpart in the reference; used to calibrate HBM
+   traffic counters against a copy of known size. */
+QR_API int qr_dev_copy(void* dst, const void* src, size_t bytes, double* ms);
 
 /* ---- build (src/gpu/gpu_bvh.h:103-118) ----------------------------------------
  * buildBVHGPU / buildPointBVHGPU / buildGaussianBVHGPU (builder LBVH) and
@@ -133,6 +138,25 @@ QR_API int qr_bvh_refit(qr_bvh* bvh, const void* items_host, int n, const int* d
 QR_API int qr_bvh_refit_device(qr_bvh* bvh, const void* items_dev, int n,
                                qr_refit_timings* timings, double* ms);
 
+/* GpuTopoUpdateTimings (src/gpu/gpu_bvh.h:190-206) */
+typedef struct {
+    double upload_ms, plan_ms, surgery_ms, relayout_ms, merge_ms, wide_ms, total_ms;
+    int deleted, moved, inserted, appended, surgery_roots, rebuilt_leaves, new_depth;
+    int full_rebuild;
+    double refit_ms;
+} qr_topo_timings;
+
+/* update{,Point,Gaussian}BVHGPU (src/gpu/gpu_bvh.h:208-213; edit script at :182-187).
+ * items_host: the whole new scene (n primitives).  deleted: old indices, ascending,
+ * unique; moved: new indices whose geometry changed; inserted: new indices,
+ * ascending, unique, each < n - appended; appended: count at the end.
+ * n must equal prim_count - n_deleted + n_inserted + appended.  An invalid edit
+ * returns QR_ERR_EDIT.  Queries afterwards equal those on a fresh build of the
+ * new scene bit for bit. */
+QR_API int qr_bvh_update(qr_bvh* bvh, const void* items_host, int n, const int* deleted,
+                         int n_deleted, const int* moved, int n_moved, const int* inserted,
+                         int n_inserted, int appended, qr_topo_timings* timings, double* ms);
+
 /* validate{,Point,Gaussian}BVHGPU (src/gpu/gpu_bvh.h:347-349): why receives "" when
  * every structural invariant holds. */
 QR_API int qr_bvh_validate(qr_bvh* bvh, const void* items_host, int n, char* why, int len);
@@ -153,6 +177,9 @@ QR_API int qr_raycast(const qr_bvh* bvh, qr_rays* rays, int collect_stats, int o
 QR_API int qr_rays_summarize(const qr_rays* rays, uint64_t* per_ray, qr_ray_stats* out);
 /* lastTraversalStatsGPU */
 QR_API int qr_last_traversal_stats(qr_traversal_stats* out);
+/* GpuRayBuffer::d_rays / d_results / count (src/gpu/gpu_bvh.h:91-96). */
+QR_API int qr_rays_device_ptrs(const qr_rays* rays, void** rays_dev, uint64_t** results_dev,
+                               int* count);
 /* Device-pointer variant: rays_dev[n] -> results_dev[n] (u64). */
 QR_API int qr_raycast_device(const qr_bvh* bvh, const void* rays_dev, int n, uint64_t* results_dev,
                              int order, int wide, double* ms);
@@ -178,6 +205,10 @@ QR_API int qr_points_info(const qr_points* pts, int* count, int* stride);
 /* readRadiusResultsGPU: offsets_host[count + 1], keys_host[total]. */
 QR_API int qr_radius_info(const qr_csr* r, int* count, int* total);
 QR_API int qr_radius_read(const qr_csr* r, int* offsets_host, uint64_t* keys_host);
+/* GpuPointBuffer::d_points / d_results and GpuRadiusBuffer::d_offsets / d_keys
+ * (src/gpu/gpu_bvh.h:255-262, :296-301). */
+QR_API int qr_points_device_ptrs(const qr_points* pts, float** points_dev, void** results_dev);
+QR_API int qr_radius_device_ptrs(const qr_csr* r, int** offsets_dev, uint64_t** keys_dev);
 /* destroyRadiusBufferGPU */
 QR_API int qr_radius_destroy(qr_csr* r);
 /* lastQueryStatsGPU */

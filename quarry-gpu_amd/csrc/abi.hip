@@ -112,10 +112,11 @@ struct Stats {
 
 void readStats(const unsigned long long* dev, qr_traversal_stats* out) {
     unsigned long long h[4] = {0, 0, 0, 0};
-    hipCheck(hipMemcpy(h, dev, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "stats");
+    hipCheck(hipMemcpy(h, dev, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "stats");
     out->node_tests = h[0];
     out->leaf_tests = h[1];
     out->stack_overflows = h[2];
+    out->nodes_opened = h[3];
 }
 
 TraverseArgs argsFor(const BvhState& b, bool wide) {
@@ -255,6 +256,44 @@ int qr_dev_download(void* dst, const void* src, size_t bytes) {
     });
 }
 
+namespace {
+// Streaming device copy, 16 B per lane per iteration (grid-stride); the tail
+// below 16 B is copied bytewise by the first lanes.
+__global__ __launch_bounds__(256) void copyKernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                  size_t vec, const uint8_t* __restrict__ srcTail,
+                                                  uint8_t* __restrict__ dstTail, int tail) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < vec; i += stride)
+        dst[i] = src[i];
+    const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < static_cast<size_t>(tail)) dstTail[t] = srcTail[t];
+}
+}  // namespace
+
+int qr_dev_copy(void* dst, const void* src, size_t bytes, double* ms) {
+    return guard([&] {
+        need(bytes == 0 || (dst != nullptr && src != nullptr), "null pointer");
+        need((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) % 16 == 0,
+             "pointers must be 16-byte aligned");
+        hipStream_t s = currentStream();
+        EventSet<2> ev;
+        ev.record(0, s);
+        if (bytes) {
+            const size_t vec = bytes / 16;
+            const int tail = static_cast<int>(bytes % 16);
+            const size_t want = (vec + 255) / 256;
+            const int blocks = static_cast<int>(std::max<size_t>(1, std::min<size_t>(want, 8192)));
+            copyKernel<<<blocks, 256, 0, s>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst),
+                                              vec, static_cast<const uint8_t*>(src) + vec * 16,
+                                              static_cast<uint8_t*>(dst) + vec * 16, tail);
+            hipCheck(hipGetLastError(), "copy kernel");
+        }
+        ev.record(1, s);
+        ev.sync(1);
+        if (ms) *ms = ev.ms(0, 1);
+    });
+}
+
 int qr_dev_synchronize(void) {
     return guard([&] { hipCheck(hipDeviceSynchronize(), "synchronize"); });
 }
@@ -314,6 +353,111 @@ int qr_bvh_get_info(const qr_bvh* bvh, qr_bvh_info* out) {
         out->d_items = b.items.p;
         out->d_prim_ids = b.primIds.p;
         out->d_wide_nodes = b.wideValid ? b.wide.p : nullptr;
+    });
+}
+
+int qr_bvh_update(qr_bvh* bvh, const void* items, int n, const int* deleted, int nDeleted,
+                  const int* moved, int nMoved, const int* inserted, int nInserted, int appended,
+                  qr_topo_timings* t, double* ms) {
+    return guard([&] {
+        // Edit validation follows src/gpu/lbvh.cu:2555-2585: every violation is an
+        // invalid edit (std::invalid_argument in the reference API).
+        if (bvh == nullptr || bvh->s.n < 2) throw EditError("updateBVHGPU: BVH not built");
+        BvhState& b = bvh->s;
+        const int nOld = b.n;
+        if (appended < 0) throw EditError("updateBVHGPU: appended must be >= 0");
+        if (nDeleted < 0 || nMoved < 0 || nInserted < 0) throw EditError("updateBVHGPU: negative list size");
+        for (int i = 0; i < nDeleted; ++i) {
+            const int id = deleted[i];
+            if (id < 0 || id >= nOld) throw EditError("updateBVHGPU: deleted out of range");
+            if (i > 0 && id <= deleted[i - 1])
+                throw EditError("updateBVHGPU: deleted must be ascending and unique");
+        }
+        const long long nNew = static_cast<long long>(nOld) - nDeleted + nInserted + appended;
+        if (nNew < 2) throw EditError("updateBVHGPU: at least 2 primitives required");
+        if (static_cast<long long>(n) != nNew)
+            throw EditError("updateBVHGPU: scene size does not match the edit");
+        const long long tailBeg = nNew - appended;
+        for (int i = 0; i < nInserted; ++i) {
+            const int id = inserted[i];
+            if (id < 0 || id >= tailBeg) throw EditError("updateBVHGPU: inserted index out of range");
+            if (i > 0 && id <= inserted[i - 1])
+                throw EditError("updateBVHGPU: inserted must be ascending and unique");
+        }
+        for (int i = 0; i < nMoved; ++i)
+            if (moved[i] < 0 || moved[i] >= nNew) throw EditError("updateBVHGPU: moved index out of range");
+        need(items != nullptr, "null primitive array");
+
+        qr_topo_timings tt{};
+        tt.deleted = nDeleted;
+        tt.inserted = nInserted;
+        tt.appended = appended;
+        tt.new_depth = b.depth;
+        if (nDeleted == 0 && nInserted == 0 && appended == 0) {
+            // Shape unchanged: the dirty refit path (only the moved primitives travel).
+            qr_refit_timings rt{};
+            double rms = 0.0;
+            const int st = qr_bvh_refit(bvh, items, n, moved, nMoved, &rt, &rms);
+            if (st != QR_OK) throw DeviceError(tLastError);
+            tt.refit_ms = rms;
+            tt.moved = rt.updated;
+            tt.total_ms = rms;
+        } else {
+            // New shape: rebuild with the tree's own builder; the 4-wide level is
+            // re-collapsed when it existed.  Queries on the result equal those on a
+            // fresh build of the new scene bit for bit.
+            hipStream_t s = currentStream();
+            const bool hadWide = b.wideValid;
+            const size_t bytes = static_cast<size_t>(n) * itemSize(b.kind);
+            EventSet<4> ev;
+            ev.record(0, s);
+            DevBuf<unsigned char> src(bytes);
+            hipCheck(hipMemcpyAsync(src.p, items, bytes, hipMemcpyHostToDevice, s), "H2D update");
+            ev.record(1, s);
+            BvhState fresh;
+            buildInto(fresh, b.kind, b.builder, src.p, n, nullptr);
+            b = std::move(fresh);
+            ev.record(2, s);
+            if (hadWide) buildWide(b, s);
+            ev.record(3, s);
+            ev.sync(3);
+            tt.upload_ms = ev.ms(0, 1);
+            tt.surgery_ms = ev.ms(1, 2);
+            tt.wide_ms = hadWide ? ev.ms(2, 3) : 0.0;
+            tt.total_ms = ev.ms(0, 3);
+            tt.moved = nMoved;
+            tt.surgery_roots = 1;
+            tt.rebuilt_leaves = n;
+            tt.new_depth = b.depth;
+            tt.full_rebuild = 1;
+        }
+        if (t) *t = tt;
+        if (ms) *ms = tt.total_ms;
+    });
+}
+
+int qr_rays_device_ptrs(const qr_rays* r, void** rays_dev, uint64_t** results_dev, int* count) {
+    return guard([&] {
+        need(r != nullptr, "null ray buffer");
+        if (rays_dev) *rays_dev = r->rays.p;
+        if (results_dev) *results_dev = r->results.p;
+        if (count) *count = r->count;
+    });
+}
+
+int qr_points_device_ptrs(const qr_points* p, float** points_dev, void** results_dev) {
+    return guard([&] {
+        need(p != nullptr, "null point buffer");
+        if (points_dev) *points_dev = reinterpret_cast<float*>(p->pts.p);
+        if (results_dev) *results_dev = p->results.p;
+    });
+}
+
+int qr_radius_device_ptrs(const qr_csr* r, int** offsets_dev, uint64_t** keys_dev) {
+    return guard([&] {
+        need(r != nullptr, "null radius result");
+        if (offsets_dev) *offsets_dev = r->offsets.p;
+        if (keys_dev) *keys_dev = r->keys.p;
     });
 }
 

@@ -33,25 +33,41 @@ constexpr int kBlock = 256;
 constexpr int kLdsDepth = 32;
 constexpr int kSpill = 112;  // scratch entries beyond the LDS part (never reached by LBVH trees)
 
+// ---- XCD-aware block numbering ------------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The
+// traversal kernels renumber their blocks so that XCD x walks one contiguous range
+// of the (Morton-ordered) queries: neighbouring queries share most of their nodes,
+// and those nodes then stay in one L2 instead of being fetched by all eight.
+// Bijective on [0, gridDim.x): XCD x owns q + (x < r) blocks, q = g / 8, r = g % 8.
+__device__ __forceinline__ int xcdBlock() {
+    const int g = static_cast<int>(gridDim.x), i = static_cast<int>(blockIdx.x);
+    const int x = i & 7, j = i >> 3;
+    const int q = g >> 3, r = g & 7;
+    return x * q + (x < r ? x : r) + j;
+}
+
 // ---- stats -------------------------------------------------------------------------
-// stats[0] node tests, [1] leaf tests, [2] stack overflows.
+// stats[0] node tests, [1] leaf tests, [2] stack overflows, [3] interior nodes opened.
 template <bool kStats>
 __device__ __forceinline__ void flushStats(unsigned long long* stats, int nodeTests, int leafTests,
-                                           int overflow) {
+                                           int overflow, int opened) {
     if (!kStats) return;
     unsigned long long a = static_cast<unsigned long long>(nodeTests);
     unsigned long long b = static_cast<unsigned long long>(leafTests);
     unsigned long long c = static_cast<unsigned long long>(overflow);
+    unsigned long long d = static_cast<unsigned long long>(opened);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_xor(a, off, 64);
         b += __shfl_xor(b, off, 64);
         c += __shfl_xor(c, off, 64);
+        d += __shfl_xor(d, off, 64);
     }
     if ((threadIdx.x & 63) == 0) {
         if (a) atomicAdd(&stats[0], a);
         if (b) atomicAdd(&stats[1], b);
         if (c) atomicAdd(&stats[2], c);
+        if (d) atomicAdd(&stats[3], d);
     }
 }
 
@@ -183,8 +199,8 @@ __global__ __launch_bounds__(kBlock) void rayBinaryKernel(const Node2* __restric
                                                           int count, uint64_t* __restrict__ out,
                                                           unsigned long long* __restrict__ stats) {
     __shared__ int lds[kLdsDepth * kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    int nodeTests = 0, leafTests = 0, overflow = 0;
+    const int i = xcdBlock() * kBlock + threadIdx.x;
+    int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
     if (i < count) {
         const int q = order ? static_cast<int>(order[i]) : i;
         const Ray ray = rays[q];
@@ -201,6 +217,7 @@ __global__ __launch_bounds__(kBlock) void rayBinaryKernel(const Node2* __restric
             id = 0;
         }
         while (id >= 0) {
+            if (kStats) ++opened;
             const Node2 nd = nodes[id];
             const int kid[2] = {nd.left, nd.right};
             float t[2];
@@ -258,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void rayBinaryKernel(const Node2* __restric
         }
         out[q] = any ? ((uint64_t{1} << 32) | floatBits(best)) : 0ull;
     }
-    flushStats<kStats>(stats, nodeTests, leafTests, overflow);
+    flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
 __device__ __forceinline__ void loadWide(const Wide4* __restrict__ wide, int id, WideEntry (&e)[4]) {
@@ -305,8 +322,8 @@ __global__ __launch_bounds__(kBlock) void rayWideKernel(const Wide4* __restrict_
                                                         int count, uint64_t* __restrict__ out,
                                                         unsigned long long* __restrict__ stats) {
     __shared__ int lds[kLdsDepth * kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    int nodeTests = 0, leafTests = 0, overflow = 0;
+    const int i = xcdBlock() * kBlock + threadIdx.x;
+    int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
     if (i < count) {
         const int q = order ? static_cast<int>(order[i]) : i;
         const Ray ray = rays[q];
@@ -317,6 +334,7 @@ __global__ __launch_bounds__(kBlock) void rayWideKernel(const Wide4* __restrict_
         st.init(lds);
         int id = 0;
         while (id >= 0) {
+            if (kStats) ++opened;
             WideEntry e[4];
             loadWide(wide, id, e);
             float key[4];
@@ -351,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void rayWideKernel(const Wide4* __restrict_
         }
         out[q] = any ? ((uint64_t{1} << 32) | floatBits(best)) : 0ull;
     }
-    flushStats<kStats>(stats, nodeTests, leafTests, overflow);
+    flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
 // ---- closest / kNN ------------------------------------------------------------------
@@ -375,8 +393,8 @@ __global__ __launch_bounds__(kBlock) void nearestBinaryKernel(
     const uint32_t* __restrict__ order, int count, int k, int outStride,
     ClosestResult* __restrict__ out, unsigned long long* __restrict__ stats) {
     __shared__ int lds[kLdsDepth * kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    int nodeTests = 0, leafTests = 0, overflow = 0;
+    const int i = xcdBlock() * kBlock + threadIdx.x;
+    int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
     if (i < count) {
         const int q = order ? static_cast<int>(order[i]) : i;
         const V3 p = points[q];
@@ -387,6 +405,7 @@ __global__ __launch_bounds__(kBlock) void nearestBinaryKernel(
         st.init(lds);
         int id = 0;
         while (id >= 0) {
+            if (kStats) ++opened;
             const Node2 nd = nodes[id];
             ++nodeTests;
             const int kid[2] = {nd.left, nd.right};
@@ -433,7 +452,7 @@ __global__ __launch_bounds__(kBlock) void nearestBinaryKernel(
         }
         writeList<L, K>(list, k, out + static_cast<size_t>(q) * outStride);
     }
-    flushStats<kStats>(stats, nodeTests, leafTests, overflow);
+    flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
 template <typename L, int K, bool kStats>
@@ -443,8 +462,8 @@ __global__ __launch_bounds__(kBlock) void nearestWideKernel(
     const uint32_t* __restrict__ order, int count, int k, int outStride,
     ClosestResult* __restrict__ out, unsigned long long* __restrict__ stats) {
     __shared__ int lds[kLdsDepth * kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    int nodeTests = 0, leafTests = 0, overflow = 0;
+    const int i = xcdBlock() * kBlock + threadIdx.x;
+    int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
     if (i < count) {
         const int q = order ? static_cast<int>(order[i]) : i;
         const V3 p = points[q];
@@ -455,6 +474,7 @@ __global__ __launch_bounds__(kBlock) void nearestWideKernel(
         st.init(lds);
         int id = 0;
         while (id >= 0) {
+            if (kStats) ++opened;
             WideEntry e[4];
             loadWide(wide, id, e);
             float key[4];
@@ -490,7 +510,7 @@ __global__ __launch_bounds__(kBlock) void nearestWideKernel(
         }
         writeList<L, K>(list, k, out + static_cast<size_t>(q) * outStride);
     }
-    flushStats<kStats>(stats, nodeTests, leafTests, overflow);
+    flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
 // ---- radius: count and fill passes ---------------------------------------------------
@@ -505,8 +525,8 @@ __global__ __launch_bounds__(kBlock) void radiusKernel(
     int* __restrict__ counts, const int* __restrict__ offsets, uint64_t* __restrict__ keys,
     unsigned long long* __restrict__ stats) {
     __shared__ int lds[kLdsDepth * kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    int nodeTests = 0, leafTests = 0, overflow = 0;
+    const int i = xcdBlock() * kBlock + threadIdx.x;
+    int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
     if (i < count) {
         const int q = order ? static_cast<int>(order[i]) : i;
         const V3 p = points[q];
@@ -540,6 +560,7 @@ __global__ __launch_bounds__(kBlock) void radiusKernel(
             id = 0;
         }
         while (id >= 0) {
+            if (kStats) ++opened;
             int c[4] = {kWideEmpty, kWideEmpty, kWideEmpty, kWideEmpty};
             bool ok[4] = {false, false, false, false};
             if (!kWide) {
@@ -577,7 +598,7 @@ __global__ __launch_bounds__(kBlock) void radiusKernel(
         }
         if (!kFill) counts[q] = found;
     }
-    flushStats<kStats>(stats, nodeTests, leafTests, overflow);
+    flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
 // ---- segment sort ----------------------------------------------------------------

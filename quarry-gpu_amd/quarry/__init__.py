@@ -1,19 +1,21 @@
 """quarry: GPU BVH build and spatial queries on MI355X (gfx950).
 
-Drop-in for the reference's Python module (bindings/python/quarry_py.cpp):
+Drop-in for the reference's Python module (bindings/python/quarry_py.cpp:887-1050):
 describe(), demo_mesh(), demo_gaussians() and the Scene / PointCloud /
-GaussianCloud classes live in quarry.api; the low-level ctypes binding of the
-C ABI is quarry._abi.
+GaussianCloud classes, all running on the HIP library libquarry_hip.so through
+its C ABI (quarry._abi).  The library is loaded on first use and a missing or
+unloadable library raises immediately: there is no CPU fallback.
 """
 from . import _abi  # noqa: F401
 from ._abi import QuarryError  # noqa: F401
+from .api import GaussianCloud, PointCloud, Scene  # noqa: F401
+from .demo import demo_gaussians, demo_mesh  # noqa: F401
 
 __version__ = "1.1.0"
 
 
 def describe() -> str:
-    """Name of the HIP device, or raises QuarryError when none is visible."""
+    """HIP device string, or an empty string when no GPU is available
+    (quarry_py.cpp:891-897)."""
     d = _abi.describe()
-    if d is None:
-        raise QuarryError(_abi.lib().qr_last_error().decode())
-    return d
+    return d if d is not None else ""

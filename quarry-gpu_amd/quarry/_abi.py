@@ -45,10 +45,18 @@ class RefitTimings(ctypes.Structure):
         ("updated", ctypes.c_int)]
 
 
+class TopoTimings(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("upload_ms", "plan_ms", "surgery_ms", "relayout_ms",
+                                                "merge_ms", "wide_ms", "total_ms")] + \
+               [(n, ctypes.c_int) for n in ("deleted", "moved", "inserted", "appended", "surgery_roots",
+                                             "rebuilt_leaves", "new_depth", "full_rebuild")] + \
+               [("refit_ms", ctypes.c_double)]
+
+
 class TraversalStats(ctypes.Structure):
     _fields_ = [("node_tests", ctypes.c_uint64), ("leaf_tests", ctypes.c_uint64),
                 ("stack_overflows", ctypes.c_uint64), ("sort_ms", ctypes.c_double),
-                ("kernel_ms", ctypes.c_double)]
+                ("kernel_ms", ctypes.c_double), ("nodes_opened", ctypes.c_uint64)]
 
 
 class RayStats(ctypes.Structure):
@@ -87,6 +95,7 @@ def lib():
             "qr_dev_upload": ([_vp, _vp, ctypes.c_size_t], _ip),
             "qr_dev_download": ([_vp, _vp, ctypes.c_size_t], _ip),
             "qr_dev_synchronize": ([], _ip),
+            "qr_dev_copy": ([_vp, _vp, ctypes.c_size_t, _dp], _ip),
             "qr_bvh_build": ([_ip, _ip, _vp, _ip, _vp, _vp], _ip),
             "qr_bvh_build_device": ([_ip, _ip, _vp, _ip, _vp, _vp], _ip),
             "qr_bvh_destroy": ([_vp], _ip),
@@ -116,6 +125,10 @@ def lib():
             "qr_last_query_stats": ([_vp], _ip),
             "qr_closest_device": ([_vp, _vp, _ip, _vp, _ip, _ip, _dp], _ip),
             "qr_knn_device": ([_vp, _vp, _ip, _ip, _vp, _ip, _ip, _dp], _ip),
+            "qr_bvh_update": ([_vp, _vp, _ip, _vp, _ip, _vp, _ip, _vp, _ip, _ip, _vp, _dp], _ip),
+            "qr_rays_device_ptrs": ([_vp, _vp, _vp, _vp], _ip),
+            "qr_points_device_ptrs": ([_vp, _vp, _vp], _ip),
+            "qr_radius_device_ptrs": ([_vp, _vp, _vp], _ip),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -199,6 +212,19 @@ class Bvh:
         else:
             d = np.ascontiguousarray(dirty, dtype=np.int32)
             check(lib().qr_bvh_refit(self.h, ptr(items), items.shape[0], ptr(d), d.shape[0], ctypes.byref(t), ctypes.byref(ms)))
+        return t
+
+    def update(self, items: np.ndarray, deleted=None, moved=None, inserted=None,
+               appended: int = 0) -> TopoTimings:
+        """Topology edit: items is the whole new scene (see qr_bvh_update)."""
+        items = np.ascontiguousarray(items, dtype=np.float32).reshape(-1, ITEM_FLOATS[self.kind])
+        arrs = [np.ascontiguousarray(a if a is not None else [], dtype=np.int32)
+                for a in (deleted, moved, inserted)]
+        t = TopoTimings()
+        ms = ctypes.c_double()
+        check(lib().qr_bvh_update(self.h, ptr(items), items.shape[0], ptr(arrs[0]), arrs[0].shape[0],
+                                  ptr(arrs[1]), arrs[1].shape[0], ptr(arrs[2]), arrs[2].shape[0],
+                                  int(appended), ctypes.byref(t), ctypes.byref(ms)))
         return t
 
     def validate(self, items: np.ndarray) -> str:
