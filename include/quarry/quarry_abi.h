@@ -219,6 +219,31 @@ QR_API int qr_closest_device(const qr_bvh* bvh, const float* points_dev, int n, 
 QR_API int qr_knn_device(const qr_bvh* bvh, const float* points_dev, int n, int k,
                          void* results_dev, int order, int wide, double* ms);
 
+/* ---- differentiable mode ------------------------------------------------------------
+ * No reference counterpart: the reference is forward-only and its ray kernel
+ * discards the hit index (src/gpu/raycast.cu:108-111).  All pointers are device
+ * pointers; work runs on the stream set with qr_set_stream.  Forward results are
+ * the plain entry points' results bit for bit. */
+/* Ray cast that also writes the hit triangle's scene index (-1: miss) per ray. */
+QR_API int qr_raycast_save_device(const qr_bvh* bvh, const void* rays_dev, int n, uint64_t* results_dev,
+                                  int* hit_prim_dev, int order, int wide, double* ms);
+/* Saved word per closest result (from qr_closest_device): triangle -> closest-feature
+ * region 0..9 (vertex A, B, edge AB, vertex C, edge AC, edge BC, face, degenerate
+ * A/B/C), Gaussian -> bits of the Eberly root (-2: inside), point -> 0; -1: no result. */
+QR_API int qr_closest_save_aux(qr_bvh* bvh, const float* points_dev, int n, const void* results_dev,
+                               int* aux_dev);
+/* Adjoint of dist2: grad_points (n x 3, overwritten, may be NULL) and grad_items
+ * (prim_count x 9 / 3 / 10 floats in scene order, accumulated, may be NULL;
+ * Gaussians: mean 3, quaternion 4, scale 3) from grad_dist2 (n). */
+QR_API int qr_adjoint_closest(qr_bvh* bvh, const float* points_dev, int n, const void* results_dev,
+                              const int* aux_dev, const float* grad_dist2_dev, float* grad_points_dev,
+                              float* grad_items_dev);
+/* Adjoint of hit t: grad_origins, grad_dirs (n x 3, overwritten, may be NULL),
+ * grad_tris (prim_count x 9, accumulated, may be NULL) from grad_t (n). */
+QR_API int qr_adjoint_ray(qr_bvh* bvh, const void* rays_dev, int n, const uint64_t* results_dev,
+                          const int* hit_prim_dev, const float* grad_t_dev, float* grad_origins_dev,
+                          float* grad_dirs_dev, float* grad_tris_dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/quarry/quarry_abi.h"
+#include "qr_adjoint.h"
 #include "qr_bvh.h"
 #include "qr_traverse.h"
 
@@ -671,7 +672,7 @@ int qr_rays_destroy(qr_rays* r) {
 
 namespace {
 double raycastImpl(const qr_bvh* bvh, const Ray* rays, int n, uint64_t* out, qr_rays* scratch,
-                   int collectStats, int order, int wide) {
+                   int collectStats, int order, int wide, int* hitPrim = nullptr) {
     checkTree(bvh, wide != 0);
     need(bvh->s.kind == kPrimTri, "raycast needs a triangle BVH");
     need(order >= 0 && order <= 2, "unknown ray order");
@@ -691,7 +692,11 @@ double raycastImpl(const qr_bvh* bvh, const Ray* rays, int n, uint64_t* out, qr_
         a.order = reorder(*scratch, base, 8, n, mode, s);
     }
     ev.record(1, s);
-    launchRays(a, rays, n, out, s);
+    if (hitPrim) {
+        launchRaysSave(a, rays, n, out, hitPrim, s);
+    } else {
+        launchRays(a, rays, n, out, s);
+    }
     ev.record(2, s);
     ev.sync(2);
     const double total = ev.ms(0, 2);
@@ -717,6 +722,50 @@ int qr_raycast_device(const qr_bvh* bvh, const void* rays, int n, uint64_t* resu
         qr_rays scratch;
         const double t = raycastImpl(bvh, static_cast<const Ray*>(rays), n, results, &scratch, 0, order, wide);
         if (ms) *ms = t;
+    });
+}
+
+int qr_raycast_save_device(const qr_bvh* bvh, const void* rays, int n, uint64_t* results, int* hitPrim,
+                           int order, int wide, double* ms) {
+    return guard([&] {
+        need(n >= 0, "negative ray count");
+        need(n == 0 || (rays && results && hitPrim), "null device pointer");
+        qr_rays scratch;
+        const double t =
+            raycastImpl(bvh, static_cast<const Ray*>(rays), n, results, &scratch, 0, order, wide, hitPrim);
+        if (ms) *ms = t;
+    });
+}
+
+int qr_closest_save_aux(qr_bvh* bvh, const float* points, int n, const void* results, int* aux) {
+    return guard([&] {
+        checkTree(bvh, false);
+        need(n >= 0, "negative point count");
+        need(n == 0 || (points && results && aux), "null device pointer");
+        const int bad = saveClosestAux(bvh->s, points, n, results, aux, currentStream());
+        if (bad != 0) throw DeviceError("closest save: saved results disagree with their primitives");
+    });
+}
+
+int qr_adjoint_closest(qr_bvh* bvh, const float* points, int n, const void* results, const int* aux,
+                       const float* grad_dist2, float* grad_points, float* grad_items) {
+    return guard([&] {
+        checkTree(bvh, false);
+        need(n >= 0, "negative point count");
+        need(n == 0 || (points && results && grad_dist2), "null device pointer");
+        need(n == 0 || bvh->s.kind == kPrimPoint || aux != nullptr, "saved region / root required");
+        adjointClosest(bvh->s, points, n, results, aux, grad_dist2, grad_points, grad_items, currentStream());
+    });
+}
+
+int qr_adjoint_ray(qr_bvh* bvh, const void* rays, int n, const uint64_t* results, const int* hit_prim,
+                   const float* grad_t, float* grad_origins, float* grad_dirs, float* grad_tris) {
+    return guard([&] {
+        checkTree(bvh, false);
+        need(n >= 0, "negative ray count");
+        need(n == 0 || (rays && results && hit_prim && grad_t), "null device pointer");
+        adjointRay(bvh->s, rays, n, results, hit_prim, grad_t, grad_origins, grad_dirs, grad_tris,
+                   currentStream());
     });
 }
 

@@ -369,6 +369,32 @@ void buildLbvhDevice(BvhState& b, int kind, const void* dItems, int n, hipStream
     }
 }
 
+// Leaf gather, leaf nodes and bottom-up boxes for a topology built elsewhere (the
+// SAH builder): internal nodes' left/right, parent[] (root -1) and zeroed arrival
+// counters must be in place; order[k] is the scene index of leaf slot k.
+void finishTopology(BvhState& b, const void* dSrc, const uint32_t* order, hipStream_t s) {
+    const int n = b.n;
+    const int grid = blocksFor(n, kBlock);
+    switch (b.kind) {
+        case kPrimTri:
+            leafUpKernel<TriLeaf, true><<<grid, kBlock, 0, s>>>(
+                static_cast<const Tri*>(dSrc), order, n, reinterpret_cast<Tri*>(b.items.p), b.primIds.p,
+                b.nodes.p, b.parent.p, b.arrive.p, b.height.p);
+            break;
+        case kPrimPoint:
+            leafUpKernel<PointLeaf, true><<<grid, kBlock, 0, s>>>(
+                static_cast<const V3*>(dSrc), order, n, reinterpret_cast<V3*>(b.items.p), b.primIds.p,
+                b.nodes.p, b.parent.p, b.arrive.p, b.height.p);
+            break;
+        default:
+            leafUpKernel<GaussLeaf, true><<<grid, kBlock, 0, s>>>(
+                static_cast<const Gaussian*>(dSrc), order, n, reinterpret_cast<Gaussian*>(b.items.p),
+                b.primIds.p, b.nodes.p, b.parent.p, b.arrive.p, b.height.p);
+            break;
+    }
+    hipCheck(hipGetLastError(), "finish topology");
+}
+
 // Refit of every leaf from the leaf-ordered items already in b.items: the same
 // bottom-up pass without the gather (arrival counters are zero between kernels).
 void propagateAll(BvhState& b, hipStream_t s) {

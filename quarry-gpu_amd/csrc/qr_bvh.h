@@ -68,6 +68,11 @@ void computeDepth(BvhState& b, hipStream_t s);
 // leaf order (refit path shares it).
 void propagateAll(BvhState& b, hipStream_t s);
 
+// Leaf gather + leaf nodes + bottom-up boxes and heights for a topology already in
+// b.nodes (internal left/right), b.parent (root -1) and zeroed b.arrive; order[k]
+// is the scene index of leaf slot k.  The root's height (b.height[0]) is the depth.
+void finishTopology(BvhState& b, const void* dSrc, const uint32_t* order, hipStream_t s);
+
 void buildWide(BvhState& b, hipStream_t s);
 void reemitWide(BvhState& b, hipStream_t s);
 void ensurePrimSlot(BvhState& b, hipStream_t s);

@@ -34,6 +34,9 @@ void mortonOrder(const float* base, int stride, int count, int boxMode, SortWork
                  hipStream_t s);
 
 void launchRays(const TraverseArgs& a, const Ray* rays, int count, uint64_t* out, hipStream_t s);
+// Same, also writing the hit triangle's scene index (-1 for a miss) to hitPrim.
+void launchRaysSave(const TraverseArgs& a, const Ray* rays, int count, uint64_t* out, int* hitPrim,
+                    hipStream_t s);
 void launchClosest(const TraverseArgs& a, const V3* pts, int count, ClosestResult* out,
                    hipStream_t s);
 // Result j of query q lands at out[q * outStride + j] (outStride >= k).

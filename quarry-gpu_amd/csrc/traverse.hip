@@ -191,13 +191,33 @@ struct KnnList {
 // intersected nearest first, and the nearest surviving interior child becomes the
 // next node without a stack round trip; the other survivors are pushed far first.
 
-template <bool kStats>
+// One ray-triangle leaf test.  With kSave the hit's scene index is kept too
+// (equal t: the smaller scene index), for the differentiable mode; t is the same.
+template <bool kSave>
+__device__ __forceinline__ void rayLeaf(const Ray& ray, const Tri* __restrict__ tris,
+                                        const int* __restrict__ primIds, int slot, float& best,
+                                        bool& any, int& bestId) {
+    float t;
+    if (!rayTri(ray, tris[slot], t)) return;
+    if (t < best) {
+        best = t;
+        any = true;
+        if (kSave) bestId = primIds[slot];
+    } else if (kSave && any && t == best) {
+        const int sid = primIds[slot];
+        if (sid < bestId) bestId = sid;
+    }
+}
+
+template <bool kStats, bool kSave>
 __global__ __launch_bounds__(kBlock) void rayBinaryKernel(const Node2* __restrict__ nodes,
                                                           const Tri* __restrict__ tris,
                                                           const Ray* __restrict__ rays,
                                                           const uint32_t* __restrict__ order,
                                                           int count, uint64_t* __restrict__ out,
-                                                          unsigned long long* __restrict__ stats) {
+                                                          unsigned long long* __restrict__ stats,
+                                                          const int* __restrict__ primIds,
+                                                          int* __restrict__ hitPrim) {
     __shared__ int lds[kLdsDepth * kBlock];
     const int i = xcdBlock() * kBlock + threadIdx.x;
     int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
@@ -206,6 +226,7 @@ __global__ __launch_bounds__(kBlock) void rayBinaryKernel(const Node2* __restric
         const Ray ray = rays[q];
         float best = ray.tMax;
         bool any = false;
+        int bestId = -1;
         LaneStack st;
         SpillArea spill;
         st.init(lds);
@@ -239,20 +260,13 @@ __global__ __launch_bounds__(kBlock) void rayBinaryKernel(const Node2* __restric
             const bool swap = t[1] < t[0];
             const int c0 = swap ? c[1] : c[0], c1 = swap ? c[0] : c[1];
             const bool l0 = swap ? leaf[1] : leaf[0], l1 = swap ? leaf[0] : leaf[1];
-            float tt;
             if (c0 != kWideEmpty && l0) {
                 ++leafTests;
-                if (rayTri(ray, tris[c0], tt) && tt < best) {
-                    best = tt;
-                    any = true;
-                }
+                rayLeaf<kSave>(ray, tris, primIds, c0, best, any, bestId);
             }
             if (c1 != kWideEmpty && l1) {
                 ++leafTests;
-                if (rayTri(ray, tris[c1], tt) && tt < best) {
-                    best = tt;
-                    any = true;
-                }
+                rayLeaf<kSave>(ray, tris, primIds, c1, best, any, bestId);
             }
             const bool in0 = c0 != kWideEmpty && !l0;
             const bool in1 = c1 != kWideEmpty && !l1;
@@ -274,6 +288,7 @@ __global__ __launch_bounds__(kBlock) void rayBinaryKernel(const Node2* __restric
             }
         }
         out[q] = any ? ((uint64_t{1} << 32) | floatBits(best)) : 0ull;
+        if (kSave) hitPrim[q] = any ? bestId : -1;
     }
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
@@ -314,13 +329,15 @@ __device__ __forceinline__ int descend(const int (&c)[4], const bool (&ok)[4], L
     return next;
 }
 
-template <bool kStats>
+template <bool kStats, bool kSave>
 __global__ __launch_bounds__(kBlock) void rayWideKernel(const Wide4* __restrict__ wide,
                                                         const Tri* __restrict__ tris,
                                                         const Ray* __restrict__ rays,
                                                         const uint32_t* __restrict__ order,
                                                         int count, uint64_t* __restrict__ out,
-                                                        unsigned long long* __restrict__ stats) {
+                                                        unsigned long long* __restrict__ stats,
+                                                        const int* __restrict__ primIds,
+                                                        int* __restrict__ hitPrim) {
     __shared__ int lds[kLdsDepth * kBlock];
     const int i = xcdBlock() * kBlock + threadIdx.x;
     int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
@@ -329,6 +346,7 @@ __global__ __launch_bounds__(kBlock) void rayWideKernel(const Wide4* __restrict_
         const Ray ray = rays[q];
         float best = ray.tMax;
         bool any = false;
+        int bestId = -1;
         LaneStack st;
         SpillArea spill;
         st.init(lds);
@@ -358,16 +376,13 @@ __global__ __launch_bounds__(kBlock) void rayWideKernel(const Wide4* __restrict_
                 ok[s] = c[s] != kWideEmpty;
                 if (ok[s] && c[s] < 0) {
                     ++leafTests;
-                    float t;
-                    if (rayTri(ray, tris[-c[s] - 1], t) && t < best) {
-                        best = t;
-                        any = true;
-                    }
+                    rayLeaf<kSave>(ray, tris, primIds, -c[s] - 1, best, any, bestId);
                 }
             }
             id = descend(c, ok, st, spill, overflow);
         }
         out[q] = any ? ((uint64_t{1} << 32) | floatBits(best)) : 0ull;
+        if (kSave) hitPrim[q] = any ? bestId : -1;
     }
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
@@ -797,22 +812,37 @@ void mortonOrder(const float* base, int stride, int count, int boxMode, SortWork
     sortPairs(ws, keys, nullptr, keysSorted, orderOut, static_cast<size_t>(count), 4, s);
 }
 
-void launchRays(const TraverseArgs& a, const Ray* rays, int count, uint64_t* out, hipStream_t s) {
+template <bool kSave>
+static void launchRaysT(const TraverseArgs& a, const Ray* rays, int count, uint64_t* out, int* hitPrim,
+                        hipStream_t s) {
     const int grid = blocksFor(count, kBlock);
     if (grid == 0) return;
     const Tri* tris = static_cast<const Tri*>(a.items);
     if (a.wide) {
         if (a.stats)
-            rayWideKernel<true><<<grid, kBlock, 0, s>>>(a.wideNodes, tris, rays, a.order, count, out, a.stats);
+            rayWideKernel<true, kSave><<<grid, kBlock, 0, s>>>(a.wideNodes, tris, rays, a.order, count, out, a.stats,
+                                                              a.primIds, hitPrim);
         else
-            rayWideKernel<false><<<grid, kBlock, 0, s>>>(a.wideNodes, tris, rays, a.order, count, out, nullptr);
+            rayWideKernel<false, kSave><<<grid, kBlock, 0, s>>>(a.wideNodes, tris, rays, a.order, count, out,
+                                                               nullptr, a.primIds, hitPrim);
     } else {
         if (a.stats)
-            rayBinaryKernel<true><<<grid, kBlock, 0, s>>>(a.nodes, tris, rays, a.order, count, out, a.stats);
+            rayBinaryKernel<true, kSave><<<grid, kBlock, 0, s>>>(a.nodes, tris, rays, a.order, count, out, a.stats,
+                                                                a.primIds, hitPrim);
         else
-            rayBinaryKernel<false><<<grid, kBlock, 0, s>>>(a.nodes, tris, rays, a.order, count, out, nullptr);
+            rayBinaryKernel<false, kSave><<<grid, kBlock, 0, s>>>(a.nodes, tris, rays, a.order, count, out,
+                                                                 nullptr, a.primIds, hitPrim);
     }
     hipCheck(hipGetLastError(), "raycast");
+}
+
+void launchRays(const TraverseArgs& a, const Ray* rays, int count, uint64_t* out, hipStream_t s) {
+    launchRaysT<false>(a, rays, count, out, nullptr, s);
+}
+
+void launchRaysSave(const TraverseArgs& a, const Ray* rays, int count, uint64_t* out, int* hitPrim,
+                    hipStream_t s) {
+    launchRaysT<true>(a, rays, count, out, hitPrim, s);
 }
 
 template <typename L, int K>

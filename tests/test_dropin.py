@@ -78,7 +78,10 @@ def test_example_main():
 @pytest.mark.gpu
 def test_example_dynamic_scene():
     out = _run("example_dynamic_scene")
-    assert "MISMATCH" not in out.upper().replace("NO MISMATCH", "")
+    assert "checks ok" in out
+    counts = re.findall(r"mismatch(?:es)? (\d+)/\d+", out)
+    assert counts and all(c == "0" for c in counts), counts
+    assert "validate failures 0" in out
 
 
 FAST = ("--runs", "1", "--warmup-ms", "0")

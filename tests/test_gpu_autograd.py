@@ -1,0 +1,206 @@
+"""Differentiable mode on the GPU: forward bit-exact, adjoints against the FP64 CPU
+adjoint and against FP64 central finite differences (h = 1e-3, Richardson-refined
+with h/2) on 1,024 sampled parameters per scene.
+
+Tolerance: |gpu - reference| <= 1e-4 * max(|reference|, 5% of the gradient
+array's rms) -- relative 1e-4, with components far below the typical magnitude
+judged against that floor.  The loss keeps the primitive each query was assigned
+in the forward pass (that assignment is piecewise constant, and the gradient is
+the gradient of the active piece).
+"""
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+TOL = 1e-4
+
+
+def rel(a, b, floor):
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+def rms(x):
+    return float(np.sqrt(np.mean(np.square(x))))
+
+
+def central(fun, x, idx, h):
+    xp, xm = x.copy(), x.copy()
+    xp.flat[idx] += h
+    xm.flat[idx] -= h
+    return (fun(xp) - fun(xm)) / (2 * h)
+
+
+def fd(fun, x, idx, h=1e-3):
+    return (4.0 * central(fun, x, idx, h / 2) - central(fun, x, idx, h)) / 3.0
+
+
+@pytest.fixture(scope="module")
+def AG():
+    from quarry import autograd
+
+    if not torch.cuda.is_available():
+        pytest.fail("no GPU visible to torch")
+    return autograd
+
+
+@pytest.fixture(scope="module")
+def sphere(scenes):
+    V, F, tris = scenes.icosphere(levels=3, radius=10.0, noise=0.1, seed=2)
+    rng = np.random.default_rng(3)
+    q = rng.uniform(-15, 15, (4096, 3)).astype(np.float32)
+    rays = scenes.sphere_rays(4096, seed=4)
+    return V, F, tris, q, rays
+
+
+def test_forward_bit_exact(AG, orc, sphere):
+    V, F, tris, q, rays = sphere
+    dev = torch.device("cuda")
+    d2, idx = AG.closest_point_dist2(torch.from_numpy(q).to(dev), torch.from_numpy(V).to(dev),
+                                     torch.from_numpy(F).to(dev))
+    want = orc.knn_brute(0, tris, q, 0, threads=THREADS)
+    assert np.array_equal(d2.cpu().numpy().view(np.uint32), want[:, 0, 0])
+    assert np.array_equal(idx.cpu().numpy().astype(np.int32), want[:, 0, 1].view(np.int32))
+    t, hit = AG.ray_hit_t(torch.from_numpy(rays[:, 0:3].copy()).to(dev), torch.from_numpy(rays[:, 3:6].copy()).to(dev),
+                          torch.from_numpy(V).to(dev), torch.from_numpy(F).to(dev))
+    wr = orc.ray_brute(tris, rays, threads=THREADS)
+    wh = (wr >> np.uint64(32)) != 0
+    tt = t.cpu().numpy()
+    assert np.array_equal(np.isfinite(tt), wh)
+    assert np.array_equal(tt[wh].view(np.uint32), (wr[wh] & np.uint64(0xFFFFFFFF)).astype(np.uint32))
+    assert np.all(hit.cpu().numpy()[~wh] == -1) and np.all(hit.cpu().numpy()[wh] >= 0)
+
+
+def test_mesh_gradients(AG, sphere):
+    from oracle import adjoint as J
+
+    V, F, tris, q, rays = sphere
+    dev = torch.device("cuda")
+    Vt = torch.from_numpy(V).to(dev).requires_grad_(True)
+    qt = torch.from_numpy(q).to(dev).requires_grad_(True)
+    ot = torch.from_numpy(rays[:, 0:3].copy()).to(dev).requires_grad_(True)
+    dt = torch.from_numpy(rays[:, 3:6].copy()).to(dev).requires_grad_(True)
+    Ft = torch.from_numpy(F).to(dev)
+    d2, idx = AG.closest_point_dist2(qt, Vt, Ft)
+    t, hit = AG.ray_hit_t(ot, dt, Vt, Ft)
+    mask = torch.isfinite(t)
+    loss = d2.sum() + torch.where(mask, t, torch.zeros_like(t)).sum()
+    loss.backward()
+    gV, gq = Vt.grad.cpu().numpy().astype(np.float64), qt.grad.cpu().numpy().astype(np.float64)
+    go, gd = ot.grad.cpu().numpy().astype(np.float64), dt.grad.cpu().numpy().astype(np.float64)
+    prim = idx.cpu().numpy()
+    hp = hit.cpu().numpy()
+    assert (hp >= 0).sum() > 3000
+
+    # CPU adjoint in FP64 with the same assignment
+    q64, V64, o64, d64 = (np.asarray(x, np.float64) for x in (q, V, rays[:, 0:3], rays[:, 3:6]))
+    T64 = V64[F].reshape(-1, 9)
+    cq, ct = J.tri_adjoint(q64, T64, prim, np.ones(len(q)))
+    co, cd, ct2 = J.ray_adjoint(o64, d64, T64, hp, np.ones(len(hp)))
+    cT = (ct + ct2).reshape(-1, 3, 3)
+    cV = np.zeros_like(V64)
+    for k in range(3):
+        np.add.at(cV, F[:, k], cT[:, k])
+    for got, ref in ((gq, cq), (gV, cV), (go, co), (gd, cd)):
+        err = rel(got, ref, 0.05 * rms(ref))
+        print("cpu-adjoint max rel err", float(err.max()), flush=True)
+        assert err.max() < TOL
+
+    # central differences of the FP64 loss on 1,024 sampled parameters
+    def loss_of(Vx=V64, qx=q64, ox=o64, dx=d64):
+        Tx = Vx[F].reshape(-1, 9)
+        s = J.tri_dist2_fixed(qx, Tx, prim).sum()
+        ok = hp >= 0
+        return s + J.ray_t_fixed(ox[ok], dx[ok], Tx, hp[ok]).sum()
+
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for arr, grad, fn, count in (
+            (V64, gV, lambda x: loss_of(Vx=x), 512),
+            (q64, gq, lambda x: loss_of(qx=x), 256),
+            (o64, go, lambda x: loss_of(ox=x), 128),
+            (d64, gd, lambda x: loss_of(dx=x), 128)):
+        floor = 0.05 * rms(grad)
+        for idx_ in rng.choice(arr.size, count, replace=False):
+            e = rel(grad.flat[idx_], fd(fn, arr, idx_), floor)
+            worst = max(worst, float(e))
+            assert e < TOL, (idx_, grad.flat[idx_], fd(fn, arr, idx_))
+    print("FD max rel err", worst, flush=True)
+
+
+def test_gaussian_gradients(AG, orc, scenes):
+    from oracle import adjoint as J
+
+    g = scenes.gaussians(3000, 5, side=24.0)
+    g[:, 3:7] *= np.float32(1.7)  # unnormalised quaternions
+    rng = np.random.default_rng(8)
+    q = rng.uniform(-3, 27, (2048, 3)).astype(np.float32)
+    dev = torch.device("cuda")
+    m = torch.from_numpy(g[:, 0:3].copy()).to(dev).requires_grad_(True)
+    r = torch.from_numpy(g[:, 3:7].copy()).to(dev).requires_grad_(True)
+    s = torch.from_numpy(g[:, 7:10].copy()).to(dev).requires_grad_(True)
+    qt = torch.from_numpy(q).to(dev).requires_grad_(True)
+    d2, idx = AG.gaussian_closest_dist2(qt, m, r, s)
+    want = orc.knn_brute(2, g, q, 0, threads=THREADS)
+    assert np.array_equal(d2.detach().cpu().numpy().view(np.uint32), want[:, 0, 0])
+    d2.sum().backward()
+    prim = idx.cpu().numpy()
+    G64, q64 = g.astype(np.float64), q.astype(np.float64)
+    cq, cg = J.gauss_adjoint(q64, G64, prim, np.ones(len(q)))
+    grads = {"q": qt.grad.cpu().numpy().astype(np.float64), "m": m.grad.cpu().numpy().astype(np.float64),
+             "r": r.grad.cpu().numpy().astype(np.float64), "s": s.grad.cpu().numpy().astype(np.float64)}
+    for name, got, ref in (("q", grads["q"], cq), ("m", grads["m"], cg[:, 0:3]), ("r", grads["r"], cg[:, 3:7]),
+                           ("s", grads["s"], cg[:, 7:10])):
+        err = rel(got, ref, 0.05 * rms(ref))
+        print(name, "cpu-adjoint max rel err", float(err.max()), flush=True)
+        assert err.max() < TOL, name
+
+    def loss_of(gx=G64, qx=q64):
+        return J.gauss_dist2_fixed(qx, gx, prim).sum()
+
+    rng = np.random.default_rng(9)
+    gflat = np.concatenate([grads["m"], grads["r"], grads["s"]], 1)
+    floor_g = {c: 0.05 * rms(gflat[:, c]) for c in range(10)}
+    for idx_ in rng.choice(G64.size, 768, replace=False):
+        e = rel(gflat.flat[idx_], fd(lambda x: loss_of(gx=x), G64, idx_), floor_g[idx_ % 10])
+        assert e < TOL, (idx_ % 10, gflat.flat[idx_])
+    for idx_ in rng.choice(q64.size, 256, replace=False):
+        e = rel(grads["q"].flat[idx_], fd(lambda x: loss_of(qx=x), q64, idx_), 0.05 * rms(grads["q"]))
+        assert e < TOL
+
+
+def test_point_gradients(AG, orc, scenes):
+    pts = scenes.point_cloud(5000, 3)
+    q = np.random.default_rng(10).uniform(-4, 68, (1500, 3)).astype(np.float32)
+    dev = torch.device("cuda")
+    ct = torch.from_numpy(pts).to(dev).requires_grad_(True)
+    qt = torch.from_numpy(q).to(dev).requires_grad_(True)
+    d2, idx = AG.point_closest_dist2(qt, ct)
+    want = orc.knn_brute(1, pts, q, 0, threads=THREADS)
+    assert np.array_equal(d2.detach().cpu().numpy().view(np.uint32), want[:, 0, 0])
+    (2.0 * d2).sum().backward()
+    prim = idx.cpu().numpy()
+    gq = qt.grad.cpu().numpy()
+    assert np.allclose(gq, 4.0 * (q - pts[prim]), rtol=1e-5, atol=1e-5)
+    gc = ct.grad.cpu().numpy()
+    ref = np.zeros_like(pts)
+    np.add.at(ref, prim, -4.0 * (q - pts[prim]))
+    assert np.allclose(gc, ref, rtol=1e-5, atol=1e-4)
+
+
+def test_gradcheck_small(AG):
+    """torch.autograd.gradcheck-style sanity on a tiny scene in float32: the
+    analytic Jacobian-vector products match directional differences."""
+    dev = torch.device("cuda")
+    V = torch.tensor([[0, 0, 0], [4, 0, 0], [0, 4, 0], [4, 4, 1]], dtype=torch.float32, device=dev,
+                     requires_grad=True)
+    F = torch.tensor([[0, 1, 2], [1, 3, 2]], dtype=torch.int32, device=dev)
+    q = torch.tensor([[1.0, 1.0, 2.0], [3.5, 3.0, -1.0], [-2.0, -1.0, 0.5]], device=dev, requires_grad=True)
+    d2, _ = AG.closest_point_dist2(q, V, F, order="input", wide=False)
+    d2.sum().backward()
+    # point above the first triangle's interior: gradient 2 (q - p) with p = (1, 1, 0)
+    assert torch.allclose(q.grad[0], torch.tensor([0.0, 0.0, 4.0], device=dev), atol=1e-6)
