@@ -219,6 +219,42 @@ QR_API int qr_closest_device(const qr_bvh* bvh, const float* points_dev, int n, 
 QR_API int qr_knn_device(const qr_bvh* bvh, const float* points_dev, int n, int k,
                          void* results_dev, int order, int wide, double* ms);
 
+/* ---- batched mode ----------------------------------------------------------------
+ * No reference counterpart (the reference builds and queries one scene per call,
+ * src/gpu/gpu_bvh.h:103-118, :276-342).  Thousands of independent scenes of one
+ * primitive kind are built with one segmented launch per stage, and queried with a
+ * persistent work-queue kernel replayed from a captured hipGraph.  Every tree and
+ * every result equals the plain per-scene path's (scene-local indices). */
+typedef struct qr_batch qr_batch;
+typedef struct qr_batch_plan qr_batch_plan;
+/* items: scene-major primitives; offsets_host[scenes + 1] (0 first, >= 2 per scene). */
+QR_API int qr_batch_build(int prim, const void* items_host, const int* offsets_host, int scenes,
+                          qr_build_timings* timings, qr_batch** out);
+QR_API int qr_batch_build_device(int prim, const void* items_dev, const int* offsets_host, int scenes,
+                                 qr_build_timings* timings, qr_batch** out);
+QR_API int qr_batch_destroy(qr_batch* b);
+QR_API int qr_batch_info(const qr_batch* b, int* scenes, int* total_prims, int* max_depth);
+/* One scene's node array (2 n_s - 1 nodes, local ids) and leaf slot -> local index. */
+QR_API int qr_batch_read_scene(const qr_batch* b, int scene, void* nodes_host, int* prim_ids_host);
+/* FNV-1a over one scene's node bytes: equals qr_bvh_tree_hash of that scene built alone. */
+QR_API int qr_batch_tree_hash(const qr_batch* b, int scene, uint64_t* out);
+/* Query plan over device points (scene-major, qoffsets_host[scenes + 1]); k = 0: closest,
+ * 1..32: kNN.  tune = 1 times every kernel variant (stack in LDS with 256 or 64 lanes
+ * per workgroup, or in scratch) with and without per-scene Morton order and keeps the
+ * fastest; tune = 0 takes variant 0 with Morton order; tune = 2 + 2 v + m forces
+ * variant v with Morton order m.  The chosen pipeline is captured in a hipGraph.
+ * The points must stay valid while the plan lives. */
+QR_API int qr_batch_plan_create(qr_batch* b, const float* points_dev, const int* qoffsets_host, int k,
+                                int tune, qr_batch_plan** out);
+/* Replays the graph; results stay in the plan (n * max(k, 1) records, scene-local ids). */
+QR_API int qr_batch_plan_run(qr_batch_plan* p, double* ms);
+/* The same pipeline launched directly on the plan's stream (no graph). */
+QR_API int qr_batch_plan_run_eager(qr_batch_plan* p, double* ms);
+QR_API int qr_batch_plan_read(const qr_batch_plan* p, void* results_host);
+/* variant: 0 LDS stack / 256 lanes, 1 LDS stack / 64 lanes, 2 scratch stack / 64 lanes. */
+QR_API int qr_batch_plan_config(const qr_batch_plan* p, int* variant, int* morton, double* tuned_ms);
+QR_API int qr_batch_plan_destroy(qr_batch_plan* p);
+
 /* ---- differentiable mode ------------------------------------------------------------
  * No reference counterpart: the reference is forward-only and its ray kernel
  * discards the hit index (src/gpu/raycast.cu:108-111).  All pointers are device

@@ -48,6 +48,34 @@ struct qr_csr {
     int total = 0;
 };
 
+struct qr_batch {
+    BatchState s;
+};
+
+// A prepared batched query: the query set, its scratch, the chosen kernel variant
+// and the captured graph (work-queue reset -> per-scene Morton order of the queries
+// -> persistent traversal) that each run replays.
+struct qr_batch_plan {
+    qr_batch* batch = nullptr;
+    const V3* points = nullptr;
+    int n = 0, k = 1, stride = 1, scenes = 0;
+    int variant = kBatchLds256;
+    bool morton = true;
+    double tunedMs = 0.0;
+    DevBuf<int> qOff, qScene, counter;
+    DevBuf<ClosestResult> results;
+    DevBuf<uint32_t> cb, codes, tmpA, tmpB, sceneSorted, order;
+    SortWorkspace ws;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~qr_batch_plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
 namespace qr {
 
 void hipCheck(hipError_t e, const char* what) {
@@ -767,6 +795,254 @@ int qr_adjoint_ray(qr_bvh* bvh, const void* rays, int n, const uint64_t* results
         adjointRay(bvh->s, rays, n, results, hit_prim, grad_t, grad_origins, grad_dirs, grad_tris,
                    currentStream());
     });
+}
+
+namespace {
+void enqueuePlan(qr_batch_plan& p, int variant, bool morton, hipStream_t s) {
+    const BatchState& b = p.batch->s;
+    launchBatchReset(p.counter.p, s);
+    if (morton)
+        batchMortonOrder(kPrimPoint, p.points, p.n, p.qScene.p, p.scenes, p.ws, p.cb.p, p.codes.p, p.tmpA.p,
+                         p.tmpB.p, p.sceneSorted.p, p.order.p, s);
+    BatchQueryArgs a;
+    a.kind = b.kind;
+    a.nodes = b.nodes.p;
+    a.items = b.items.p;
+    a.primIds = b.primIds.p;
+    a.primOff = b.primOff.p;
+    a.points = p.points;
+    a.qScene = p.qScene.p;
+    a.order = morton ? p.order.p : nullptr;
+    a.count = p.n;
+    a.k = p.k < 1 ? 1 : p.k;
+    a.outStride = p.stride;
+    a.out = p.results.p;
+    a.counter = p.counter.p;
+    launchBatchNearest(a, variant, s);
+}
+
+double timePlan(qr_batch_plan& p, int variant, bool morton) {
+    EventSet<2> ev;
+    double best = 1e30;
+    for (int r = 0; r < 2; ++r) {
+        ev.record(0, p.stream);
+        enqueuePlan(p, variant, morton, p.stream);
+        ev.record(1, p.stream);
+        ev.sync(1);
+        const double ms = ev.ms(0, 1);
+        best = ms < best ? ms : best;
+    }
+    return best;
+}
+}  // namespace
+
+int qr_batch_build_device(int prim, const void* items, const int* offsets, int scenes, qr_build_timings* t,
+                          qr_batch** out) {
+    return guard([&] {
+        need(out != nullptr, "null output handle");
+        need(prim >= 0 && prim <= 2, "unknown primitive kind");
+        need(scenes >= 1 && offsets != nullptr && items != nullptr, "empty batch");
+        need(scenes < 65536, "at most 65535 scenes per batch");
+        need(offsets[0] == 0, "offsets must start at 0");
+        for (int i = 0; i < scenes; ++i)
+            need(offsets[i + 1] - offsets[i] >= 2, "every scene needs at least 2 primitives");
+        auto* h = new qr_batch();
+        try {
+            BatchState& b = h->s;
+            b.scenes = scenes;
+            b.total = offsets[scenes];
+            b.primOffH.assign(offsets, offsets + scenes + 1);
+            b.primOff.alloc(static_cast<size_t>(scenes) + 1);
+            hipCheck(hipMemcpy(b.primOff.p, offsets, sizeof(int) * (static_cast<size_t>(scenes) + 1),
+                               hipMemcpyHostToDevice), "batch offsets");
+            BuildTimingsC bt{};
+            buildBatchDevice(b, prim, items, currentStream(), &bt);
+            if (t) {
+                *t = qr_build_timings{};
+                t->morton_ms = bt.mortonMs;
+                t->tree_ms = bt.treeMs;
+                t->refit_ms = bt.refitMs;
+                t->total_ms = bt.totalMs;
+            }
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int qr_batch_build(int prim, const void* items, const int* offsets, int scenes, qr_build_timings* t,
+                   qr_batch** out) {
+    return guard([&] {
+        need(prim >= 0 && prim <= 2, "unknown primitive kind");
+        need(scenes >= 1 && offsets != nullptr && items != nullptr, "empty batch");
+        const size_t bytes = static_cast<size_t>(offsets[scenes]) * itemSize(prim);
+        DevBuf<unsigned char> src(bytes);
+        hipCheck(hipMemcpy(src.p, items, bytes, hipMemcpyHostToDevice), "H2D batch items");
+        const int st = qr_batch_build_device(prim, src.p, offsets, scenes, t, out);
+        if (st != QR_OK) throw ArgError(tLastError);
+    });
+}
+
+int qr_batch_destroy(qr_batch* b) {
+    return guard([&] { delete b; });
+}
+
+int qr_batch_info(const qr_batch* b, int* scenes, int* total, int* maxDepth) {
+    return guard([&] {
+        need(b != nullptr, "null batch");
+        if (scenes) *scenes = b->s.scenes;
+        if (total) *total = b->s.total;
+        if (maxDepth) *maxDepth = b->s.maxDepth;
+    });
+}
+
+int qr_batch_read_scene(const qr_batch* b, int scene, void* nodes, int* primIds) {
+    return guard([&] {
+        need(b != nullptr, "null batch");
+        need(scene >= 0 && scene < b->s.scenes, "scene out of range");
+        const int ps = b->s.primOffH[static_cast<size_t>(scene)];
+        const int ns = b->s.primOffH[static_cast<size_t>(scene) + 1] - ps;
+        if (nodes)
+            hipCheck(hipMemcpy(nodes, b->s.nodes.p + (2 * ps - scene), sizeof(Node2) * static_cast<size_t>(2 * ns - 1),
+                               hipMemcpyDeviceToHost), "batch nodes D2H");
+        if (primIds)
+            hipCheck(hipMemcpy(primIds, b->s.primIds.p + ps, sizeof(int) * static_cast<size_t>(ns),
+                               hipMemcpyDeviceToHost), "batch ids D2H");
+    });
+}
+
+int qr_batch_tree_hash(const qr_batch* b, int scene, uint64_t* out) {
+    return guard([&] {
+        need(b != nullptr && out != nullptr, "null argument");
+        need(scene >= 0 && scene < b->s.scenes, "scene out of range");
+        const int ps = b->s.primOffH[static_cast<size_t>(scene)];
+        const int ns = b->s.primOffH[static_cast<size_t>(scene) + 1] - ps;
+        std::vector<unsigned char> bytes(sizeof(Node2) * static_cast<size_t>(2 * ns - 1));
+        hipCheck(hipMemcpy(bytes.data(), b->s.nodes.p + (2 * ps - scene), bytes.size(), hipMemcpyDeviceToHost),
+                 "batch nodes D2H");
+        uint64_t h = 0xcbf29ce484222325ull;
+        for (unsigned char c : bytes) {
+            h ^= static_cast<uint64_t>(c);
+            h *= 0x100000001b3ull;
+        }
+        *out = h;
+    });
+}
+
+int qr_batch_plan_create(qr_batch* b, const float* points, const int* qoffsets, int k, int tune,
+                         qr_batch_plan** out) {
+    return guard([&] {
+        need(b != nullptr && out != nullptr && qoffsets != nullptr, "null argument");
+        need(k >= 0 && k <= 32, "k must be in [0, 32] (0: closest)");
+        const int S = b->s.scenes;
+        need(qoffsets[0] == 0, "query offsets must start at 0");
+        for (int i = 0; i < S; ++i) need(qoffsets[i + 1] >= qoffsets[i], "query offsets must not decrease");
+        const int n = qoffsets[S];
+        need(n == 0 || points != nullptr, "null points");
+        auto* p = new qr_batch_plan();
+        try {
+            p->batch = b;
+            p->points = reinterpret_cast<const V3*>(points);
+            p->n = n;
+            p->k = k;
+            p->stride = k < 1 ? 1 : k;
+            p->scenes = S;
+            p->qOff.alloc(static_cast<size_t>(S) + 1);
+            hipCheck(hipMemcpy(p->qOff.p, qoffsets, sizeof(int) * (static_cast<size_t>(S) + 1), hipMemcpyHostToDevice),
+                     "query offsets");
+            p->qScene.alloc(static_cast<size_t>(n));
+            p->counter.alloc(1);
+            p->results.alloc(static_cast<size_t>(n) * p->stride);
+            p->cb.alloc(static_cast<size_t>(S) * 6);
+            for (DevBuf<uint32_t>* d : {&p->codes, &p->tmpA, &p->tmpB, &p->sceneSorted, &p->order})
+                d->alloc(static_cast<size_t>(n));
+            p->ws.reserve(static_cast<size_t>(n));
+            hipCheck(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "plan stream");
+            batchSceneIds(p->qOff.p, S, p->qScene.p, p->stream);
+            // warm every candidate once (occupancy, code objects), then choose
+            int bestV = kBatchLds256;
+            bool bestM = true;
+            double best = 1e30;
+            if (tune >= 2) {
+                need(tune < 2 + 2 * kBatchVariants, "unknown forced batch configuration");
+                bestV = (tune - 2) / 2;
+                bestM = ((tune - 2) % 2) != 0;
+                best = timePlan(*p, bestV, bestM);
+            } else if (tune == 1) {
+                for (int v = 0; v < kBatchVariants; ++v)
+                    for (int m = 0; m < 2; ++m) {
+                        const double ms = timePlan(*p, v, m != 0);
+                        if (ms < best) {
+                            best = ms;
+                            bestV = v;
+                            bestM = m != 0;
+                        }
+                    }
+            } else {
+                best = timePlan(*p, bestV, bestM);
+            }
+            p->variant = bestV;
+            p->morton = bestM;
+            p->tunedMs = best;
+            hipCheck(hipStreamSynchronize(p->stream), "plan sync");
+            hipCheck(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+            enqueuePlan(*p, p->variant, p->morton, p->stream);
+            hipCheck(hipStreamEndCapture(p->stream, &p->graph), "end capture");
+            hipCheck(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0), "graph instantiate");
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+int qr_batch_plan_run(qr_batch_plan* p, double* ms) {
+    return guard([&] {
+        need(p != nullptr && p->exec != nullptr, "null plan");
+        EventSet<2> ev;
+        ev.record(0, p->stream);
+        hipCheck(hipGraphLaunch(p->exec, p->stream), "graph launch");
+        ev.record(1, p->stream);
+        ev.sync(1);
+        if (ms) *ms = ev.ms(0, 1);
+    });
+}
+
+int qr_batch_plan_run_eager(qr_batch_plan* p, double* ms) {
+    return guard([&] {
+        need(p != nullptr, "null plan");
+        EventSet<2> ev;
+        ev.record(0, p->stream);
+        enqueuePlan(*p, p->variant, p->morton, p->stream);
+        ev.record(1, p->stream);
+        ev.sync(1);
+        if (ms) *ms = ev.ms(0, 1);
+    });
+}
+
+int qr_batch_plan_read(const qr_batch_plan* p, void* host) {
+    return guard([&] {
+        need(p != nullptr && host != nullptr, "null argument");
+        if (p->n > 0)
+            hipCheck(hipMemcpy(host, p->results.p, sizeof(ClosestResult) * static_cast<size_t>(p->n) * p->stride,
+                               hipMemcpyDeviceToHost), "plan results D2H");
+    });
+}
+
+int qr_batch_plan_config(const qr_batch_plan* p, int* variant, int* morton, double* tunedMs) {
+    return guard([&] {
+        need(p != nullptr, "null plan");
+        if (variant) *variant = p->variant;
+        if (morton) *morton = p->morton ? 1 : 0;
+        if (tunedMs) *tunedMs = p->tunedMs;
+    });
+}
+
+int qr_batch_plan_destroy(qr_batch_plan* p) {
+    return guard([&] { delete p; });
 }
 
 int qr_rays_summarize(const qr_rays* r, uint64_t* perRay, qr_ray_stats* out) {

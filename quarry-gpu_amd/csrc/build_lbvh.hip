@@ -172,13 +172,12 @@ __device__ __forceinline__ int prefixLen(const uint32_t* __restrict__ code,
     return x == 0u ? 64 : 32 + __clz(x);
 }
 
-__global__ __launch_bounds__(kBlock) void karrasKernel(const uint32_t* __restrict__ code,
-                                                       const uint32_t* __restrict__ idx, int n,
-                                                       Node2* __restrict__ nodes,
-                                                       int* __restrict__ parent,
-                                                       int* __restrict__ arrive) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n - 1) return;
+// Karras node i of one tree whose sorted keys are (code[j] << 32 | idx[j]), j < n.
+// Node ids, children and parent[] are local to the tree (root 0, leaf n-1+slot).
+__device__ __forceinline__ void karrasNode(const uint32_t* __restrict__ code,
+                                           const uint32_t* __restrict__ idx, int n, int i,
+                                           Node2* __restrict__ nodes, int* __restrict__ parent,
+                                           int* __restrict__ arrive) {
     arrive[i] = 0;
     if (i == 0) parent[0] = -1;
 
@@ -219,6 +218,16 @@ __global__ __launch_bounds__(kBlock) void karrasKernel(const uint32_t* __restric
     parent[right] = i;
 }
 
+__global__ __launch_bounds__(kBlock) void karrasKernel(const uint32_t* __restrict__ code,
+                                                       const uint32_t* __restrict__ idx, int n,
+                                                       Node2* __restrict__ nodes,
+                                                       int* __restrict__ parent,
+                                                       int* __restrict__ arrive) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    karrasNode(code, idx, n, i, nodes, parent, arrive);
+}
+
 // ---- leaf scatter + bottom-up propagation -------------------------------------------
 // One thread per leaf slot k: gather the item into leaf order, write leaf node
 // n-1+k, then climb.  At each parent the arrival counter decides: the first child
@@ -227,13 +236,12 @@ __global__ __launch_bounds__(kBlock) void karrasKernel(const uint32_t* __restric
 // (edges to the deepest leaf) rides along, so the root's height is the tree
 // depth.  Counters start at zero (karrasKernel) and end at zero.
 template <typename L, bool kGather>
-__global__ __launch_bounds__(kBlock) void leafUpKernel(
-    const typename L::Item* __restrict__ src, const uint32_t* __restrict__ order, int n,
-    typename L::Item* __restrict__ sorted, int* __restrict__ primIds, Node2* __restrict__ nodes,
-    const int* __restrict__ parent, int* __restrict__ arrive, int* __restrict__ height) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= n) return;
-
+__device__ __forceinline__ void leafClimb(const typename L::Item* __restrict__ src,
+                                          const uint32_t* __restrict__ order, int n, int k,
+                                          typename L::Item* __restrict__ sorted,
+                                          int* __restrict__ primIds, Node2* __restrict__ nodes,
+                                          const int* __restrict__ parent, int* __restrict__ arrive,
+                                          int* __restrict__ height) {
     typename L::Item item;
     if (kGather) {
         const int sceneIdx = static_cast<int>(order[k]);
@@ -272,6 +280,166 @@ __global__ __launch_bounds__(kBlock) void leafUpKernel(
         stAgentI(&arrive[p], 0);
         node = p;
     }
+}
+
+template <typename L, bool kGather>
+__global__ __launch_bounds__(kBlock) void leafUpKernel(
+    const typename L::Item* __restrict__ src, const uint32_t* __restrict__ order, int n,
+    typename L::Item* __restrict__ sorted, int* __restrict__ primIds, Node2* __restrict__ nodes,
+    const int* __restrict__ parent, int* __restrict__ arrive, int* __restrict__ height) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    leafClimb<L, kGather>(src, order, n, k, sorted, primIds, nodes, parent, arrive, height);
+}
+
+// ---- batched (segmented) LBVH: many independent scenes per launch -----------------------
+// Scene s owns items [primOff[s], primOff[s+1]) (n_s >= 2), nodes from 2 primOff[s] - s
+// (2 n_s - 1 of them) and internal-node slots (arrival counters, heights) from
+// primOff[s] - s.  Each tree is built by the plain build's own device code on local
+// ids through offset pointers, so it equals the plain build of that scene alone,
+// byte for byte.  The per-scene Morton order is a least-significant-digit sort:
+// first by code, then (stably) by scene id.
+
+__global__ void batchSceneIdKernel(const int* __restrict__ primOff, int* __restrict__ sceneId) {
+    const int sc = blockIdx.x;
+    for (int i = primOff[sc] + threadIdx.x; i < primOff[sc + 1]; i += kBlock) sceneId[i] = sc;
+}
+
+__global__ void batchBoundsInitKernel(uint32_t* __restrict__ cb, int scenes) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= scenes * 6) return;
+    cb[i] = (i % 6) < 3 ? orderedBits(kInf) : orderedBits(-kInf);
+}
+
+// Per-scene centroid bounds.  A wave whose 64 items all belong to one scene reduces
+// in registers and issues 6 atomics; mixed waves fall back to per-lane atomics.
+template <typename L>
+__global__ __launch_bounds__(kBlock) void batchBoundsKernel(const typename L::Item* __restrict__ items, int n,
+                                                            const int* __restrict__ sceneId,
+                                                            uint32_t* __restrict__ cb) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < n;
+    const int s0 = __shfl(live ? sceneId[i] : -1, 0, 64);
+    int sc = live ? sceneId[i] : s0;
+    V3 c{kInf, kInf, kInf};
+    V3 ch{-kInf, -kInf, -kInf};
+    if (live) {
+        c = L::centroid(items[i]);
+        ch = c;
+    }
+    if (__all(sc == s0) && s0 >= 0) {
+        float v[6] = {c.x, c.y, c.z, ch.x, ch.y, ch.z};
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                v[k] = fminf(v[k], __shfl_xor(v[k], off, 64));
+                v[3 + k] = fmaxf(v[3 + k], __shfl_xor(v[3 + k], off, 64));
+            }
+        }
+        if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(&cb[s0 * 6 + k], orderedBits(v[k]));
+                atomicMax(&cb[s0 * 6 + 3 + k], orderedBits(v[3 + k]));
+            }
+        }
+    } else if (live) {
+        atomicMin(&cb[sc * 6 + 0], orderedBits(c.x));
+        atomicMin(&cb[sc * 6 + 1], orderedBits(c.y));
+        atomicMin(&cb[sc * 6 + 2], orderedBits(c.z));
+        atomicMax(&cb[sc * 6 + 3], orderedBits(c.x));
+        atomicMax(&cb[sc * 6 + 4], orderedBits(c.y));
+        atomicMax(&cb[sc * 6 + 5], orderedBits(c.z));
+    }
+}
+
+template <typename L>
+__global__ __launch_bounds__(kBlock) void batchMortonKernel(const typename L::Item* __restrict__ items, int n,
+                                                            const int* __restrict__ sceneId,
+                                                            const uint32_t* __restrict__ cb,
+                                                            uint32_t* __restrict__ codes,
+                                                            uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[4][256];
+    for (int i = threadIdx.x; i < 4 * 256; i += kBlock) (&h[0][0])[i] = 0u;
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const int sc = sceneId[i];
+        const uint32_t* b = cb + sc * 6;
+        const float lo0 = fromOrderedBits(b[0]), lo1 = fromOrderedBits(b[1]), lo2 = fromOrderedBits(b[2]);
+        const float ex0 = fromOrderedBits(b[3]) - lo0, ex1 = fromOrderedBits(b[4]) - lo1,
+                    ex2 = fromOrderedBits(b[5]) - lo2;
+        const V3 c = L::centroid(items[i]);
+        const uint32_t code = spread10(quantize1023(c.x, lo0, ex0)) |
+                              (spread10(quantize1023(c.y, lo1, ex1)) << 1) |
+                              (spread10(quantize1023(c.z, lo2, ex2)) << 2);
+        codes[i] = code;
+        atomicAdd(&h[0][code & 255u], 1u);
+        atomicAdd(&h[1][(code >> 8) & 255u], 1u);
+        atomicAdd(&h[2][(code >> 16) & 255u], 1u);
+        atomicAdd(&h[3][code >> 24], 1u);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < 4 * 256; j += kBlock) {
+        const uint32_t v = (&h[0][0])[j];
+        if (v) atomicAdd(&hist[j], v);
+    }
+}
+
+__global__ void batchGatherKernel(const uint32_t* __restrict__ idx, const int* __restrict__ src, int n,
+                                  uint32_t* __restrict__ dst) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[i] = static_cast<uint32_t>(src[idx[i]]);
+}
+
+__global__ void batchLocalKernel(const uint32_t* __restrict__ order2, const uint32_t* __restrict__ s2,
+                                 const uint32_t* __restrict__ codes, const int* __restrict__ primOff, int n,
+                                 uint32_t* __restrict__ codes2, uint32_t* __restrict__ idxLocal) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t o = order2[i];
+    codes2[i] = codes[o];
+    idxLocal[i] = o - static_cast<uint32_t>(primOff[s2[i]]);
+}
+
+__global__ __launch_bounds__(kBlock) void batchKarrasKernel(const uint32_t* __restrict__ codes2,
+                                                            const uint32_t* __restrict__ idxLocal,
+                                                            const uint32_t* __restrict__ s2,
+                                                            const int* __restrict__ primOff, int n,
+                                                            Node2* __restrict__ nodes, int* __restrict__ parent,
+                                                            int* __restrict__ arrive) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int sc = static_cast<int>(s2[i]);
+    const int ps = primOff[sc], ns = primOff[sc + 1] - ps;
+    const int il = i - ps;
+    if (il >= ns - 1) return;
+    karrasNode(codes2 + ps, idxLocal + ps, ns, il, nodes + (2 * ps - sc), parent + (2 * ps - sc),
+               arrive + (ps - sc));
+}
+
+template <typename L>
+__global__ __launch_bounds__(kBlock) void batchLeafKernel(const typename L::Item* __restrict__ items,
+                                                          const uint32_t* __restrict__ idxLocal,
+                                                          const uint32_t* __restrict__ s2,
+                                                          const int* __restrict__ primOff, int n,
+                                                          typename L::Item* __restrict__ sorted,
+                                                          int* __restrict__ primIds, Node2* __restrict__ nodes,
+                                                          const int* __restrict__ parent, int* __restrict__ arrive,
+                                                          int* __restrict__ height) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const int sc = static_cast<int>(s2[k]);
+    const int ps = primOff[sc], ns = primOff[sc + 1] - ps;
+    leafClimb<L, true>(items + ps, idxLocal + ps, ns, k - ps, sorted + ps, primIds + ps, nodes + (2 * ps - sc),
+                       parent + (2 * ps - sc), arrive + (ps - sc), height + (ps - sc));
+}
+
+__global__ void batchDepthKernel(const int* __restrict__ height, const int* __restrict__ primOff, int scenes,
+                                 int* __restrict__ depth) {
+    const int sc = blockIdx.x * kBlock + threadIdx.x;
+    if (sc < scenes) depth[sc] = height[primOff[sc] - sc];
 }
 
 }  // namespace
@@ -469,6 +637,113 @@ void computeDepth(BvhState& b, hipStream_t s) {
     hipCheck(hipGetLastError(), "depth");
     hipCheck(hipStreamSynchronize(s), "depth sync");
     b.depthValid = true;
+}
+
+
+// ---- batched build ---------------------------------------------------------------------------
+
+template <typename L>
+static void batchOrderT(const typename L::Item* items, int n, const int* sceneId, int scenes, SortWorkspace& ws,
+                        uint32_t* cb, uint32_t* codes, uint32_t* tmpA, uint32_t* tmpB, uint32_t* s2,
+                        uint32_t* order2, hipStream_t s) {
+    batchBoundsInitKernel<<<blocksFor(static_cast<long long>(scenes) * 6, kBlock), kBlock, 0, s>>>(cb, scenes);
+    batchBoundsKernel<L><<<blocksFor(n, kBlock), kBlock, 0, s>>>(items, n, sceneId, cb);
+    sortClear(ws, static_cast<size_t>(n), 4, s);
+    batchMortonKernel<L><<<blocksFor(n, kBlock), kBlock, 0, s>>>(items, n, sceneId, cb, codes, ws.hist.p);
+    hipCheck(hipGetLastError(), "batch morton");
+    // by code: tmpA = sorted codes (unused), tmpB = order1
+    sortPairs(ws, codes, nullptr, tmpA, tmpB, static_cast<size_t>(n), 4, s);
+    // then stably by scene: keys s1 = scene of order1
+    batchGatherKernel<<<blocksFor(n, kBlock), kBlock, 0, s>>>(tmpB, sceneId, n, tmpA);
+    sortClear(ws, static_cast<size_t>(n), 2, s);
+    sortHistogram(ws, tmpA, static_cast<size_t>(n), 2, s);
+    sortPairs(ws, tmpA, tmpB, s2, order2, static_cast<size_t>(n), 2, s);
+}
+
+void batchMortonOrder(int kind, const void* items, int n, const int* sceneId, int scenes, SortWorkspace& ws,
+                      uint32_t* cb, uint32_t* codes, uint32_t* tmpA, uint32_t* tmpB, uint32_t* sceneSorted,
+                      uint32_t* order, hipStream_t s) {
+    if (n <= 0) return;
+    switch (kind) {
+        case kPrimTri:
+            batchOrderT<TriLeaf>(static_cast<const Tri*>(items), n, sceneId, scenes, ws, cb, codes, tmpA, tmpB,
+                                 sceneSorted, order, s);
+            break;
+        case kPrimPoint:
+            batchOrderT<PointLeaf>(static_cast<const V3*>(items), n, sceneId, scenes, ws, cb, codes, tmpA, tmpB,
+                                   sceneSorted, order, s);
+            break;
+        default:
+            batchOrderT<GaussLeaf>(static_cast<const Gaussian*>(items), n, sceneId, scenes, ws, cb, codes, tmpA,
+                                   tmpB, sceneSorted, order, s);
+            break;
+    }
+}
+
+void batchSceneIds(const int* primOff, int scenes, int* sceneId, hipStream_t s) {
+    if (scenes <= 0) return;
+    batchSceneIdKernel<<<scenes, kBlock, 0, s>>>(primOff, sceneId);
+    hipCheck(hipGetLastError(), "batch scene ids");
+}
+
+template <typename L>
+static void buildBatchT(BatchState& b, const typename L::Item* dSrc, hipStream_t s, BuildTimingsC* t) {
+    using Item = typename L::Item;
+    const int n = b.total, S = b.scenes;
+    b.nodes.alloc(static_cast<size_t>(2 * n - S));
+    b.items.alloc(static_cast<size_t>(n) * sizeof(Item));
+    b.primIds.alloc(static_cast<size_t>(n));
+    b.parent.alloc(static_cast<size_t>(2 * n - S));
+    b.arrive.alloc(static_cast<size_t>(n - S));
+    b.height.alloc(static_cast<size_t>(n - S));
+    b.depth.alloc(static_cast<size_t>(S));
+    b.sceneId.alloc(static_cast<size_t>(n));
+    b.ws.reserve(static_cast<size_t>(n));
+    DevBuf<uint32_t> cb(static_cast<size_t>(S) * 6), codes(n), tmpA(n), tmpB(n), s2(n), order2(n), codes2(n),
+        idxLocal(n);
+
+    EventSet<5> ev;
+    ev.record(0, s);
+    batchSceneIds(b.primOff.p, S, b.sceneId.p, s);
+    batchOrderT<L>(dSrc, n, b.sceneId.p, S, b.ws, cb.p, codes.p, tmpA.p, tmpB.p, s2.p, order2.p, s);
+    ev.record(1, s);
+    batchLocalKernel<<<blocksFor(n, kBlock), kBlock, 0, s>>>(order2.p, s2.p, codes.p, b.primOff.p, n, codes2.p,
+                                                             idxLocal.p);
+    batchKarrasKernel<<<blocksFor(n, kBlock), kBlock, 0, s>>>(codes2.p, idxLocal.p, s2.p, b.primOff.p, n,
+                                                              b.nodes.p, b.parent.p, b.arrive.p);
+    hipCheck(hipGetLastError(), "batch karras");
+    ev.record(2, s);
+    batchLeafKernel<L><<<blocksFor(n, kBlock), kBlock, 0, s>>>(dSrc, idxLocal.p, s2.p, b.primOff.p, n,
+                                                               reinterpret_cast<Item*>(b.items.p), b.primIds.p,
+                                                               b.nodes.p, b.parent.p, b.arrive.p, b.height.p);
+    hipCheck(hipGetLastError(), "batch leaf up");
+    ev.record(3, s);
+    batchDepthKernel<<<blocksFor(S, kBlock), kBlock, 0, s>>>(b.height.p, b.primOff.p, S, b.depth.p);
+    std::vector<int> d(static_cast<size_t>(S));
+    hipCheck(hipMemcpyAsync(d.data(), b.depth.p, sizeof(int) * d.size(), hipMemcpyDeviceToHost, s), "batch depth");
+    ev.record(4, s);
+    ev.sync(4);
+    b.maxDepth = 0;
+    for (int v : d) b.maxDepth = v > b.maxDepth ? v : b.maxDepth;
+    if (t) {
+        t->mortonMs = ev.ms(0, 1);
+        t->sortMs = 0.0;
+        t->treeMs = ev.ms(1, 2);
+        t->scatterMs = 0.0;
+        t->refitMs = ev.ms(2, 3);
+        t->sahBinMs = t->sahSplitMs = t->sahPartMs = 0.0;
+        t->totalMs = ev.ms(0, 3);
+    }
+}
+
+void buildBatchDevice(BatchState& b, int kind, const void* dItems, hipStream_t s, BuildTimingsC* t) {
+    b.kind = kind;
+    switch (kind) {
+        case kPrimTri: buildBatchT<TriLeaf>(b, static_cast<const Tri*>(dItems), s, t); break;
+        case kPrimPoint: buildBatchT<PointLeaf>(b, static_cast<const V3*>(dItems), s, t); break;
+        case kPrimGauss: buildBatchT<GaussLeaf>(b, static_cast<const Gaussian*>(dItems), s, t); break;
+        default: throw ArgError("batch build: unknown primitive kind");
+    }
 }
 
 }  // namespace qr

@@ -1,6 +1,8 @@
 // The state behind an opaque qr_bvh handle and the build entry points.
 #pragma once
 
+#include <vector>
+
 #include "qr_internal.h"
 
 namespace qr {
@@ -54,6 +56,35 @@ struct BvhState {
     // scene-order primitives kept for validate / refit-by-device-pointer paths
     SortWorkspace sortWs;
 };
+
+// Many independent trees built at once (the batched mode).  Scene s owns items
+// [primOff[s], primOff[s+1]), nodes from 2 primOff[s] - s, internal slots from
+// primOff[s] - s; node ids, leaf slots and prim ids are local to the scene.
+struct BatchState {
+    int kind = kPrimTri;
+    int scenes = 0;
+    int total = 0;
+    int maxDepth = 0;
+    std::vector<int> primOffH;
+    DevBuf<int> primOff;             // scenes + 1
+    DevBuf<Node2> nodes;             // 2 total - scenes
+    DevBuf<unsigned char> items;     // leaf order per scene
+    DevBuf<int> primIds;             // per slot: scene-local index
+    DevBuf<int> parent, arrive, height, depth, sceneId;
+    SortWorkspace ws;
+};
+
+// Segmented LBVH of all scenes; b.primOff / primOffH / scenes / total must be set.
+void buildBatchDevice(BatchState& b, int kind, const void* dItems, hipStream_t s, BuildTimingsC* t);
+// sceneId[i] = scene of item i, for scene-major items with offsets primOff.
+void batchSceneIds(const int* primOff, int scenes, int* sceneId, hipStream_t s);
+// Stable order of items (any primitive kind; query points use kPrimPoint) by
+// (scene, Morton code of the centroid within its scene's centroid bounds, index).
+// cb: 6 x scenes words; codes, tmpA, tmpB: n words of scratch; outputs
+// sceneSorted[i] = scene of the i-th item in that order and order[i] = its index.
+void batchMortonOrder(int kind, const void* items, int n, const int* sceneId, int scenes, SortWorkspace& ws,
+                      uint32_t* cb, uint32_t* codes, uint32_t* tmpA, uint32_t* tmpB, uint32_t* sceneSorted,
+                      uint32_t* order, hipStream_t s);
 
 // Builds into an empty state from scene-order items already on the device.
 void buildLbvhDevice(BvhState& b, int kind, const void* dItems, int n, hipStream_t s,

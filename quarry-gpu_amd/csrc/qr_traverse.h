@@ -51,4 +51,25 @@ void launchRadiusFill(const TraverseArgs& a, const V3* pts, int count, float r2,
 void segmentSort(const int* offsets, uint64_t* keys, int count, int maxSeg, int* scratchInts,
                  hipStream_t s);
 
+// ---- batched mode: persistent work-queue traversal over many trees ------------
+struct BatchQueryArgs {
+    int kind = 0;
+    const Node2* nodes = nullptr;   // all trees (scene s from 2 primOff[s] - s)
+    const void* items = nullptr;    // leaf-order items of all trees
+    const int* primIds = nullptr;   // per slot: scene-local index
+    const int* primOff = nullptr;   // scenes + 1
+    const V3* points = nullptr;     // queries, scene-major
+    const int* qScene = nullptr;    // scene of each query
+    const uint32_t* order = nullptr;  // optional processing order
+    int count = 0;
+    int k = 1;                      // 1: closest; up to 32
+    int outStride = 1;
+    ClosestResult* out = nullptr;   // per query: outStride records, scene-local indices
+    int* counter = nullptr;         // work-queue head (reset before each launch)
+};
+// Stack placement / workgroup size variants the autotuner chooses from.
+enum BatchVariant : int { kBatchLds256 = 0, kBatchLds64 = 1, kBatchScratch64 = 2, kBatchVariants = 3 };
+void launchBatchNearest(const BatchQueryArgs& a, int variant, hipStream_t s);
+void launchBatchReset(int* counter, hipStream_t s);
+
 }  // namespace qr

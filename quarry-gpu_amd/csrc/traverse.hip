@@ -81,7 +81,10 @@ __device__ __forceinline__ void flushStats(unsigned long long* stats, int nodeTe
 typedef __attribute__((address_space(3))) volatile int LdsInt;      // workgroup LDS
 typedef __attribute__((address_space(5))) volatile int PrivateInt;  // per-lane scratch
 
-struct LaneStack {
+template <int kS, int kD>
+struct LaneStackT {
+    static constexpr int kStride = kS;  // lanes per workgroup: distance between a lane's entries
+    static constexpr int kDepth = kD;   // entries kept in LDS before spilling to scratch
     LdsInt* base;  // this lane's column of the workgroup's LDS stack
     int sp;
     __device__ __forceinline__ void init(int* lds) {
@@ -89,8 +92,10 @@ struct LaneStack {
         sp = 0;
     }
     __device__ __forceinline__ bool empty() const { return sp == 0; }
-    __device__ __forceinline__ bool room(int n) const { return sp + n <= kLdsDepth + kSpill; }
+    __device__ __forceinline__ bool room(int n) const { return sp + n <= kD + kSpill; }
 };
+using LaneStack = LaneStackT<kBlock, kLdsDepth>;
+
 struct SpillArea {
     int v[kSpill];
     __device__ __forceinline__ PrivateInt* at(int i) {
@@ -101,10 +106,10 @@ struct SpillArea {
 #define QR_PUSH(st, spill, val)                               \
     do {                                                      \
         const int qrv_ = (val);                               \
-        if ((st).sp < kLdsDepth) {                            \
-            (st).base[(st).sp * kBlock] = qrv_;               \
+        if ((st).sp < (st).kDepth) {                          \
+            (st).base[(st).sp * (st).kStride] = qrv_;         \
         } else {                                              \
-            *(spill).at((st).sp - kLdsDepth) = qrv_;          \
+            *(spill).at((st).sp - (st).kDepth) = qrv_;        \
         }                                                     \
         ++(st).sp;                                            \
     } while (0)
@@ -112,10 +117,10 @@ struct SpillArea {
 #define QR_POP(st, spill, out)                                \
     do {                                                      \
         --(st).sp;                                            \
-        if ((st).sp < kLdsDepth) {                            \
-            (out) = (st).base[(st).sp * kBlock];              \
+        if ((st).sp < (st).kDepth) {                          \
+            (out) = (st).base[(st).sp * (st).kStride];        \
         } else {                                              \
-            (out) = *(spill).at((st).sp - kLdsDepth);         \
+            (out) = *(spill).at((st).sp - (st).kDepth);       \
         }                                                     \
     } while (0)
 
@@ -401,6 +406,65 @@ __device__ __forceinline__ void writeList(const KnnList<K>& list, int k, Closest
     }
 }
 
+// Closest / kNN walk of one query over a binary tree whose node ids, leaf slots
+// and prim ids are local to `nodes`, `prims` and `primIds` (a plain tree, or one
+// tree of a batch addressed through offset pointers).
+template <typename L, int K, bool kStats, typename Stack>
+__device__ __forceinline__ void nearestBinaryWalk(const Node2* __restrict__ nodes,
+                                                  const typename L::Item* __restrict__ prims,
+                                                  const int* __restrict__ primIds, V3 p,
+                                                  KnnList<K>& list, Stack& st, SpillArea& spill,
+                                                  int& nodeTests, int& leafTests, int& overflow,
+                                                  int& opened) {
+    int id = 0;
+    while (id >= 0) {
+        if (kStats) ++opened;
+        const Node2 nd = nodes[id];
+        ++nodeTests;
+        const int kid[2] = {nd.left, nd.right};
+        float key[2];
+        int c[2];
+        bool leaf[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const Node2 cn = nodes[kid[s]];
+            leaf[s] = cn.left < 0;
+            c[s] = leaf[s] ? -cn.left - 1 : kid[s];
+            key[s] = boxDist2(p, nodeBox(cn));
+        }
+        const bool swap = key[1] < key[0];
+        const int c0 = swap ? c[1] : c[0], c1 = swap ? c[0] : c[1];
+        const bool l0 = swap ? leaf[1] : leaf[0], l1 = swap ? leaf[0] : leaf[1];
+        const float k0 = swap ? key[1] : key[0], k1 = swap ? key[0] : key[1];
+        if (l0 && list.admits(k0)) {
+            ++leafTests;
+            list.offer(L::dist2(p, prims[c0]), primIds[c0]);
+        }
+        if (l1 && list.admits(k1)) {
+            ++leafTests;
+            list.offer(L::dist2(p, prims[c1]), primIds[c1]);
+        }
+        const bool in0 = !l0 && list.admits(k0);
+        const bool in1 = !l1 && list.admits(k1);
+        if (in0) {
+            if (in1) {
+                if (!st.room(1)) {
+                    overflow = 1;
+                    break;
+                }
+                QR_PUSH(st, spill, c1);
+            }
+            id = c0;
+        } else if (in1) {
+            id = c1;
+        } else if (!st.empty()) {
+            QR_POP(st, spill, id);
+        } else {
+            id = -1;
+        }
+    }
+}
+
 template <typename L, int K, bool kStats>
 __global__ __launch_bounds__(kBlock) void nearestBinaryKernel(
     const Node2* __restrict__ nodes, const typename L::Item* __restrict__ prims,
@@ -412,62 +476,57 @@ __global__ __launch_bounds__(kBlock) void nearestBinaryKernel(
     int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
     if (i < count) {
         const int q = order ? static_cast<int>(order[i]) : i;
-        const V3 p = points[q];
         KnnList<K> list;
         list.init();
         LaneStack st;
         SpillArea spill;
         st.init(lds);
-        int id = 0;
-        while (id >= 0) {
-            if (kStats) ++opened;
-            const Node2 nd = nodes[id];
-            ++nodeTests;
-            const int kid[2] = {nd.left, nd.right};
-            float key[2];
-            int c[2];
-            bool leaf[2];
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                const Node2 cn = nodes[kid[s]];
-                leaf[s] = cn.left < 0;
-                c[s] = leaf[s] ? -cn.left - 1 : kid[s];
-                key[s] = boxDist2(p, nodeBox(cn));
-            }
-            const bool swap = key[1] < key[0];
-            const int c0 = swap ? c[1] : c[0], c1 = swap ? c[0] : c[1];
-            const bool l0 = swap ? leaf[1] : leaf[0], l1 = swap ? leaf[0] : leaf[1];
-            const float k0 = swap ? key[1] : key[0], k1 = swap ? key[0] : key[1];
-            if (l0 && list.admits(k0)) {
-                ++leafTests;
-                list.offer(L::dist2(p, prims[c0]), primIds[c0]);
-            }
-            if (l1 && list.admits(k1)) {
-                ++leafTests;
-                list.offer(L::dist2(p, prims[c1]), primIds[c1]);
-            }
-            const bool in0 = !l0 && list.admits(k0);
-            const bool in1 = !l1 && list.admits(k1);
-            if (in0) {
-                if (in1) {
-                    if (!st.room(1)) {
-                        overflow = 1;
-                        break;
-                    }
-                    QR_PUSH(st, spill, c1);
-                }
-                id = c0;
-            } else if (in1) {
-                id = c1;
-            } else if (!st.empty()) {
-                QR_POP(st, spill, id);
-            } else {
-                id = -1;
-            }
-        }
+        nearestBinaryWalk<L, K, kStats>(nodes, prims, primIds, points[q], list, st, spill, nodeTests,
+                                        leafTests, overflow, opened);
         writeList<L, K>(list, k, out + static_cast<size_t>(q) * outStride);
     }
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
+}
+
+// ---- batched scenes: persistent work-queue traversal ------------------------------------
+// Queries of all scenes form one queue (scene-major, optionally Morton-ordered inside
+// each scene).  Each wave takes the next 64 queries with one atomic and walks, per
+// lane, the tree of that query's scene (scene s: items from primOff[s], nodes from
+// 2 primOff[s] - s); the grid is sized to the resident occupancy and every wave
+// leaves when the queue is exhausted.  kD = 0 keeps the whole stack in scratch.
+template <typename L, int K, int kBT, int kD>
+__global__ __launch_bounds__(kBT) void batchNearestKernel(
+    const Node2* __restrict__ nodes, const typename L::Item* __restrict__ prims,
+    const int* __restrict__ primIds, const int* __restrict__ primOff, const V3* __restrict__ points,
+    const int* __restrict__ qScene, const uint32_t* __restrict__ order, int count, int k, int outStride,
+    ClosestResult* __restrict__ out, int* __restrict__ counter) {
+    __shared__ int lds[(kD > 0 ? kD : 1) * kBT];
+    const int lane = threadIdx.x & 63;
+    while (true) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(counter, 64);
+        base = __shfl(base, 0, 64);
+        if (base >= count) break;
+        const int i = base + lane;
+        if (i < count) {
+            const int q = order ? static_cast<int>(order[i]) : i;
+            const int s = qScene[q];
+            const int ps = primOff[s];
+            KnnList<K> list;
+            list.init();
+            LaneStackT<kBT, kD> st;
+            SpillArea spill;
+            st.init(lds);
+            int a = 0, b = 0, c = 0, d = 0;
+            nearestBinaryWalk<L, K, false>(nodes + (2 * ps - s), prims + ps, primIds + ps, points[q], list,
+                                           st, spill, a, b, c, d);
+            writeList<L, K>(list, k, out + static_cast<size_t>(q) * outStride);
+        }
+    }
+}
+
+__global__ void batchResetKernel(int* __restrict__ counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
 }
 
 template <typename L, int K, bool kStats>
@@ -965,6 +1024,79 @@ void segmentSort(const int* offsets, uint64_t* keys, int count, int maxSeg, int*
     if (h[1] > 0) {
         throw ArgError("radius: a query matched more than 8192 primitives; reduce the radius");
     }
+}
+
+
+// ---- batched traversal launchers ---------------------------------------------------------
+
+namespace {
+
+int residentCus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipCheck(hipGetDevice(&dev), "get device");
+        hipDeviceProp_t prop{};
+        hipCheck(hipGetDeviceProperties(&prop, dev), "device properties");
+        cus = prop.multiProcessorCount;
+    }
+    return cus;
+}
+
+template <typename L, int K, int kBT, int kD>
+void launchBatchTKV(const BatchQueryArgs& a, hipStream_t s) {
+    auto kern = batchNearestKernel<L, K, kBT, kD>;
+    static int perCu = 0;
+    if (perCu == 0) {
+        hipCheck(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void*>(kern), kBT, 0),
+                 "occupancy");
+        if (perCu < 1) perCu = 1;
+    }
+    const int waves = (a.count + 63) / 64;
+    int grid = perCu * residentCus();
+    const int need = (waves * 64 + kBT - 1) / kBT;
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+    kern<<<grid, kBT, 0, s>>>(a.nodes, static_cast<const typename L::Item*>(a.items), a.primIds, a.primOff,
+                              a.points, a.qScene, a.order, a.count, a.k, a.outStride, a.out, a.counter);
+}
+
+template <typename L, int K>
+void launchBatchTK(const BatchQueryArgs& a, int variant, hipStream_t s) {
+    switch (variant) {
+        case kBatchLds256: launchBatchTKV<L, K, 256, kLdsDepth>(a, s); break;
+        case kBatchLds64: launchBatchTKV<L, K, 64, kLdsDepth>(a, s); break;
+        default: launchBatchTKV<L, K, 64, 0>(a, s); break;
+    }
+}
+
+template <typename L>
+void launchBatchT(const BatchQueryArgs& a, int variant, hipStream_t s) {
+    if (a.k <= 1) {
+        launchBatchTK<L, 1>(a, variant, s);
+    } else if (a.k <= 8) {
+        launchBatchTK<L, 8>(a, variant, s);
+    } else {
+        launchBatchTK<L, 32>(a, variant, s);
+    }
+}
+
+}  // namespace
+
+void launchBatchNearest(const BatchQueryArgs& a, int variant, hipStream_t s) {
+    if (a.count <= 0) return;
+    if (a.k > 32) throw ArgError("k must be in [1, 32]");
+    switch (a.kind) {
+        case 0: launchBatchT<TriLeaf>(a, variant, s); break;
+        case 1: launchBatchT<PointLeaf>(a, variant, s); break;
+        default: launchBatchT<GaussLeaf>(a, variant, s); break;
+    }
+    hipCheck(hipGetLastError(), "batch traversal");
+}
+
+void launchBatchReset(int* counter, hipStream_t s) {
+    batchResetKernel<<<1, 64, 0, s>>>(counter);
+    hipCheck(hipGetLastError(), "batch reset");
 }
 
 }  // namespace qr

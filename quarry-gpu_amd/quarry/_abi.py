@@ -129,6 +129,22 @@ def lib():
             "qr_rays_device_ptrs": ([_vp, _vp, _vp, _vp], _ip),
             "qr_points_device_ptrs": ([_vp, _vp, _vp], _ip),
             "qr_radius_device_ptrs": ([_vp, _vp, _vp], _ip),
+            "qr_batch_build": ([_ip, _vp, _vp, _ip, _vp, _vp], _ip),
+            "qr_batch_build_device": ([_ip, _vp, _vp, _ip, _vp, _vp], _ip),
+            "qr_batch_destroy": ([_vp], _ip),
+            "qr_batch_info": ([_vp, _vp, _vp, _vp], _ip),
+            "qr_batch_read_scene": ([_vp, _ip, _vp, _vp], _ip),
+            "qr_batch_tree_hash": ([_vp, _ip, _vp], _ip),
+            "qr_batch_plan_create": ([_vp, _vp, _vp, _ip, _ip, _vp], _ip),
+            "qr_batch_plan_run": ([_vp, _dp], _ip),
+            "qr_batch_plan_run_eager": ([_vp, _dp], _ip),
+            "qr_batch_plan_read": ([_vp, _vp], _ip),
+            "qr_batch_plan_config": ([_vp, _vp, _vp, _dp], _ip),
+            "qr_batch_plan_destroy": ([_vp], _ip),
+            "qr_raycast_save_device": ([_vp, _vp, _ip, _vp, _vp, _ip, _ip, _dp], _ip),
+            "qr_closest_save_aux": ([_vp, _vp, _ip, _vp, _vp], _ip),
+            "qr_adjoint_closest": ([_vp, _vp, _ip, _vp, _vp, _vp, _vp, _vp], _ip),
+            "qr_adjoint_ray": ([_vp, _vp, _ip, _vp, _vp, _vp, _vp, _vp, _vp], _ip),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -443,6 +459,91 @@ class DeviceArray:
         if self.ptr.value:
             lib().qr_dev_free(self.ptr)
             self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Owning wrapper of a qr_batch: many scenes of one kind built together."""
+
+    def __init__(self, kind: int, items: np.ndarray, offsets: np.ndarray):
+        items = np.ascontiguousarray(items, dtype=np.float32).reshape(-1, ITEM_FLOATS[kind])
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+        self.kind = kind
+        self.h = ctypes.c_void_p()
+        self.timings = BuildTimings()
+        check(lib().qr_batch_build(kind, ptr(items), ptr(self.offsets), self.offsets.shape[0] - 1,
+                                   ctypes.byref(self.timings), ctypes.byref(self.h)))
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().qr_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        s, t, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().qr_batch_info(self.h, ctypes.byref(s), ctypes.byref(t), ctypes.byref(d)))
+        return s.value, t.value, d.value
+
+    def tree_hash(self, scene: int) -> int:
+        out = ctypes.c_uint64()
+        check(lib().qr_batch_tree_hash(self.h, scene, ctypes.byref(out)))
+        return int(out.value)
+
+    def read_scene(self, scene: int):
+        n = int(self.offsets[scene + 1] - self.offsets[scene])
+        nodes = np.zeros((2 * n - 1) * 32, np.uint8)
+        ids = np.zeros(n, np.int32)
+        check(lib().qr_batch_read_scene(self.h, scene, ptr(nodes), ptr(ids)))
+        return nodes, ids
+
+
+class BatchPlan:
+    """A prepared batched query (device copy of the points, captured graph)."""
+
+    def __init__(self, batch: Batch, points: np.ndarray, qoffsets: np.ndarray, k: int, tune=True):
+        """tune: True autotunes, False takes the default, or (variant, morton) forces one."""
+        self.batch = batch
+        self.points = DeviceArray(np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3))
+        self.qoffsets = np.ascontiguousarray(qoffsets, dtype=np.int32)
+        self.n = int(self.qoffsets[-1])
+        self.stride = max(k, 1)
+        self.h = ctypes.c_void_p()
+        code = 2 + 2 * tune[0] + int(tune[1]) if isinstance(tune, tuple) else int(bool(tune))
+        check(lib().qr_batch_plan_create(batch.h, self.points.ptr, ptr(self.qoffsets), k, code,
+                                         ctypes.byref(self.h)))
+
+    def run(self, graph: bool = True) -> float:
+        ms = ctypes.c_double()
+        fn = lib().qr_batch_plan_run if graph else lib().qr_batch_plan_run_eager
+        check(fn(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def read(self) -> np.ndarray:
+        out = np.zeros((self.n, self.stride, 2), np.uint32)
+        check(lib().qr_batch_plan_read(self.h, ptr(out)))
+        return out
+
+    def config(self):
+        v, m, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(lib().qr_batch_plan_config(self.h, ctypes.byref(v), ctypes.byref(m), ctypes.byref(t)))
+        return {"variant": ("lds256", "lds64", "scratch64")[v.value], "morton": bool(m.value),
+                "tuned_ms": t.value}
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().qr_batch_plan_destroy(self.h)
+            self.h = ctypes.c_void_p()
 
     def __del__(self):
         try:

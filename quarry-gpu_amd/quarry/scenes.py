@@ -22,6 +22,11 @@ SPEC below:
   gaussians means uniform in [0, 64]^3, per-axis scale log-uniform [0.05, 1],
             Shoemake uniform quaternions
   icosphere radius 10, 7 subdivisions (327,680 triangles), per-vertex N(0, 0.1)
+  batch     S scenes, n_s = floor(exp(U[ln 256, ln 65536])) triangles; even s: a
+            terrain patch of ceil(sqrt(n_s/2))^2 cells (first n_s triangles kept),
+            odd s: a city block with an 8x8-cell ground and (n_s - 128) // 10
+            buildings; q_s = floor(exp(U[ln 64, ln 8192])) queries uniform in the
+            scene's box, the first ceil(q_s/2) for closest, the rest for kNN k=8
 """
 from __future__ import annotations
 
@@ -287,6 +292,43 @@ def sphere_rays(count: int, seed: int = 10, origin_radius: float = 30.0,
     r[:, 6] = np.float32(1e-6)
     r[:, 7] = np.inf
     return r
+
+
+# ---- batched scenes -------------------------------------------------------------------------
+
+def batch_scene(s: int, n: int, seed: int) -> np.ndarray:
+    """Scene s of the batched workload with about n triangles (n exactly for terrain)."""
+    if s % 2 == 0:
+        cells = int(math.ceil(math.sqrt(n / 2.0)))
+        return np.ascontiguousarray(terrain(cells, seed * 100003 + s)[:n])
+    b = max(1, (n - 128) // 10)
+    return city(b, seed * 100003 + s, ground_cells=8)
+
+
+def workload_c5(seed: int = 1, scenes: int = 4096, nmin: int = 256, nmax: int = 65536,
+                qmin: int = 64, qmax: int = 8192):
+    """Returns (items (N, 9), prim offsets (S+1), closest points, closest offsets,
+    kNN points, kNN offsets); every array is scene-major."""
+    rng = _rng(seed, 11)
+    ns = np.floor(np.exp(rng.uniform(math.log(nmin), math.log(nmax), scenes))).astype(np.int64)
+    qs = np.floor(np.exp(rng.uniform(math.log(qmin), math.log(qmax), scenes))).astype(np.int64)
+    parts, cq, kq = [], [], []
+    for s in range(scenes):
+        tris = batch_scene(s, int(ns[s]), seed)
+        parts.append(tris)
+        b = bounds_of(tris)
+        q = query_points(b[:3], b[3:], int(qs[s]), seed * 7919 + s)
+        h = (int(qs[s]) + 1) // 2
+        cq.append(q[:h])
+        kq.append(q[h:])
+    poff = np.zeros(scenes + 1, np.int32)
+    np.cumsum([p.shape[0] for p in parts], out=poff[1:])
+    coff = np.zeros(scenes + 1, np.int32)
+    np.cumsum([c.shape[0] for c in cq], out=coff[1:])
+    koff = np.zeros(scenes + 1, np.int32)
+    np.cumsum([k.shape[0] for k in kq], out=koff[1:])
+    return (np.ascontiguousarray(np.concatenate(parts)), poff, np.ascontiguousarray(np.concatenate(cq)), coff,
+            np.ascontiguousarray(np.concatenate(kq)), koff)
 
 
 # ---- the benchmark workloads ----------------------------------------------------------------
