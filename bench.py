@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity legs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--profile", action="store_true", help="short run for rocprof: no CPU legs, no extras")
+    ap.add_argument("--no-extras", action="store_true", help="skip the C3 / C4 / C5 legs")
     return ap.parse_args()
 
 
@@ -256,6 +257,18 @@ def main() -> int:
         log(f"rays: {extra['rays_c1']} parity={parity['rays']}")
         del cbr, rb, b1
 
+        # ---- the other workloads: non-uniform scenes, gradients, batches --------------
+        if not args.no_extras:
+            for name, leg in (("c3", leg_c3), ("c4_grad", leg_c4), ("c5_batched", leg_c5)):
+                t0 = time.time()
+                try:
+                    extra[name], ok = leg(A, S, O, threads)
+                    parity[name] = ok
+                except Exception as e:  # reported, never hidden: the parity flag fails
+                    extra[name] = {"error": f"{type(e).__name__}: {e}"}
+                    parity[name] = False
+                log(f"{name} ({time.time() - t0:.1f}s): {extra[name]}")
+
         # ---- CPU baseline on a bounded sample ------------------------------------------
         if not args.no_cpu:
             cpu_baseline = cpu_leg(O, tris, queries, args, threads)
@@ -293,6 +306,201 @@ def main() -> int:
     out.update(extra)
     print(json.dumps(out), flush=True)
     return 0 if all(parity.values()) else 3
+
+
+def _median_ms(fn, reps=5):
+    fn()
+    return statistics.median(fn() for _ in range(reps))
+
+
+def leg_c3(A, S, O, threads):
+    """Non-uniform scenes: city rays (LBVH and 32-bin SAH), 1M-point kNN8 / radius 1,
+    1M-Gaussian closest / kNN8; 20,000 uniform queries per scene."""
+    out, ok = {}, True
+    city = S.city(131072, 1)
+    rays = S.cosine_down_rays(city, 1_000_000, 2)
+    want = O.CpuBvh(0, city).ray(rays, threads)
+    rb = A.RayBuffer(rays)
+    for name, builder in (("lbvh", A.BUILDER_LBVH), ("sah", A.BUILDER_SAH)):
+        b = A.Bvh(0, city, builder)
+        b.build_wide()
+        ms = _median_ms(lambda: rb.cast(b))
+        rb.cast(b, stats=True)
+        st = A.last_ray_stats()
+        got, _ = rb.summarize()
+        same = bool(np.array_equal(got, want))
+        ok &= same
+        out[f"city_rays_{name}"] = {"mrays_per_s": round(rays.shape[0] / ms / 1e3, 1), "ms": round(ms, 4),
+                                    "build_ms": round(b.timings.total_ms, 3),
+                                    "node_tests_per_ray": round(st.node_tests / rays.shape[0], 2),
+                                    "parity": same}
+        b.close()
+    out["city_triangles"] = int(city.shape[0])
+    rng = np.random.default_rng(12)
+    for kind, items, label in ((1, S.point_cloud(1 << 20, 1), "points"), (2, S.gaussians(1 << 20, 1), "gaussians")):
+        q = rng.uniform(0.0, 64.0, (20_000, 3)).astype(np.float32)
+        b = A.Bvh(kind, items)
+        b.build_wide()
+        ref = O.CpuBvh(kind, items)
+        pb1, pb8 = A.PointBuffer(q, 1), A.PointBuffer(q, 8)
+        res = {"build_ms": round(b.timings.total_ms, 3)}
+        ms_c = _median_ms(lambda: pb1.closest(b))
+        same_c = bool(np.array_equal(pb1.read(), ref.knn(q, 0, threads)))
+        ms_k = _median_ms(lambda: pb8.knn(b, 8))
+        same_k = bool(np.array_equal(pb8.read(), ref.knn(q, 8, threads)))
+        res.update({"closest_mq_per_s": round(q.shape[0] / ms_c / 1e3, 2),
+                    "knn8_mq_per_s": round(q.shape[0] / ms_k / 1e3, 2), "parity_closest": same_c,
+                    "parity_knn8": same_k})
+        ok &= same_c and same_k
+        if kind == 1:
+            ms_r = _median_ms(lambda: pb1.radius(b, 1.0))
+            _, offs, keys = pb1.radius(b, 1.0, keep=True)
+            wo, wk = ref.radius(q, 1.0, threads)
+            same_r = bool(np.array_equal(offs, wo) and np.array_equal(keys, wk))
+            res.update({"radius1_mq_per_s": round(q.shape[0] / ms_r / 1e3, 2), "parity_radius": same_r})
+            ok &= same_r
+        out[label] = res
+        b.close()
+        del ref
+    return out, bool(ok)
+
+
+def leg_c4(A, S, O, threads):
+    """Differentiable closest-point and ray queries on the 327,680-triangle icosphere:
+    forward-with-save + adjoint against the plain forward, through the C ABI."""
+    import ctypes
+
+    from oracle import adjoint as J
+
+    L = A.lib()
+    V, F, tris = S.icosphere(levels=7, radius=10.0, noise=0.1, seed=1)
+    rng = np.random.default_rng(13)
+    q = rng.uniform(-15, 15, (262_144, 3)).astype(np.float32)
+    rays = S.sphere_rays(1_000_000, seed=14)
+    dt, dq, dr = A.DeviceArray(tris), A.DeviceArray(q), A.DeviceArray(rays)
+    b = A.Bvh.from_device(0, dt.ptr.value, tris.shape[0])
+    b.build_wide()
+    nq, nr = q.shape[0], rays.shape[0]
+    res = A.DeviceArray.empty((nq, 2), np.uint32)
+    aux = A.DeviceArray.empty(nq, np.int32)
+    rres = A.DeviceArray.empty(nr, np.uint64)
+    hit = A.DeviceArray.empty(nr, np.int32)
+    ones_q, ones_r = A.DeviceArray(np.ones(nq, np.float32)), A.DeviceArray(np.ones(nr, np.float32))
+    gq = A.DeviceArray.empty((nq, 3), np.float32)
+    go, gd = A.DeviceArray.empty((nr, 3), np.float32), A.DeviceArray.empty((nr, 3), np.float32)
+    gt = A.DeviceArray(np.zeros(tris.shape, np.float32))
+    ms = ctypes.c_double()
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        A.check(L.qr_dev_synchronize())
+        return 1000.0 * (time.perf_counter() - t0)
+
+    def plain():
+        A.check(L.qr_closest_device(b.h, dq.ptr, nq, res.ptr, A.ORDER_MORTON, 1, ctypes.byref(ms)))
+        A.check(L.qr_raycast_device(b.h, dr.ptr, nr, rres.ptr, A.ORDER_MORTON, 1, ctypes.byref(ms)))
+
+    def forward_save():
+        A.check(L.qr_closest_device(b.h, dq.ptr, nq, res.ptr, A.ORDER_MORTON, 1, ctypes.byref(ms)))
+        A.check(L.qr_closest_save_aux(b.h, dq.ptr, nq, res.ptr, aux.ptr))
+        A.check(L.qr_raycast_save_device(b.h, dr.ptr, nr, rres.ptr, hit.ptr, A.ORDER_MORTON, 1, ctypes.byref(ms)))
+
+    def backward():
+        A.check(L.qr_adjoint_closest(b.h, dq.ptr, nq, res.ptr, aux.ptr, ones_q.ptr, gq.ptr, gt.ptr))
+        A.check(L.qr_adjoint_ray(b.h, dr.ptr, nr, rres.ptr, hit.ptr, ones_r.ptr, go.ptr, gd.ptr, gt.ptr))
+
+    t_plain = _median_ms(lambda: timed(plain))
+    t_fwd = _median_ms(lambda: timed(forward_save))
+    t_bwd = _median_ms(lambda: timed(backward))
+    plain()
+    plain_res = res.download()
+    forward_save()
+    same_fwd = bool(np.array_equal(res.download(), plain_res))
+    want = O.CpuBvh(0, tris).knn(q[:20000], 0, threads)
+    same_fwd &= bool(np.array_equal(plain_res[:20000], want[:, 0, :]))
+    # gradient spot check against the FP64 CPU adjoint on the first 4,096 queries
+    A.check(L.qr_adjoint_closest(b.h, dq.ptr, nq, res.ptr, aux.ptr, ones_q.ptr, gq.ptr, None))
+    prim = plain_res[:, 1].view(np.int32)
+    cq, _ = J.tri_adjoint(q[:4096].astype(np.float64), tris.astype(np.float64), prim[:4096], np.ones(4096))
+    g = gq.download()[:4096].astype(np.float64)
+    floor = 0.05 * float(np.sqrt(np.mean(cq ** 2)))
+    err = float(np.max(np.abs(g - cq) / np.maximum(np.abs(cq), floor)))
+    out = {"triangles": int(tris.shape[0]), "queries": nq, "rays": nr,
+           "plain_forward_ms": round(t_plain, 3), "forward_with_save_ms": round(t_fwd, 3),
+           "adjoint_ms": round(t_bwd, 3),
+           "grad_step_vs_two_plain_passes": round((t_fwd + t_bwd) / (2.0 * t_plain), 3),
+           "forward_bit_exact": same_fwd, "adjoint_max_rel_err_vs_fp64": err}
+    return out, bool(same_fwd and err < 1e-4)
+
+
+def leg_c5(A, S, O, threads):
+    """4,096 uneven scenes: one segmented build + persistent-kernel queries (hipGraph,
+    autotuned) against the looped plain per-scene path; results compared whole."""
+    import ctypes
+
+    L = A.lib()
+    items, poff, cp, coff, kp, koff = S.workload_c5(seed=1)
+    S_ = poff.shape[0] - 1
+    di = A.DeviceArray(items)
+    builds = []
+    for _ in range(3):
+        bt = A.Batch.from_device(0, di.ptr.value, poff)
+        builds.append(bt.timings.total_ms)
+        if len(builds) < 3:
+            bt.close()
+    plan_c = A.BatchPlan(bt, cp, coff, 0, tune=True)
+    plan_k = A.BatchPlan(bt, kp, koff, 8, tune=True)
+    ms_c = _median_ms(lambda: plan_c.run())
+    ms_k = _median_ms(lambda: plan_k.run())
+    got_c, got_k = plan_c.read(), plan_k.read()
+    nq = int(coff[-1] + koff[-1])
+    # looped plain path: one build and two query calls per scene, device pointers
+    dc, dk = plan_c.points, plan_k.points
+    oc = A.DeviceArray.empty((int(coff[-1]), 2), np.uint32)
+    ok_ = A.DeviceArray.empty((int(koff[-1]), 8, 2), np.uint32)
+    ms = ctypes.c_double()
+    trees = []
+    t0 = time.perf_counter()
+    for s in range(S_):
+        trees.append(A.Bvh.from_device(0, di.ptr.value + int(poff[s]) * 36, int(poff[s + 1] - poff[s])))
+    A.check(L.qr_dev_synchronize())
+    t_build = 1000.0 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for s in range(S_):
+        h = trees[s].h
+        n1, n2 = int(coff[s + 1] - coff[s]), int(koff[s + 1] - koff[s])
+        if n1:
+            A.check(L.qr_closest_device(h, dc.at(int(coff[s]) * 12), n1, oc.at(int(coff[s]) * 8),
+                                        A.ORDER_MORTON, 0, ctypes.byref(ms)))
+        if n2:
+            A.check(L.qr_knn_device(h, dk.at(int(koff[s]) * 12), n2, 8, ok_.at(int(koff[s]) * 64),
+                                    A.ORDER_MORTON, 0, ctypes.byref(ms)))
+    A.check(L.qr_dev_synchronize())
+    t_query = 1000.0 * (time.perf_counter() - t0)
+    loop_c, loop_k = oc.download(), ok_.download()
+    same_c = bool(np.array_equal(got_c.reshape(-1, 2), loop_c))
+    same_k = bool(np.array_equal(got_k, loop_k))
+    hashes_equal = sum(int(O.knn_hash(got_k[koff[s]:koff[s + 1]], 8)[0] ==
+                           O.knn_hash(loop_k[koff[s]:koff[s + 1]], 8)[0]) for s in range(S_))
+    trees_equal = sum(int(bt.tree_hash(s) == trees[s].tree_hash()) for s in range(0, S_, 64))
+    for tr in trees:
+        tr.close()
+    out = {"scenes": S_, "triangles": int(poff[-1]), "queries": nq,
+           "batched_build_ms": round(statistics.median(builds), 3),
+           "batched_query_ms": round(ms_c + ms_k, 3),
+           "batched_mq_per_s": round(nq / (ms_c + ms_k) / 1e3, 2),
+           "plain_loop_build_ms": round(t_build, 1), "plain_loop_query_ms": round(t_query, 1),
+           "plain_loop_mq_per_s": round(nq / t_query / 1e3, 2),
+           "speedup_queries": round(t_query / (ms_c + ms_k), 1),
+           "speedup_build_plus_queries": round((t_build + t_query) / (statistics.median(builds) + ms_c + ms_k), 1),
+           "tuned": {"closest": plan_c.config(), "knn8": plan_k.config()},
+           "results_equal": {"closest": same_c, "knn8": same_k},
+           "knn8_fingerprints_equal": hashes_equal, "tree_hashes_equal_sampled": trees_equal}
+    plan_c.close()
+    plan_k.close()
+    bt.close()
+    return out, bool(same_c and same_k)
 
 
 def calibration_copy(A, mib=256):

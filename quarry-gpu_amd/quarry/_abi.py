@@ -441,6 +441,19 @@ def last_ray_stats() -> TraversalStats:
 class DeviceArray:
     """A device allocation holding a copy of a host array (for *_device calls)."""
 
+    @classmethod
+    def empty(cls, shape, dtype) -> "DeviceArray":
+        self = cls.__new__(cls)
+        self.shape = tuple(np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = ctypes.c_void_p()
+        check(lib().qr_dev_alloc(max(self.nbytes, 1), ctypes.byref(self.ptr)))
+        return self
+
+    def at(self, byte_offset: int) -> ctypes.c_void_p:
+        return ctypes.c_void_p(self.ptr.value + int(byte_offset))
+
     def __init__(self, host: np.ndarray):
         host = np.ascontiguousarray(host)
         self.nbytes = host.nbytes
@@ -478,6 +491,18 @@ class Batch:
         self.timings = BuildTimings()
         check(lib().qr_batch_build(kind, ptr(items), ptr(self.offsets), self.offsets.shape[0] - 1,
                                    ctypes.byref(self.timings), ctypes.byref(self.h)))
+
+    @classmethod
+    def from_device(cls, kind: int, dev_ptr: int, offsets: np.ndarray) -> "Batch":
+        self = cls.__new__(cls)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+        self.kind = kind
+        self.h = ctypes.c_void_p()
+        self.timings = BuildTimings()
+        check(lib().qr_batch_build_device(kind, ctypes.c_void_p(dev_ptr), ptr(self.offsets),
+                                          self.offsets.shape[0] - 1, ctypes.byref(self.timings),
+                                          ctypes.byref(self.h)))
+        return self
 
     def close(self) -> None:
         if getattr(self, "h", None) is not None and self.h.value:
