@@ -432,28 +432,55 @@ int qr_bvh_update(qr_bvh* bvh, const void* items, int n, const int* deleted, int
             tt.moved = rt.updated;
             tt.total_ms = rms;
         } else {
-            // New shape: rebuild with the tree's own builder; the 4-wide level is
-            // re-collapsed when it existed.  Queries on the result equal those on a
-            // fresh build of the new scene bit for bit.
+            // New shape.  Only the new and moved primitives travel; the new scene is
+            // composed on the device from the old tree's items, then the topology is
+            // rebuilt with the tree's own builder and the 4-wide level re-collapsed
+            // when it existed.  Queries on the result equal those on a fresh build of
+            // the new scene bit for bit.
             hipStream_t s = currentStream();
             const bool hadWide = b.wideValid;
-            const size_t bytes = static_cast<size_t>(n) * itemSize(b.kind);
-            EventSet<4> ev;
+            const size_t isz = itemSize(b.kind);
+            const auto* src = static_cast<const unsigned char*>(items);
+            const int nStaged = nInserted + appended;
+            std::vector<unsigned char> staged(static_cast<size_t>(nStaged + nMoved) * isz + 1);
+            for (int i = 0; i < nInserted; ++i)
+                std::memcpy(&staged[static_cast<size_t>(i) * isz], src + static_cast<size_t>(inserted[i]) * isz, isz);
+            for (int i = 0; i < appended; ++i)
+                std::memcpy(&staged[static_cast<size_t>(nInserted + i) * isz],
+                            src + static_cast<size_t>(tailBeg + i) * isz, isz);
+            for (int i = 0; i < nMoved; ++i)
+                std::memcpy(&staged[static_cast<size_t>(nStaged + i) * isz], src + static_cast<size_t>(moved[i]) * isz,
+                            isz);
+            std::vector<int> lists;
+            lists.insert(lists.end(), deleted, deleted + nDeleted);
+            lists.insert(lists.end(), inserted, inserted + nInserted);
+            lists.insert(lists.end(), moved, moved + nMoved);
+            lists.push_back(0);
+            EventSet<5> ev;
             ev.record(0, s);
-            DevBuf<unsigned char> src(bytes);
-            hipCheck(hipMemcpyAsync(src.p, items, bytes, hipMemcpyHostToDevice, s), "H2D update");
+            DevBuf<unsigned char> dStaged(staged.size());
+            DevBuf<int> dLists(lists.size());
+            hipCheck(hipMemcpyAsync(dStaged.p, staged.data(), staged.size(), hipMemcpyHostToDevice, s), "H2D edit");
+            hipCheck(hipMemcpyAsync(dLists.p, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice, s),
+                     "H2D edit lists");
             ev.record(1, s);
-            BvhState fresh;
-            buildInto(fresh, b.kind, b.builder, src.p, n, nullptr);
-            b = std::move(fresh);
+            DevBuf<unsigned char> composed(static_cast<size_t>(n) * isz);
+            composeEdit(b, dLists.p, nDeleted, dLists.p + nDeleted, nInserted, static_cast<int>(tailBeg), n,
+                        dStaged.p, dLists.p + nDeleted + nInserted, nMoved,
+                        dStaged.p + static_cast<size_t>(nStaged) * isz, composed.p, s);
             ev.record(2, s);
-            if (hadWide) buildWide(b, s);
+            BvhState fresh;
+            buildInto(fresh, b.kind, b.builder, composed.p, n, nullptr);
+            b = std::move(fresh);
             ev.record(3, s);
-            ev.sync(3);
+            if (hadWide) buildWide(b, s);
+            ev.record(4, s);
+            ev.sync(4);
             tt.upload_ms = ev.ms(0, 1);
-            tt.surgery_ms = ev.ms(1, 2);
-            tt.wide_ms = hadWide ? ev.ms(2, 3) : 0.0;
-            tt.total_ms = ev.ms(0, 3);
+            tt.plan_ms = ev.ms(1, 2);
+            tt.surgery_ms = ev.ms(2, 3);
+            tt.wide_ms = hadWide ? ev.ms(3, 4) : 0.0;
+            tt.total_ms = ev.ms(0, 4);
             tt.moved = nMoved;
             tt.surgery_roots = 1;
             tt.rebuilt_leaves = n;

@@ -177,7 +177,7 @@ __device__ __forceinline__ int prefixLen(const uint32_t* __restrict__ code,
 __device__ __forceinline__ void karrasNode(const uint32_t* __restrict__ code,
                                            const uint32_t* __restrict__ idx, int n, int i,
                                            Node2* __restrict__ nodes, int* __restrict__ parent,
-                                           int* __restrict__ arrive) {
+                                           int* __restrict__ arrive, int* __restrict__ farEnd) {
     arrive[i] = 0;
     if (i == 0) parent[0] = -1;
 
@@ -193,6 +193,7 @@ __device__ __forceinline__ void karrasNode(const uint32_t* __restrict__ code,
         if (prefixLen(code, idx, n, i, i + (len + step) * dir) > dmin) len += step;
     }
     const int j = i + len * dir;
+    if (farEnd) farEnd[i] = j;  // node i covers leaf slots [min(i, j), max(i, j)]
     const int first = i < j ? i : j;
     const int last = i < j ? j : i;
 
@@ -222,10 +223,11 @@ __global__ __launch_bounds__(kBlock) void karrasKernel(const uint32_t* __restric
                                                        const uint32_t* __restrict__ idx, int n,
                                                        Node2* __restrict__ nodes,
                                                        int* __restrict__ parent,
-                                                       int* __restrict__ arrive) {
+                                                       int* __restrict__ arrive,
+                                                       int* __restrict__ farEnd) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n - 1) return;
-    karrasNode(code, idx, n, i, nodes, parent, arrive);
+    karrasNode(code, idx, n, i, nodes, parent, arrive, farEnd);
 }
 
 // ---- leaf scatter + bottom-up propagation -------------------------------------------
@@ -282,14 +284,122 @@ __device__ __forceinline__ void leafClimb(const typename L::Item* __restrict__ s
     }
 }
 
+// Bottom-up boxes with the block-local part in LDS.  Workgroup b owns leaf slots
+// [256 b, 256 b + 255].  An internal node p whose leaf range [min(p, far[p]),
+// max(p, far[p])] lies inside that window has both subtrees in this workgroup: its
+// arrival counter, box and height live in LDS (the stores to the node array are
+// ordinary write-back stores).  The first node whose range leaves the window is
+// published with agent-scope stores and from there on the climb follows the
+// global protocol of leafClimb.  Boxes are merged in the same order with the same
+// operands, so the node array is identical to leafClimb's.
 template <typename L, bool kGather>
-__global__ __launch_bounds__(kBlock) void leafUpKernel(
+__global__ __launch_bounds__(kBlock) void leafUpLocalKernel(
     const typename L::Item* __restrict__ src, const uint32_t* __restrict__ order, int n,
     typename L::Item* __restrict__ sorted, int* __restrict__ primIds, Node2* __restrict__ nodes,
-    const int* __restrict__ parent, int* __restrict__ arrive, int* __restrict__ height) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
+    const int* __restrict__ parent, int* __restrict__ arrive, int* __restrict__ height,
+    const int* __restrict__ farEnd) {
+    __shared__ float sLeaf[kBlock][6];
+    __shared__ float sNode[kBlock][6];
+    __shared__ int sCount[kBlock];
+    __shared__ int sHeight[kBlock];
+    const int base = blockIdx.x * kBlock;
+    const int last = (base + kBlock < n ? base + kBlock : n) - 1;
+    sCount[threadIdx.x] = 0;
+    __syncthreads();
+    const int k = base + threadIdx.x;
     if (k >= n) return;
-    leafClimb<L, kGather>(src, order, n, k, sorted, primIds, nodes, parent, arrive, height);
+
+    typename L::Item item;
+    if (kGather) {
+        const int sceneIdx = static_cast<int>(order[k]);
+        item = src[sceneIdx];
+        sorted[k] = item;
+        primIds[k] = sceneIdx;
+    } else {
+        item = sorted[k];
+    }
+    Box box = L::bounds(item);
+    {
+        Node2 leaf;
+        leaf.lo[0] = box.lo.x;
+        leaf.lo[1] = box.lo.y;
+        leaf.lo[2] = box.lo.z;
+        leaf.hi[0] = box.hi.x;
+        leaf.hi[1] = box.hi.y;
+        leaf.hi[2] = box.hi.z;
+        leaf.left = -(k + 1);
+        leaf.right = 1;
+        nodes[n - 1 + k] = leaf;
+    }
+    float* mine = sLeaf[threadIdx.x];
+    mine[0] = box.lo.x;
+    mine[1] = box.lo.y;
+    mine[2] = box.lo.z;
+    mine[3] = box.hi.x;
+    mine[4] = box.hi.y;
+    mine[5] = box.hi.z;
+
+    int node = n - 1 + k;
+    int h = 0;
+    bool published = false;
+    while (true) {
+        const int p = parent[node];
+        if (p < 0) break;
+        const int f = farEnd[p];
+        const int lo = p < f ? p : f, hi = p < f ? f : p;
+        if (lo >= base && hi <= last) {
+            const int prior = __hip_atomic_fetch_add(&sCount[p - base], 1, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (prior == 0) break;
+            const int l = nodes[p].left;
+            const int r = nodes[p].right;
+            const int sib = (l == node) ? r : l;
+            const float* sb = sib >= n - 1 ? sLeaf[sib - (n - 1) - base] : sNode[sib - base];
+            const Box other{{sb[0], sb[1], sb[2]}, {sb[3], sb[4], sb[5]}};
+            const int sh = sib >= n - 1 ? 0 : sHeight[sib - base];
+            box = (l == node) ? mergeChildren(box, other) : mergeChildren(other, box);
+            h = (h > sh ? h : sh) + 1;
+            float* dst = sNode[p - base];
+            dst[0] = box.lo.x;
+            dst[1] = box.lo.y;
+            dst[2] = box.lo.z;
+            dst[3] = box.hi.x;
+            dst[4] = box.hi.y;
+            dst[5] = box.hi.z;
+            sHeight[p - base] = h;
+            Node2* nd = &nodes[p];
+            nd->lo[0] = box.lo.x;
+            nd->lo[1] = box.lo.y;
+            nd->lo[2] = box.lo.z;
+            nd->hi[0] = box.hi.x;
+            nd->hi[1] = box.hi.y;
+            nd->hi[2] = box.hi.z;
+            height[p] = h;
+            node = p;
+            continue;
+        }
+        // p's range leaves this workgroup's window: publish this subtree's root for the
+        // other child's workgroup, then take the global arrival counter
+        if (!published) {
+            writeBoxAgent(&nodes[node], box);
+            if (node < n - 1) stAgentI(&height[node], h);
+            published = true;
+        }
+        storesDone();
+        const int prior = __hip_atomic_fetch_add(&arrive[p], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (prior == 0) break;
+        const int l = nodes[p].left;
+        const int r = nodes[p].right;
+        const int sib = (l == node) ? r : l;
+        const Box sb = readBoxAgent(&nodes[sib]);
+        const int sh = sib >= n - 1 ? 0 : ldAgentI(&height[sib]);
+        box = (l == node) ? mergeChildren(box, sb) : mergeChildren(sb, box);
+        h = (h > sh ? h : sh) + 1;
+        writeBoxAgent(&nodes[p], box);
+        stAgentI(&height[p], h);
+        stAgentI(&arrive[p], 0);
+        node = p;
+    }
 }
 
 // ---- batched (segmented) LBVH: many independent scenes per launch -----------------------
@@ -416,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void batchKarrasKernel(const uint32_t* __re
     const int il = i - ps;
     if (il >= ns - 1) return;
     karrasNode(codes2 + ps, idxLocal + ps, ns, il, nodes + (2 * ps - sc), parent + (2 * ps - sc),
-               arrive + (ps - sc));
+               arrive + (ps - sc), nullptr);
 }
 
 template <typename L>
@@ -456,6 +566,7 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     b.parent.alloc(static_cast<size_t>(b.nodeCount));
     b.arrive.alloc(static_cast<size_t>(n - 1));
     b.height.alloc(static_cast<size_t>(n - 1));
+    b.farEnd.alloc(static_cast<size_t>(n - 1));
     b.sortWs.reserve(static_cast<size_t>(n));
     b.counters.reserve(16);
 
@@ -484,13 +595,13 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     ev.record(2, s);
 
     karrasKernel<<<blocksFor(n - 1, kBlock), kBlock, 0, s>>>(codesSorted.p, order.p, n, b.nodes.p,
-                                                            b.parent.p, b.arrive.p);
+                                                            b.parent.p, b.arrive.p, b.farEnd.p);
     hipCheck(hipGetLastError(), "karras");
     ev.record(3, s);
 
-    leafUpKernel<L, true><<<blocksFor(n, kBlock), kBlock, 0, s>>>(
+    leafUpLocalKernel<L, true><<<blocksFor(n, kBlock), kBlock, 0, s>>>(
         dSrc, order.p, n, reinterpret_cast<Item*>(b.items.p), b.primIds.p, b.nodes.p, b.parent.p,
-        b.arrive.p, b.height.p);
+        b.arrive.p, b.height.p, b.farEnd.p);
     hipCheck(hipGetLastError(), "leaf up");
     ev.record(4, s);
 
@@ -545,19 +656,19 @@ void finishTopology(BvhState& b, const void* dSrc, const uint32_t* order, hipStr
     const int grid = blocksFor(n, kBlock);
     switch (b.kind) {
         case kPrimTri:
-            leafUpKernel<TriLeaf, true><<<grid, kBlock, 0, s>>>(
+            leafUpLocalKernel<TriLeaf, true><<<grid, kBlock, 0, s>>>(
                 static_cast<const Tri*>(dSrc), order, n, reinterpret_cast<Tri*>(b.items.p), b.primIds.p,
-                b.nodes.p, b.parent.p, b.arrive.p, b.height.p);
+                b.nodes.p, b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
             break;
         case kPrimPoint:
-            leafUpKernel<PointLeaf, true><<<grid, kBlock, 0, s>>>(
+            leafUpLocalKernel<PointLeaf, true><<<grid, kBlock, 0, s>>>(
                 static_cast<const V3*>(dSrc), order, n, reinterpret_cast<V3*>(b.items.p), b.primIds.p,
-                b.nodes.p, b.parent.p, b.arrive.p, b.height.p);
+                b.nodes.p, b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
             break;
         default:
-            leafUpKernel<GaussLeaf, true><<<grid, kBlock, 0, s>>>(
+            leafUpLocalKernel<GaussLeaf, true><<<grid, kBlock, 0, s>>>(
                 static_cast<const Gaussian*>(dSrc), order, n, reinterpret_cast<Gaussian*>(b.items.p),
-                b.primIds.p, b.nodes.p, b.parent.p, b.arrive.p, b.height.p);
+                b.primIds.p, b.nodes.p, b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
             break;
     }
     hipCheck(hipGetLastError(), "finish topology");
@@ -570,19 +681,19 @@ void propagateAll(BvhState& b, hipStream_t s) {
     const int grid = blocksFor(n, kBlock);
     switch (b.kind) {
         case kPrimTri:
-            leafUpKernel<TriLeaf, false><<<grid, kBlock, 0, s>>>(
+            leafUpLocalKernel<TriLeaf, false><<<grid, kBlock, 0, s>>>(
                 nullptr, nullptr, n, reinterpret_cast<Tri*>(b.items.p), nullptr, b.nodes.p,
-                b.parent.p, b.arrive.p, b.height.p);
+                b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
             break;
         case kPrimPoint:
-            leafUpKernel<PointLeaf, false><<<grid, kBlock, 0, s>>>(
+            leafUpLocalKernel<PointLeaf, false><<<grid, kBlock, 0, s>>>(
                 nullptr, nullptr, n, reinterpret_cast<V3*>(b.items.p), nullptr, b.nodes.p,
-                b.parent.p, b.arrive.p, b.height.p);
+                b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
             break;
         default:
-            leafUpKernel<GaussLeaf, false><<<grid, kBlock, 0, s>>>(
+            leafUpLocalKernel<GaussLeaf, false><<<grid, kBlock, 0, s>>>(
                 nullptr, nullptr, n, reinterpret_cast<Gaussian*>(b.items.p), nullptr, b.nodes.p,
-                b.parent.p, b.arrive.p, b.height.p);
+                b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
             break;
     }
     hipCheck(hipGetLastError(), "propagate");

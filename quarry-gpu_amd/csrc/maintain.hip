@@ -298,6 +298,90 @@ void reemitWide(BvhState& b, hipStream_t s) {
     hipCheck(hipGetLastError(), "wide re-emit");
 }
 
+namespace {
+// New-scene item j of a topology edit: appended and inserted items come from the
+// upload (inserted first, then appended); a survivor is the old item, found
+// through the old tree's scene -> slot map.  The r-th survivor (0-based) is old
+// index r + d with d = #{i : deleted[i] - i <= r} (deleted is ascending and unique,
+// so deleted[i] - i does not decrease).
+template <typename L>
+__global__ void composeKernel(const typename L::Item* __restrict__ oldItems, const int* __restrict__ oldSlot,
+                              const int* __restrict__ deleted, int nDel, const int* __restrict__ inserted,
+                              int nIns, int tailBeg, int nNew, const typename L::Item* __restrict__ staged,
+                              typename L::Item* __restrict__ out) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= nNew) return;
+    if (j >= tailBeg) {
+        out[j] = staged[nIns + (j - tailBeg)];
+        return;
+    }
+    int lo = 0, hi = nIns;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (inserted[mid] < j) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    if (lo < nIns && inserted[lo] == j) {
+        out[j] = staged[lo];
+        return;
+    }
+    const int r = j - lo;
+    int a = 0, c = nDel;
+    while (a < c) {
+        const int mid = (a + c) >> 1;
+        if (deleted[mid] - mid <= r) {
+            a = mid + 1;
+        } else {
+            c = mid;
+        }
+    }
+    out[j] = oldItems[oldSlot[r + a]];
+}
+
+template <typename L>
+__global__ void scatterMovedKernel(const int* __restrict__ moved, int nMoved,
+                                   const typename L::Item* __restrict__ staged, typename L::Item* __restrict__ out) {
+    const int m = blockIdx.x * kBlock + threadIdx.x;
+    if (m < nMoved) out[moved[m]] = staged[m];
+}
+
+template <typename L>
+void composeT(BvhState& b, const int* dDel, int nDel, const int* dIns, int nIns, int tailBeg, int nNew,
+              const void* dStaged, const int* dMoved, int nMoved, const void* dStagedMoved, void* dOut,
+              hipStream_t s) {
+    using Item = typename L::Item;
+    composeKernel<L><<<blocksFor(nNew, kBlock), kBlock, 0, s>>>(
+        reinterpret_cast<const Item*>(b.items.p), b.primSlot.p, dDel, nDel, dIns, nIns, tailBeg, nNew,
+        static_cast<const Item*>(dStaged), static_cast<Item*>(dOut));
+    if (nMoved > 0)
+        scatterMovedKernel<L><<<blocksFor(nMoved, kBlock), kBlock, 0, s>>>(
+            dMoved, nMoved, static_cast<const Item*>(dStagedMoved), static_cast<Item*>(dOut));
+    hipCheck(hipGetLastError(), "compose edit");
+}
+}  // namespace
+
+void composeEdit(BvhState& b, const int* dDel, int nDel, const int* dIns, int nIns, int tailBeg, int nNew,
+                 const void* dStaged, const int* dMoved, int nMoved, const void* dStagedMoved, void* dOut,
+                 hipStream_t s) {
+    ensurePrimSlot(b, s);
+    switch (b.kind) {
+        case kPrimTri:
+            composeT<TriLeaf>(b, dDel, nDel, dIns, nIns, tailBeg, nNew, dStaged, dMoved, nMoved, dStagedMoved, dOut, s);
+            break;
+        case kPrimPoint:
+            composeT<PointLeaf>(b, dDel, nDel, dIns, nIns, tailBeg, nNew, dStaged, dMoved, nMoved, dStagedMoved, dOut,
+                                s);
+            break;
+        default:
+            composeT<GaussLeaf>(b, dDel, nDel, dIns, nIns, tailBeg, nNew, dStaged, dMoved, nMoved, dStagedMoved, dOut,
+                                s);
+            break;
+    }
+}
+
 void ensurePrimSlot(BvhState& b, hipStream_t s) {
     if (b.primSlotValid) return;
     b.primSlot.reserve(static_cast<size_t>(b.n));

@@ -31,6 +31,7 @@ struct BvhState {
     DevBuf<int> parent;             // per node, root -1
     DevBuf<int> arrive;             // per internal node: bottom-up arrival counters
     DevBuf<int> height;             // per internal node: subtree height (leaves are 0)
+    DevBuf<int> farEnd;             // per internal node i: the other end j of its leaf range [i, j] or [j, i]
 
     // scene index -> leaf slot; rebuilt lazily after a build or topology update
     DevBuf<int> primSlot;
@@ -103,6 +104,13 @@ void propagateAll(BvhState& b, hipStream_t s);
 // b.nodes (internal left/right), b.parent (root -1) and zeroed b.arrive; order[k]
 // is the scene index of leaf slot k.  The root's height (b.height[0]) is the depth.
 void finishTopology(BvhState& b, const void* dSrc, const uint32_t* order, hipStream_t s);
+
+// Scene-order items of the new scene of a topology edit, on the device, from the
+// old tree's items (survivors) and the uploaded items: dStaged = inserted items
+// then appended ones, dStagedMoved[m] = the new item at new index dMoved[m].
+void composeEdit(BvhState& b, const int* dDel, int nDel, const int* dIns, int nIns, int tailBeg, int nNew,
+                 const void* dStaged, const int* dMoved, int nMoved, const void* dStagedMoved, void* dOut,
+                 hipStream_t s);
 
 void buildWide(BvhState& b, hipStream_t s);
 void reemitWide(BvhState& b, hipStream_t s);

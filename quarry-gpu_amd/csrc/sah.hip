@@ -68,7 +68,7 @@ struct Chunk {
 };
 
 struct NodeTriple {
-    int id, left, right, pad;
+    int id, left, right, far;  // far: the other end of the node's leaf range
 };
 
 __device__ __forceinline__ int hidx(int a, int b, int v) { return (a * kBins + b) * kBinVals + v; }
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void sahLocalKernel(const PrimRec* __restri
 
 // Internal nodes decided on the host (levels of large and local tasks).
 __global__ void sahWriteNodesKernel(const NodeTriple* __restrict__ tri, int count, Node2* __restrict__ nodes,
-                                    int* __restrict__ parent) {
+                                    int* __restrict__ parent, int* __restrict__ farEnd) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= count) return;
     const NodeTriple t = tri[i];
@@ -476,6 +476,7 @@ __global__ void sahWriteNodesKernel(const NodeTriple* __restrict__ tri, int coun
     nodes[t.id].right = t.right;
     parent[t.left] = t.id;
     parent[t.right] = t.id;
+    farEnd[t.id] = t.far;
 }
 
 struct SerialTask {
@@ -488,7 +489,7 @@ struct SerialTask {
 __global__ void sahSerialKernel(const PrimRec* __restrict__ recA, const PrimRec* __restrict__ recB,
                                 const SerialTask* __restrict__ tasks, int count, int n,
                                 uint32_t* __restrict__ order, Node2* __restrict__ nodes,
-                                int* __restrict__ parent) {
+                                int* __restrict__ parent, int* __restrict__ farEnd) {
     const int ti = blockIdx.x * kBlock + threadIdx.x;
     if (ti >= count) return;
     const SerialTask t = tasks[ti];
@@ -511,6 +512,7 @@ __global__ void sahSerialKernel(const PrimRec* __restrict__ recA, const PrimRec*
         nodes[id].right = r;
         parent[l] = id;
         parent[r] = id;
+        farEnd[id] = id == s ? e : s;
         if (g > s) {
             stS[sp] = s;
             stE[sp] = g;
@@ -544,6 +546,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     b.parent.alloc(static_cast<size_t>(b.nodeCount));
     b.arrive.alloc(static_cast<size_t>(n - 1));
     b.height.alloc(static_cast<size_t>(n - 1));
+    b.farEnd.alloc(static_cast<size_t>(n - 1));
     b.sortWs.reserve(static_cast<size_t>(n));
     b.counters.reserve(16);
     uint32_t* cb = reinterpret_cast<uint32_t*>(b.counters.p);
@@ -693,7 +696,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
             const int e = h.start + h.count - 1;
             const int left = (g == h.start) ? n - 1 + g : g;
             const int right = (g + 1 == e) ? n - 1 + e : g + 1;
-            triples.push_back({h.node, left, right, 0});
+            triples.push_back({h.node, left, right, h.node == h.start ? e : h.start});
             pushChild(h.start, nL, left, r.lLo, r.lHi, cur);
             pushChild(g + 1, h.count - nL, right, r.rLo, r.rHi, cur);
         }
@@ -708,12 +711,13 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
         hipCheck(hipMemcpyAsync(dTri.p, triples.data(), sizeof(NodeTriple) * triples.size(), hipMemcpyHostToDevice, s),
                  "sah nodes");
         sahWriteNodesKernel<<<blocksFor(static_cast<long long>(triples.size()), kBlock), kBlock, 0, s>>>(
-            dTri.p, static_cast<int>(triples.size()), b.nodes.p, b.parent.p);
+            dTri.p, static_cast<int>(triples.size()), b.nodes.p, b.parent.p, b.farEnd.p);
     }
     hipCheck(hipMemcpyAsync(dSerial.p, serial.data(), sizeof(SerialTask) * serial.size(), hipMemcpyHostToDevice, s),
              "sah serial");
     sahSerialKernel<<<blocksFor(static_cast<long long>(serial.size()), kBlock), kBlock, 0, s>>>(
-        rec[0].p, rec[1].p, dSerial.p, static_cast<int>(serial.size()), n, order.p, b.nodes.p, b.parent.p);
+        rec[0].p, rec[1].p, dSerial.p, static_cast<int>(serial.size()), n, order.p, b.nodes.p, b.parent.p,
+        b.farEnd.p);
     hipCheck(hipGetLastError(), "sah serial");
     ev.record(2, s);
 
