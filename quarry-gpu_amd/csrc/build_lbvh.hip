@@ -742,11 +742,10 @@ void computeDepth(BvhState& b, hipStream_t s) {
     if (b.depthValid) return;
     const int nc = b.nodeCount;
     b.depthArr.reserve(static_cast<size_t>(nc));
-    DevBuf<unsigned long long> jump(static_cast<size_t>(nc));
-    depthJumpInitKernel<<<blocksFor(nc, kBlock), kBlock, 0, s>>>(b.parent.p, nc, jump.p);
-    depthJumpKernel<<<blocksFor(nc, kBlock), kBlock, 0, s>>>(b.parent.p, nc, jump.p, b.depthArr.p);
+    b.jump.reserve(static_cast<size_t>(nc));
+    depthJumpInitKernel<<<blocksFor(nc, kBlock), kBlock, 0, s>>>(b.parent.p, nc, b.jump.p);
+    depthJumpKernel<<<blocksFor(nc, kBlock), kBlock, 0, s>>>(b.parent.p, nc, b.jump.p, b.depthArr.p);
     hipCheck(hipGetLastError(), "depth");
-    hipCheck(hipStreamSynchronize(s), "depth sync");
     b.depthValid = true;
 }
 

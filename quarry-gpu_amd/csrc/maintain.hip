@@ -60,31 +60,35 @@ __device__ __forceinline__ void clearEntry(WideEntry& e) {
     e.pad = 0;
 }
 
-__global__ void wideEmitKernel(const Node2* __restrict__ nodes, int n,
-                               const int* __restrict__ depth, const int* __restrict__ wideId,
-                               Wide4* __restrict__ wide) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+// Four lanes per internal node: lane s writes entry s (32 B) of the node's 4-wide
+// node, so a wide node is one contiguous 128 B write.  Even-depth internal node i
+// holds its children, with each interior child replaced by its two children.
+// A node id >= n - 1 is a leaf, so the entry layout needs no node reads.
+__global__ __launch_bounds__(kBlock) void wideEmitKernel(const Node2* __restrict__ nodes, int n,
+                                                         const int* __restrict__ depth,
+                                                         const int* __restrict__ wideId,
+                                                         Wide4* __restrict__ wide) {
+    const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const int i = static_cast<int>(t >> 2);
+    const int slot = static_cast<int>(t & 3);
     if (i >= n - 1 || (depth[i] & 1) != 0) return;
-    Wide4 out;
-#pragma unroll
-    for (int s = 0; s < 4; ++s) clearEntry(out.c[s]);
     const Node2 nd = nodes[i];
-    const int kids[2] = {nd.left, nd.right};
-    int slot = 0;
-    for (int k = 0; k < 2; ++k) {
-        const int c = kids[k];
-        const Node2 cn = nodes[c];
-        if (c >= n - 1) {
-            setEntry(out.c[slot++], cn, cn.left);
-            continue;
-        }
-        const int g[2] = {cn.left, cn.right};
-        for (int j = 0; j < 2; ++j) {
-            const Node2 gn = nodes[g[j]];
-            setEntry(out.c[slot++], gn, g[j] >= n - 1 ? gn.left : wideId[g[j]]);
-        }
+    const int a0 = nd.left >= n - 1 ? 1 : 2;
+    const int a1 = nd.right >= n - 1 ? 1 : 2;
+    WideEntry e;
+    clearEntry(e);
+    int src = -1;
+    if (slot < a0) {
+        src = a0 == 1 ? nd.left : (slot == 0 ? nodes[nd.left].left : nodes[nd.left].right);
+    } else if (slot - a0 < a1) {
+        const int j = slot - a0;
+        src = a1 == 1 ? nd.right : (j == 0 ? nodes[nd.right].left : nodes[nd.right].right);
     }
-    wide[wideId[i]] = out;
+    if (src >= 0) {
+        const Node2 sn = nodes[src];
+        setEntry(e, sn, src >= n - 1 ? sn.left : wideId[src]);
+    }
+    wide[wideId[i]].c[slot] = e;
 }
 
 // ---- refit --------------------------------------------------------------------------
@@ -272,20 +276,21 @@ void buildWide(BvhState& b, hipStream_t s) {
     computeDepth(b, s);
     const int internal = b.n - 1;
     b.wideId.reserve(static_cast<size_t>(internal));
-    DevBuf<int> flags(static_cast<size_t>(internal));
-    DevBuf<int> scratch(static_cast<size_t>(scanScratchInts(internal)));
-    DevBuf<int> total(1);
-    wideFlagKernel<<<blocksFor(internal, kBlock), kBlock, 0, s>>>(b.depthArr.p, internal, flags.p);
-    scanExclusive(flags.p, b.wideId.p, internal, total.p, scratch.p, s);
-    int count = 0;
-    hipCheck(hipMemcpyAsync(&count, total.p, sizeof(int), hipMemcpyDeviceToHost, s), "wide count");
-    hipCheck(hipStreamSynchronize(s), "wide count sync");
-    if (count <= 0) throw DeviceError("buildWideNodes: no wide nodes");
-    b.wide.reserve(static_cast<size_t>(count));
-    b.wideCount = count;
-    wideEmitKernel<<<blocksFor(internal, kBlock), kBlock, 0, s>>>(b.nodes.p, b.n, b.depthArr.p,
-                                                                 b.wideId.p, b.wide.p);
+    b.wideFlags.reserve(static_cast<size_t>(internal));
+    b.wideScan.reserve(static_cast<size_t>(scanScratchInts(internal)) + 1);
+    // at most one wide node per internal node: no count round trip before the emit
+    b.wide.reserve(static_cast<size_t>(internal));
+    wideFlagKernel<<<blocksFor(internal, kBlock), kBlock, 0, s>>>(b.depthArr.p, internal, b.wideFlags.p);
+    int* total = b.wideScan.p + scanScratchInts(internal);
+    scanExclusive(b.wideFlags.p, b.wideId.p, internal, total, b.wideScan.p, s);
+    wideEmitKernel<<<blocksFor(4LL * internal, kBlock), kBlock, 0, s>>>(b.nodes.p, b.n, b.depthArr.p,
+                                                                       b.wideId.p, b.wide.p);
     hipCheck(hipGetLastError(), "wide emit");
+    int count = 0;
+    hipCheck(hipMemcpyAsync(&count, total, sizeof(int), hipMemcpyDeviceToHost, s), "wide count");
+    hipCheck(hipStreamSynchronize(s), "wide sync");
+    if (count <= 0) throw DeviceError("buildWideNodes: no wide nodes");
+    b.wideCount = count;
     b.wideValid = true;
 }
 
@@ -293,8 +298,8 @@ void buildWide(BvhState& b, hipStream_t s) {
 void reemitWide(BvhState& b, hipStream_t s) {
     if (!b.wideValid) return;
     const int internal = b.n - 1;
-    wideEmitKernel<<<blocksFor(internal, kBlock), kBlock, 0, s>>>(b.nodes.p, b.n, b.depthArr.p,
-                                                                 b.wideId.p, b.wide.p);
+    wideEmitKernel<<<blocksFor(4LL * internal, kBlock), kBlock, 0, s>>>(b.nodes.p, b.n, b.depthArr.p,
+                                                                       b.wideId.p, b.wide.p);
     hipCheck(hipGetLastError(), "wide re-emit");
 }
 

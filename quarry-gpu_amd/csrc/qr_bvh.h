@@ -40,10 +40,12 @@ struct BvhState {
     // per node depth (root 0); computed on demand for wide collapse and validate
     DevBuf<int> depthArr;
     bool depthValid = false;
+    DevBuf<unsigned long long> jump;  // pointer-jumping scratch of the depth pass
 
     // 4-wide collapse
     DevBuf<Wide4> wide;
     DevBuf<int> wideId;             // per internal node: wide index or -1
+    DevBuf<int> wideFlags, wideScan;  // collapse scratch (flags, scan partials + total)
     int wideCount = 0;
     bool wideValid = false;
 
