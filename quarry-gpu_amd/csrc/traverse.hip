@@ -23,6 +23,9 @@
 // Queries may be visited in a permuted order (Morton reordering): lane i works
 // on query order[i] and writes its result straight to slot order[i], so no
 // gather/scatter copies of queries or results are made.
+#include <algorithm>
+#include <vector>
+
 #include "qr_traverse.h"
 
 namespace qr {
@@ -894,6 +897,28 @@ __global__ void segClassifyKernel(const int* __restrict__ offsets, int count,
     if (m > kBlockCap) hugeList[atomicAdd(&info[1], 1)] = s;
 }
 
+__global__ void splitKeysKernel(const uint64_t* __restrict__ keys, int m, uint32_t* __restrict__ lo,
+                                uint32_t* __restrict__ hi) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    const uint64_t k = keys[i];
+    lo[i] = static_cast<uint32_t>(k);
+    hi[i] = static_cast<uint32_t>(k >> 32);
+}
+
+__global__ void gatherU32Kernel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, int m,
+                                uint32_t* __restrict__ dst) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < m) dst[i] = src[idx[i]];
+}
+
+// keys[i] = hiSorted[i] << 32 | lo[ord[i]]
+__global__ void joinKeysKernel(const uint32_t* __restrict__ hiSorted, const uint32_t* __restrict__ lo,
+                               const uint32_t* __restrict__ ord, int m, uint64_t* __restrict__ keys) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < m) keys[i] = (static_cast<uint64_t>(hiSorted[i]) << 32) | lo[ord[i]];
+}
+
 // ---- Morton reordering of queries / rays ------------------------------------------------
 // quantize10 over the query (or ray-origin) bounds (src/gpu/morton.h:8-29).
 __device__ __forceinline__ uint32_t quant10(float v, float lo, float hi) {
@@ -1162,7 +1187,36 @@ void segmentSort(const int* offsets, uint64_t* keys, int count, int maxSeg, int*
         hipCheck(hipGetLastError(), "segment sort (block)");
     }
     if (h[1] > 0) {
-        throw ArgError("radius: a query matched more than 8192 primitives; reduce the radius");
+        // Segments above the workgroup network's capacity: each is sorted by the
+        // device radix sort as two stable 32-bit passes (index half, then distance
+        // half), which orders the 64-bit keys exactly.
+        std::vector<int> huge(static_cast<size_t>(h[1]));
+        hipCheck(hipMemcpyAsync(huge.data(), hugeList, sizeof(int) * huge.size(), hipMemcpyDeviceToHost, s),
+                 "huge list");
+        std::vector<int> offs(static_cast<size_t>(count) + 1);
+        hipCheck(hipMemcpyAsync(offs.data(), offsets, sizeof(int) * offs.size(), hipMemcpyDeviceToHost, s),
+                 "huge offsets");
+        hipCheck(hipStreamSynchronize(s), "huge sync");
+        int maxM = 0;
+        for (int sg : huge) maxM = std::max(maxM, offs[static_cast<size_t>(sg) + 1] - offs[static_cast<size_t>(sg)]);
+        SortWorkspace ws;
+        ws.reserve(static_cast<size_t>(maxM));
+        DevBuf<uint32_t> lo(maxM), hi(maxM), tmp(maxM), ord1(maxM), ord2(maxM);
+        for (int sg : huge) {
+            const int off = offs[static_cast<size_t>(sg)];
+            const int m = offs[static_cast<size_t>(sg) + 1] - off;
+            splitKeysKernel<<<blocksFor(m, kBlock), kBlock, 0, s>>>(keys + off, m, lo.p, hi.p);
+            sortClear(ws, static_cast<size_t>(m), 4, s);
+            sortHistogram(ws, lo.p, static_cast<size_t>(m), 4, s);
+            sortPairs(ws, lo.p, nullptr, tmp.p, ord1.p, static_cast<size_t>(m), 4, s);
+            gatherU32Kernel<<<blocksFor(m, kBlock), kBlock, 0, s>>>(hi.p, ord1.p, m, tmp.p);
+            sortClear(ws, static_cast<size_t>(m), 4, s);
+            sortHistogram(ws, tmp.p, static_cast<size_t>(m), 4, s);
+            sortPairs(ws, tmp.p, ord1.p, hi.p, ord2.p, static_cast<size_t>(m), 4, s);
+            joinKeysKernel<<<blocksFor(m, kBlock), kBlock, 0, s>>>(hi.p, lo.p, ord2.p, m, keys + off);
+            hipCheck(hipGetLastError(), "segment sort (huge)");
+        }
+        hipCheck(hipStreamSynchronize(s), "huge sort sync");
     }
 }
 

@@ -252,3 +252,19 @@ def test_errors(A, scenes):
     assert o.tolist() == [0] * 5 and k.size == 0
     # empty query sets are fine
     assert b.closest(np.zeros((0, 3), np.float32))[0].shape == (0, 1, 2)
+
+
+def test_radius_segments_above_workgroup_capacity(A, orc):
+    """Segments longer than 8,192 keys take the device radix-sort path; a few
+    queries see every one of 20,000 points (with ties on distance)."""
+    rng = np.random.default_rng(21)
+    pts = rng.uniform(-1, 1, (20000, 3)).astype(np.float32)
+    pts[5000:5100] = pts[17]  # equal distances, ordered by index
+    q = np.concatenate([np.zeros((3, 3)), rng.uniform(-3, 3, (60, 3))]).astype(np.float32)
+    b = A.Bvh(1, pts)
+    b.build_wide()
+    want = orc.radius_brute(1, pts, q, 4.0, threads=THREADS)
+    assert int(np.diff(want[0]).max()) > 8192
+    for wide in (False, True):
+        o, k, _, _ = b.radius(q, 4.0, order=A.ORDER_MORTON, wide=wide)
+        assert eq(o, want[0]) and eq(k, want[1]), wide
