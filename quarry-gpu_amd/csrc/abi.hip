@@ -254,6 +254,13 @@ int qr_warmup(void) {
     });
 }
 
+int qr_set_traversal_mode(int mode) {
+    return guard([&] {
+        need(mode >= 0 && mode <= 2, "traversal mode must be 0, 1 or 2");
+        setTraversalMode(mode);
+    });
+}
+
 int qr_set_stream(void* s) {
     tStream = static_cast<hipStream_t>(s);
     return QR_OK;

@@ -72,4 +72,8 @@ enum BatchVariant : int { kBatchLds256 = 0, kBatchLds64 = 1, kBatchScratch64 = 2
 void launchBatchNearest(const BatchQueryArgs& a, int variant, hipStream_t s);
 void launchBatchReset(int* counter, hipStream_t s);
 
+// Traversal shape of the wide point queries: 0 automatic, 1 one lane per query,
+// 2 four lanes per query (process-wide).
+void setTraversalMode(int mode);
+
 }  // namespace qr

@@ -676,6 +676,128 @@ __global__ __launch_bounds__(kBlock) void nearestWideKernel(
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
+// ---- four lanes per query -------------------------------------------------------------
+// For query sets too small to fill the GPU one lane per query, a quad of lanes
+// walks one query: lane s takes entry s of each 4-wide node (one 128 B load per
+// quad), tests its box and, if it is a leaf, its primitive -- in parallel.  Every
+// quantity that steers the walk (the k-best list, the stack pointer, the next
+// node) is kept identical in the four lanes: candidates are exchanged within the
+// quad (DPP quad broadcasts) and offered in slot order by all four lanes.  The
+// quad's stack is a private LDS column written by the pushing lanes.
+
+template <int S>
+__device__ __forceinline__ int quadGet(int v) {
+    return __builtin_amdgcn_mov_dpp(v, S | (S << 2) | (S << 4) | (S << 6), 0xF, 0xF, false);
+}
+template <int S>
+__device__ __forceinline__ float quadGetF(float v) {
+    return __int_as_float(quadGet<S>(__float_as_int(v)));
+}
+
+constexpr int kQuadDepth = 96;  // 3 x the deepest 4-wide level a tree here can have
+
+template <typename L, int K, bool kStats>
+__global__ __launch_bounds__(kBlock) void nearestQuadKernel(
+    const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
+    const int* __restrict__ primIds, const V3* __restrict__ points,
+    const uint32_t* __restrict__ order, int count, int k, int outStride,
+    ClosestResult* __restrict__ out, unsigned long long* __restrict__ stats) {
+    __shared__ int lds[kQuadDepth * (kBlock / 4)];
+    const int t = xcdBlock() * kBlock + threadIdx.x;
+    const int qi = t >> 2;
+    const int slot = threadIdx.x & 3;
+    int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
+    if (qi < count) {  // the same for the four lanes of a quad
+        const int q = order ? static_cast<int>(order[qi]) : qi;
+        const V3 p = points[q];
+        KnnList<K> list;
+        list.init();
+        LdsInt* stk = (LdsInt*)(lds + (threadIdx.x >> 2) * kQuadDepth);
+        int sp = 0;
+        int id = 0;
+        while (true) {
+            if (kStats && slot == 0) ++opened;
+            const WideEntry e = wide[id].c[slot];
+            const int ch = e.child;
+            const bool leaf = ch < 0 && ch != kWideEmpty;
+            typename L::Item it;
+            int pid = 0;
+            if (leaf) {
+                it = prims[-ch - 1];
+                pid = primIds[-ch - 1];
+            }
+            float d = kInf;
+            bool adm = false;
+            if (ch != kWideEmpty) {
+                d = boxDist2(p, entryBox(e));
+                adm = list.admits(d);
+            }
+            if (adm) ++nodeTests;
+            float cd = kInf;
+            int cid = 0x7fffffff;
+            int cand = 0;
+            if (adm && leaf) {
+                ++leafTests;
+                cd = L::dist2(p, it);
+                cid = pid;
+                cand = 1;
+            }
+            {
+                const float d0 = quadGetF<0>(cd), d1 = quadGetF<1>(cd), d2 = quadGetF<2>(cd), d3 = quadGetF<3>(cd);
+                const int i0 = quadGet<0>(cid), i1 = quadGet<1>(cid), i2 = quadGet<2>(cid), i3 = quadGet<3>(cid);
+                const int o0 = quadGet<0>(cand), o1 = quadGet<1>(cand), o2 = quadGet<2>(cand), o3 = quadGet<3>(cand);
+                if (o0) list.offer(d0, i0);
+                if (o1) list.offer(d1, i1);
+                if (o2) list.offer(d2, i2);
+                if (o3) list.offer(d3, i3);
+            }
+            const bool in = adm && !leaf && list.admits(d);
+            const float key = in ? d : kInf;
+            const int child = in ? ch : -1;
+            const float k0 = quadGetF<0>(key), k1 = quadGetF<1>(key), k2 = quadGetF<2>(key), k3 = quadGetF<3>(key);
+            const int c0 = quadGet<0>(child), c1 = quadGet<1>(child), c2 = quadGet<2>(child), c3 = quadGet<3>(child);
+            // rank of this lane's child among the quad's surviving children (ties by slot)
+            int rank = 0;
+            rank += (slot != 0) && (k0 < key || (k0 == key && 0 < slot));
+            rank += (slot != 1) && (k1 < key || (k1 == key && 1 < slot));
+            rank += (slot != 2) && (k2 < key || (k2 == key && 2 < slot));
+            rank += (slot != 3) && (k3 < key || (k3 == key && 3 < slot));
+            const int m = (c0 >= 0) + (c1 >= 0) + (c2 >= 0) + (c3 >= 0);
+            if (m > 0) {
+                if (sp + m - 1 > kQuadDepth) {
+                    if (slot == 0) overflow = 1;
+                    break;
+                }
+                if (in && rank > 0) stk[sp + (m - 1 - rank)] = ch;
+                sp += m - 1;
+                // the nearest survivor: smallest key, then smallest slot
+                int next = -1;
+                float bk = kInf;
+                if (c3 >= 0) { next = c3; bk = k3; }
+                if (c2 >= 0 && !(bk < k2)) { next = c2; bk = k2; }
+                if (c1 >= 0 && !(bk < k1)) { next = c1; bk = k1; }
+                if (c0 >= 0 && !(bk < k0)) { next = c0; bk = k0; }
+                id = next;
+            } else if (sp > 0) {
+                --sp;
+                id = stk[sp];
+            } else {
+                break;
+            }
+        }
+        ClosestResult* dst = out + static_cast<size_t>(q) * outStride;
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            if ((s & 3) == slot && s < k) {
+                ClosestResult r{0u, -1};
+                if (s < list.n) r = {floatBits(list.d[s]), list.id[s]};
+                dst[s] = r;
+            }
+        }
+    }
+    flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
+}
+
 // ---- radius: count and fill passes ---------------------------------------------------
 // kFill = false counts matches per query; kFill = true writes the 64-bit keys
 // (dist2 bits << 32 | index) into the query's CSR segment (bounded by the next
@@ -1069,12 +1191,32 @@ void launchRaysSave(const TraverseArgs& a, const Ray* rays, int count, uint64_t*
     launchRaysT<true>(a, rays, count, out, hitPrim, s);
 }
 
+namespace {
+int gTraversalMode = 0;  // 0: automatic, 1: one lane per query, 2: four lanes per query
+}
+
+void setTraversalMode(int mode) { gTraversalMode = mode; }
+
+// Four lanes per query while the query set leaves most of the GPU idle one lane
+// per query (fewer than 4 waves per SIMD): 256 CUs x 4 SIMDs x 4 waves x 64 lanes.
+static bool useQuad(int count) {
+    if (gTraversalMode == 1) return false;
+    if (gTraversalMode == 2) return true;
+    return count < 256 * 4 * 4 * 64;
+}
+
 template <typename L, int K>
 static void launchNearestTK(const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
                             ClosestResult* out, hipStream_t s) {
     const int grid = blocksFor(count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
-    if (a.wide) {
+    if (a.wide && useQuad(count)) {
+        const int qgrid = blocksFor(4LL * count, kBlock);
+        if (a.stats)
+            nearestQuadKernel<L, K, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats);
+        else
+            nearestQuadKernel<L, K, false><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr);
+    } else if (a.wide) {
         if (a.stats)
             nearestWideKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats);
         else

@@ -268,3 +268,24 @@ def test_radius_segments_above_workgroup_capacity(A, orc):
     for wide in (False, True):
         o, k, _, _ = b.radius(q, 4.0, order=A.ORDER_MORTON, wide=wide)
         assert eq(o, want[0]) and eq(k, want[1]), wide
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_traversal_modes_agree(A, orc, scenes, kind):
+    """One lane per query and four lanes per query give the same bits."""
+    rng = np.random.default_rng(31 + kind)
+    items = {0: lambda: np.concatenate([scenes.terrain(20, 3), random_tris(rng, 400, 10)]),
+             1: lambda: scenes.point_cloud(5000, 4), 2: lambda: scenes.gaussians(3000, 5)}[kind]()
+    q = rng.uniform(-4, 68 if kind else 24, (900, 3)).astype(np.float32)
+    b = A.Bvh(kind, items)
+    b.build_wide()
+    want = {k: orc.knn_brute(kind, items, q, k, threads=THREADS) for k in (0, 1, 5, 16, 32)}
+    try:
+        for mode in (1, 2, 0):
+            A.check(A.lib().qr_set_traversal_mode(mode))
+            for k, w in want.items():
+                got = b.closest(q, order=A.ORDER_MORTON, wide=True)[0] if k == 0 else \
+                    b.knn(q, k, order=A.ORDER_MORTON, wide=True)[0]
+                assert eq(got, w), (mode, k)
+    finally:
+        A.check(A.lib().qr_set_traversal_mode(0))
