@@ -798,6 +798,88 @@ __global__ __launch_bounds__(kBlock) void nearestQuadKernel(
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
+// Radius with four lanes per query: lane s tests entry s; hits of a node are
+// counted (count pass) or written at the quad's cursor plus the number of hits in
+// lower slots (fill pass, bounded by the query's segment end).
+template <typename L, bool kFill, bool kStats>
+__global__ __launch_bounds__(kBlock) void radiusQuadKernel(
+    const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
+    const int* __restrict__ primIds, const V3* __restrict__ points, const uint32_t* __restrict__ order,
+    int count, float r2, int* __restrict__ counts, const int* __restrict__ offsets,
+    uint64_t* __restrict__ keys, unsigned long long* __restrict__ stats) {
+    __shared__ int lds[kQuadDepth * (kBlock / 4)];
+    const int t = xcdBlock() * kBlock + threadIdx.x;
+    const int qi = t >> 2;
+    const int slot = threadIdx.x & 3;
+    int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
+    if (qi < count) {
+        const int q = order ? static_cast<int>(order[qi]) : qi;
+        const V3 p = points[q];
+        int found = 0;
+        int write = kFill ? offsets[q] : 0;
+        const int end = kFill ? offsets[q + 1] : 0;
+        LdsInt* stk = (LdsInt*)(lds + (threadIdx.x >> 2) * kQuadDepth);
+        int sp = 0;
+        int id = 0;
+        while (true) {
+            if (kStats && slot == 0) ++opened;
+            const WideEntry e = wide[id].c[slot];
+            const int ch = e.child;
+            const bool leaf = ch < 0 && ch != kWideEmpty;
+            typename L::Item it;
+            int pid = 0;
+            if (leaf) {
+                it = prims[-ch - 1];
+                if (kFill) pid = primIds[-ch - 1];
+            }
+            const bool adm = ch != kWideEmpty && boxDist2(p, entryBox(e)) <= r2;
+            if (adm) ++nodeTests;
+            int hit = 0;
+            float d2 = 0.0f;
+            if (adm && leaf) {
+                ++leafTests;
+                d2 = L::dist2(p, it);
+                hit = d2 <= r2 ? 1 : 0;
+            }
+            const int h0 = quadGet<0>(hit), h1 = quadGet<1>(hit), h2 = quadGet<2>(hit), h3 = quadGet<3>(hit);
+            if (kFill) {
+                const int before = (slot > 0 ? h0 : 0) + (slot > 1 ? h1 : 0) + (slot > 2 ? h2 : 0);
+                const int at = write + before;
+                if (hit && at < end)
+                    keys[at] = (static_cast<uint64_t>(floatBits(d2)) << 32) | static_cast<uint32_t>(pid);
+                write += h0 + h1 + h2 + h3;
+            } else {
+                found += hit;
+            }
+            const int child = (adm && !leaf) ? ch : -1;
+            const int c0 = quadGet<0>(child), c1 = quadGet<1>(child), c2 = quadGet<2>(child), c3 = quadGet<3>(child);
+            const int m = (c0 >= 0) + (c1 >= 0) + (c2 >= 0) + (c3 >= 0);
+            if (m > 0) {
+                if (sp + m - 1 > kQuadDepth) {
+                    if (slot == 0) overflow = 1;
+                    break;
+                }
+                // the lowest surviving slot continues; the others are pushed in slot order
+                const int first = c0 >= 0 ? 0 : (c1 >= 0 ? 1 : (c2 >= 0 ? 2 : 3));
+                const int below = (slot > 0 && c0 >= 0) + (slot > 1 && c1 >= 0) + (slot > 2 && c2 >= 0);
+                if (child >= 0 && slot != first) stk[sp + below - 1] = child;
+                sp += m - 1;
+                id = first == 0 ? c0 : (first == 1 ? c1 : (first == 2 ? c2 : c3));
+            } else if (sp > 0) {
+                --sp;
+                id = stk[sp];
+            } else {
+                break;
+            }
+        }
+        if (!kFill) {
+            const int total = quadGet<0>(found) + quadGet<1>(found) + quadGet<2>(found) + quadGet<3>(found);
+            if (slot == 0) counts[q] = total;
+        }
+    }
+    flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
+}
+
 // ---- radius: count and fill passes ---------------------------------------------------
 // kFill = false counts matches per query; kFill = true writes the 64-bit keys
 // (dist2 bits << 32 | index) into the query's CSR segment (bounded by the next
@@ -1272,7 +1354,13 @@ static void launchRadiusT(const TraverseArgs& a, const V3* pts, int count, float
     const int grid = blocksFor(count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
     unsigned long long* st = kFill ? nullptr : a.stats;
-    if (a.wide) {
+    if (a.wide && useQuad(count)) {
+        const int qgrid = blocksFor(4LL * count, kBlock);
+        if (st)
+            radiusQuadKernel<L, kFill, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, st);
+        else
+            radiusQuadKernel<L, kFill, false><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, nullptr);
+    } else if (a.wide) {
         if (st)
             radiusKernel<L, true, kFill, true><<<grid, kBlock, 0, s>>>(a.nodes, a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, st);
         else

@@ -289,3 +289,21 @@ def test_traversal_modes_agree(A, orc, scenes, kind):
                 assert eq(got, w), (mode, k)
     finally:
         A.check(A.lib().qr_set_traversal_mode(0))
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_radius_modes_agree(A, orc, scenes, kind):
+    rng = np.random.default_rng(41 + kind)
+    items = {0: lambda: np.concatenate([scenes.terrain(20, 3), random_tris(rng, 400, 10)]),
+             1: lambda: scenes.point_cloud(5000, 4), 2: lambda: scenes.gaussians(3000, 5)}[kind]()
+    q = rng.uniform(-4, 68 if kind else 24, (700, 3)).astype(np.float32)
+    b = A.Bvh(kind, items)
+    b.build_wide()
+    want = orc.radius_brute(kind, items, q, 2.5, threads=THREADS)
+    try:
+        for mode in (1, 2):
+            A.check(A.lib().qr_set_traversal_mode(mode))
+            o, k, _, _ = b.radius(q, 2.5, order=A.ORDER_MORTON, wide=True)
+            assert eq(o, want[0]) and eq(k, want[1]), mode
+    finally:
+        A.check(A.lib().qr_set_traversal_mode(0))
