@@ -337,22 +337,6 @@ __device__ __forceinline__ int descend(const int (&c)[4], const bool (&ok)[4], L
     return next;
 }
 
-// Same pipelining as the wide point queries: leaf triangles are requested with the
-// node, the nearest interior child before the leaves are intersected.
-template <bool kSave>
-__device__ __forceinline__ void rayLeafItem(const Ray& ray, const Tri& tri, int sid, float& best, bool& any,
-                                            int& bestId) {
-    float t;
-    if (!rayTri(ray, tri, t)) return;
-    if (t < best) {
-        best = t;
-        any = true;
-        if (kSave) bestId = sid;
-    } else if (kSave && any && t == best) {
-        if (sid < bestId) bestId = sid;
-    }
-}
-
 template <bool kStats, bool kSave>
 __global__ __launch_bounds__(kBlock) void rayWideKernel(const Wide4* __restrict__ wide,
                                                         const Tri* __restrict__ tris,
@@ -374,68 +358,36 @@ __global__ __launch_bounds__(kBlock) void rayWideKernel(const Wide4* __restrict_
         LaneStack st;
         SpillArea spill;
         st.init(lds);
-        WideEntry e[4];
-        loadWide(wide, 0, e);
-        while (true) {
+        int id = 0;
+        while (id >= 0) {
             if (kStats) ++opened;
-            Tri it[4];
-            int pid[4];
-            bool leaf[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int ch = e[s].child;
-                leaf[s] = ch < 0 && ch != kWideEmpty;
-                pid[s] = 0;
-                if (leaf[s]) {
-                    it[s] = tris[-ch - 1];
-                    if (kSave) pid[s] = primIds[-ch - 1];
-                }
-            }
+            WideEntry e[4];
+            loadWide(wide, id, e);
             float key[4];
             int c[4];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                c[s] = kWideEmpty;
+                c[s] = e[s].child;
                 key[s] = kInf;
                 float t;
-                if (e[s].child != kWideEmpty && rayBox(ray, entryBox(e[s]), best, t)) {
+                if (c[s] != kWideEmpty && rayBox(ray, entryBox(e[s]), best, t)) {
+                    key[s] = t;
                     ++nodeTests;
-                    if (!leaf[s]) {
-                        key[s] = t;
-                        c[s] = e[s].child;
-                    }
                 } else {
-                    leaf[s] = false;
+                    c[s] = kWideEmpty;
                 }
             }
             sort4(key, c);
-            const bool hasNext = c[0] != kWideEmpty;
-            WideEntry en[4];
-            if (hasNext) loadWide(wide, c[0], en);
+            bool ok[4];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                if (leaf[s]) {
+                ok[s] = c[s] != kWideEmpty;
+                if (ok[s] && c[s] < 0) {
                     ++leafTests;
-                    rayLeafItem<kSave>(ray, it[s], pid[s], best, any, bestId);
+                    rayLeaf<kSave>(ray, tris, primIds, -c[s] - 1, best, any, bestId);
                 }
             }
-            if (!st.room(3)) {
-                overflow = 1;
-                break;
-            }
-#pragma unroll
-            for (int s = 3; s >= 1; --s)
-                if (c[s] != kWideEmpty) QR_PUSH(st, spill, c[s]);
-            if (hasNext) {
-#pragma unroll
-                for (int s = 0; s < 4; ++s) e[s] = en[s];
-            } else if (!st.empty()) {
-                int nid;
-                QR_POP(st, spill, nid);
-                loadWide(wide, nid, e);
-            } else {
-                break;
-            }
+            id = descend(c, ok, st, spill, overflow);
         }
         out[q] = any ? ((uint64_t{1} << 32) | floatBits(best)) : 0ull;
         if (kSave) hitPrim[q] = any ? bestId : -1;
@@ -580,12 +532,6 @@ __global__ void batchResetKernel(int* __restrict__ counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
 }
 
-// Software-pipelined wide walk: the leaf children's primitives are requested as
-// soon as a node arrives (before its box tests), and the nearest interior child
-// is requested before the leaves are evaluated, so each iteration waits for one
-// memory round trip instead of one per leaf plus one per node.  Leaves are
-// offered in slot order (the k-best list is exact whatever the order); the
-// prefetched child is used only if it still survives the tightened bound.
 template <typename L, int K, bool kStats>
 __global__ __launch_bounds__(kBlock) void nearestWideKernel(
     const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
@@ -603,73 +549,41 @@ __global__ __launch_bounds__(kBlock) void nearestWideKernel(
         LaneStack st;
         SpillArea spill;
         st.init(lds);
-        WideEntry e[4];
-        loadWide(wide, 0, e);
-        while (true) {
+        int id = 0;
+        while (id >= 0) {
             if (kStats) ++opened;
-            typename L::Item it[4];
-            int pid[4];
-            bool leaf[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int ch = e[s].child;
-                leaf[s] = ch < 0 && ch != kWideEmpty;
-                pid[s] = 0;
-                if (leaf[s]) {
-                    it[s] = prims[-ch - 1];
-                    pid[s] = primIds[-ch - 1];
-                }
-            }
-            float key[4], lkey[4];
+            WideEntry e[4];
+            loadWide(wide, id, e);
+            float key[4];
             int c[4];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                c[s] = kWideEmpty;
+                c[s] = e[s].child;
                 key[s] = kInf;
-                lkey[s] = kInf;
-                if (e[s].child != kWideEmpty) {
+                if (c[s] != kWideEmpty) {
                     const float d = boxDist2(p, entryBox(e[s]));
                     if (list.admits(d)) {
+                        key[s] = d;
                         ++nodeTests;
-                        if (leaf[s]) {
-                            lkey[s] = d;
-                        } else {
-                            key[s] = d;
-                            c[s] = e[s].child;
-                        }
                     } else {
-                        leaf[s] = false;
+                        c[s] = kWideEmpty;
                     }
                 }
             }
             sort4(key, c);
-            const bool hasNext = c[0] != kWideEmpty;
-            WideEntry en[4];
-            if (hasNext) loadWide(wide, c[0], en);
+            // leaves now, nearest first, against the live bound
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                if (leaf[s] && list.admits(lkey[s])) {
+                if (c[s] != kWideEmpty && c[s] < 0 && list.admits(key[s])) {
+                    const int slot = -c[s] - 1;
                     ++leafTests;
-                    list.offer(L::dist2(p, it[s]), pid[s]);
+                    list.offer(L::dist2(p, prims[slot]), primIds[slot]);
                 }
             }
-            if (!st.room(3)) {
-                overflow = 1;
-                break;
-            }
+            bool ok[4];
 #pragma unroll
-            for (int s = 3; s >= 1; --s)
-                if (c[s] != kWideEmpty && list.admits(key[s])) QR_PUSH(st, spill, c[s]);
-            if (hasNext && list.admits(key[0])) {
-#pragma unroll
-                for (int s = 0; s < 4; ++s) e[s] = en[s];
-            } else if (!st.empty()) {
-                int nid;
-                QR_POP(st, spill, nid);
-                loadWide(wide, nid, e);
-            } else {
-                break;
-            }
+            for (int s = 0; s < 4; ++s) ok[s] = c[s] != kWideEmpty && list.admits(key[s]);
+            id = descend(c, ok, st, spill, overflow);
         }
         writeList<L, K>(list, k, out + static_cast<size_t>(q) * outStride);
     }
@@ -916,17 +830,21 @@ __global__ __launch_bounds__(kBlock) void radiusKernel(
         LaneStack st;
         SpillArea spill;
         st.init(lds);
+        int id = -1;
         if (!kWide) {
-            int id = -1;
             const Node2 root = nodes[0];
             if (boxDist2(p, nodeBox(root)) <= r2) {
                 ++nodeTests;
                 id = 0;
             }
-            while (id >= 0) {
-                if (kStats) ++opened;
-                int c[4] = {kWideEmpty, kWideEmpty, kWideEmpty, kWideEmpty};
-                bool ok[4] = {false, false, false, false};
+        } else {
+            id = 0;
+        }
+        while (id >= 0) {
+            if (kStats) ++opened;
+            int c[4] = {kWideEmpty, kWideEmpty, kWideEmpty, kWideEmpty};
+            bool ok[4] = {false, false, false, false};
+            if (!kWide) {
                 const Node2 nd = nodes[id];
                 const int kid[2] = {nd.left, nd.right};
 #pragma unroll
@@ -941,81 +859,23 @@ __global__ __launch_bounds__(kBlock) void radiusKernel(
                         ok[s] = true;
                     }
                 }
-                id = descend(c, ok, st, spill, overflow);
-            }
-        } else {
-            // pipelined like the wide point queries: leaf items with the node, the
-            // first surviving interior child before the leaves are evaluated
-            WideEntry e[4];
-            loadWide(wide, 0, e);
-            while (true) {
-                if (kStats) ++opened;
-                typename L::Item it[4];
-                int pid[4];
-                bool leaf[4];
+            } else {
+                WideEntry e[4];
+                loadWide(wide, id, e);
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     const int ch = e[s].child;
-                    leaf[s] = ch < 0 && ch != kWideEmpty;
-                    pid[s] = 0;
-                    if (leaf[s]) {
-                        it[s] = prims[-ch - 1];
-                        if (kFill) pid[s] = primIds[-ch - 1];
-                    }
-                }
-                int c[4] = {kWideEmpty, kWideEmpty, kWideEmpty, kWideEmpty};
-                int next = -1;
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const int ch = e[s].child;
-                    if (ch == kWideEmpty || boxDist2(p, entryBox(e[s])) > r2) {
-                        leaf[s] = false;
-                        continue;
-                    }
+                    if (ch == kWideEmpty || boxDist2(p, entryBox(e[s])) > r2) continue;
                     ++nodeTests;
-                    if (!leaf[s]) {
-                        if (next < 0) {
-                            next = ch;
-                        } else {
-                            c[s] = ch;
-                        }
+                    if (ch < 0) {
+                        leafTest(-ch - 1);
+                    } else {
+                        c[s] = ch;
+                        ok[s] = true;
                     }
-                }
-                WideEntry en[4];
-                if (next >= 0) loadWide(wide, next, en);
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    if (!leaf[s]) continue;
-                    ++leafTests;
-                    const float d2 = L::dist2(p, it[s]);
-                    if (d2 <= r2) {
-                        if (kFill) {
-                            if (write < end)
-                                keys[write++] = (static_cast<uint64_t>(floatBits(d2)) << 32) |
-                                                static_cast<uint32_t>(pid[s]);
-                        } else {
-                            ++found;
-                        }
-                    }
-                }
-                if (!st.room(3)) {
-                    overflow = 1;
-                    break;
-                }
-#pragma unroll
-                for (int s = 3; s >= 0; --s)
-                    if (c[s] != kWideEmpty) QR_PUSH(st, spill, c[s]);
-                if (next >= 0) {
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) e[s] = en[s];
-                } else if (!st.empty()) {
-                    int nid;
-                    QR_POP(st, spill, nid);
-                    loadWide(wide, nid, e);
-                } else {
-                    break;
                 }
             }
+            id = descend(c, ok, st, spill, overflow);
         }
         if (!kFill) counts[q] = found;
     }
@@ -1279,8 +1139,9 @@ int gTraversalMode = 0;  // 0: automatic, 1: one lane per query, 2: four lanes p
 
 void setTraversalMode(int mode) { gTraversalMode = mode; }
 
-// Four lanes per query while the query set leaves most of the GPU idle one lane
-// per query (fewer than 4 waves per SIMD): 256 CUs x 4 SIMDs x 4 waves x 64 lanes.
+// Closest / kNN: four lanes per query while the query set leaves most of the GPU
+// idle one lane per query (fewer than 4 waves per SIMD: 256 CUs x 4 SIMDs x 4 waves
+// x 64 lanes).
 static bool useQuad(int count) {
     if (gTraversalMode == 1) return false;
     if (gTraversalMode == 2) return true;
@@ -1354,7 +1215,8 @@ static void launchRadiusT(const TraverseArgs& a, const V3* pts, int count, float
     const int grid = blocksFor(count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
     unsigned long long* st = kFill ? nullptr : a.stats;
-    if (a.wide && useQuad(count)) {
+    // radius walks few nodes per query: one lane per query measured faster unless forced
+    if (a.wide && gTraversalMode == 2) {
         const int qgrid = blocksFor(4LL * count, kBlock);
         if (st)
             radiusQuadKernel<L, kFill, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, st);

@@ -39,6 +39,22 @@ def fd(fun, x, idx, h=1e-3):
     return (4.0 * central(fun, x, idx, h / 2) - central(fun, x, idx, h)) / 3.0
 
 
+def check_fd(grad, fn, arr, idx, floor, stats):
+    """The gradient component against the FP64 central difference at h = 1e-3
+    (Richardson-refined with h/2).  Where a closest-feature region boundary lies
+    within h of the sample the loss is C1 but not C2 there and that difference is
+    off by O(h); such samples are checked at h = 1e-6 instead (counted in
+    stats["fine"])."""
+    e = rel(grad, fd(fn, arr, idx), floor)
+    if e < TOL:
+        stats["coarse"] += 1
+        return e
+    e2 = rel(grad, central(fn, arr, idx, 1e-6), floor)
+    stats["fine"] += 1
+    assert e2 < TOL, (idx, grad, fd(fn, arr, idx), central(fn, arr, idx, 1e-6))
+    return e2
+
+
 @pytest.fixture(scope="module")
 def AG():
     from quarry import autograd
@@ -119,6 +135,7 @@ def test_mesh_gradients(AG, sphere):
 
     rng = np.random.default_rng(7)
     worst = 0.0
+    stats = {"coarse": 0, "fine": 0}
     for arr, grad, fn, count in (
             (V64, gV, lambda x: loss_of(Vx=x), 512),
             (q64, gq, lambda x: loss_of(qx=x), 256),
@@ -126,10 +143,9 @@ def test_mesh_gradients(AG, sphere):
             (d64, gd, lambda x: loss_of(dx=x), 128)):
         floor = 0.05 * rms(grad)
         for idx_ in rng.choice(arr.size, count, replace=False):
-            e = rel(grad.flat[idx_], fd(fn, arr, idx_), floor)
-            worst = max(worst, float(e))
-            assert e < TOL, (idx_, grad.flat[idx_], fd(fn, arr, idx_))
-    print("FD max rel err", worst, flush=True)
+            worst = max(worst, float(check_fd(grad.flat[idx_], fn, arr, idx_, floor, stats)))
+    print("FD max rel err", worst, stats, flush=True)
+    assert stats["coarse"] + stats["fine"] == 1024 and stats["fine"] <= 16
 
 
 def test_gaussian_gradients(AG, orc, scenes):
@@ -165,12 +181,13 @@ def test_gaussian_gradients(AG, orc, scenes):
     rng = np.random.default_rng(9)
     gflat = np.concatenate([grads["m"], grads["r"], grads["s"]], 1)
     floor_g = {c: 0.05 * rms(gflat[:, c]) for c in range(10)}
+    stats = {"coarse": 0, "fine": 0}
     for idx_ in rng.choice(G64.size, 768, replace=False):
-        e = rel(gflat.flat[idx_], fd(lambda x: loss_of(gx=x), G64, idx_), floor_g[idx_ % 10])
-        assert e < TOL, (idx_ % 10, gflat.flat[idx_])
+        check_fd(gflat.flat[idx_], lambda x: loss_of(gx=x), G64, idx_, floor_g[idx_ % 10], stats)
     for idx_ in rng.choice(q64.size, 256, replace=False):
-        e = rel(grads["q"].flat[idx_], fd(lambda x: loss_of(qx=x), q64, idx_), 0.05 * rms(grads["q"]))
-        assert e < TOL
+        check_fd(grads["q"].flat[idx_], lambda x: loss_of(qx=x), q64, idx_, 0.05 * rms(grads["q"]), stats)
+    print("FD checks", stats, flush=True)
+    assert stats["coarse"] + stats["fine"] == 1024 and stats["fine"] <= 16
 
 
 def test_point_gradients(AG, orc, scenes):
