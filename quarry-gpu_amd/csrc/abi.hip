@@ -39,6 +39,9 @@ struct qr_points {
     DevBuf<uint32_t> keys, keysSorted, order, bb;
     bool bbValid = false;
     SortWorkspace ws;
+    // radius scratch, kept across calls: per-query counts, scan / classification
+    // scratch, and [0] = largest per-query count
+    DevBuf<int> rCounts, rScratch, rMax;
 };
 
 struct qr_csr {
@@ -1254,14 +1257,19 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
                 a.order = reorder(*p, reinterpret_cast<const float*>(p->pts.p), 3, n,
                                   p->bbValid ? kBoxGiven : kBoxMeasure, s);
             ev.record(1, s);
-            DevBuf<int> counts(static_cast<size_t>(n));
-            DevBuf<int> scratch(static_cast<size_t>(std::max(scanScratchInts(n), 4 + 2 * n)));
+            p->rCounts.reserve(static_cast<size_t>(n));
+            p->rScratch.reserve(static_cast<size_t>(std::max(scanScratchInts(n), 4 + 2 * n)));
+            p->rMax.reserve(1);
+            int* counts = p->rCounts.p;
+            int* scratch = p->rScratch.p;
+            zeroInts(p->rMax.p, 1, s);
             kev.record(0, s);
-            launchRadiusCount(a, p->pts.p, n, r2, counts.p, s);
+            launchRadiusCount(a, p->pts.p, n, r2, counts, p->rMax.p, s);
             kev.record(1, s);
-            scanExclusive(counts.p, res->offsets.p, n, res->offsets.p + n, scratch.p, s);
-            int total = 0;
+            scanExclusive(counts, res->offsets.p, n, res->offsets.p + n, scratch, s);
+            int total = 0, maxSeg = 0;
             hipCheck(hipMemcpyAsync(&total, res->offsets.p + n, sizeof(int), hipMemcpyDeviceToHost, s), "D2H total");
+            hipCheck(hipMemcpyAsync(&maxSeg, p->rMax.p, sizeof(int), hipMemcpyDeviceToHost, s), "D2H max");
             hipCheck(hipStreamSynchronize(s), "radius total sync");
             res->total = total;
             res->keys.alloc(static_cast<size_t>(total > 0 ? total : 1));
@@ -1271,7 +1279,7 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
                 launchRadiusFill(a, p->pts.p, n, r2, res->offsets.p, res->keys.p, s);
             }
             kev.record(3, s);
-            if (total > 0) segmentSort(res->offsets.p, res->keys.p, n, 1 << 30, scratch.p, s);
+            if (total > 0) segmentSort(res->offsets.p, res->keys.p, n, maxSeg, scratch, s);
             ev.record(2, s);
             ev.sync(2);
             const double total_ms = ev.ms(0, 2);

@@ -18,6 +18,7 @@ struct TraverseArgs {
 };
 
 void zeroStats(unsigned long long* stats, hipStream_t s);
+void zeroInts(int* p, int n, hipStream_t s);
 
 // Quantisation domain of a Morton reordering.
 enum MortonBox : int {
@@ -42,7 +43,8 @@ void launchClosest(const TraverseArgs& a, const V3* pts, int count, ClosestResul
 // Result j of query q lands at out[q * outStride + j] (outStride >= k).
 void launchKnn(const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
                ClosestResult* out, hipStream_t s);
-void launchRadiusCount(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts,
+// maxCount (may be null): atomically raised to the largest per-query count.
+void launchRadiusCount(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts, int* maxCount,
                        hipStream_t s);
 // offsets has count + 1 entries (CSR, in query order).
 void launchRadiusFill(const TraverseArgs& a, const V3* pts, int count, float r2,

@@ -49,6 +49,17 @@ __device__ __forceinline__ int xcdBlock() {
     return x * q + (x < r ? x : r) + j;
 }
 
+// Largest v over the wave, one atomic per wave (nothing when dst is null).
+__device__ __forceinline__ void waveMaxAtomic(int* dst, int v) {
+    if (dst == nullptr) return;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(dst, v);
+}
+
 // ---- stats -------------------------------------------------------------------------
 // stats[0] node tests, [1] leaf tests, [2] stack overflows, [3] interior nodes opened.
 template <bool kStats>
@@ -720,12 +731,13 @@ __global__ __launch_bounds__(kBlock) void radiusQuadKernel(
     const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
     const int* __restrict__ primIds, const V3* __restrict__ points, const uint32_t* __restrict__ order,
     int count, float r2, int* __restrict__ counts, const int* __restrict__ offsets,
-    uint64_t* __restrict__ keys, unsigned long long* __restrict__ stats) {
+    uint64_t* __restrict__ keys, unsigned long long* __restrict__ stats, int* __restrict__ maxCount) {
     __shared__ int lds[kQuadDepth * (kBlock / 4)];
     const int t = xcdBlock() * kBlock + threadIdx.x;
     const int qi = t >> 2;
     const int slot = threadIdx.x & 3;
     int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
+    int queryHits = 0;
     if (qi < count) {
         const int q = order ? static_cast<int>(order[qi]) : qi;
         const V3 p = points[q];
@@ -789,8 +801,10 @@ __global__ __launch_bounds__(kBlock) void radiusQuadKernel(
         if (!kFill) {
             const int total = quadGet<0>(found) + quadGet<1>(found) + quadGet<2>(found) + quadGet<3>(found);
             if (slot == 0) counts[q] = total;
+            queryHits = total;
         }
     }
+    if (!kFill) waveMaxAtomic(maxCount, queryHits);
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
@@ -804,8 +818,9 @@ __global__ __launch_bounds__(kBlock) void radiusKernel(
     const typename L::Item* __restrict__ prims, const int* __restrict__ primIds,
     const V3* __restrict__ points, const uint32_t* __restrict__ order, int count, float r2,
     int* __restrict__ counts, const int* __restrict__ offsets, uint64_t* __restrict__ keys,
-    unsigned long long* __restrict__ stats) {
+    unsigned long long* __restrict__ stats, int* __restrict__ maxCount) {
     __shared__ int lds[kLdsDepth * kBlock];
+    int queryHits = 0;
     const int i = xcdBlock() * kBlock + threadIdx.x;
     int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
     if (i < count) {
@@ -878,7 +893,9 @@ __global__ __launch_bounds__(kBlock) void radiusKernel(
             id = descend(c, ok, st, spill, overflow);
         }
         if (!kFill) counts[q] = found;
+        queryHits = found;
     }
+    if (!kFill) waveMaxAtomic(maxCount, queryHits);
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
@@ -934,19 +951,26 @@ __global__ __launch_bounds__(kBlock) void segSortWaveKernel(const int* __restric
     for (int i = lane; i < m; i += 64) keys[begin + i] = buf[i];
 }
 
+// Segments of kWaveCap+1 .. kBlockCap keys, listed by segClassifyKernel (info[0] of
+// them); each workgroup takes list entries blockIdx.x, blockIdx.x + gridDim.x, ...
 __global__ __launch_bounds__(kBlock) void segSortBlockKernel(const int* __restrict__ offsets,
                                                              const int* __restrict__ segList,
+                                                             const int* __restrict__ info,
                                                              uint64_t* __restrict__ keys) {
     extern __shared__ uint64_t dyn[];
-    const int seg = segList[blockIdx.x];
-    const int begin = offsets[seg];
-    const int m = offsets[seg + 1] - begin;
-    int cap = 2;
-    while (cap < m) cap <<= 1;
-    for (int i = threadIdx.x; i < cap; i += kBlock) dyn[i] = i < m ? keys[begin + i] : ~0ull;
-    __syncthreads();
-    bitonicLds(dyn, cap, threadIdx.x, kBlock, true);
-    for (int i = threadIdx.x; i < m; i += kBlock) keys[begin + i] = dyn[i];
+    const int listed = info[0];
+    for (int e = blockIdx.x; e < listed; e += gridDim.x) {
+        const int seg = segList[e];
+        const int begin = offsets[seg];
+        const int m = offsets[seg + 1] - begin;
+        int cap = 2;
+        while (cap < m) cap <<= 1;
+        for (int i = threadIdx.x; i < cap; i += kBlock) dyn[i] = i < m ? keys[begin + i] : ~0ull;
+        __syncthreads();
+        bitonicLds(dyn, cap, threadIdx.x, kBlock, true);
+        for (int i = threadIdx.x; i < m; i += kBlock) keys[begin + i] = dyn[i];
+        __syncthreads();
+    }
 }
 
 // Collects segments longer than the wave cap; info[0] = count of block-sized ones,
@@ -959,6 +983,11 @@ __global__ void segClassifyKernel(const int* __restrict__ offsets, int count,
     const int m = offsets[s + 1] - offsets[s];
     if (m > kWaveCap && m <= kBlockCap) blockList[atomicAdd(&info[0], 1)] = s;
     if (m > kBlockCap) hugeList[atomicAdd(&info[1], 1)] = s;
+}
+
+__global__ void zeroIntsKernel(int* __restrict__ p, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 __global__ void splitKeysKernel(const uint64_t* __restrict__ keys, int m, uint32_t* __restrict__ lo,
@@ -1077,6 +1106,10 @@ __global__ void zeroU64Kernel(unsigned long long* __restrict__ p, int n) {
 
 // ---------------------------------------------------------------------------------------
 // Host launchers.
+
+void zeroInts(int* p, int n, hipStream_t s) {
+    zeroIntsKernel<<<blocksFor(n, kBlock), kBlock, 0, s>>>(p, n);
+}
 
 void zeroStats(unsigned long long* stats, hipStream_t s) {
     zeroU64Kernel<<<1, kBlock, 0, s>>>(stats, 4);
@@ -1211,7 +1244,7 @@ void launchKnn(const TraverseArgs& a, const V3* pts, int count, int k, int outSt
 
 template <typename L, bool kFill>
 static void launchRadiusT(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts,
-                          const int* offsets, uint64_t* keys, hipStream_t s) {
+                          const int* offsets, uint64_t* keys, int* maxCount, hipStream_t s) {
     const int grid = blocksFor(count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
     unsigned long long* st = kFill ? nullptr : a.stats;
@@ -1219,29 +1252,29 @@ static void launchRadiusT(const TraverseArgs& a, const V3* pts, int count, float
     if (a.wide && gTraversalMode == 2) {
         const int qgrid = blocksFor(4LL * count, kBlock);
         if (st)
-            radiusQuadKernel<L, kFill, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, st);
+            radiusQuadKernel<L, kFill, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, st, maxCount);
         else
-            radiusQuadKernel<L, kFill, false><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, nullptr);
+            radiusQuadKernel<L, kFill, false><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, nullptr, maxCount);
     } else if (a.wide) {
         if (st)
-            radiusKernel<L, true, kFill, true><<<grid, kBlock, 0, s>>>(a.nodes, a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, st);
+            radiusKernel<L, true, kFill, true><<<grid, kBlock, 0, s>>>(a.nodes, a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, st, maxCount);
         else
-            radiusKernel<L, true, kFill, false><<<grid, kBlock, 0, s>>>(a.nodes, a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, nullptr);
+            radiusKernel<L, true, kFill, false><<<grid, kBlock, 0, s>>>(a.nodes, a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, nullptr, maxCount);
     } else {
         if (st)
-            radiusKernel<L, false, kFill, true><<<grid, kBlock, 0, s>>>(a.nodes, a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, st);
+            radiusKernel<L, false, kFill, true><<<grid, kBlock, 0, s>>>(a.nodes, a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, st, maxCount);
         else
-            radiusKernel<L, false, kFill, false><<<grid, kBlock, 0, s>>>(a.nodes, a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, nullptr);
+            radiusKernel<L, false, kFill, false><<<grid, kBlock, 0, s>>>(a.nodes, a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, nullptr, maxCount);
     }
 }
 
 void launchRadiusCount(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts,
-                       hipStream_t s) {
+                       int* maxCount, hipStream_t s) {
     if (count <= 0) return;
     switch (a.kind) {
-        case 0: launchRadiusT<TriLeaf, false>(a, pts, count, r2, counts, nullptr, nullptr, s); break;
-        case 1: launchRadiusT<PointLeaf, false>(a, pts, count, r2, counts, nullptr, nullptr, s); break;
-        default: launchRadiusT<GaussLeaf, false>(a, pts, count, r2, counts, nullptr, nullptr, s); break;
+        case 0: launchRadiusT<TriLeaf, false>(a, pts, count, r2, counts, nullptr, nullptr, maxCount, s); break;
+        case 1: launchRadiusT<PointLeaf, false>(a, pts, count, r2, counts, nullptr, nullptr, maxCount, s); break;
+        default: launchRadiusT<GaussLeaf, false>(a, pts, count, r2, counts, nullptr, nullptr, maxCount, s); break;
     }
     hipCheck(hipGetLastError(), "radius count");
 }
@@ -1250,16 +1283,16 @@ void launchRadiusFill(const TraverseArgs& a, const V3* pts, int count, float r2,
                       const int* offsets, uint64_t* keys, hipStream_t s) {
     if (count <= 0) return;
     switch (a.kind) {
-        case 0: launchRadiusT<TriLeaf, true>(a, pts, count, r2, nullptr, offsets, keys, s); break;
-        case 1: launchRadiusT<PointLeaf, true>(a, pts, count, r2, nullptr, offsets, keys, s); break;
-        default: launchRadiusT<GaussLeaf, true>(a, pts, count, r2, nullptr, offsets, keys, s); break;
+        case 0: launchRadiusT<TriLeaf, true>(a, pts, count, r2, nullptr, offsets, keys, nullptr, s); break;
+        case 1: launchRadiusT<PointLeaf, true>(a, pts, count, r2, nullptr, offsets, keys, nullptr, s); break;
+        default: launchRadiusT<GaussLeaf, true>(a, pts, count, r2, nullptr, offsets, keys, nullptr, s); break;
     }
     hipCheck(hipGetLastError(), "radius fill");
 }
 
 void segmentSort(const int* offsets, uint64_t* keys, int count, int maxSeg, int* scratchInts,
                  hipStream_t s) {
-    if (count <= 0) return;
+    if (count <= 0 || maxSeg <= 1) return;
     const int wavesPerBlock = kBlock / 64;
     segSortWaveKernel<<<blocksFor(count, wavesPerBlock), kBlock, 0, s>>>(offsets, keys, count);
     hipCheck(hipGetLastError(), "segment sort (wave)");
@@ -1271,13 +1304,14 @@ void segmentSort(const int* offsets, uint64_t* keys, int count, int maxSeg, int*
     zeroU64Kernel<<<1, kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(info), 2);
     segClassifyKernel<<<blocksFor(count, kBlock), kBlock, 0, s>>>(offsets, count, blockList,
                                                                   hugeList, info);
+    // the block-sized segments: a fixed grid walks the device-side list (no host round trip)
+    const int grid = count < 1024 ? count : 1024;
+    segSortBlockKernel<<<grid, kBlock, kBlockCap * sizeof(uint64_t), s>>>(offsets, blockList, info, keys);
+    hipCheck(hipGetLastError(), "segment sort (block)");
+    if (maxSeg <= kBlockCap) return;
     int h[2] = {0, 0};
     hipCheck(hipMemcpyAsync(h, info, sizeof(h), hipMemcpyDeviceToHost, s), "seg info");
     hipCheck(hipStreamSynchronize(s), "seg info sync");
-    if (h[0] > 0) {
-        segSortBlockKernel<<<h[0], kBlock, kBlockCap * sizeof(uint64_t), s>>>(offsets, blockList, keys);
-        hipCheck(hipGetLastError(), "segment sort (block)");
-    }
     if (h[1] > 0) {
         // Segments above the workgroup network's capacity: each is sorted by the
         // device radix sort as two stable 32-bit passes (index half, then distance
