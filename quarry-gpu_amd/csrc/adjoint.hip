@@ -176,27 +176,14 @@ __global__ void adjPointKernel(const V3* __restrict__ items, const int* __restri
     }
 }
 
-// d(axes)/d(normalised quaternion): the rotation columns u, v, w of gaussAxes.
-__device__ void quatGrad(const Quat& q, V3 gu, V3 gv, V3 gw, float out[4]) {
-    const float lenSq = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
-    const float inv = lenSq > 0.0f ? 1.0f / sqrtf(lenSq) : 1.0f;
-    const float x = q.x * inv, y = q.y * inv, z = q.z * inv, w = q.w * inv;
-    // u = (1-2(yy+zz), 2(xy+zw), 2(xz-yw)); v = (2(xy-zw), 1-2(xx+zz), 2(yz+xw));
-    // w = (2(xz+yw), 2(yz-xw), 1-2(xx+yy))
-    const float gx = gu.y * 2 * y + gu.z * 2 * z + gv.x * 2 * y + gv.y * -4 * x + gv.z * 2 * w + gw.x * 2 * z +
-                     gw.y * -2 * w + gw.z * -4 * x;
-    const float gy = gu.x * -4 * y + gu.y * 2 * x + gu.z * -2 * w + gv.x * 2 * x + gv.z * 2 * z + gw.x * 2 * w +
-                     gw.y * 2 * z + gw.z * -4 * y;
-    const float gz = gu.x * -4 * z + gu.y * 2 * w + gu.z * 2 * x + gv.x * -2 * w + gv.y * -4 * z + gv.z * 2 * y +
-                     gw.x * 2 * x + gw.y * 2 * y;
-    const float gw_ = gu.y * 2 * z + gu.z * -2 * y + gv.x * -2 * z + gv.z * 2 * x + gw.x * 2 * y + gw.y * -2 * x;
-    // through the normalisation n = q / |q|: dL/dq = (gn - n (n . gn)) / |q|
-    const float dotng = x * gx + y * gy + z * gz + w * gw_;
-    out[0] = (gx - x * dotng) * inv;
-    out[1] = (gy - y * dotng) * inv;
-    out[2] = (gz - z * dotng) * inv;
-    out[3] = (gw_ - w * dotng) * inv;
-}
+// The Gaussian adjoint runs in double precision: the quaternion gradient is a sum
+// of terms much larger than the result (it is projected onto the tangent of the
+// unit sphere), which single precision would leave with ~1e-4 relative error.  The
+// saved root t comes from the FP32 forward.
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ double dotd(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
 __global__ void adjGaussKernel(const Gaussian* __restrict__ items, const int* __restrict__ primSlot,
                                const V3* __restrict__ pts, const ClosestResult* __restrict__ res,
@@ -206,40 +193,65 @@ __global__ void adjGaussKernel(const Gaussian* __restrict__ items, const int* __
     if (i >= n) return;
     const ClosestResult r = res[i];
     const float g = gout[i];
-    V3 gq{0.0f, 0.0f, 0.0f};
+    double gq[3] = {0.0, 0.0, 0.0};
     if (r.prim >= 0 && g != 0.0f && aux[i] != -2) {
         const Gaussian gs = items[primSlot[r.prim]];
-        const float t = bitsFloat(static_cast<uint32_t>(aux[i]));
-        V3 u, v, w;
-        gaussAxes(gs.rot, u, v, w);
-        const V3 d = sub(pts[i], gs.mean);
-        const float y[3] = {dot(d, u), dot(d, v), dot(d, w)};
-        const float s[3] = {gs.scale.x, gs.scale.y, gs.scale.z};
-        float e[3], pk[3];
+        const double t = static_cast<double>(bitsFloat(static_cast<uint32_t>(aux[i])));
+        const double qx0 = gs.rot.x, qy0 = gs.rot.y, qz0 = gs.rot.z, qw0 = gs.rot.w;
+        const double len2 = qx0 * qx0 + qy0 * qy0 + qz0 * qz0 + qw0 * qw0;
+        const double inv = len2 > 0.0 ? 1.0 / sqrt(len2) : 1.0;
+        const double x = qx0 * inv, y = qy0 * inv, z = qz0 * inv, w = qw0 * inv;
+        const D3 u{1 - 2 * (y * y + z * z), 2 * (x * y + z * w), 2 * (x * z - y * w)};
+        const D3 v{2 * (x * y - z * w), 1 - 2 * (x * x + z * z), 2 * (y * z + x * w)};
+        const D3 ww{2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y)};
+        const V3 q = pts[i];
+        const D3 d{static_cast<double>(q.x) - gs.mean.x, static_cast<double>(q.y) - gs.mean.y,
+                   static_cast<double>(q.z) - gs.mean.z};
+        const double yy[3] = {dotd(d, u), dotd(d, v), dotd(d, ww)};
+        const double sc[3] = {gs.scale.x, gs.scale.y, gs.scale.z};
+        double e[3], pk[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const float ss = s[k] * s[k];
-            pk[k] = ss * y[k] / (t + ss);
-            e[k] = y[k] - pk[k];
+            const double ss = sc[k] * sc[k];
+            pk[k] = ss * yy[k] / (t + ss);
+            e[k] = yy[k] - pk[k];
         }
-        const float g2 = 2.0f * g;
-        gq = scale(add(add(scale(u, e[0]), scale(v, e[1])), scale(w, e[2])), g2);
+        const double g2 = 2.0 * static_cast<double>(g);
+        gq[0] = g2 * (e[0] * u.x + e[1] * v.x + e[2] * ww.x);
+        gq[1] = g2 * (e[0] * u.y + e[1] * v.y + e[2] * ww.y);
+        gq[2] = g2 * (e[0] * u.z + e[1] * v.z + e[2] * ww.z);
         if (gitems) {
             float* dst = gitems + static_cast<size_t>(r.prim) * 10;
-            atomicAdd3(dst + 0, scale(gq, -1.0f));
-            float gqt[4];
-            quatGrad(gs.rot, scale(d, g2 * e[0]), scale(d, g2 * e[1]), scale(d, g2 * e[2]), gqt);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) atomicAdd(dst + 3 + k, gqt[k]);
+            atomicAdd(dst + 0, static_cast<float>(-gq[0]));
+            atomicAdd(dst + 1, static_cast<float>(-gq[1]));
+            atomicAdd(dst + 2, static_cast<float>(-gq[2]));
+            // dL/d(axis k) = g2 e_k d; chained through the rotation columns (see the
+            // matrix above) and the normalisation n = q / |q|
+            const D3 gu{g2 * e[0] * d.x, g2 * e[0] * d.y, g2 * e[0] * d.z};
+            const D3 gv{g2 * e[1] * d.x, g2 * e[1] * d.y, g2 * e[1] * d.z};
+            const D3 gw{g2 * e[2] * d.x, g2 * e[2] * d.y, g2 * e[2] * d.z};
+            const double gx = gu.y * 2 * y + gu.z * 2 * z + gv.x * 2 * y + gv.y * -4 * x + gv.z * 2 * w +
+                              gw.x * 2 * z + gw.y * -2 * w + gw.z * -4 * x;
+            const double gy = gu.x * -4 * y + gu.y * 2 * x + gu.z * -2 * w + gv.x * 2 * x + gv.z * 2 * z +
+                              gw.x * 2 * w + gw.y * 2 * z + gw.z * -4 * y;
+            const double gz = gu.x * -4 * z + gu.y * 2 * w + gu.z * 2 * x + gv.x * -2 * w + gv.y * -4 * z +
+                              gv.z * 2 * y + gw.x * 2 * x + gw.y * 2 * y;
+            const double gwq = gu.y * 2 * z + gu.z * -2 * y + gv.x * -2 * z + gv.z * 2 * x + gw.x * 2 * y +
+                               gw.y * -2 * x;
+            const double dn = x * gx + y * gy + z * gz + w * gwq;
+            atomicAdd(dst + 3, static_cast<float>((gx - x * dn) * inv));
+            atomicAdd(dst + 4, static_cast<float>((gy - y * dn) * inv));
+            atomicAdd(dst + 5, static_cast<float>((gz - z * dn) * inv));
+            atomicAdd(dst + 6, static_cast<float>((gwq - w * dn) * inv));
 #pragma unroll
             for (int k = 0; k < 3; ++k)
-                atomicAdd(dst + 7 + k, -g2 * t * pk[k] * pk[k] / (s[k] * s[k] * s[k]));
+                atomicAdd(dst + 7 + k, static_cast<float>(-g2 * t * pk[k] * pk[k] / (sc[k] * sc[k] * sc[k])));
         }
     }
     if (gpts) {
-        gpts[3 * i + 0] = gq.x;
-        gpts[3 * i + 1] = gq.y;
-        gpts[3 * i + 2] = gq.z;
+        gpts[3 * i + 0] = static_cast<float>(gq[0]);
+        gpts[3 * i + 1] = static_cast<float>(gq[1]);
+        gpts[3 * i + 2] = static_cast<float>(gq[2]);
     }
 }
 
