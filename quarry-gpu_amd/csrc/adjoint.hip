@@ -196,7 +196,7 @@ __global__ void adjGaussKernel(const Gaussian* __restrict__ items, const int* __
     double gq[3] = {0.0, 0.0, 0.0};
     if (r.prim >= 0 && g != 0.0f && aux[i] != -2) {
         const Gaussian gs = items[primSlot[r.prim]];
-        const double t = static_cast<double>(bitsFloat(static_cast<uint32_t>(aux[i])));
+        const double t32 = static_cast<double>(bitsFloat(static_cast<uint32_t>(aux[i])));
         const double qx0 = gs.rot.x, qy0 = gs.rot.y, qz0 = gs.rot.z, qw0 = gs.rot.w;
         const double len2 = qx0 * qx0 + qy0 * qy0 + qz0 * qz0 + qw0 * qw0;
         const double inv = len2 > 0.0 ? 1.0 / sqrt(len2) : 1.0;
@@ -209,6 +209,25 @@ __global__ void adjGaussKernel(const Gaussian* __restrict__ items, const int* __
                    static_cast<double>(q.z) - gs.mean.z};
         const double yy[3] = {dotd(d, u), dotd(d, v), dotd(d, ww)};
         const double sc[3] = {gs.scale.x, gs.scale.y, gs.scale.z};
+        // The forward pass solves the multiplier in FP32; the gradient wants the root
+        // of f(t) = sum_k (s_k y_k / (t + s_k^2))^2 - 1 itself.  f is convex and
+        // decreasing for t > -min s_k^2, so Newton from the FP32 root converges
+        // quadratically (three steps reach FP64 precision from ~1e-6 relative).
+        double t = t32;
+        for (int it = 0; it < 3; ++it) {
+            double f = -1.0, df = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double ss = sc[k] * sc[k];
+                const double a = sc[k] * yy[k] / (t + ss);
+                f += a * a;
+                df -= 2.0 * a * a / (t + ss);
+            }
+            if (!(df < 0.0)) break;
+            const double tn = t - f / df;
+            if (!(tn >= 0.0)) break;
+            t = tn;
+        }
         double e[3], pk[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
