@@ -1,15 +1,18 @@
 // LBVH build for gfx950: centroid bounds -> 30-bit Morton codes (fused with the
-// radix histogram) -> onesweep sort -> Karras 2012 hierarchy emission ->
-// fused leaf scatter + bottom-up box propagation with per-node arrival counters.
+// radix histogram) -> onesweep sort -> one pass that emits the hierarchy, gathers
+// the leaves and merges boxes bottom-up (lbvhEmitKernel).
 //
 // Topology and node bytes follow the reference's LBVH exactly
 // (src/gpu/lbvh.cu:277-315 Morton, :366-414 Karras, :345-354 leaf boxes,
 // :472-514 parent box merge), so treeHash equals the reference's for the same
 // input.  What differs is the schedule: the reference sorts 64-bit keys with 8
-// digit passes, computes depths by walking parent chains, depth-sorts the nodes
-// and launches one merge kernel per level; here the sort runs 4 passes and the
-// boxes are merged in a single bottom-up pass where the second child to arrive
-// at a node merges it (no depth table is needed to build).
+// digit passes, runs a Karras kernel (binary searches per node), computes depths
+// by walking parent chains, depth-sorts the nodes and launches one merge kernel
+// per level; here the sort runs 4 passes and the Karras-numbered tree is built
+// agglomeratively in a single bottom-up pass where the second child to arrive at
+// a split merges it.  The two-pass form (karrasNode, then a bottom-up climb over
+// the given parent[]) stays for the batched build and for boxes over a topology
+// that is already known (SAH, refits).
 #include "qr_bvh.h"
 
 namespace qr {
@@ -52,18 +55,29 @@ __device__ __forceinline__ Box readBoxAgent(const Node2* nd) {
 }
 
 // ---- centroid bounds -----------------------------------------------------------
-// cb[0..2] = ordered(min), cb[3..5] = ordered(max) (src/gpu/lbvh.cu:233-275).
+// Per copy c < kBoundsCopies: cb[c * kBoundsStride + 0..2] = ordered(min),
+// [3..5] = ordered(max) (src/gpu/lbvh.cu:233-275).  Workgroup w reduces into copy
+// w % kBoundsCopies (one 128 B line each), which spreads the same-address atomics;
+// the Morton pass folds the copies.
+constexpr int kBoundsCopies = 8;
+constexpr int kBoundsStride = 32;
+constexpr int kBoundsWords = kBoundsCopies * kBoundsStride;
+constexpr int kUnroll = 4;  // items in flight per thread in the streaming passes
 
 __global__ void buildPrepKernel(uint32_t* __restrict__ cb, uint32_t* __restrict__ hist,
                                 uint32_t* __restrict__ desc, long long descWords,
-                                uint32_t* __restrict__ tickets) {
+                                uint32_t* __restrict__ tickets, int* __restrict__ arrive,
+                                int slots) {
     const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
     const long long stride = static_cast<long long>(gridDim.x) * kBlock;
     for (long long j = i; j < descWords; j += stride) desc[j] = 0u;
-    if (i < 4 * 256) hist[i] = 0u;
+    for (long long j = i; j < slots; j += stride) arrive[j] = 0;
+    for (long long j = i; j < kHistCopies * kHistCopyWords; j += stride) hist[j] = 0u;
     if (i < 4) tickets[i] = 0u;
-    if (i < 3) cb[i] = orderedBits(kInf);
-    if (i >= 3 && i < 6) cb[i] = orderedBits(-kInf);
+    if (i < kBoundsWords) {
+        const int c = static_cast<int>(i % kBoundsStride);
+        cb[i] = c < 3 ? orderedBits(kInf) : orderedBits(-kInf);
+    }
 }
 
 template <typename L>
@@ -71,7 +85,24 @@ __global__ __launch_bounds__(kBlock) void centroidBoundsKernel(
     const typename L::Item* __restrict__ items, int n, uint32_t* __restrict__ cb) {
     float lo0 = kInf, lo1 = kInf, lo2 = kInf;
     float hi0 = -kInf, hi1 = -kInf, hi2 = -kInf;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kBlock * gridDim.x) {
+    const int stride = kBlock * gridDim.x;
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    for (; i + (kUnroll - 1) * stride < n; i += kUnroll * stride) {
+        typename L::Item it[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) it[u] = items[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const V3 c = L::centroid(it[u]);
+            lo0 = fminf(lo0, c.x);
+            lo1 = fminf(lo1, c.y);
+            lo2 = fminf(lo2, c.z);
+            hi0 = fmaxf(hi0, c.x);
+            hi1 = fmaxf(hi1, c.y);
+            hi2 = fmaxf(hi2, c.z);
+        }
+    }
+    for (; i < n; i += stride) {
         const V3 c = L::centroid(items[i]);
         lo0 = fminf(lo0, c.x);
         lo1 = fminf(lo1, c.y);
@@ -104,10 +135,11 @@ __global__ __launch_bounds__(kBlock) void centroidBoundsKernel(
         const int c = threadIdx.x;
         float v = s[c][0];
         for (int k = 1; k < kBlock / 64; ++k) v = c < 3 ? fminf(v, s[c][k]) : fmaxf(v, s[c][k]);
+        uint32_t* dst = cb + (blockIdx.x % kBoundsCopies) * kBoundsStride;
         if (c < 3) {
-            atomicMin(&cb[c], orderedBits(v));
+            atomicMin(&dst[c], orderedBits(v));
         } else {
-            atomicMax(&cb[c], orderedBits(v));
+            atomicMax(&dst[c], orderedBits(v));
         }
     }
 }
@@ -136,14 +168,23 @@ __global__ __launch_bounds__(kBlock) void mortonHistKernel(const typename L::Ite
                                                            uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[4][256];
     for (int i = threadIdx.x; i < 4 * 256; i += kBlock) (&h[0][0])[i] = 0u;
-    const float lo0 = fromOrderedBits(cb[0]), lo1 = fromOrderedBits(cb[1]),
-                lo2 = fromOrderedBits(cb[2]);
-    const float hi0 = fromOrderedBits(cb[3]), hi1 = fromOrderedBits(cb[4]),
-                hi2 = fromOrderedBits(cb[5]);
+    uint32_t b[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) b[c] = cb[c];
+#pragma unroll
+    for (int k = 1; k < kBoundsCopies; ++k) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const uint32_t v = cb[k * kBoundsStride + c];
+            b[c] = c < 3 ? (v < b[c] ? v : b[c]) : (v > b[c] ? v : b[c]);
+        }
+    }
+    const float lo0 = fromOrderedBits(b[0]), lo1 = fromOrderedBits(b[1]), lo2 = fromOrderedBits(b[2]);
+    const float hi0 = fromOrderedBits(b[3]), hi1 = fromOrderedBits(b[4]), hi2 = fromOrderedBits(b[5]);
     const float ex0 = hi0 - lo0, ex1 = hi1 - lo1, ex2 = hi2 - lo2;
     __syncthreads();
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kBlock * gridDim.x) {
-        const V3 c = L::centroid(items[i]);
+    auto emit = [&](int i, const typename L::Item& item) {
+        const V3 c = L::centroid(item);
         const uint32_t code = spread10(quantize1023(c.x, lo0, ex0)) |
                               (spread10(quantize1023(c.y, lo1, ex1)) << 1) |
                               (spread10(quantize1023(c.z, lo2, ex2)) << 2);
@@ -152,12 +193,19 @@ __global__ __launch_bounds__(kBlock) void mortonHistKernel(const typename L::Ite
         atomicAdd(&h[1][(code >> 8) & 255u], 1u);
         atomicAdd(&h[2][(code >> 16) & 255u], 1u);
         atomicAdd(&h[3][code >> 24], 1u);
+    };
+    const int stride = kBlock * gridDim.x;
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    for (; i + (kUnroll - 1) * stride < n; i += kUnroll * stride) {
+        typename L::Item it[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) it[u] = items[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) emit(i + u * stride, it[u]);
     }
+    for (; i < n; i += stride) emit(i, items[i]);
     __syncthreads();
-    for (int i = threadIdx.x; i < 4 * 256; i += kBlock) {
-        const uint32_t v = (&h[0][0])[i];
-        if (v) atomicAdd(&hist[i], v);
-    }
+    flushHistogram(&h[0][0], 4 * 256, hist);
 }
 
 // ---- Karras 2012 hierarchy ---------------------------------------------------------
@@ -219,24 +267,13 @@ __device__ __forceinline__ void karrasNode(const uint32_t* __restrict__ code,
     parent[right] = i;
 }
 
-__global__ __launch_bounds__(kBlock) void karrasKernel(const uint32_t* __restrict__ code,
-                                                       const uint32_t* __restrict__ idx, int n,
-                                                       Node2* __restrict__ nodes,
-                                                       int* __restrict__ parent,
-                                                       int* __restrict__ arrive,
-                                                       int* __restrict__ farEnd) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n - 1) return;
-    karrasNode(code, idx, n, i, nodes, parent, arrive, farEnd);
-}
-
 // ---- leaf scatter + bottom-up propagation -------------------------------------------
 // One thread per leaf slot k: gather the item into leaf order, write leaf node
 // n-1+k, then climb.  At each parent the arrival counter decides: the first child
 // to arrive stops, the second merges both child boxes (left operand = left child,
 // as the reference's refit does), resets the counter and continues.  Height
 // (edges to the deepest leaf) rides along, so the root's height is the tree
-// depth.  Counters start at zero (karrasKernel) and end at zero.
+// depth.  Counters start at zero and end at zero.
 template <typename L, bool kGather>
 __device__ __forceinline__ void leafClimb(const typename L::Item* __restrict__ src,
                                           const uint32_t* __restrict__ order, int n, int k,
@@ -285,13 +322,17 @@ __device__ __forceinline__ void leafClimb(const typename L::Item* __restrict__ s
 }
 
 // Bottom-up boxes with the block-local part in LDS.  Workgroup b owns leaf slots
-// [256 b, 256 b + 255].  An internal node p whose leaf range [min(p, far[p]),
-// max(p, far[p])] lies inside that window has both subtrees in this workgroup: its
-// arrival counter, box and height live in LDS (the stores to the node array are
-// ordinary write-back stores).  The first node whose range leaves the window is
-// published with agent-scope stores and from there on the climb follows the
-// global protocol of leafClimb.  Boxes are merged in the same order with the same
-// operands, so the node array is identical to leafClimb's.
+// [256 b, 256 b + 255] = [base, last].  An internal node p whose leaf range
+// [min(p, far[p]), max(p, far[p])] lies inside that window has both subtrees in this
+// workgroup: its arrival counter, box and height live in LDS (the stores to the node
+// array are ordinary write-back stores).  Node ids are range endpoints (Karras
+// numbering, which the SAH builder follows too), so the topology of every node that
+// can be local -- parent[], far[], children of ids base..last, parents of the
+// window's leaves -- is read into LDS up front with coalesced loads and the local
+// climb makes no dependent global loads.  The first node whose range leaves the
+// window is published with agent-scope stores and from there on the climb follows
+// the global protocol of leafClimb.  Boxes are merged in the same order with the
+// same operands, so the node array is identical to leafClimb's.
 template <typename L, bool kGather>
 __global__ __launch_bounds__(kBlock) void leafUpLocalKernel(
     const typename L::Item* __restrict__ src, const uint32_t* __restrict__ order, int n,
@@ -302,24 +343,38 @@ __global__ __launch_bounds__(kBlock) void leafUpLocalKernel(
     __shared__ float sNode[kBlock][6];
     __shared__ int sCount[kBlock];
     __shared__ int sHeight[kBlock];
+    __shared__ int sLeafParent[kBlock];
+    __shared__ int sParent[kBlock];
+    __shared__ int sFar[kBlock];
+    __shared__ int sLeft[kBlock];
+    __shared__ int sRight[kBlock];
     const int base = blockIdx.x * kBlock;
     const int last = (base + kBlock < n ? base + kBlock : n) - 1;
-    sCount[threadIdx.x] = 0;
-    __syncthreads();
     const int k = base + threadIdx.x;
-    if (k >= n) return;
-
-    typename L::Item item;
-    if (kGather) {
-        const int sceneIdx = static_cast<int>(order[k]);
-        item = src[sceneIdx];
-        sorted[k] = item;
-        primIds[k] = sceneIdx;
+    const bool live = k < n;
+    sCount[threadIdx.x] = 0;
+    if (live) sLeafParent[threadIdx.x] = parent[n - 1 + k];
+    if (k < n - 1) {
+        sParent[threadIdx.x] = parent[k];
+        sFar[threadIdx.x] = farEnd[k];
+        sLeft[threadIdx.x] = nodes[k].left;
+        sRight[threadIdx.x] = nodes[k].right;
     } else {
-        item = sorted[k];
+        sFar[threadIdx.x] = -1;  // not an internal node: never local
     }
-    Box box = L::bounds(item);
-    {
+
+    Box box{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+    if (live) {
+        typename L::Item item;
+        if (kGather) {
+            const int sceneIdx = static_cast<int>(order[k]);
+            item = src[sceneIdx];
+            sorted[k] = item;
+            primIds[k] = sceneIdx;
+        } else {
+            item = sorted[k];
+        }
+        box = L::bounds(item);
         Node2 leaf;
         leaf.lo[0] = box.lo.x;
         leaf.lo[1] = box.lo.y;
@@ -330,61 +385,60 @@ __global__ __launch_bounds__(kBlock) void leafUpLocalKernel(
         leaf.left = -(k + 1);
         leaf.right = 1;
         nodes[n - 1 + k] = leaf;
+        float* mine = sLeaf[threadIdx.x];
+        mine[0] = box.lo.x;
+        mine[1] = box.lo.y;
+        mine[2] = box.lo.z;
+        mine[3] = box.hi.x;
+        mine[4] = box.hi.y;
+        mine[5] = box.hi.z;
     }
-    float* mine = sLeaf[threadIdx.x];
-    mine[0] = box.lo.x;
-    mine[1] = box.lo.y;
-    mine[2] = box.lo.z;
-    mine[3] = box.hi.x;
-    mine[4] = box.hi.y;
-    mine[5] = box.hi.z;
+    __syncthreads();
+    if (!live) return;
 
     int node = n - 1 + k;
+    int p = sLeafParent[threadIdx.x];
     int h = 0;
-    bool published = false;
+    // local part: node is a leaf of the window or an internal node with a local range
+    while (p >= base && p <= last) {
+        const int f = sFar[p - base];
+        if (f < base || f > last) break;
+        const int prior = __hip_atomic_fetch_add(&sCount[p - base], 1, __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (prior == 0) return;
+        const int l = sLeft[p - base];
+        const int r = sRight[p - base];
+        const int sib = (l == node) ? r : l;
+        const float* sb = sib >= n - 1 ? sLeaf[sib - (n - 1) - base] : sNode[sib - base];
+        const Box other{{sb[0], sb[1], sb[2]}, {sb[3], sb[4], sb[5]}};
+        const int sh = sib >= n - 1 ? 0 : sHeight[sib - base];
+        box = (l == node) ? mergeChildren(box, other) : mergeChildren(other, box);
+        h = (h > sh ? h : sh) + 1;
+        float* dst = sNode[p - base];
+        dst[0] = box.lo.x;
+        dst[1] = box.lo.y;
+        dst[2] = box.lo.z;
+        dst[3] = box.hi.x;
+        dst[4] = box.hi.y;
+        dst[5] = box.hi.z;
+        sHeight[p - base] = h;
+        Node2* nd = &nodes[p];
+        nd->lo[0] = box.lo.x;
+        nd->lo[1] = box.lo.y;
+        nd->lo[2] = box.lo.z;
+        nd->hi[0] = box.hi.x;
+        nd->hi[1] = box.hi.y;
+        nd->hi[2] = box.hi.z;
+        height[p] = h;
+        node = p;
+        p = sParent[p - base];
+    }
+    if (p < 0) return;  // the whole tree lies in this window
+    // p's range leaves this workgroup's window: publish this subtree's root for the
+    // other child's workgroup, then take the global arrival counters
+    writeBoxAgent(&nodes[node], box);
+    if (node < n - 1) stAgentI(&height[node], h);
     while (true) {
-        const int p = parent[node];
-        if (p < 0) break;
-        const int f = farEnd[p];
-        const int lo = p < f ? p : f, hi = p < f ? f : p;
-        if (lo >= base && hi <= last) {
-            const int prior = __hip_atomic_fetch_add(&sCount[p - base], 1, __ATOMIC_ACQ_REL,
-                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (prior == 0) break;
-            const int l = nodes[p].left;
-            const int r = nodes[p].right;
-            const int sib = (l == node) ? r : l;
-            const float* sb = sib >= n - 1 ? sLeaf[sib - (n - 1) - base] : sNode[sib - base];
-            const Box other{{sb[0], sb[1], sb[2]}, {sb[3], sb[4], sb[5]}};
-            const int sh = sib >= n - 1 ? 0 : sHeight[sib - base];
-            box = (l == node) ? mergeChildren(box, other) : mergeChildren(other, box);
-            h = (h > sh ? h : sh) + 1;
-            float* dst = sNode[p - base];
-            dst[0] = box.lo.x;
-            dst[1] = box.lo.y;
-            dst[2] = box.lo.z;
-            dst[3] = box.hi.x;
-            dst[4] = box.hi.y;
-            dst[5] = box.hi.z;
-            sHeight[p - base] = h;
-            Node2* nd = &nodes[p];
-            nd->lo[0] = box.lo.x;
-            nd->lo[1] = box.lo.y;
-            nd->lo[2] = box.lo.z;
-            nd->hi[0] = box.hi.x;
-            nd->hi[1] = box.hi.y;
-            nd->hi[2] = box.hi.z;
-            height[p] = h;
-            node = p;
-            continue;
-        }
-        // p's range leaves this workgroup's window: publish this subtree's root for the
-        // other child's workgroup, then take the global arrival counter
-        if (!published) {
-            writeBoxAgent(&nodes[node], box);
-            if (node < n - 1) stAgentI(&height[node], h);
-            published = true;
-        }
         storesDone();
         const int prior = __hip_atomic_fetch_add(&arrive[p], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (prior == 0) break;
@@ -399,6 +453,220 @@ __global__ __launch_bounds__(kBlock) void leafUpLocalKernel(
         stAgentI(&height[p], h);
         stAgentI(&arrive[p], 0);
         node = p;
+        p = parent[p];
+        if (p < 0) break;
+    }
+}
+
+// ---- fused hierarchy emission + bottom-up boxes ------------------------------------------
+// delta(i) = common-prefix length of sorted keys i and i+1, -1 outside [0, n-2].
+__device__ __forceinline__ int adjPrefix(const uint32_t* __restrict__ code,
+                                         const uint32_t* __restrict__ idx, int n, int i) {
+    if (i < 0 || i >= n - 1) return -1;
+    return prefixLen(code, idx, n, i, i + 1);
+}
+
+// Where node [l, r] attaches: the binary radix tree over distinct keys joins a node
+// with the neighbour it shares the longer prefix with (delta(r) vs delta(l-1); they
+// are never equal).  As a left child its parent splits at r, as a right child at l-1.
+// Karras numbering names an internal node by the end of its range that faces its
+// parent's split: a left child is node r, a right child node l, the root node 0
+// (karrasNode above produces the same ids).
+struct Attach {
+    int split;  // parent's split position (keys split..split+1 straddle it)
+    bool left;  // this node is the parent's left child
+    int id;     // this node's Karras id (internal nodes)
+};
+__device__ __forceinline__ Attach attachOf(int l, int r, int dl, int dr) {
+    const bool left = dr > dl;
+    return {left ? r : l - 1, left, left ? r : l};
+}
+
+// One thread per leaf slot k: gather the item into leaf order, write leaf node
+// n-1+k, then build the tree bottom-up without a separate hierarchy pass
+// (agglomerative order, as in Apetrei 2014): a finished node [l, r] knows its
+// parent's split from two adjacent-key prefixes, and the split slot's arrival
+// exchange hands the second child the far end of its sibling, from which both
+// child ids, the parent range and the parent's Karras id follow.  The result is
+// byte-identical to karrasNode + leafUpLocalKernel (same ids, parent[], farEnd[],
+// boxes merged with the left child's box as the left operand, heights).
+//
+// Workgroup b owns leaf slots [256 b, 256 b + 255] = [base, last].  Split slots
+// base..last-1 have both neighbouring leaves in the window: arrivals from this
+// workgroup meet there in LDS (prefixes of the window come from an LDS table).  A
+// node whose parent splits outside the window, or the first child at an LDS slot
+// whose sibling was not finished here (known after the workgroup barrier),
+// continues on the global slot: arrive[split] takes the far end + 1 by an
+// agent-scope exchange (0 = empty; the second child resets it, so every slot is
+// zero between kernels), boxes and heights cross workgroups with agent-scope
+// stores.
+template <typename L>
+__global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
+    const typename L::Item* __restrict__ src, const uint32_t* __restrict__ code,
+    const uint32_t* __restrict__ order, int n, typename L::Item* __restrict__ sorted,
+    int* __restrict__ primIds, Node2* __restrict__ nodes, int* __restrict__ parent,
+    int* __restrict__ arrive, int* __restrict__ height, int* __restrict__ farEnd) {
+    __shared__ float sLeaf[kBlock][6];
+    __shared__ float sNode[kBlock][6];
+    __shared__ int sSlot[kBlock];
+    __shared__ int sHeight[kBlock];
+    __shared__ int sDelta[kBlock + 1];  // delta(base - 1 + j)
+    const int base = blockIdx.x * kBlock;
+    const int last = (base + kBlock < n ? base + kBlock : n) - 1;
+    const int k = base + threadIdx.x;
+    const bool live = k < n;
+    sSlot[threadIdx.x] = 0;
+    sDelta[threadIdx.x] = adjPrefix(code, order, n, base - 1 + threadIdx.x);
+    if (threadIdx.x == 0) {
+        sDelta[kBlock] = adjPrefix(code, order, n, base + kBlock - 1);
+        if (blockIdx.x == 0) parent[0] = -1;
+    }
+
+    Box box{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+    if (live) {
+        const int sceneIdx = static_cast<int>(order[k]);
+        const typename L::Item item = src[sceneIdx];
+        sorted[k] = item;
+        primIds[k] = sceneIdx;
+        box = L::bounds(item);
+        Node2 leaf;
+        leaf.lo[0] = box.lo.x;
+        leaf.lo[1] = box.lo.y;
+        leaf.lo[2] = box.lo.z;
+        leaf.hi[0] = box.hi.x;
+        leaf.hi[1] = box.hi.y;
+        leaf.hi[2] = box.hi.z;
+        leaf.left = -(k + 1);
+        leaf.right = 1;
+        nodes[n - 1 + k] = leaf;
+        float* mine = sLeaf[threadIdx.x];
+        mine[0] = box.lo.x;
+        mine[1] = box.lo.y;
+        mine[2] = box.lo.z;
+        mine[3] = box.hi.x;
+        mine[4] = box.hi.y;
+        mine[5] = box.hi.z;
+    }
+    __syncthreads();
+
+    int l = k, r = k, my = n - 1 + k, h = 0;
+    Attach at{0, false, 0};
+    bool done = !live, global = false;
+    int dangling = -1;
+    if (live) at = attachOf(l, r, sDelta[l - base], sDelta[r - base + 1]);
+    while (!done) {
+        const int g = at.split;
+        if (g < base || g >= last) {
+            global = true;
+            break;
+        }
+        const int myFar = at.left ? l : r;
+        const int prior = __hip_atomic_exchange(&sSlot[g - base], myFar + 1, __ATOMIC_ACQ_REL,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (prior == 0) {
+            dangling = g;
+            break;
+        }
+        const int sibFar = prior - 1;
+        int lo, hi, lid, rid, sib;
+        if (at.left) {
+            lo = l;
+            hi = sibFar;
+            lid = my;
+            rid = sib = (sibFar == g + 1) ? n + g : g + 1;
+        } else {
+            lo = sibFar;
+            hi = r;
+            lid = sib = (sibFar == g) ? n - 1 + g : g;
+            rid = my;
+        }
+        const float* sb = sib >= n - 1 ? sLeaf[sib - (n - 1) - base] : sNode[sib - base];
+        const Box other{{sb[0], sb[1], sb[2]}, {sb[3], sb[4], sb[5]}};
+        const int sh = sib >= n - 1 ? 0 : sHeight[sib - base];
+        box = at.left ? mergeChildren(box, other) : mergeChildren(other, box);
+        h = (h > sh ? h : sh) + 1;
+        int pid = 0;
+        if (lo == 0 && hi == n - 1) {
+            done = true;
+        } else {
+            at = attachOf(lo, hi, sDelta[lo - base], sDelta[hi - base + 1]);
+            pid = at.id;
+        }
+        float* dst = sNode[pid - base];
+        dst[0] = box.lo.x;
+        dst[1] = box.lo.y;
+        dst[2] = box.lo.z;
+        dst[3] = box.hi.x;
+        dst[4] = box.hi.y;
+        dst[5] = box.hi.z;
+        sHeight[pid - base] = h;
+        Node2 nd;
+        nd.lo[0] = box.lo.x;
+        nd.lo[1] = box.lo.y;
+        nd.lo[2] = box.lo.z;
+        nd.hi[0] = box.hi.x;
+        nd.hi[1] = box.hi.y;
+        nd.hi[2] = box.hi.z;
+        nd.left = lid;
+        nd.right = rid;
+        nodes[pid] = nd;
+        height[pid] = h;
+        parent[lid] = pid;
+        parent[rid] = pid;
+        farEnd[pid] = pid == lo ? hi : lo;
+        l = lo;
+        r = hi;
+        my = pid;
+    }
+    __syncthreads();
+    // a first arrival whose sibling never came in this workgroup moves to the global slot
+    if (dangling >= 0 && sSlot[dangling - base] == (at.left ? l : r) + 1) global = true;
+    if (!global) return;
+
+    writeBoxAgent(&nodes[my], box);
+    if (my < n - 1) stAgentI(&height[my], h);
+    while (true) {
+        const int g = at.split;
+        const int myFar = at.left ? l : r;
+        storesDone();
+        const int prior = __hip_atomic_exchange(&arrive[g], myFar + 1, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+        if (prior == 0) break;
+        stAgentI(&arrive[g], 0);
+        const int sibFar = prior - 1;
+        int lo, hi, lid, rid, sib;
+        if (at.left) {
+            lo = l;
+            hi = sibFar;
+            lid = my;
+            rid = sib = (sibFar == g + 1) ? n + g : g + 1;
+        } else {
+            lo = sibFar;
+            hi = r;
+            lid = sib = (sibFar == g) ? n - 1 + g : g;
+            rid = my;
+        }
+        const Box sb = readBoxAgent(&nodes[sib]);
+        const int sh = sib >= n - 1 ? 0 : ldAgentI(&height[sib]);
+        box = at.left ? mergeChildren(box, sb) : mergeChildren(sb, box);
+        h = (h > sh ? h : sh) + 1;
+        const bool root = lo == 0 && hi == n - 1;
+        int pid = 0;
+        if (!root) {
+            at = attachOf(lo, hi, adjPrefix(code, order, n, lo - 1), adjPrefix(code, order, n, hi));
+            pid = at.id;
+        }
+        writeBoxAgent(&nodes[pid], box);
+        nodes[pid].left = lid;
+        nodes[pid].right = rid;
+        stAgentI(&height[pid], h);
+        parent[lid] = pid;
+        parent[rid] = pid;
+        farEnd[pid] = pid == lo ? hi : lo;
+        if (root) break;
+        l = lo;
+        r = hi;
+        my = pid;
     }
 }
 
@@ -491,10 +759,7 @@ __global__ __launch_bounds__(kBlock) void batchMortonKernel(const typename L::It
         atomicAdd(&h[3][code >> 24], 1u);
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < 4 * 256; j += kBlock) {
-        const uint32_t v = (&h[0][0])[j];
-        if (v) atomicAdd(&hist[j], v);
-    }
+    flushHistogram(&h[0][0], 4 * 256, hist);
 }
 
 __global__ void batchGatherKernel(const uint32_t* __restrict__ idx, const int* __restrict__ src, int n,
@@ -568,7 +833,7 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     b.height.alloc(static_cast<size_t>(n - 1));
     b.farEnd.alloc(static_cast<size_t>(n - 1));
     b.sortWs.reserve(static_cast<size_t>(n));
-    b.counters.reserve(16);
+    b.counters.reserve(kBoundsWords);
 
     DevBuf<uint32_t> codes(static_cast<size_t>(n));
     DevBuf<uint32_t> codesSorted(static_cast<size_t>(n));
@@ -580,12 +845,15 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
 
     const int tiles = sortTiles(static_cast<size_t>(n));
     const long long descWords = 4LL * tiles * 256;
-    int prepGrid = blocksFor(descWords > 1024 ? descWords : 1024, kBlock);
+    const long long prepWords = descWords > n ? descWords : n;
+    int prepGrid = blocksFor(prepWords > 1024 ? prepWords : 1024, kBlock);
     if (prepGrid > 1024) prepGrid = 1024;
     buildPrepKernel<<<prepGrid, kBlock, 0, s>>>(cb, b.sortWs.hist.p, b.sortWs.desc.p, descWords,
-                                                b.sortWs.tickets.p);
-    int grid = blocksFor(n, kBlock * 4);
-    if (grid > 2048) grid = 2048;
+                                                b.sortWs.tickets.p, b.arrive.p, n - 1);
+    // two workgroups per CU, 16 items per thread: enough loads in flight to stream,
+    // few enough workgroups that the bounds / histogram atomics stay cheap
+    int grid = blocksFor(n, kBlock * 4 * kUnroll);
+    if (grid > 512) grid = 512;
     centroidBoundsKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb);
     mortonHistKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb, codes.p, b.sortWs.hist.p);
     hipCheck(hipGetLastError(), "morton");
@@ -594,15 +862,12 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     sortPairs(b.sortWs, codes.p, nullptr, codesSorted.p, order.p, static_cast<size_t>(n), 4, s);
     ev.record(2, s);
 
-    karrasKernel<<<blocksFor(n - 1, kBlock), kBlock, 0, s>>>(codesSorted.p, order.p, n, b.nodes.p,
-                                                            b.parent.p, b.arrive.p, b.farEnd.p);
-    hipCheck(hipGetLastError(), "karras");
+    // hierarchy, leaf gather and boxes in one pass (no separate Karras kernel)
     ev.record(3, s);
-
-    leafUpLocalKernel<L, true><<<blocksFor(n, kBlock), kBlock, 0, s>>>(
-        dSrc, order.p, n, reinterpret_cast<Item*>(b.items.p), b.primIds.p, b.nodes.p, b.parent.p,
-        b.arrive.p, b.height.p, b.farEnd.p);
-    hipCheck(hipGetLastError(), "leaf up");
+    lbvhEmitKernel<L><<<blocksFor(n, kBlock), kBlock, 0, s>>>(
+        dSrc, codesSorted.p, order.p, n, reinterpret_cast<Item*>(b.items.p), b.primIds.p, b.nodes.p,
+        b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
+    hipCheck(hipGetLastError(), "lbvh emit");
     ev.record(4, s);
 
     int depth = 0;

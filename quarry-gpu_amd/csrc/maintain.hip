@@ -138,6 +138,17 @@ __global__ void dirtyMarkKernel(const typename L::Item* __restrict__ staged,
     }
 }
 
+// Dense dirty refit: only the new items are written; every box is then rebuilt by
+// the bottom-up pass over the whole tree (same merge order, so the same bytes).
+template <typename L>
+__global__ void scatterDirtyKernel(const typename L::Item* __restrict__ staged,
+                                   const int* __restrict__ sceneIds, int count,
+                                   const int* __restrict__ primSlot,
+                                   typename L::Item* __restrict__ items) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < count) items[primSlot[sceneIds[i]]] = staged[i];
+}
+
 // Dirty refit, pass 2: climb from every dirty leaf; at a node the last of its
 // dirty children to arrive merges it and resets the node's counters, so the
 // counters are all zero again when the pass ends.
@@ -412,9 +423,24 @@ void refitFull(BvhState& b, const void* dSceneItems, hipStream_t s) {
     }
 }
 
+// The sparse path (mark the dirty paths, then merge only the marked nodes) is a
+// chain of dependent device-scope atomics about as long as the tree is deep, whatever
+// the dirty count; the bottom-up pass over all leaves streams the tree with its
+// local part in LDS.  Past 1 dirty leaf in 256 the full pass is the faster one.
+static bool denseRefit(int count, int n) { return static_cast<long long>(count) * 256 > n; }
+
 template <typename L>
 static void refitDirtyT(BvhState& b, const void* dStaged, const int* dSceneIds, int count,
                         int* dLeaves, hipStream_t s, hipEvent_t mid) {
+    if (denseRefit(count, b.n)) {
+        scatterDirtyKernel<L><<<blocksFor(count, kBlock), kBlock, 0, s>>>(
+            static_cast<const typename L::Item*>(dStaged), dSceneIds, count, b.primSlot.p,
+            reinterpret_cast<typename L::Item*>(b.items.p));
+        hipCheck(hipGetLastError(), "dirty scatter");
+        if (mid) hipCheck(hipEventRecord(mid, s), "event");
+        propagateAll(b, s);
+        return;
+    }
     dirtyMarkKernel<L><<<blocksFor(count, kBlock), kBlock, 0, s>>>(
         static_cast<const typename L::Item*>(dStaged), dSceneIds, count, b.primSlot.p, b.n,
         reinterpret_cast<typename L::Item*>(b.items.p), b.nodes.p, b.parent.p, b.need.p, dLeaves);
@@ -429,7 +455,7 @@ void refitDirty(BvhState& b, const void* dStaged, const int* dSceneIds, int coun
                 hipStream_t s, hipEvent_t mid) {
     if (count <= 0) return;
     ensurePrimSlot(b, s);
-    if (!b.need.p || b.need.n < static_cast<size_t>(b.n - 1)) {
+    if (!denseRefit(count, b.n) && (!b.need.p || b.need.n < static_cast<size_t>(b.n - 1))) {
         b.need.alloc(static_cast<size_t>(b.n - 1));
         zeroIntKernel<<<blocksFor(b.n - 1, kBlock), kBlock, 0, s>>>(b.need.p, b.n - 1);
     }

@@ -152,16 +152,35 @@ struct EventSet {
 
 // ---- sort ---------------------------------------------------------------------
 // Workspace for the onesweep LSD radix sort of 32-bit keys with 32-bit values.
+// The sort's global digit histogram is kept in kHistCopies copies of
+// [4 passes][256 digits]: a producing workgroup adds its counts into copy
+// blockIdx.x % kHistCopies, which divides the same-address atomics by the copy
+// count, and the sort sums the copies when it reads a digit's total.
+constexpr int kHistCopies = 8;
+constexpr int kHistCopyWords = 4 * 256;
+
+__device__ __forceinline__ void flushHistogram(const uint32_t* lds, int words, uint32_t* hist) {
+    uint32_t* dst = hist + (blockIdx.x % kHistCopies) * kHistCopyWords;
+    for (int i = threadIdx.x; i < words; i += blockDim.x) {
+        const uint32_t v = lds[i];
+        if (v) atomicAdd(&dst[i], v);
+    }
+}
+
 struct SortWorkspace {
     DevBuf<uint32_t> keysAlt, valsAlt;   // ping-pong buffers
-    DevBuf<uint32_t> hist;               // [passes][256] global digit counts
+    DevBuf<uint32_t> hist;               // [kHistCopies][4][256] global digit counts
     DevBuf<uint32_t> desc;               // [passes][tiles][256] look-back descriptors
     DevBuf<uint32_t> tickets;            // [passes] tile tickets
     size_t capacity = 0;
     void reserve(size_t n);
 };
 
-constexpr int kSortTile = 4096;  // keys per onesweep tile (256 threads x 16)
+// Keys per onesweep tile: 2,048 (8 per thread) up to 8M keys, so that a few-million-key
+// sort still spreads over every CU; 4,096 (16 per thread) above.
+constexpr int kSortTileSmall = 2048;
+constexpr int kSortTileLarge = 4096;
+constexpr size_t kSortLargeFrom = size_t(1) << 23;
 
 int sortTiles(size_t n);
 

@@ -1,8 +1,9 @@
 // LSD onesweep radix sort of 32-bit keys with 32-bit payloads, written for
 // gfx950 wave64 (one global histogram pass + one scatter pass per 8-bit digit).
 //
-// Each 256-thread workgroup owns a 4096-key tile.  Keys are ranked inside a
-// wave with 9 ballots (8 digit bits + a validity bit) and a per-wave LDS digit
+// Each 256-thread workgroup owns a tile of 2,048 or 4,096 keys (8 or 16 per
+// thread).  Keys are ranked inside a wave with 9 ballots (8 digit bits + a
+// validity bit) and a per-wave LDS digit
 // counter, then across waves with an LDS prefix, then across tiles with a
 // decoupled look-back over per-tile digit descriptors.  Tile ids come from an
 // atomic ticket, so every predecessor a tile waits on is already resident and
@@ -22,7 +23,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kKpt = kSortTile / kBlock;  // 16 keys per thread
 constexpr uint32_t kStatAgg = 1u << 30;
 constexpr uint32_t kStatIncl = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
@@ -36,13 +36,13 @@ __device__ __forceinline__ void storeAgent(uint32_t* p, uint32_t v) {
 
 // Clears histogram, descriptors and tickets (a kernel, not a memset, so the
 // whole build can be captured in a graph).
-__global__ void sortClearKernel(uint32_t* __restrict__ hist, int histWords,
-                                uint32_t* __restrict__ desc, long long descWords,
+__global__ void sortClearKernel(uint32_t* __restrict__ hist, uint32_t* __restrict__ desc,
+                                long long descWords,
                                 uint32_t* __restrict__ tickets, int passes) {
     const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
     const long long stride = static_cast<long long>(gridDim.x) * kBlock;
     for (long long j = i; j < descWords; j += stride) desc[j] = 0u;
-    for (long long j = i; j < histWords; j += stride) hist[j] = 0u;
+    for (long long j = i; j < kHistCopies * kHistCopyWords; j += stride) hist[j] = 0u;
     if (i < passes) tickets[i] = 0u;
 }
 
@@ -58,10 +58,7 @@ __global__ void sortHistogramKernel(const uint32_t* __restrict__ keys, long long
         for (int p = 0; p < passes; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < passes * 256; i += kBlock) {
-        const uint32_t v = (&h[0][0])[i];
-        if (v) atomicAdd(&hist[i], v);
-    }
+    flushHistogram(&h[0][0], passes * 256, hist);
 }
 
 // Exclusive scan of 256 values held one per thread; returns the prefix and the
@@ -93,6 +90,7 @@ __device__ __forceinline__ uint32_t blockScan256(uint32_t v, uint32_t* scratch,
     return waveBase + x - v;
 }
 
+template <int kKpt>
 __global__ __launch_bounds__(kBlock) void onesweepKernel(
     const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
     uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, long long n, int shift,
@@ -102,8 +100,9 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(
     __shared__ uint32_t tileStart[256];
     __shared__ uint32_t globalBase[256];
     __shared__ uint32_t scanScratch[kWaves];
-    __shared__ uint32_t stageKeys[kSortTile];
-    __shared__ uint32_t stageVals[kSortTile];
+    constexpr int kTile = kKpt * kBlock;
+    __shared__ uint32_t stageKeys[kTile];
+    __shared__ uint32_t stageVals[kTile];
     __shared__ int tileShared;
 
     const int t = threadIdx.x;
@@ -114,14 +113,14 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(
     for (int i = t; i < kWaves * 256; i += kBlock) (&waveCnt[0][0])[i] = 0u;
     __syncthreads();
     const int tile = tileShared;
-    const long long tileBase = static_cast<long long>(tile) * kSortTile;
+    const long long tileBase = static_cast<long long>(tile) * kTile;
     const long long remain = n - tileBase;
-    const int validCount = remain >= kSortTile ? kSortTile : static_cast<int>(remain);
+    const int validCount = remain >= kTile ? kTile : static_cast<int>(remain);
 
     uint32_t key[kKpt];
     uint32_t val[kKpt];
     uint32_t rank[kKpt];
-    const long long waveBase = tileBase + static_cast<long long>(w) * (kSortTile / kWaves);
+    const long long waveBase = tileBase + static_cast<long long>(w) * (kTile / kWaves);
 #pragma unroll
     for (int r = 0; r < kKpt; ++r) {
         const long long idx = waveBase + r * 64 + lane;
@@ -177,7 +176,10 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(
     uint32_t dummy;
     const uint32_t startInTile = blockScan256(tileTotal, scanScratch, &dummy);
     tileStart[t] = startInTile;
-    const uint32_t bucketStart = blockScan256(histPass[t], scanScratch, &dummy);
+    uint32_t digitTotal = 0;
+#pragma unroll
+    for (int c = 0; c < kHistCopies; ++c) digitTotal += histPass[c * kHistCopyWords + t];
+    const uint32_t bucketStart = blockScan256(digitTotal, scanScratch, &dummy);
 
     uint32_t exclusive = 0;
     if (tile > 0) {
@@ -313,15 +315,18 @@ __global__ void scanApplyKernel(const int* __restrict__ in, int n, const int* __
 
 }  // namespace
 
-int sortTiles(size_t n) { return static_cast<int>((n + kSortTile - 1) / kSortTile); }
+static size_t sortTileKeys(size_t n) { return n < kSortLargeFrom ? kSortTileSmall : kSortTileLarge; }
+
+int sortTiles(size_t n) { return static_cast<int>((n + sortTileKeys(n) - 1) / sortTileKeys(n)); }
 
 void SortWorkspace::reserve(size_t n) {
     if (n <= capacity && keysAlt.p) return;
     const size_t cap = n < 4096 ? 4096 : n;
     keysAlt.alloc(cap);
     valsAlt.alloc(cap);
-    hist.alloc(4 * 256);
-    desc.alloc(static_cast<size_t>(4) * sortTiles(cap) * 256);
+    hist.alloc(static_cast<size_t>(kHistCopies) * kHistCopyWords);
+    // enough descriptors for any n <= cap at the smaller tile
+    desc.alloc(static_cast<size_t>(4) * ((cap + kSortTileSmall - 1) / kSortTileSmall) * 256);
     tickets.alloc(4);
     capacity = cap;
 }
@@ -331,8 +336,7 @@ void sortClear(SortWorkspace& ws, size_t n, int passes, hipStream_t s) {
     long long work = descWords > 1024 ? descWords : 1024;
     int grid = blocksFor(work, kBlock);
     if (grid > 2048) grid = 2048;
-    sortClearKernel<<<grid, kBlock, 0, s>>>(ws.hist.p, passes * 256, ws.desc.p, descWords,
-                                            ws.tickets.p, passes);
+    sortClearKernel<<<grid, kBlock, 0, s>>>(ws.hist.p, ws.desc.p, descWords, ws.tickets.p, passes);
     hipCheck(hipGetLastError(), "sort clear");
 }
 
@@ -360,9 +364,15 @@ void sortPairs(SortWorkspace& ws, const uint32_t* keys, const uint32_t* vals, ui
         uint32_t* dstK = toOut ? keysOut : ws.keysAlt.p;
         uint32_t* dstV = toOut ? valsOut : ws.valsAlt.p;
         (void)last;
-        onesweepKernel<<<tiles, kBlock, 0, s>>>(
-            srcK, srcV, dstK, dstV, static_cast<long long>(n), 8 * p, ws.hist.p + p * 256,
-            ws.desc.p + static_cast<size_t>(p) * tiles * 256, ws.tickets.p + p);
+        uint32_t* desc = ws.desc.p + static_cast<size_t>(p) * tiles * 256;
+        if (sortTileKeys(n) == kSortTileSmall)
+            onesweepKernel<kSortTileSmall / kBlock><<<tiles, kBlock, 0, s>>>(
+                srcK, srcV, dstK, dstV, static_cast<long long>(n), 8 * p, ws.hist.p + p * 256, desc,
+                ws.tickets.p + p);
+        else
+            onesweepKernel<kSortTileLarge / kBlock><<<tiles, kBlock, 0, s>>>(
+                srcK, srcV, dstK, dstV, static_cast<long long>(n), 8 * p, ws.hist.p + p * 256, desc,
+                ws.tickets.p + p);
         hipCheck(hipGetLastError(), "onesweep");
         srcK = dstK;
         srcV = dstV;

@@ -621,6 +621,101 @@ __device__ __forceinline__ float quadGetF(float v) {
 
 constexpr int kQuadDepth = 96;  // 3 x the deepest 4-wide level a tree here can have
 
+// DPP quad permutation: lane s of each quad receives lane P[s] (2 bits per lane).
+template <int P>
+__device__ __forceinline__ int quadPerm(int v) {
+    return __builtin_amdgcn_mov_dpp(v, P, 0xF, 0xF, false);
+}
+constexpr int kQuadXor1 = 1 | (0 << 2) | (3 << 4) | (2 << 6);
+constexpr int kQuadXor2 = 2 | (3 << 2) | (0 << 4) | (1 << 6);
+constexpr int kQuadPrev = 0 | (0 << 2) | (1 << 4) | (2 << 6);
+
+// The k-best list of a quad, spread over its four lanes: lane s holds entries
+// [s E, s E + E) of the sorted list (E = K / 4).  Same contents and order as
+// KnnList<K> after the same offers: an offer (the same candidate in all four lanes)
+// goes to position pos = number of entries not farther than it (a quad sum of the
+// lanes' counts); entries at pos and after move one place down, the one crossing a
+// lane boundary coming from the previous lane.  Every lane keeps the list's last
+// entry (kd, kid) for pruning.  Per offer this is E compares and 2 E selects per
+// lane instead of K of each in every lane.
+template <int K>
+struct QuadKnnList {
+    static constexpr int E = K / 4;
+    static_assert(K % 4 == 0, "K must be a multiple of 4");
+    float d[E];
+    int id[E];
+    int n;
+    float kd;
+    int kid;
+    __device__ __forceinline__ void init() {
+        n = 0;
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            d[j] = kInf;
+            id[j] = 0x7fffffff;
+        }
+        kd = kInf;
+        kid = 0x7fffffff;
+    }
+    __device__ __forceinline__ bool admits(float lb) const { return n < K || lb <= kd; }
+    __device__ __forceinline__ void offer(float nd, int nid, int slot) {
+        if (n >= K && !closer(nd, nid, kd, kid)) return;
+        int before = 0;
+#pragma unroll
+        for (int j = 0; j < E; ++j) before += closer(nd, nid, d[j], id[j]) ? 0 : 1;
+        int pos = before + quadPerm<kQuadXor1>(before);
+        pos += quadPerm<kQuadXor2>(pos);
+        const float pd = __int_as_float(quadPerm<kQuadPrev>(__float_as_int(d[E - 1])));
+        const int pi = quadPerm<kQuadPrev>(id[E - 1]);
+#pragma unroll
+        for (int j = E - 1; j >= 0; --j) {
+            const int g = slot * E + j;
+            const float od = j > 0 ? d[j - 1] : pd;
+            const int oi = j > 0 ? id[j - 1] : pi;
+            d[j] = g > pos ? od : (g == pos ? nd : d[j]);
+            id[j] = g > pos ? oi : (g == pos ? nid : id[j]);
+        }
+        n = n < K ? n + 1 : K;
+        kd = quadGetF<3>(d[E - 1]);
+        kid = quadGet<3>(id[E - 1]);
+    }
+    __device__ __forceinline__ void store(ClosestResult* dst, int k, int slot) const {
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            const int g = slot * E + j;
+            if (g < k) {
+                ClosestResult r{0u, -1};
+                if (g < n) r = {floatBits(d[j]), id[j]};
+                dst[g] = r;
+            }
+        }
+    }
+};
+
+// k = 1: one entry, kept in every lane.
+template <>
+struct QuadKnnList<1> {
+    float kd;
+    int kid;
+    int n;
+    __device__ __forceinline__ void init() {
+        kd = kInf;
+        kid = 0x7fffffff;
+        n = 0;
+    }
+    __device__ __forceinline__ bool admits(float lb) const { return n == 0 || lb <= kd; }
+    __device__ __forceinline__ void offer(float nd, int nid, int) {
+        if (n == 0 || closer(nd, nid, kd, kid)) {
+            kd = nd;
+            kid = nid;
+        }
+        n = 1;
+    }
+    __device__ __forceinline__ void store(ClosestResult* dst, int k, int slot) const {
+        if (slot == 0 && k > 0) dst[0] = n ? ClosestResult{floatBits(kd), kid} : ClosestResult{0u, -1};
+    }
+};
+
 template <typename L, int K, bool kStats>
 __global__ __launch_bounds__(kBlock) void nearestQuadKernel(
     const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
@@ -635,7 +730,7 @@ __global__ __launch_bounds__(kBlock) void nearestQuadKernel(
     if (qi < count) {  // the same for the four lanes of a quad
         const int q = order ? static_cast<int>(order[qi]) : qi;
         const V3 p = points[q];
-        KnnList<K> list;
+        QuadKnnList<K> list;
         list.init();
         LdsInt* stk = (LdsInt*)(lds + (threadIdx.x >> 2) * kQuadDepth);
         int sp = 0;
@@ -671,10 +766,10 @@ __global__ __launch_bounds__(kBlock) void nearestQuadKernel(
                 const float d0 = quadGetF<0>(cd), d1 = quadGetF<1>(cd), d2 = quadGetF<2>(cd), d3 = quadGetF<3>(cd);
                 const int i0 = quadGet<0>(cid), i1 = quadGet<1>(cid), i2 = quadGet<2>(cid), i3 = quadGet<3>(cid);
                 const int o0 = quadGet<0>(cand), o1 = quadGet<1>(cand), o2 = quadGet<2>(cand), o3 = quadGet<3>(cand);
-                if (o0) list.offer(d0, i0);
-                if (o1) list.offer(d1, i1);
-                if (o2) list.offer(d2, i2);
-                if (o3) list.offer(d3, i3);
+                if (o0) list.offer(d0, i0, slot);
+                if (o1) list.offer(d1, i1, slot);
+                if (o2) list.offer(d2, i2, slot);
+                if (o3) list.offer(d3, i3, slot);
             }
             const bool in = adm && !leaf && list.admits(d);
             const float key = in ? d : kInf;
@@ -710,15 +805,7 @@ __global__ __launch_bounds__(kBlock) void nearestQuadKernel(
                 break;
             }
         }
-        ClosestResult* dst = out + static_cast<size_t>(q) * outStride;
-#pragma unroll
-        for (int s = 0; s < K; ++s) {
-            if ((s & 3) == slot && s < k) {
-                ClosestResult r{0u, -1};
-                if (s < list.n) r = {floatBits(list.d[s]), list.id[s]};
-                dst[s] = r;
-            }
-        }
+        list.store(out + static_cast<size_t>(q) * outStride, k, slot);
     }
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
@@ -1091,10 +1178,7 @@ __global__ void mortonKeysKernel(const float* __restrict__ base, int stride, int
         atomicAdd(&h[3][code >> 24], 1u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < 4 * 256; i += kBlock) {
-        const uint32_t v = (&h[0][0])[i];
-        if (v) atomicAdd(&hist[i], v);
-    }
+    flushHistogram(&h[0][0], 4 * 256, hist);
 }
 
 __global__ void zeroU64Kernel(unsigned long long* __restrict__ p, int n) {
@@ -1248,8 +1332,10 @@ static void launchRadiusT(const TraverseArgs& a, const V3* pts, int count, float
     const int grid = blocksFor(count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
     unsigned long long* st = kFill ? nullptr : a.stats;
-    // radius walks few nodes per query: one lane per query measured faster unless forced
-    if (a.wide && gTraversalMode == 2) {
+    // radius: four lanes per query measured faster at every query count from 20k to 1M
+    // on the 2M-triangle terrain (the lanes of a quad test the four children of a node
+    // and its leaves together instead of walking them one after another)
+    if (a.wide && gTraversalMode != 1) {
         const int qgrid = blocksFor(4LL * count, kBlock);
         if (st)
             radiusQuadKernel<L, kFill, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, st, maxCount);

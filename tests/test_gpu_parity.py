@@ -219,6 +219,22 @@ def test_refit_dirty_rounds(A, orc, scenes):
     assert eq(b.read()[0], cn)
 
 
+@pytest.mark.parametrize("dirty_count", [40, 3000])
+def test_refit_sparse_and_dense_paths(A, scenes, dirty_count):
+    """Dirty refit takes the marked-path merge below 1 dirty leaf in 256 and the
+    full bottom-up pass above it; both must give the full refit's node bytes."""
+    tris = scenes.terrain(128, 5)  # 32,768 triangles
+    rng = np.random.default_rng(dirty_count)
+    idx = rng.choice(tris.shape[0], dirty_count, replace=False).astype(np.int32)
+    moved = tris.copy()
+    moved[idx] += rng.normal(0, 0.3, (dirty_count, 9)).astype(np.float32)
+    a, b = A.Bvh(0, tris), A.Bvh(0, tris)
+    a.refit(moved, idx)
+    b.refit(moved)
+    assert a.validate(moved) == ""
+    assert eq(a.read()[0], b.read()[0])
+
+
 def test_refit_points_and_gaussians(A, orc, scenes):
     rng = np.random.default_rng(8)
     for kind, items in ((1, scenes.point_cloud(5000, 4)), (2, scenes.gaussians(3000, 4))):
