@@ -157,6 +157,7 @@ TraverseArgs argsFor(const BvhState& b, bool wide) {
     a.wide = wide;
     a.nodes = b.nodes.p;
     a.wideNodes = b.wide.p;
+    a.wideCount = b.wideCount;
     a.items = b.items.p;
     a.primIds = b.primIds.p;
     return a;

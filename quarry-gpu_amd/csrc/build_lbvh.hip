@@ -828,6 +828,7 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     b.nodes.alloc(static_cast<size_t>(b.nodeCount));
     b.items.alloc(static_cast<size_t>(n) * sizeof(Item));
     b.primIds.alloc(static_cast<size_t>(n));
+    b.primSlot.reserve(static_cast<size_t>(n));  // filled on first use (refits, adjoints)
     b.parent.alloc(static_cast<size_t>(b.nodeCount));
     b.arrive.alloc(static_cast<size_t>(n - 1));
     b.height.alloc(static_cast<size_t>(n - 1));

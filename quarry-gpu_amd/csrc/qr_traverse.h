@@ -11,6 +11,7 @@ struct TraverseArgs {
     bool wide = false;
     const Node2* nodes = nullptr;
     const Wide4* wideNodes = nullptr;
+    int wideCount = 0;
     const void* items = nullptr;        // leaf-ordered primitives
     const int* primIds = nullptr;       // leaf slot -> scene index
     const uint32_t* order = nullptr;    // optional visiting order (lane i -> query order[i])

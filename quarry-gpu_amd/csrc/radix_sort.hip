@@ -26,6 +26,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kStatAgg = 1u << 30;
 constexpr uint32_t kStatIncl = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
+constexpr int kLookBack = 8;  // predecessors read per look-back round
 
 __device__ __forceinline__ uint32_t loadAgent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -181,19 +182,34 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(
     for (int c = 0; c < kHistCopies; ++c) digitTotal += histPass[c * kHistCopyWords + t];
     const uint32_t bucketStart = blockScan256(digitTotal, scanScratch, &dummy);
 
+    // Look-back over a window of kLookBack predecessors per round (loads in flight
+    // together): aggregates are summed in order up to the first inclusive prefix; a
+    // predecessor that has not published yet ends the round, which resumes there.
+    // Tile 0 publishes an inclusive prefix at once, so the walk ends.
     uint32_t exclusive = 0;
     if (tile > 0) {
         int j = tile - 1;
         while (true) {
-            const uint32_t v = loadAgent(&desc[static_cast<size_t>(j) * 256 + t]);
-            const uint32_t st = v & ~kValMask;
-            if (st == 0u) {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
+            uint32_t v[kLookBack];
+#pragma unroll
+            for (int u = 0; u < kLookBack; ++u)
+                v[u] = j - u >= 0 ? loadAgent(&desc[static_cast<size_t>(j - u) * 256 + t]) : 0u;
+            int used = 0;
+            bool done = false;
+#pragma unroll
+            for (int u = 0; u < kLookBack; ++u) {
+                const uint32_t st = v[u] & ~kValMask;
+                if (st == 0u) break;
+                exclusive += v[u] & kValMask;
+                ++used;
+                if (st == kStatIncl) {
+                    done = true;
+                    break;
+                }
             }
-            exclusive += v & kValMask;
-            if (st == kStatIncl) break;
-            --j;
+            if (done) break;
+            j -= used;
+            if (used < kLookBack) __builtin_amdgcn_s_sleep(1);
         }
         storeAgent(&myDesc[t], kStatIncl | (exclusive + tileTotal));
     }

@@ -621,6 +621,25 @@ __device__ __forceinline__ float quadGetF(float v) {
 
 constexpr int kQuadDepth = 96;  // 3 x the deepest 4-wide level a tree here can have
 
+// Stack entries of the quad walk carry the child's box distance above its id,
+// truncated to the bits the id leaves free: the leading bits of a non-negative
+// float are a lower bound of it.  A popped node whose bound already exceeds the
+// k-th distance is dropped without being opened -- opening it would find every
+// child beyond the k-th distance too (a child's box lies inside its parent's), so
+// the walk and its results are unchanged.  idBits = 32 turns the keys off (trees
+// whose wide ids need more than 23 bits).
+__device__ __forceinline__ int packEntry(int id, float key, int idBits) {
+    if (idBits >= 32) return id;
+    const uint32_t q = floatBits(key) >> (idBits - 1);
+    return static_cast<int>((q << idBits) | static_cast<uint32_t>(id));
+}
+__device__ __forceinline__ int entryId(int v, int idBits) {
+    return idBits >= 32 ? v : static_cast<int>(static_cast<uint32_t>(v) & ((1u << idBits) - 1u));
+}
+__device__ __forceinline__ float entryKey(int v, int idBits) {
+    return idBits >= 32 ? 0.0f : bitsFloat((static_cast<uint32_t>(v) >> idBits) << (idBits - 1));
+}
+
 // DPP quad permutation: lane s of each quad receives lane P[s] (2 bits per lane).
 template <int P>
 __device__ __forceinline__ int quadPerm(int v) {
@@ -721,7 +740,7 @@ __global__ __launch_bounds__(kBlock) void nearestQuadKernel(
     const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
     const int* __restrict__ primIds, const V3* __restrict__ points,
     const uint32_t* __restrict__ order, int count, int k, int outStride,
-    ClosestResult* __restrict__ out, unsigned long long* __restrict__ stats) {
+    ClosestResult* __restrict__ out, unsigned long long* __restrict__ stats, int idBits) {
     __shared__ int lds[kQuadDepth * (kBlock / 4)];
     const int t = xcdBlock() * kBlock + threadIdx.x;
     const int qi = t >> 2;
@@ -734,10 +753,14 @@ __global__ __launch_bounds__(kBlock) void nearestQuadKernel(
         list.init();
         LdsInt* stk = (LdsInt*)(lds + (threadIdx.x >> 2) * kQuadDepth);
         int sp = 0;
-        int id = 0;
+        // The successor is requested before this node's leaves are tested: offers can
+        // only shrink the k-th distance, so the walk continues either with the nearest
+        // child that survives the box tests now or, if none survives the offers,
+        // with the top of the stack -- that one is requested speculatively and the
+        // entry is reloaded only when the offers pruned it.
+        WideEntry e = wide[0].c[slot];
         while (true) {
             if (kStats && slot == 0) ++opened;
-            const WideEntry e = wide[id].c[slot];
             const int ch = e.child;
             const bool leaf = ch < 0 && ch != kWideEmpty;
             typename L::Item it;
@@ -753,6 +776,25 @@ __global__ __launch_bounds__(kBlock) void nearestQuadKernel(
                 adm = list.admits(d);
             }
             if (adm) ++nodeTests;
+            int spec = -1;
+            {
+                const bool in0 = adm && !leaf;
+                const float key0 = in0 ? d : kInf;
+                const int child0 = in0 ? ch : -1;
+                const float k0 = quadGetF<0>(key0), k1 = quadGetF<1>(key0), k2 = quadGetF<2>(key0), k3 = quadGetF<3>(key0);
+                const int c0 = quadGet<0>(child0), c1 = quadGet<1>(child0), c2 = quadGet<2>(child0), c3 = quadGet<3>(child0);
+                float bk = kInf;
+                if (c3 >= 0) { spec = c3; bk = k3; }
+                if (c2 >= 0 && !(bk < k2)) { spec = c2; bk = k2; }
+                if (c1 >= 0 && !(bk < k1)) { spec = c1; bk = k1; }
+                if (c0 >= 0 && !(bk < k0)) { spec = c0; bk = k0; }
+                for (int j = sp - 1; spec < 0 && j >= 0; --j) {
+                    const int v = stk[j];
+                    if (list.admits(entryKey(v, idBits))) spec = entryId(v, idBits);
+                }
+            }
+            // (the root when there is no successor: a cached, harmless read)
+            const WideEntry se = wide[spec >= 0 ? spec : 0].c[slot];
             float cd = kInf;
             int cid = 0x7fffffff;
             int cand = 0;
@@ -788,7 +830,7 @@ __global__ __launch_bounds__(kBlock) void nearestQuadKernel(
                     if (slot == 0) overflow = 1;
                     break;
                 }
-                if (in && rank > 0) stk[sp + (m - 1 - rank)] = ch;
+                if (in && rank > 0) stk[sp + (m - 1 - rank)] = packEntry(ch, d, idBits);
                 sp += m - 1;
                 // the nearest survivor: smallest key, then smallest slot
                 int next = -1;
@@ -797,12 +839,15 @@ __global__ __launch_bounds__(kBlock) void nearestQuadKernel(
                 if (c2 >= 0 && !(bk < k2)) { next = c2; bk = k2; }
                 if (c1 >= 0 && !(bk < k1)) { next = c1; bk = k1; }
                 if (c0 >= 0 && !(bk < k0)) { next = c0; bk = k0; }
-                id = next;
-            } else if (sp > 0) {
-                --sp;
-                id = stk[sp];
+                if (next == spec) e = se;
+                else e = wide[next].c[slot];
             } else {
-                break;
+                while (sp > 0 && !list.admits(entryKey(stk[sp - 1], idBits))) --sp;
+                if (sp == 0) break;
+                --sp;
+                const int next = entryId(stk[sp], idBits);
+                if (next == spec) e = se;
+                else e = wide[next].c[slot];
             }
         }
         list.store(out + static_cast<size_t>(q) * outStride, k, slot);
@@ -833,10 +878,12 @@ __global__ __launch_bounds__(kBlock) void radiusQuadKernel(
         const int end = kFill ? offsets[q + 1] : 0;
         LdsInt* stk = (LdsInt*)(lds + (threadIdx.x >> 2) * kQuadDepth);
         int sp = 0;
-        int id = 0;
+        // The next node depends on the box tests only, so its entry is requested
+        // before this node's leaves are tested: the leaf loads and tests overlap the
+        // next node's load.
+        WideEntry e = wide[0].c[slot];
         while (true) {
             if (kStats && slot == 0) ++opened;
-            const WideEntry e = wide[id].c[slot];
             const int ch = e.child;
             const bool leaf = ch < 0 && ch != kWideEmpty;
             typename L::Item it;
@@ -847,6 +894,27 @@ __global__ __launch_bounds__(kBlock) void radiusQuadKernel(
             }
             const bool adm = ch != kWideEmpty && boxDist2(p, entryBox(e)) <= r2;
             if (adm) ++nodeTests;
+            const int child = (adm && !leaf) ? ch : -1;
+            const int c0 = quadGet<0>(child), c1 = quadGet<1>(child), c2 = quadGet<2>(child), c3 = quadGet<3>(child);
+            const int m = (c0 >= 0) + (c1 >= 0) + (c2 >= 0) + (c3 >= 0);
+            int next = -1;
+            if (m > 0) {
+                if (sp + m - 1 > kQuadDepth) {
+                    if (slot == 0) overflow = 1;
+                    break;
+                }
+                // the lowest surviving slot continues; the others are pushed in slot order
+                const int first = c0 >= 0 ? 0 : (c1 >= 0 ? 1 : (c2 >= 0 ? 2 : 3));
+                const int below = (slot > 0 && c0 >= 0) + (slot > 1 && c1 >= 0) + (slot > 2 && c2 >= 0);
+                if (child >= 0 && slot != first) stk[sp + below - 1] = child;
+                sp += m - 1;
+                next = first == 0 ? c0 : (first == 1 ? c1 : (first == 2 ? c2 : c3));
+            } else if (sp > 0) {
+                --sp;
+                next = stk[sp];
+            }
+            WideEntry ne;
+            if (next >= 0) ne = wide[next].c[slot];
             int hit = 0;
             float d2 = 0.0f;
             if (adm && leaf) {
@@ -864,26 +932,8 @@ __global__ __launch_bounds__(kBlock) void radiusQuadKernel(
             } else {
                 found += hit;
             }
-            const int child = (adm && !leaf) ? ch : -1;
-            const int c0 = quadGet<0>(child), c1 = quadGet<1>(child), c2 = quadGet<2>(child), c3 = quadGet<3>(child);
-            const int m = (c0 >= 0) + (c1 >= 0) + (c2 >= 0) + (c3 >= 0);
-            if (m > 0) {
-                if (sp + m - 1 > kQuadDepth) {
-                    if (slot == 0) overflow = 1;
-                    break;
-                }
-                // the lowest surviving slot continues; the others are pushed in slot order
-                const int first = c0 >= 0 ? 0 : (c1 >= 0 ? 1 : (c2 >= 0 ? 2 : 3));
-                const int below = (slot > 0 && c0 >= 0) + (slot > 1 && c1 >= 0) + (slot > 2 && c2 >= 0);
-                if (child >= 0 && slot != first) stk[sp + below - 1] = child;
-                sp += m - 1;
-                id = first == 0 ? c0 : (first == 1 ? c1 : (first == 2 ? c2 : c3));
-            } else if (sp > 0) {
-                --sp;
-                id = stk[sp];
-            } else {
-                break;
-            }
+            if (next < 0) break;
+            e = ne;
         }
         if (!kFill) {
             const int total = quadGet<0>(found) + quadGet<1>(found) + quadGet<2>(found) + quadGet<3>(found);
@@ -1256,13 +1306,24 @@ int gTraversalMode = 0;  // 0: automatic, 1: one lane per query, 2: four lanes p
 
 void setTraversalMode(int mode) { gTraversalMode = mode; }
 
-// Closest / kNN: four lanes per query while the query set leaves most of the GPU
-// idle one lane per query (fewer than 4 waves per SIMD: 256 CUs x 4 SIMDs x 4 waves
-// x 64 lanes).
-static bool useQuad(int count) {
+// kNN (k > 1): four lanes per query at every size -- the quad's spread k-best list
+// makes each offer a quarter of the per-lane work, and it measured faster from 20k
+// to 1M queries on the 2M-triangle terrain.  Closest (k = 1): four lanes per query
+// while the query set leaves most of the GPU idle one lane per query (fewer than 4
+// waves per SIMD: 256 CUs x 4 SIMDs x 4 waves x 64 lanes).
+// Bits of a stack entry taken by the wide node id (the rest hold the distance key);
+// 32 = no key, when ids need more than 23 bits (fewer than 9 would be left, too few
+// for the float's exponent and a mantissa bit).
+static int stackIdBits(int wideCount) {
+    int bits = 1;
+    while (bits < 32 && (1LL << bits) < static_cast<long long>(wideCount)) ++bits;
+    return bits <= 23 ? bits : 32;
+}
+
+static bool useQuad(int count, int K) {
     if (gTraversalMode == 1) return false;
     if (gTraversalMode == 2) return true;
-    return count < 256 * 4 * 4 * 64;
+    return K > 1 || count < 256 * 4 * 4 * 64;
 }
 
 template <typename L, int K>
@@ -1270,12 +1331,13 @@ static void launchNearestTK(const TraverseArgs& a, const V3* pts, int count, int
                             ClosestResult* out, hipStream_t s) {
     const int grid = blocksFor(count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
-    if (a.wide && useQuad(count)) {
+    if (a.wide && useQuad(count, K)) {
         const int qgrid = blocksFor(4LL * count, kBlock);
+        const int idBits = stackIdBits(a.wideCount);
         if (a.stats)
-            nearestQuadKernel<L, K, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats);
+            nearestQuadKernel<L, K, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats, idBits);
         else
-            nearestQuadKernel<L, K, false><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr);
+            nearestQuadKernel<L, K, false><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, idBits);
     } else if (a.wide) {
         if (a.stats)
             nearestWideKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats);
