@@ -156,6 +156,7 @@ TraverseArgs argsFor(const BvhState& b, bool wide) {
     a.kind = b.kind;
     a.wide = wide;
     a.nodes = b.nodes.p;
+    a.nodeCount = b.nodeCount;
     a.wideNodes = b.wide.p;
     a.wideCount = b.wideCount;
     a.items = b.items.p;

@@ -10,6 +10,7 @@ struct TraverseArgs {
     int kind = 0;                       // 0 tri, 1 point, 2 Gaussian
     bool wide = false;
     const Node2* nodes = nullptr;
+    int nodeCount = 0;
     const Wide4* wideNodes = nullptr;
     int wideCount = 0;
     const void* items = nullptr;        // leaf-ordered primitives

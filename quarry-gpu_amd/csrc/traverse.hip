@@ -420,6 +420,34 @@ __device__ __forceinline__ void writeList(const KnnList<K>& list, int k, Closest
     }
 }
 
+// Stack entries of the closest / kNN walks carry the child's box distance above its id,
+// truncated to the bits the id leaves free: the leading bits of a non-negative
+// float are a lower bound of it.  A popped node whose bound already exceeds the
+// k-th distance is dropped without being opened -- opening it would find every
+// child beyond the k-th distance too (a child's box lies inside its parent's), so
+// the walk and its results are unchanged.  idBits = 32 turns the keys off (trees
+// whose wide ids need more than 23 bits).
+// Bits of a stack entry taken by a node id below `idCount` (the rest hold the
+// distance key); 32 = no key, when ids need more than 23 bits (fewer than 9 would
+// be left, too few for the float's exponent and a mantissa bit).
+__host__ __device__ __forceinline__ int stackIdBitsDev(int idCount) {
+    int bits = 1;
+    while (bits < 32 && (1LL << bits) < static_cast<long long>(idCount)) ++bits;
+    return bits <= 23 ? bits : 32;
+}
+
+__device__ __forceinline__ int packEntry(int id, float key, int idBits) {
+    if (idBits >= 32) return id;
+    const uint32_t q = floatBits(key) >> (idBits - 1);
+    return static_cast<int>((q << idBits) | static_cast<uint32_t>(id));
+}
+__device__ __forceinline__ int entryId(int v, int idBits) {
+    return idBits >= 32 ? v : static_cast<int>(static_cast<uint32_t>(v) & ((1u << idBits) - 1u));
+}
+__device__ __forceinline__ float entryKey(int v, int idBits) {
+    return idBits >= 32 ? 0.0f : bitsFloat((static_cast<uint32_t>(v) >> idBits) << (idBits - 1));
+}
+
 // Closest / kNN walk of one query over a binary tree whose node ids, leaf slots
 // and prim ids are local to `nodes`, `prims` and `primIds` (a plain tree, or one
 // tree of a batch addressed through offset pointers).
@@ -429,7 +457,7 @@ __device__ __forceinline__ void nearestBinaryWalk(const Node2* __restrict__ node
                                                   const int* __restrict__ primIds, V3 p,
                                                   KnnList<K>& list, Stack& st, SpillArea& spill,
                                                   int& nodeTests, int& leafTests, int& overflow,
-                                                  int& opened) {
+                                                  int& opened, int idBits) {
     int id = 0;
     while (id >= 0) {
         if (kStats) ++opened;
@@ -466,15 +494,21 @@ __device__ __forceinline__ void nearestBinaryWalk(const Node2* __restrict__ node
                     overflow = 1;
                     break;
                 }
-                QR_PUSH(st, spill, c1);
+                QR_PUSH(st, spill, packEntry(c1, k1, idBits));
             }
             id = c0;
         } else if (in1) {
             id = c1;
-        } else if (!st.empty()) {
-            QR_POP(st, spill, id);
         } else {
             id = -1;
+            while (!st.empty()) {
+                int v;
+                QR_POP(st, spill, v);
+                if (list.admits(entryKey(v, idBits))) {
+                    id = entryId(v, idBits);
+                    break;
+                }
+            }
         }
     }
 }
@@ -484,7 +518,7 @@ __global__ __launch_bounds__(kBlock) void nearestBinaryKernel(
     const Node2* __restrict__ nodes, const typename L::Item* __restrict__ prims,
     const int* __restrict__ primIds, const V3* __restrict__ points,
     const uint32_t* __restrict__ order, int count, int k, int outStride,
-    ClosestResult* __restrict__ out, unsigned long long* __restrict__ stats) {
+    ClosestResult* __restrict__ out, unsigned long long* __restrict__ stats, int idBits) {
     __shared__ int lds[kLdsDepth * kBlock];
     const int i = xcdBlock() * kBlock + threadIdx.x;
     int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
@@ -496,7 +530,7 @@ __global__ __launch_bounds__(kBlock) void nearestBinaryKernel(
         SpillArea spill;
         st.init(lds);
         nearestBinaryWalk<L, K, kStats>(nodes, prims, primIds, points[q], list, st, spill, nodeTests,
-                                        leafTests, overflow, opened);
+                                        leafTests, overflow, opened, idBits);
         writeList<L, K>(list, k, out + static_cast<size_t>(q) * outStride);
     }
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
@@ -532,8 +566,10 @@ __global__ __launch_bounds__(kBT) void batchNearestKernel(
             SpillArea spill;
             st.init(lds);
             int a = 0, b = 0, c = 0, d = 0;
+            // node ids of the scene's tree stay below 2 n_s - 1
+            const int idBits = stackIdBitsDev(2 * (primOff[s + 1] - ps) - 1);
             nearestBinaryWalk<L, K, false>(nodes + (2 * ps - s), prims + ps, primIds + ps, points[q], list,
-                                           st, spill, a, b, c, d);
+                                           st, spill, a, b, c, d, idBits);
             writeList<L, K>(list, k, out + static_cast<size_t>(q) * outStride);
         }
     }
@@ -548,7 +584,7 @@ __global__ __launch_bounds__(kBlock) void nearestWideKernel(
     const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
     const int* __restrict__ primIds, const V3* __restrict__ points,
     const uint32_t* __restrict__ order, int count, int k, int outStride,
-    ClosestResult* __restrict__ out, unsigned long long* __restrict__ stats) {
+    ClosestResult* __restrict__ out, unsigned long long* __restrict__ stats, int idBits) {
     __shared__ int lds[kLdsDepth * kBlock];
     const int i = xcdBlock() * kBlock + threadIdx.x;
     int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
@@ -591,10 +627,35 @@ __global__ __launch_bounds__(kBlock) void nearestWideKernel(
                     list.offer(L::dist2(p, prims[slot]), primIds[slot]);
                 }
             }
-            bool ok[4];
+            // nearest surviving interior child next; the others pushed far first with
+            // their distance keys; pops skip entries the k-th distance has overtaken
+            int first = -1;
 #pragma unroll
-            for (int s = 0; s < 4; ++s) ok[s] = c[s] != kWideEmpty && list.admits(key[s]);
-            id = descend(c, ok, st, spill, overflow);
+            for (int s = 0; s < 4; ++s) {
+                if (c[s] >= 0 && list.admits(key[s]) && first < 0) first = s;
+            }
+            if (!st.room(3)) {
+                overflow = 1;
+                break;
+            }
+#pragma unroll
+            for (int s = 3; s >= 0; --s) {
+                if (c[s] >= 0 && list.admits(key[s]) && s != first) QR_PUSH(st, spill, packEntry(c[s], key[s], idBits));
+            }
+            id = -1;
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (s == first) id = c[s];
+            if (first < 0) {
+                while (!st.empty()) {
+                    int v;
+                    QR_POP(st, spill, v);
+                    if (list.admits(entryKey(v, idBits))) {
+                        id = entryId(v, idBits);
+                        break;
+                    }
+                }
+            }
         }
         writeList<L, K>(list, k, out + static_cast<size_t>(q) * outStride);
     }
@@ -620,25 +681,6 @@ __device__ __forceinline__ float quadGetF(float v) {
 }
 
 constexpr int kQuadDepth = 96;  // 3 x the deepest 4-wide level a tree here can have
-
-// Stack entries of the quad walk carry the child's box distance above its id,
-// truncated to the bits the id leaves free: the leading bits of a non-negative
-// float are a lower bound of it.  A popped node whose bound already exceeds the
-// k-th distance is dropped without being opened -- opening it would find every
-// child beyond the k-th distance too (a child's box lies inside its parent's), so
-// the walk and its results are unchanged.  idBits = 32 turns the keys off (trees
-// whose wide ids need more than 23 bits).
-__device__ __forceinline__ int packEntry(int id, float key, int idBits) {
-    if (idBits >= 32) return id;
-    const uint32_t q = floatBits(key) >> (idBits - 1);
-    return static_cast<int>((q << idBits) | static_cast<uint32_t>(id));
-}
-__device__ __forceinline__ int entryId(int v, int idBits) {
-    return idBits >= 32 ? v : static_cast<int>(static_cast<uint32_t>(v) & ((1u << idBits) - 1u));
-}
-__device__ __forceinline__ float entryKey(int v, int idBits) {
-    return idBits >= 32 ? 0.0f : bitsFloat((static_cast<uint32_t>(v) >> idBits) << (idBits - 1));
-}
 
 // DPP quad permutation: lane s of each quad receives lane P[s] (2 bits per lane).
 template <int P>
@@ -1311,15 +1353,6 @@ void setTraversalMode(int mode) { gTraversalMode = mode; }
 // to 1M queries on the 2M-triangle terrain.  Closest (k = 1): four lanes per query
 // while the query set leaves most of the GPU idle one lane per query (fewer than 4
 // waves per SIMD: 256 CUs x 4 SIMDs x 4 waves x 64 lanes).
-// Bits of a stack entry taken by the wide node id (the rest hold the distance key);
-// 32 = no key, when ids need more than 23 bits (fewer than 9 would be left, too few
-// for the float's exponent and a mantissa bit).
-static int stackIdBits(int wideCount) {
-    int bits = 1;
-    while (bits < 32 && (1LL << bits) < static_cast<long long>(wideCount)) ++bits;
-    return bits <= 23 ? bits : 32;
-}
-
 static bool useQuad(int count, int K) {
     if (gTraversalMode == 1) return false;
     if (gTraversalMode == 2) return true;
@@ -1333,21 +1366,21 @@ static void launchNearestTK(const TraverseArgs& a, const V3* pts, int count, int
     const auto* prims = static_cast<const typename L::Item*>(a.items);
     if (a.wide && useQuad(count, K)) {
         const int qgrid = blocksFor(4LL * count, kBlock);
-        const int idBits = stackIdBits(a.wideCount);
+        const int idBits = stackIdBitsDev(a.wideCount);
         if (a.stats)
             nearestQuadKernel<L, K, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats, idBits);
         else
             nearestQuadKernel<L, K, false><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, idBits);
     } else if (a.wide) {
         if (a.stats)
-            nearestWideKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats);
+            nearestWideKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats, stackIdBitsDev(a.wideCount));
         else
-            nearestWideKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr);
+            nearestWideKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, stackIdBitsDev(a.wideCount));
     } else {
         if (a.stats)
-            nearestBinaryKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.nodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats);
+            nearestBinaryKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.nodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats, stackIdBitsDev(a.nodeCount));
         else
-            nearestBinaryKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.nodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr);
+            nearestBinaryKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.nodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, stackIdBitsDev(a.nodeCount));
     }
 }
 
