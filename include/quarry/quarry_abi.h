@@ -102,6 +102,12 @@ QR_API int qr_warmup(void);
  * 2 four lanes per query.  Results are identical in every mode.  No reference
  * counterpart (a tuning knob of this backend). */
 QR_API int qr_set_traversal_mode(int mode);
+/* LBVH build: how many frontier nodes a workgroup (or group of workgroups) may hand
+ * up to the next level, process-wide, 0..32 (default 32).  Units with more send
+ * their nodes through the device-wide arrival protocol instead; 0 sends every node
+ * that way.  Trees are identical byte for byte at every setting.  No reference
+ * counterpart (a tuning knob of this backend; tests use it to cover both paths). */
+QR_API int qr_set_frontier_cap(int cap);
 /* Stream for subsequent calls on this host thread (a hipStream_t; NULL = default). */
 QR_API int qr_set_stream(void* hip_stream);
 

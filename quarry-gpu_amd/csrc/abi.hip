@@ -266,6 +266,10 @@ int qr_set_traversal_mode(int mode) {
     });
 }
 
+int qr_set_frontier_cap(int cap) {
+    return guard([&] { setFrontierCap(cap); });
+}
+
 int qr_set_stream(void* s) {
     tStream = static_cast<hipStream_t>(s);
     return QR_OK;

@@ -482,180 +482,100 @@ __device__ __forceinline__ Attach attachOf(int l, int r, int dl, int dr) {
     return {left ? r : l - 1, left, left ? r : l};
 }
 
-// One thread per leaf slot k: gather the item into leaf order, write leaf node
-// n-1+k, then build the tree bottom-up without a separate hierarchy pass
-// (agglomerative order, as in Apetrei 2014): a finished node [l, r] knows its
-// parent's split from two adjacent-key prefixes, and the split slot's arrival
-// exchange hands the second child the far end of its sibling, from which both
-// child ids, the parent range and the parent's Karras id follow.  The result is
-// byte-identical to karrasNode + leafUpLocalKernel (same ids, parent[], farEnd[],
-// boxes merged with the left child's box as the left operand, heights).
-//
-// Workgroup b owns leaf slots [256 b, 256 b + 255] = [base, last].  Split slots
-// base..last-1 have both neighbouring leaves in the window: arrivals from this
-// workgroup meet there in LDS (prefixes of the window come from an LDS table).  A
-// node whose parent splits outside the window, or the first child at an LDS slot
-// whose sibling was not finished here (known after the workgroup barrier),
-// continues on the global slot: arrive[split] takes the far end + 1 by an
-// agent-scope exchange (0 = empty; the second child resets it, so every slot is
-// zero between kernels), boxes and heights cross workgroups with agent-scope
-// stores.
-template <typename L>
-__global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
-    const typename L::Item* __restrict__ src, const uint32_t* __restrict__ code,
-    const uint32_t* __restrict__ order, int n, typename L::Item* __restrict__ sorted,
-    int* __restrict__ primIds, Node2* __restrict__ nodes, int* __restrict__ parent,
-    int* __restrict__ arrive, int* __restrict__ height, int* __restrict__ farEnd) {
-    __shared__ float sLeaf[kBlock][6];
-    __shared__ float sNode[kBlock][6];
-    __shared__ int sSlot[kBlock];
-    __shared__ int sHeight[kBlock];
-    __shared__ int sDelta[kBlock + 1];  // delta(base - 1 + j)
-    const int base = blockIdx.x * kBlock;
-    const int last = (base + kBlock < n ? base + kBlock : n) - 1;
-    const int k = base + threadIdx.x;
-    const bool live = k < n;
-    sSlot[threadIdx.x] = 0;
-    sDelta[threadIdx.x] = adjPrefix(code, order, n, base - 1 + threadIdx.x);
-    if (threadIdx.x == 0) {
-        sDelta[kBlock] = adjPrefix(code, order, n, base + kBlock - 1);
-        if (blockIdx.x == 0) parent[0] = -1;
-    }
+// A finished node on its way up: leaf range, Karras id, height, the prefixes
+// delta(l-1) and delta(r) that decide where it attaches, and its box.
+struct NodeState {
+    int l, r, id, h, dl, dr;
+    Box box;
+};
 
-    Box box{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-    if (live) {
-        const int sceneIdx = static_cast<int>(order[k]);
-        const typename L::Item item = src[sceneIdx];
-        sorted[k] = item;
-        primIds[k] = sceneIdx;
-        box = L::bounds(item);
-        Node2 leaf;
-        leaf.lo[0] = box.lo.x;
-        leaf.lo[1] = box.lo.y;
-        leaf.lo[2] = box.lo.z;
-        leaf.hi[0] = box.hi.x;
-        leaf.hi[1] = box.hi.y;
-        leaf.hi[2] = box.hi.z;
-        leaf.left = -(k + 1);
-        leaf.right = 1;
-        nodes[n - 1 + k] = leaf;
-        float* mine = sLeaf[threadIdx.x];
-        mine[0] = box.lo.x;
-        mine[1] = box.lo.y;
-        mine[2] = box.lo.z;
-        mine[3] = box.hi.x;
-        mine[4] = box.hi.y;
-        mine[5] = box.hi.z;
-    }
+// A node handed up to the next level (48 B).
+struct FrontierEntry {
+    int l, r, id, h, dl, dr;
+    float box[6];
+};
+static_assert(sizeof(FrontierEntry) == 48, "FrontierEntry layout");
+
+__device__ __forceinline__ FrontierEntry toEntry(const NodeState& s) {
+    return {s.l, s.r, s.id, s.h, s.dl, s.dr, {s.box.lo.x, s.box.lo.y, s.box.lo.z, s.box.hi.x, s.box.hi.y, s.box.hi.z}};
+}
+__device__ __forceinline__ NodeState fromEntry(const FrontierEntry& e) {
+    return {e.l, e.r, e.id, e.h, e.dl, e.dr, {{e.box[0], e.box[1], e.box[2]}, {e.box[3], e.box[4], e.box[5]}}};
+}
+
+// Rank of this thread's pending node among the workgroup's pending nodes by
+// position (a leaf offset or an entry index, distinct per pending node), so the
+// frontier is staged in leaf order; sWave[kThreads / 64] receives the count.
+template <int kThreads = kBlock>
+__device__ __forceinline__ int frontierRank(bool pend, int pos, int* sMark, int* sWave) {
+    const int t = threadIdx.x;
+    sMark[t] = 0;
     __syncthreads();
-
-    int l = k, r = k, my = n - 1 + k, h = 0;
-    Attach at{0, false, 0};
-    bool done = !live, global = false;
-    int dangling = -1;
-    if (live) at = attachOf(l, r, sDelta[l - base], sDelta[r - base + 1]);
-    while (!done) {
-        const int g = at.split;
-        if (g < base || g >= last) {
-            global = true;
-            break;
-        }
-        const int myFar = at.left ? l : r;
-        const int prior = __hip_atomic_exchange(&sSlot[g - base], myFar + 1, __ATOMIC_ACQ_REL,
-                                                __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (prior == 0) {
-            dangling = g;
-            break;
-        }
-        const int sibFar = prior - 1;
-        int lo, hi, lid, rid, sib;
-        if (at.left) {
-            lo = l;
-            hi = sibFar;
-            lid = my;
-            rid = sib = (sibFar == g + 1) ? n + g : g + 1;
-        } else {
-            lo = sibFar;
-            hi = r;
-            lid = sib = (sibFar == g) ? n - 1 + g : g;
-            rid = my;
-        }
-        const float* sb = sib >= n - 1 ? sLeaf[sib - (n - 1) - base] : sNode[sib - base];
-        const Box other{{sb[0], sb[1], sb[2]}, {sb[3], sb[4], sb[5]}};
-        const int sh = sib >= n - 1 ? 0 : sHeight[sib - base];
-        box = at.left ? mergeChildren(box, other) : mergeChildren(other, box);
-        h = (h > sh ? h : sh) + 1;
-        int pid = 0;
-        if (lo == 0 && hi == n - 1) {
-            done = true;
-        } else {
-            at = attachOf(lo, hi, sDelta[lo - base], sDelta[hi - base + 1]);
-            pid = at.id;
-        }
-        float* dst = sNode[pid - base];
-        dst[0] = box.lo.x;
-        dst[1] = box.lo.y;
-        dst[2] = box.lo.z;
-        dst[3] = box.hi.x;
-        dst[4] = box.hi.y;
-        dst[5] = box.hi.z;
-        sHeight[pid - base] = h;
-        Node2 nd;
-        nd.lo[0] = box.lo.x;
-        nd.lo[1] = box.lo.y;
-        nd.lo[2] = box.lo.z;
-        nd.hi[0] = box.hi.x;
-        nd.hi[1] = box.hi.y;
-        nd.hi[2] = box.hi.z;
-        nd.left = lid;
-        nd.right = rid;
-        nodes[pid] = nd;
-        height[pid] = h;
-        parent[lid] = pid;
-        parent[rid] = pid;
-        farEnd[pid] = pid == lo ? hi : lo;
-        l = lo;
-        r = hi;
-        my = pid;
-    }
+    if (pend) sMark[pos] = 1;
     __syncthreads();
-    // a first arrival whose sibling never came in this workgroup moves to the global slot
-    if (dangling >= 0 && sSlot[dangling - base] == (at.left ? l : r) + 1) global = true;
-    if (!global) return;
+    const uint64_t bal = __ballot(sMark[t]);
+    const int lane = t & 63;
+    const int before = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) sWave[t >> 6] = __popcll(bal);
+    __syncthreads();
+    int prefix = 0;
+    for (int w = 0; w < (t >> 6); ++w) prefix += sWave[w];
+    sMark[t] = prefix + before;
+    __syncthreads();
+    if (t == 0) {
+        int total = 0;
+        for (int w = 0; w < kThreads / 64; ++w) total += sWave[w];
+        sWave[kThreads / 64] = total;
+    }
+    const int rank = pend ? sMark[pos] : 0;
+    __syncthreads();
+    return rank;
+}
 
-    writeBoxAgent(&nodes[my], box);
-    if (my < n - 1) stAgentI(&height[my], h);
-    while (true) {
+// Ids of the two children merged at split g, given this node [l, r] and the far end
+// of its sibling; returns the parent's range through lo / hi and the sibling's id.
+__device__ __forceinline__ void mergeIds(int n, int l, int r, int my, int g, bool left, int sibFar,
+                                         int& lo, int& hi, int& lid, int& rid, int& sib) {
+    if (left) {
+        lo = l;
+        hi = sibFar;
+        lid = my;
+        rid = sib = (sibFar == g + 1) ? n + g : g + 1;
+    } else {
+        lo = sibFar;
+        hi = r;
+        lid = sib = (sibFar == g) ? n - 1 + g : g;
+        rid = my;
+    }
+}
+
+// The agent-scope arrival protocol, from node s up to where its thread arrives
+// first: arrive[split] takes the far end + 1 by an exchange (0 = empty; the second
+// child resets it, so every slot is zero between kernels), boxes and heights cross
+// workgroups with agent-scope stores.
+__device__ void globalClimb(NodeState s, const uint32_t* __restrict__ code,
+                            const uint32_t* __restrict__ order, int n, Node2* __restrict__ nodes,
+                            int* __restrict__ parent, int* __restrict__ arrive, int* __restrict__ height,
+                            int* __restrict__ farEnd) {
+    writeBoxAgent(&nodes[s.id], s.box);
+    if (s.id < n - 1) stAgentI(&height[s.id], s.h);
+    while (!(s.l == 0 && s.r == n - 1)) {
+        const Attach at = attachOf(s.l, s.r, s.dl, s.dr);
         const int g = at.split;
-        const int myFar = at.left ? l : r;
         storesDone();
-        const int prior = __hip_atomic_exchange(&arrive[g], myFar + 1, __ATOMIC_RELAXED,
+        const int prior = __hip_atomic_exchange(&arrive[g], (at.left ? s.l : s.r) + 1, __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_AGENT);
-        if (prior == 0) break;
+        if (prior == 0) return;
         stAgentI(&arrive[g], 0);
-        const int sibFar = prior - 1;
         int lo, hi, lid, rid, sib;
-        if (at.left) {
-            lo = l;
-            hi = sibFar;
-            lid = my;
-            rid = sib = (sibFar == g + 1) ? n + g : g + 1;
-        } else {
-            lo = sibFar;
-            hi = r;
-            lid = sib = (sibFar == g) ? n - 1 + g : g;
-            rid = my;
-        }
+        mergeIds(n, s.l, s.r, s.id, g, at.left, prior - 1, lo, hi, lid, rid, sib);
         const Box sb = readBoxAgent(&nodes[sib]);
         const int sh = sib >= n - 1 ? 0 : ldAgentI(&height[sib]);
-        box = at.left ? mergeChildren(box, sb) : mergeChildren(sb, box);
-        h = (h > sh ? h : sh) + 1;
-        const bool root = lo == 0 && hi == n - 1;
-        int pid = 0;
-        if (!root) {
-            at = attachOf(lo, hi, adjPrefix(code, order, n, lo - 1), adjPrefix(code, order, n, hi));
-            pid = at.id;
-        }
+        const Box box = at.left ? mergeChildren(s.box, sb) : mergeChildren(sb, s.box);
+        const int h = (s.h > sh ? s.h : sh) + 1;
+        const int ndl = adjPrefix(code, order, n, lo - 1);
+        const int ndr = adjPrefix(code, order, n, hi);
+        const int pid = (lo == 0 && hi == n - 1) ? 0 : attachOf(lo, hi, ndl, ndr).id;
         writeBoxAgent(&nodes[pid], box);
         nodes[pid].left = lid;
         nodes[pid].right = rid;
@@ -663,11 +583,363 @@ __global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
         parent[lid] = pid;
         parent[rid] = pid;
         farEnd[pid] = pid == lo ? hi : lo;
-        if (root) break;
-        l = lo;
-        r = hi;
-        my = pid;
+        s = {lo, hi, pid, h, ndl, ndr, box};
     }
+}
+
+// Frontier hand-up between kernels.  The nodes a workgroup cannot finish inside
+// its leaf window -- its frontier, the maximal subtrees in the window, in leaf
+// order -- are staged in global memory with their ranges, prefixes, heights and
+// boxes.  The next kernel merges the frontiers of kMergeUnits neighbouring units per
+// workgroup in LDS, with the boundaries between consecutive entries as split slots,
+// and stages the group's own frontier; the last level makes the root.  Kernel
+// boundaries order the staging, so no workgroup waits and no device-scope atomic
+// is needed.  A unit whose frontier has more than the capacity (a tree with long
+// spines across windows) sends its nodes through globalClimb instead and stages a
+// count of -1, which sends the whole group, and every group above it, the same
+// way -- so the two protocols never meet at one split.
+constexpr int kMergeUnits = 32;
+constexpr int kFrontierCap = 32;
+constexpr int kMergeBlock = kMergeUnits * kFrontierCap;  // one thread per entry
+constexpr int kEmitMaxLevels = 10;
+
+struct EmitLevels {
+    int levels;                    // merge levels above the windows (0: one window holds all)
+    int units[kEmitMaxLevels];     // units per level (level 0: the windows)
+    int stageOff[kEmitMaxLevels];  // first FrontierEntry of level L's staging (kFrontierCap per unit)
+    int countOff[kEmitMaxLevels];  // first frontier count of level L's units
+};
+
+int gFrontierCap = kFrontierCap;  // qr_set_frontier_cap
+
+// One thread per leaf slot k: gather the item into leaf order, write leaf node
+// n-1+k, then build the tree bottom-up without a separate hierarchy pass
+// (agglomerative order, as in Apetrei 2014): a finished node [l, r] knows its
+// parent's split from two adjacent-key prefixes, and the split slot's arrival hands
+// the second child the far end of its sibling, from which both child ids, the
+// parent range and the parent's Karras id follow.  The result is byte-identical to
+// karrasNode + leafUpLocalKernel (same ids, parent[], farEnd[], boxes merged with
+// the left child's box as the left operand, heights).
+//
+// Workgroup b owns leaf slots [256 b, 256 b + 255] = [base, last].  Split slots
+// base..last-1 have both neighbouring leaves in the window: arrivals from this
+// workgroup meet there in LDS (prefixes of the window come from an LDS table).  A
+// node whose parent splits outside the window, or the first child at an LDS slot
+// whose sibling was not finished here (known after the workgroup barrier), is on the
+// window's frontier and is handed up.
+template <typename L>
+__global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
+    const typename L::Item* __restrict__ src, const uint32_t* __restrict__ code,
+    const uint32_t* __restrict__ order, int n, typename L::Item* __restrict__ sorted,
+    int* __restrict__ primIds, Node2* __restrict__ nodes, int* __restrict__ parent,
+    int* __restrict__ arrive, int* __restrict__ height, int* __restrict__ farEnd,
+    FrontierEntry* __restrict__ stage, int* __restrict__ counts, int frontierCap) {
+    __shared__ float sLeaf[kBlock][6];
+    __shared__ float sNode[kBlock][6];
+    __shared__ int sSlot[kBlock];
+    __shared__ int sHeight[kBlock];
+    __shared__ int sDelta[kBlock + 1];  // delta(base - 1 + j)
+    __shared__ int sMark[kBlock];
+    __shared__ int sWave[kBlock / 64 + 1];
+    const int t = threadIdx.x;
+    const int base = blockIdx.x * kBlock;
+    const int last = (base + kBlock < n ? base + kBlock : n) - 1;
+    const int k = base + t;
+    const bool live = k < n;
+    sSlot[t] = 0;
+    sDelta[t] = adjPrefix(code, order, n, base - 1 + t);
+    if (t == 0) {
+        sDelta[kBlock] = adjPrefix(code, order, n, base + kBlock - 1);
+        if (blockIdx.x == 0) parent[0] = -1;
+    }
+
+    NodeState s{k, k, n - 1 + k, 0, -1, -1, {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}}};
+    if (live) {
+        const int sceneIdx = static_cast<int>(order[k]);
+        const typename L::Item item = src[sceneIdx];
+        sorted[k] = item;
+        primIds[k] = sceneIdx;
+        s.box = L::bounds(item);
+        Node2 leaf;
+        leaf.lo[0] = s.box.lo.x;
+        leaf.lo[1] = s.box.lo.y;
+        leaf.lo[2] = s.box.lo.z;
+        leaf.hi[0] = s.box.hi.x;
+        leaf.hi[1] = s.box.hi.y;
+        leaf.hi[2] = s.box.hi.z;
+        leaf.left = -(k + 1);
+        leaf.right = 1;
+        nodes[n - 1 + k] = leaf;
+        float* mine = sLeaf[t];
+        mine[0] = s.box.lo.x;
+        mine[1] = s.box.lo.y;
+        mine[2] = s.box.lo.z;
+        mine[3] = s.box.hi.x;
+        mine[4] = s.box.hi.y;
+        mine[5] = s.box.hi.z;
+    }
+    __syncthreads();
+
+    // ---- the window ----------------------------------------------------------------
+    bool pend = false;
+    int dangling = -1;
+    Attach at{0, false, 0};
+    if (live) {
+        s.dl = sDelta[t];
+        s.dr = sDelta[t + 1];
+    }
+    bool done = !live;
+    if (live) at = attachOf(s.l, s.r, s.dl, s.dr);
+    while (!done) {
+        const int g = at.split;
+        if (g < base || g >= last) {
+            pend = true;
+            break;
+        }
+        const int prior = __hip_atomic_exchange(&sSlot[g - base], (at.left ? s.l : s.r) + 1, __ATOMIC_ACQ_REL,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (prior == 0) {
+            dangling = g;
+            break;
+        }
+        int lo, hi, lid, rid, sib;
+        mergeIds(n, s.l, s.r, s.id, g, at.left, prior - 1, lo, hi, lid, rid, sib);
+        const float* sb = sib >= n - 1 ? sLeaf[sib - (n - 1) - base] : sNode[sib - base];
+        const Box other{{sb[0], sb[1], sb[2]}, {sb[3], sb[4], sb[5]}};
+        const int sh = sib >= n - 1 ? 0 : sHeight[sib - base];
+        s.box = at.left ? mergeChildren(s.box, other) : mergeChildren(other, s.box);
+        s.h = (s.h > sh ? s.h : sh) + 1;
+        s.dl = sDelta[lo - base];
+        s.dr = sDelta[hi - base + 1];
+        int pid = 0;
+        if (lo == 0 && hi == n - 1) {
+            done = true;
+        } else {
+            at = attachOf(lo, hi, s.dl, s.dr);
+            pid = at.id;
+        }
+        float* dst = sNode[pid - base];
+        dst[0] = s.box.lo.x;
+        dst[1] = s.box.lo.y;
+        dst[2] = s.box.lo.z;
+        dst[3] = s.box.hi.x;
+        dst[4] = s.box.hi.y;
+        dst[5] = s.box.hi.z;
+        sHeight[pid - base] = s.h;
+        Node2 nd;
+        nd.lo[0] = s.box.lo.x;
+        nd.lo[1] = s.box.lo.y;
+        nd.lo[2] = s.box.lo.z;
+        nd.hi[0] = s.box.hi.x;
+        nd.hi[1] = s.box.hi.y;
+        nd.hi[2] = s.box.hi.z;
+        nd.left = lid;
+        nd.right = rid;
+        nodes[pid] = nd;
+        if (pid == 0) height[0] = s.h;  // the tree depth; other heights stay in LDS / entries
+        parent[lid] = pid;
+        parent[rid] = pid;
+        farEnd[pid] = pid == lo ? hi : lo;
+        s.l = lo;
+        s.r = hi;
+        s.id = pid;
+    }
+    __syncthreads();
+    // a first arrival whose sibling never came in this workgroup is on the frontier
+    if (dangling >= 0 && sSlot[dangling - base] == (at.left ? s.l : s.r) + 1) pend = true;
+    int pos = pend ? s.l - base : 0;  // position of the entry inside its unit
+
+    // ---- hand-up: stage the window's frontier in leaf order ----------------------------
+    if (gridDim.x == 1) {  // one window holds the whole tree: the root is made
+        if (pend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
+        return;
+    }
+    const int rank = frontierRank(pend, pos, sMark, sWave);
+    const int total = sWave[kBlock / 64];
+    if (total > frontierCap) {
+        if (pend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
+        if (t == 0) counts[blockIdx.x] = -1;
+    } else {
+        if (pend) stage[blockIdx.x * kFrontierCap + rank] = toEntry(s);
+        if (t == 0) counts[blockIdx.x] = total;
+    }
+}
+
+// Level L >= 1 of the hand-up: workgroup g merges the frontiers of units
+// [g kMergeUnits, g kMergeUnits + kMergeUnits) of level L-1 (one thread per entry)
+// and stages its own frontier as unit g of level L.
+__global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
+    const FrontierEntry* __restrict__ inStage, const int* __restrict__ inCounts, int inUnits,
+    FrontierEntry* __restrict__ outStage, int* __restrict__ outCounts, bool top,
+    const uint32_t* __restrict__ code, const uint32_t* __restrict__ order, int n,
+    Node2* __restrict__ nodes, int* __restrict__ parent, int* __restrict__ arrive,
+    int* __restrict__ height, int* __restrict__ farEnd, int frontierCap) {
+    __shared__ float sBoxL[kMergeBlock][6];  // box of the node starting at entry j
+    __shared__ float sBoxR[kMergeBlock][6];  // box of the node ending at entry j
+    __shared__ int sHL[kMergeBlock], sHR[kMergeBlock];
+    __shared__ int sSlot[kMergeBlock];
+    __shared__ int sFarLeaf[2][kMergeBlock];  // per side of a slot, the arriving node's far end,
+    __shared__ int sFarEntry[2][kMergeBlock]; //   its far entry index
+    __shared__ int sFarD[2][kMergeBlock];     //   and its prefix at that end
+    __shared__ int sMark[kMergeBlock];
+    __shared__ int sWave[kMergeBlock / 64 + 1];
+    __shared__ int sOff[kMergeUnits + 1];
+    __shared__ int sBroken;
+    const int t = threadIdx.x;
+    const int g = blockIdx.x;
+    const int first = g * kMergeUnits;
+    const int members = inUnits - first < kMergeUnits ? inUnits - first : kMergeUnits;
+    if (t == 0) sBroken = 0;
+    __syncthreads();
+    if (t < members) {
+        const int c = inCounts[first + t];
+        if (c < 0) sBroken = 1;
+        sOff[t + 1] = c < 0 ? 0 : (c > kFrontierCap ? kFrontierCap : c);
+    }
+    __syncthreads();
+    if (t == 0) {
+        sOff[0] = 0;
+        for (int m = 1; m <= members; ++m) sOff[m] += sOff[m - 1];
+    }
+    __syncthreads();
+    const int count = sOff[members];
+    const bool broken = sBroken != 0;
+    NodeState s{};
+    bool pend = false;
+    if (t < count) {
+        int m = 0;
+        while (sOff[m + 1] <= t) ++m;
+        s = fromEntry(inStage[(first + m) * kFrontierCap + (t - sOff[m])]);
+        pend = true;
+    }
+    if (broken) {  // a unit below went global: so does the whole group
+        if (pend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
+        if (t == 0 && !top) outCounts[g] = -1;
+        return;
+    }
+
+    // climb over the group's entries; slot j lies between entries j and j + 1
+    sSlot[t] = 0;
+    if (pend) {
+        float* bl = sBoxL[t];
+        float* br = sBoxR[t];
+        bl[0] = br[0] = s.box.lo.x;
+        bl[1] = br[1] = s.box.lo.y;
+        bl[2] = br[2] = s.box.lo.z;
+        bl[3] = br[3] = s.box.hi.x;
+        bl[4] = br[4] = s.box.hi.y;
+        bl[5] = br[5] = s.box.hi.z;
+        sHL[t] = s.h;
+        sHR[t] = s.h;
+    }
+    __syncthreads();
+    int jl = t, jr = t, hang = -1;
+    while (pend) {
+        if (s.l == 0 && s.r == n - 1) {  // the root (never staged; kept for safety)
+            pend = false;
+            break;
+        }
+        const Attach a2 = attachOf(s.l, s.r, s.dl, s.dr);
+        const int slot = a2.left ? jr : jl - 1;
+        if (slot < 0 || slot >= count - 1) break;  // splits outside the group: still pending
+        const int side = a2.left ? 0 : 1;
+        sFarLeaf[side][slot] = a2.left ? s.l : s.r;
+        sFarEntry[side][slot] = a2.left ? jl : jr;
+        sFarD[side][slot] = a2.left ? s.dl : s.dr;
+        const int prior = __hip_atomic_fetch_add(&sSlot[slot], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (prior == 0) {
+            hang = slot;
+            break;
+        }
+        const int sibFar = sFarLeaf[1 - side][slot];
+        const int sibEntry = sFarEntry[1 - side][slot];
+        const int sibD = sFarD[1 - side][slot];
+        int lo, hi, lid, rid, sib;
+        mergeIds(n, s.l, s.r, s.id, a2.split, a2.left, sibFar, lo, hi, lid, rid, sib);
+        (void)sib;
+        const float* sb = a2.left ? sBoxL[jr + 1] : sBoxR[jl - 1];
+        const Box other{{sb[0], sb[1], sb[2]}, {sb[3], sb[4], sb[5]}};
+        const int sh = a2.left ? sHL[jr + 1] : sHR[jl - 1];
+        s.box = a2.left ? mergeChildren(s.box, other) : mergeChildren(other, s.box);
+        s.h = (s.h > sh ? s.h : sh) + 1;
+        if (a2.left) {
+            s.dr = sibD;
+            jr = sibEntry;
+        } else {
+            s.dl = sibD;
+            jl = sibEntry;
+        }
+        const bool root = lo == 0 && hi == n - 1;
+        const int pid = root ? 0 : attachOf(lo, hi, s.dl, s.dr).id;
+        float* bl = sBoxL[jl];
+        float* br = sBoxR[jr];
+        bl[0] = br[0] = s.box.lo.x;
+        bl[1] = br[1] = s.box.lo.y;
+        bl[2] = br[2] = s.box.lo.z;
+        bl[3] = br[3] = s.box.hi.x;
+        bl[4] = br[4] = s.box.hi.y;
+        bl[5] = br[5] = s.box.hi.z;
+        sHL[jl] = s.h;
+        sHR[jr] = s.h;
+        Node2 nd;
+        nd.lo[0] = s.box.lo.x;
+        nd.lo[1] = s.box.lo.y;
+        nd.lo[2] = s.box.lo.z;
+        nd.hi[0] = s.box.hi.x;
+        nd.hi[1] = s.box.hi.y;
+        nd.hi[2] = s.box.hi.z;
+        nd.left = lid;
+        nd.right = rid;
+        nodes[pid] = nd;
+        if (root) height[0] = s.h;  // the tree depth; other heights travel in the entries
+        parent[lid] = pid;
+        parent[rid] = pid;
+        farEnd[pid] = pid == lo ? hi : lo;
+        s.l = lo;
+        s.r = hi;
+        s.id = pid;
+        if (root) pend = false;
+    }
+    __syncthreads();
+    // pending: stopped at a split outside the group, or a first arrival left alone
+    if (hang >= 0) pend = sSlot[hang] == 1;
+    if (top) {  // the whole tree: only the root is left (kept for safety)
+        if (pend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
+        return;
+    }
+    const int rank = frontierRank<kMergeBlock>(pend, jl, sMark, sWave);
+    const int total = sWave[kMergeBlock / 64];
+    if (total > frontierCap) {
+        if (pend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
+        if (t == 0) outCounts[g] = -1;
+    } else {
+        if (pend) outStage[g * kFrontierCap + rank] = toEntry(s);
+        if (t == 0) outCounts[g] = total;
+    }
+}
+
+// Hand-up levels for n leaves: windows of kBlock leaves, then groups of kMergeUnits
+// units until one unit is left.  Returns the sizes of the staging (entries) and of
+// the counts.
+static EmitLevels emitLevels(int n, int* stageEntries, int* countInts) {
+    EmitLevels lv{};
+    int u = blocksFor(n, kBlock);
+    int stage = 0, cnt = 0, level = 0;
+    lv.units[0] = u;
+    while (u > 1) {
+        if (level + 1 >= kEmitMaxLevels) throw ArgError("build: too many primitives for the hand-up levels");
+        lv.stageOff[level] = stage;
+        stage += u * kFrontierCap;
+        lv.countOff[level] = cnt;
+        cnt += u;
+        u = (u + kMergeUnits - 1) / kMergeUnits;
+        ++level;
+        lv.units[level] = u;
+    }
+    lv.levels = level;
+    *stageEntries = stage > 0 ? stage : 1;
+    *countInts = cnt > 0 ? cnt : 1;
+    return lv;
 }
 
 // ---- batched (segmented) LBVH: many independent scenes per launch -----------------------
@@ -835,6 +1107,10 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     b.farEnd.alloc(static_cast<size_t>(n - 1));
     b.sortWs.reserve(static_cast<size_t>(n));
     b.counters.reserve(kBoundsWords);
+    int stageEntries = 0, countInts = 0;
+    const EmitLevels lv = emitLevels(n, &stageEntries, &countInts);
+    b.emitStage.reserve(static_cast<size_t>(stageEntries) * (sizeof(FrontierEntry) / sizeof(int)));
+    b.emitCounts.reserve(static_cast<size_t>(countInts));
 
     DevBuf<uint32_t> codes(static_cast<size_t>(n));
     DevBuf<uint32_t> codesSorted(static_cast<size_t>(n));
@@ -863,12 +1139,23 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     sortPairs(b.sortWs, codes.p, nullptr, codesSorted.p, order.p, static_cast<size_t>(n), 4, s);
     ev.record(2, s);
 
-    // hierarchy, leaf gather and boxes in one pass (no separate Karras kernel)
+    // hierarchy, leaf gather and boxes: the windows, then one merge kernel per level
+    // (no separate Karras kernel)
     ev.record(3, s);
-    lbvhEmitKernel<L><<<blocksFor(n, kBlock), kBlock, 0, s>>>(
+    auto* stage = reinterpret_cast<FrontierEntry*>(b.emitStage.p);
+    lbvhEmitKernel<L><<<lv.units[0], kBlock, 0, s>>>(
         dSrc, codesSorted.p, order.p, n, reinterpret_cast<Item*>(b.items.p), b.primIds.p, b.nodes.p,
-        b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
+        b.parent.p, b.arrive.p, b.height.p, b.farEnd.p, stage + lv.stageOff[0], b.emitCounts.p + lv.countOff[0],
+        gFrontierCap);
     hipCheck(hipGetLastError(), "lbvh emit");
+    for (int level = 1; level <= lv.levels; ++level) {
+        const bool top = level == lv.levels;
+        lbvhMergeKernel<<<lv.units[level], kMergeBlock, 0, s>>>(
+            stage + lv.stageOff[level - 1], b.emitCounts.p + lv.countOff[level - 1], lv.units[level - 1],
+            top ? nullptr : stage + lv.stageOff[level], top ? nullptr : b.emitCounts.p + lv.countOff[level], top,
+            codesSorted.p, order.p, n, b.nodes.p, b.parent.p, b.arrive.p, b.height.p, b.farEnd.p, gFrontierCap);
+        hipCheck(hipGetLastError(), "lbvh merge");
+    }
     ev.record(4, s);
 
     int depth = 0;
@@ -892,6 +1179,11 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     b.depthValid = false;
     b.wideValid = false;
     b.wideCount = 0;
+}
+
+void setFrontierCap(int cap) {
+    if (cap < 0 || cap > kFrontierCap) throw ArgError("frontier capacity must be in [0, 32]");
+    gFrontierCap = cap;
 }
 
 void buildLbvhDevice(BvhState& b, int kind, const void* dItems, int n, hipStream_t s,

@@ -55,6 +55,8 @@ struct BvhState {
     DevBuf<int> dirtyIds;
     DevBuf<int> dirtyLeaves;
     DevBuf<int> counters;           // small device counters
+    DevBuf<int> emitStage;          // LBVH build: frontier entries handed up between levels
+    DevBuf<int> emitCounts;         //   frontier sizes per unit (-1: the unit went device-wide)
 
     // scene-order primitives kept for validate / refit-by-device-pointer paths
     SortWorkspace sortWs;
@@ -90,6 +92,8 @@ void batchMortonOrder(int kind, const void* items, int n, const int* sceneId, in
                       uint32_t* order, hipStream_t s);
 
 // Builds into an empty state from scene-order items already on the device.
+// Frontier nodes a unit of the LBVH build may hand up (0..16, see qr_set_frontier_cap).
+void setFrontierCap(int cap);
 void buildLbvhDevice(BvhState& b, int kind, const void* dItems, int n, hipStream_t s,
                      BuildTimingsC* t);
 void buildSahDevice(BvhState& b, int kind, const void* dItems, int n, hipStream_t s,

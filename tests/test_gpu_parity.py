@@ -67,6 +67,30 @@ def test_terrain_build_deterministic(A, orc, scenes):
     assert h.pop() == orc.fnv_bytes(cn)
 
 
+@pytest.mark.parametrize("cap", [0, 1, 5, 32])
+def test_lbvh_frontier_cap(A, orc, scenes, cap):
+    """The build hands each workgroup's unfinished nodes up through groups of
+    workgroups; units with more than `cap` of them take the device-wide arrival
+    protocol instead (cap 0: every one).  The tree must not depend on the mix."""
+    log(f"frontier cap {cap}")
+    rng = np.random.default_rng(40 + cap)
+    cases = [(0, scenes.terrain(256, 2)),                                 # 131,072 triangles
+             (1, rng.uniform(-9, 9, (70001, 3)).astype(np.float32)),
+             (1, np.repeat(rng.uniform(-1, 1, (300, 3)), 37, axis=0).astype(np.float32))]  # duplicates
+    try:
+        A.check(A.lib().qr_set_frontier_cap(cap))
+        for kind, items in cases:
+            b = A.Bvh(kind, items)
+            cn, cids, depth = orc.lbvh_nodes(kind, items)
+            nodes, ids = b.read()
+            assert eq(ids, cids) and eq(nodes, cn), (kind, items.shape)
+            assert b.info().depth == depth
+    finally:
+        A.check(A.lib().qr_set_frontier_cap(32))
+    with pytest.raises(A.QuarryError):
+        A.check(A.lib().qr_set_frontier_cap(33))
+
+
 def test_duplicate_centroids(A, orc):
     pts = np.zeros((4096, 3), np.float32)  # one Morton cell: order by index only
     pts[::3] = 1.0
