@@ -40,8 +40,9 @@ struct qr_points {
     bool bbValid = false;
     SortWorkspace ws;
     // radius scratch, kept across calls: per-query counts, scan / classification
-    // scratch, and [0] = largest per-query count
+    // scratch, [0] = largest per-query count, and the captured first hits per query
     DevBuf<int> rCounts, rScratch, rMax;
+    DevBuf<uint64_t> rCapture;
 };
 
 struct qr_csr {
@@ -1266,11 +1267,13 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
             p->rCounts.reserve(static_cast<size_t>(n));
             p->rScratch.reserve(static_cast<size_t>(std::max(scanScratchInts(n), 4 + 2 * n)));
             p->rMax.reserve(1);
+            const int cap = radiusCaptureCap(n);
+            if (cap > 0) p->rCapture.reserve(static_cast<size_t>(n) * cap);
             int* counts = p->rCounts.p;
             int* scratch = p->rScratch.p;
             zeroInts(p->rMax.p, 1, s);
             kev.record(0, s);
-            launchRadiusCount(a, p->pts.p, n, r2, counts, p->rMax.p, s);
+            const bool captured = launchRadiusCapture(a, p->pts.p, n, r2, counts, p->rCapture.p, cap, p->rMax.p, s);
             kev.record(1, s);
             scanExclusive(counts, res->offsets.p, n, res->offsets.p + n, scratch, s);
             int total = 0, maxSeg = 0;
@@ -1282,7 +1285,10 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
             kev.record(2, s);
             if (total > 0) {
                 a.stats = nullptr;
-                launchRadiusFill(a, p->pts.p, n, r2, res->offsets.p, res->keys.p, s);
+                if (captured) radiusCompact(counts, res->offsets.p, p->rCapture.p, n, cap, res->keys.p, s);
+                // a second walk only for the queries the capture could not hold
+                if (!captured || maxSeg > cap)
+                    launchRadiusFill(a, p->pts.p, n, r2, counts, captured ? cap : 0, res->offsets.p, res->keys.p, s);
             }
             kev.record(3, s);
             if (total > 0) segmentSort(res->offsets.p, res->keys.p, n, maxSeg, scratch, s);

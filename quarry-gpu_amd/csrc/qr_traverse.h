@@ -46,11 +46,19 @@ void launchClosest(const TraverseArgs& a, const V3* pts, int count, ClosestResul
 void launchKnn(const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
                ClosestResult* out, hipStream_t s);
 // maxCount (may be null): atomically raised to the largest per-query count.
-void launchRadiusCount(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts, int* maxCount,
-                       hipStream_t s);
-// offsets has count + 1 entries (CSR, in query order).
-void launchRadiusFill(const TraverseArgs& a, const V3* pts, int count, float r2,
-                      const int* offsets, uint64_t* keys, hipStream_t s);
+// Radius, one walk: counts hits per query and keeps the first `cap` of each query in
+// its slot of `capture` (count * cap keys); returns false when the tree or mode takes
+// the per-lane kernels, which only count (nothing is captured).
+int radiusCaptureCap(int count);  // hits kept per query: 64, fewer for huge query sets
+bool launchRadiusCapture(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts,
+                         uint64_t* capture, int cap, int* maxCount, hipStream_t s);
+// Writes the CSR segments of the queries the capture does not hold (all queries when
+// nothing was captured: cap 0); counts are those of the capture / count pass.
+void launchRadiusFill(const TraverseArgs& a, const V3* pts, int count, float r2, const int* counts,
+                      int cap, const int* offsets, uint64_t* keys, hipStream_t s);
+// Copies the captured segments into the CSR keys.
+void radiusCompact(const int* counts, const int* offsets, const uint64_t* capture, int count, int cap,
+                   uint64_t* keys, hipStream_t s);
 // scratchInts must hold 4 + 2 * count ints.
 void segmentSort(const int* offsets, uint64_t* keys, int count, int maxSeg, int* scratchInts,
                  hipStream_t s);

@@ -898,26 +898,38 @@ __global__ __launch_bounds__(kBlock) void nearestQuadKernel(
 }
 
 // Radius with four lanes per query: lane s tests entry s; hits of a node are
-// counted (count pass) or written at the quad's cursor plus the number of hits in
-// lower slots (fill pass, bounded by the query's segment end).
-template <typename L, bool kFill, bool kStats>
+// counted, or written at the quad's cursor plus the number of hits in lower slots.
+// kMode: kRadiusCount counts; kRadiusFill writes into the query's CSR segment
+// (bounded by the next offset), skipping queries whose hits the capture pass already
+// holds (counts[q] <= cap); kRadiusCaptureMode counts and writes the first `cap` hits
+// into the query's fixed slot of `keys`, so that one walk serves every query with at
+// most that many hits.
+enum RadiusMode : int { kRadiusCount = 0, kRadiusFill = 1, kRadiusCaptureMode = 2 };
+template <typename L, int kMode, bool kStats>
 __global__ __launch_bounds__(kBlock) void radiusQuadKernel(
     const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
     const int* __restrict__ primIds, const V3* __restrict__ points, const uint32_t* __restrict__ order,
     int count, float r2, int* __restrict__ counts, const int* __restrict__ offsets,
-    uint64_t* __restrict__ keys, unsigned long long* __restrict__ stats, int* __restrict__ maxCount) {
+    uint64_t* __restrict__ keys, unsigned long long* __restrict__ stats, int* __restrict__ maxCount, int cap) {
     __shared__ int lds[kQuadDepth * (kBlock / 4)];
     const int t = xcdBlock() * kBlock + threadIdx.x;
     const int qi = t >> 2;
     const int slot = threadIdx.x & 3;
     int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
     int queryHits = 0;
-    if (qi < count) {
-        const int q = order ? static_cast<int>(order[qi]) : qi;
+    constexpr bool kFill = kMode != kRadiusCount;  // hits are written somewhere
+    bool live = qi < count;
+    int q = 0;
+    if (live) {
+        q = order ? static_cast<int>(order[qi]) : qi;
+        if (kMode == kRadiusFill && counts[q] <= cap) live = false;  // captured already
+    }
+    if (live) {
         const V3 p = points[q];
         int found = 0;
-        int write = kFill ? offsets[q] : 0;
-        const int end = kFill ? offsets[q + 1] : 0;
+        const long long segBase = kMode == kRadiusCaptureMode ? static_cast<long long>(q) * cap : 0;
+        int write = kMode == kRadiusFill ? offsets[q] : 0;
+        const int end = kMode == kRadiusFill ? offsets[q + 1] : (kMode == kRadiusCaptureMode ? cap : 0);
         LdsInt* stk = (LdsInt*)(lds + (threadIdx.x >> 2) * kQuadDepth);
         int sp = 0;
         // The next node depends on the box tests only, so its entry is requested
@@ -969,7 +981,7 @@ __global__ __launch_bounds__(kBlock) void radiusQuadKernel(
                 const int before = (slot > 0 ? h0 : 0) + (slot > 1 ? h1 : 0) + (slot > 2 ? h2 : 0);
                 const int at = write + before;
                 if (hit && at < end)
-                    keys[at] = (static_cast<uint64_t>(floatBits(d2)) << 32) | static_cast<uint32_t>(pid);
+                    keys[segBase + at] = (static_cast<uint64_t>(floatBits(d2)) << 32) | static_cast<uint32_t>(pid);
                 write += h0 + h1 + h2 + h3;
             } else {
                 found += hit;
@@ -977,13 +989,16 @@ __global__ __launch_bounds__(kBlock) void radiusQuadKernel(
             if (next < 0) break;
             e = ne;
         }
-        if (!kFill) {
+        if (kMode == kRadiusCount) {
             const int total = quadGet<0>(found) + quadGet<1>(found) + quadGet<2>(found) + quadGet<3>(found);
             if (slot == 0) counts[q] = total;
             queryHits = total;
+        } else if (kMode == kRadiusCaptureMode) {
+            if (slot == 0) counts[q] = write;
+            queryHits = write;
         }
     }
-    if (!kFill) waveMaxAtomic(maxCount, queryHits);
+    if (kMode != kRadiusFill) waveMaxAtomic(maxCount, queryHits);
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
@@ -1421,21 +1436,23 @@ void launchKnn(const TraverseArgs& a, const V3* pts, int count, int k, int outSt
     launchNearest(a, pts, count, k, outStride, out, s);
 }
 
+// Radius on four lanes per query measured faster at every query count from 20k to
+// 1M on the 2M-triangle terrain (the lanes of a quad test the four children of a
+// node and its leaves together instead of walking them one after another); it
+// serves 4-wide trees unless one lane per query is forced.
+static bool radiusOnQuads(const TraverseArgs& a) { return a.wide && gTraversalMode != 1; }
+
+// cap > 0: a fill on quads after a capture (only the queries the capture could not
+// hold are walked); cap == 0: the per-lane count / fill kernels.
 template <typename L, bool kFill>
 static void launchRadiusT(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts,
-                          const int* offsets, uint64_t* keys, int* maxCount, hipStream_t s) {
+                          const int* offsets, uint64_t* keys, int* maxCount, int cap, hipStream_t s) {
     const int grid = blocksFor(count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
     unsigned long long* st = kFill ? nullptr : a.stats;
-    // radius: four lanes per query measured faster at every query count from 20k to 1M
-    // on the 2M-triangle terrain (the lanes of a quad test the four children of a node
-    // and its leaves together instead of walking them one after another)
-    if (a.wide && gTraversalMode != 1) {
+    if (kFill && cap > 0) {
         const int qgrid = blocksFor(4LL * count, kBlock);
-        if (st)
-            radiusQuadKernel<L, kFill, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, st, maxCount);
-        else
-            radiusQuadKernel<L, kFill, false><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, nullptr, maxCount);
+        radiusQuadKernel<L, kRadiusFill, false><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, nullptr, maxCount, cap);
     } else if (a.wide) {
         if (st)
             radiusKernel<L, true, kFill, true><<<grid, kBlock, 0, s>>>(a.nodes, a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, offsets, keys, st, maxCount);
@@ -1449,26 +1466,81 @@ static void launchRadiusT(const TraverseArgs& a, const V3* pts, int count, float
     }
 }
 
-void launchRadiusCount(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts,
-                       int* maxCount, hipStream_t s) {
-    if (count <= 0) return;
-    switch (a.kind) {
-        case 0: launchRadiusT<TriLeaf, false>(a, pts, count, r2, counts, nullptr, nullptr, maxCount, s); break;
-        case 1: launchRadiusT<PointLeaf, false>(a, pts, count, r2, counts, nullptr, nullptr, maxCount, s); break;
-        default: launchRadiusT<GaussLeaf, false>(a, pts, count, r2, counts, nullptr, nullptr, maxCount, s); break;
-    }
-    hipCheck(hipGetLastError(), "radius count");
+template <typename L>
+static void launchCaptureT(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts,
+                           uint64_t* capture, int cap, int* maxCount, hipStream_t s) {
+    const int qgrid = blocksFor(4LL * count, kBlock);
+    const auto* prims = static_cast<const typename L::Item*>(a.items);
+    if (a.stats)
+        radiusQuadKernel<L, kRadiusCaptureMode, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, nullptr, capture, a.stats, maxCount, cap);
+    else
+        radiusQuadKernel<L, kRadiusCaptureMode, false><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, nullptr, capture, nullptr, maxCount, cap);
 }
 
-void launchRadiusFill(const TraverseArgs& a, const V3* pts, int count, float r2,
-                      const int* offsets, uint64_t* keys, hipStream_t s) {
-    if (count <= 0) return;
+// 64 hits per query cover the radius workloads measured here with no second walk
+// (2M-triangle terrain, r = 2: at most 43); the buffer is held to 512 MB.
+int radiusCaptureCap(int count) {
+    const long long budget = (512LL << 20) / sizeof(uint64_t);
+    long long cap = count > 0 ? budget / count : 64;
+    if (cap > 64) cap = 64;
+    return cap < 8 ? 0 : static_cast<int>(cap);
+}
+
+bool launchRadiusCapture(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts,
+                         uint64_t* capture, int cap, int* maxCount, hipStream_t s) {
+    if (count <= 0) return false;
+    if (!radiusOnQuads(a) || cap <= 0) {
+        switch (a.kind) {
+            case 0: launchRadiusT<TriLeaf, false>(a, pts, count, r2, counts, nullptr, nullptr, maxCount, 0, s); break;
+            case 1: launchRadiusT<PointLeaf, false>(a, pts, count, r2, counts, nullptr, nullptr, maxCount, 0, s); break;
+            default: launchRadiusT<GaussLeaf, false>(a, pts, count, r2, counts, nullptr, nullptr, maxCount, 0, s); break;
+        }
+        hipCheck(hipGetLastError(), "radius count");
+        return false;
+    }
     switch (a.kind) {
-        case 0: launchRadiusT<TriLeaf, true>(a, pts, count, r2, nullptr, offsets, keys, nullptr, s); break;
-        case 1: launchRadiusT<PointLeaf, true>(a, pts, count, r2, nullptr, offsets, keys, nullptr, s); break;
-        default: launchRadiusT<GaussLeaf, true>(a, pts, count, r2, nullptr, offsets, keys, nullptr, s); break;
+        case 0: launchCaptureT<TriLeaf>(a, pts, count, r2, counts, capture, cap, maxCount, s); break;
+        case 1: launchCaptureT<PointLeaf>(a, pts, count, r2, counts, capture, cap, maxCount, s); break;
+        default: launchCaptureT<GaussLeaf>(a, pts, count, r2, counts, capture, cap, maxCount, s); break;
+    }
+    hipCheck(hipGetLastError(), "radius capture");
+    return true;
+}
+
+void launchRadiusFill(const TraverseArgs& a, const V3* pts, int count, float r2, const int* counts,
+                      int cap, const int* offsets, uint64_t* keys, hipStream_t s) {
+    if (count <= 0) return;
+    int* c = const_cast<int*>(counts);  // read only by the fill pass
+    switch (a.kind) {
+        case 0: launchRadiusT<TriLeaf, true>(a, pts, count, r2, c, offsets, keys, nullptr, cap, s); break;
+        case 1: launchRadiusT<PointLeaf, true>(a, pts, count, r2, c, offsets, keys, nullptr, cap, s); break;
+        default: launchRadiusT<GaussLeaf, true>(a, pts, count, r2, c, offsets, keys, nullptr, cap, s); break;
     }
     hipCheck(hipGetLastError(), "radius fill");
+}
+
+namespace {
+// Moves every captured segment (queries with at most `cap` hits) into the CSR keys;
+// queries with more are written by the fill pass.
+__global__ void radiusCompactKernel(const int* __restrict__ counts, const int* __restrict__ offsets,
+                                    const uint64_t* __restrict__ capture, int count, int cap,
+                                    uint64_t* __restrict__ keys) {
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long long q = i / cap;
+    const int j = static_cast<int>(i % cap);
+    if (q >= count) return;
+    const int c = counts[q];
+    if (c > cap || j >= c) return;
+    keys[offsets[q] + j] = capture[i];
+}
+}  // namespace
+
+void radiusCompact(const int* counts, const int* offsets, const uint64_t* capture, int count, int cap,
+                   uint64_t* keys, hipStream_t s) {
+    if (count <= 0 || cap <= 0) return;
+    radiusCompactKernel<<<blocksFor(static_cast<long long>(count) * cap, kBlock), kBlock, 0, s>>>(
+        counts, offsets, capture, count, cap, keys);
+    hipCheck(hipGetLastError(), "radius compact");
 }
 
 void segmentSort(const int* offsets, uint64_t* keys, int count, int maxSeg, int* scratchInts,
