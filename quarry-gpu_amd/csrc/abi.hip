@@ -45,12 +45,60 @@ struct qr_points {
     DevBuf<uint64_t> rCapture;
 };
 
+// Result buffers handed to the caller: stream-ordered allocations from the device's
+// default pool, kept warm (release threshold raised once), so a radius call does not
+// stall the device at hipMalloc.  Freed on the null stream: calls are synchronous,
+// nothing still runs on them when the caller destroys the result.
+void warmDefaultPool() {
+    static thread_local int warmed = -1;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    if (warmed == dev) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    (void)hipGetLastError();
+    warmed = dev;
+}
+
+template <typename T>
+struct AsyncBuf {
+    T* p = nullptr;
+    AsyncBuf() = default;
+    AsyncBuf(const AsyncBuf&) = delete;
+    AsyncBuf& operator=(const AsyncBuf&) = delete;
+    ~AsyncBuf() { release(); }
+    void alloc(size_t count, hipStream_t s) {
+        release();
+        warmDefaultPool();
+        const hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&p), (count > 0 ? count : 1) * sizeof(T), s);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            throw OomError(std::string("hipMallocAsync failed: ") + hipGetErrorString(e));
+        }
+    }
+    void release() {
+        if (p) (void)hipFreeAsync(p, nullptr);
+        p = nullptr;
+    }
+};
+
 struct qr_csr {
-    DevBuf<int> offsets;
-    DevBuf<uint64_t> keys;
+    AsyncBuf<int> offsets;
+    AsyncBuf<uint64_t> keys;
     int count = 0;
     int total = 0;
 };
+
+// Pinned host words for the small device-to-host reads of a call (per host thread).
+int* hostWords() {
+    static thread_local int* w = nullptr;
+    if (!w) hipCheck(hipHostMalloc(reinterpret_cast<void**>(&w), 4 * sizeof(int), hipHostMallocDefault), "pinned words");
+    return w;
+}
 
 struct qr_batch {
     BatchState s;
@@ -1242,16 +1290,18 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
         try {
             const int n = p->count;
             res->count = n;
-            res->offsets.alloc(static_cast<size_t>(n) + 1);
+            hipStream_t s = currentStream();
+            res->offsets.alloc(static_cast<size_t>(n) + 1, s);
             if (n <= 0 || radius < 0.0f) {
                 // Empty CSR: every segment is empty (the reference returns no hits for r < 0).
                 std::vector<int> zeros(static_cast<size_t>(n) + 1, 0);
-                hipCheck(hipMemcpy(res->offsets.p, zeros.data(), zeros.size() * sizeof(int), hipMemcpyHostToDevice), "H2D offsets");
+                hipCheck(hipMemcpyAsync(res->offsets.p, zeros.data(), zeros.size() * sizeof(int), hipMemcpyHostToDevice, s),
+                         "H2D offsets");
+                hipCheck(hipStreamSynchronize(s), "H2D offsets");
                 *out = res;
                 if (ms) *ms = 0.0;
                 return;
             }
-            hipStream_t s = currentStream();
             Stats st;
             if (collectStats) zeroStats(st.d.p, s);
             const float r2 = radius * radius;
@@ -1276,12 +1326,13 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
             const bool captured = launchRadiusCapture(a, p->pts.p, n, r2, counts, p->rCapture.p, cap, p->rMax.p, s);
             kev.record(1, s);
             scanExclusive(counts, res->offsets.p, n, res->offsets.p + n, scratch, s);
-            int total = 0, maxSeg = 0;
-            hipCheck(hipMemcpyAsync(&total, res->offsets.p + n, sizeof(int), hipMemcpyDeviceToHost, s), "D2H total");
-            hipCheck(hipMemcpyAsync(&maxSeg, p->rMax.p, sizeof(int), hipMemcpyDeviceToHost, s), "D2H max");
+            int* hw = hostWords();
+            hipCheck(hipMemcpyAsync(hw, res->offsets.p + n, sizeof(int), hipMemcpyDeviceToHost, s), "D2H total");
+            hipCheck(hipMemcpyAsync(hw + 1, p->rMax.p, sizeof(int), hipMemcpyDeviceToHost, s), "D2H max");
             hipCheck(hipStreamSynchronize(s), "radius total sync");
+            const int total = hw[0], maxSeg = hw[1];
             res->total = total;
-            res->keys.alloc(static_cast<size_t>(total > 0 ? total : 1));
+            res->keys.alloc(static_cast<size_t>(total > 0 ? total : 1), s);
             kev.record(2, s);
             if (total > 0) {
                 a.stats = nullptr;
