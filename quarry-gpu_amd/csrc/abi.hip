@@ -186,9 +186,15 @@ struct Timer {
     }
 };
 
+// Traversal counters of a call: one buffer per host thread, allocated on first use
+// and kept for the life of the process (never freed: no HIP call runs at exit).
 struct Stats {
-    DevBuf<unsigned long long> d;
-    Stats() : d(4) {}
+    unsigned long long* p = nullptr;
+    Stats() {
+        static thread_local unsigned long long* buf = nullptr;
+        if (!buf) hipCheck(hipMalloc(reinterpret_cast<void**>(&buf), 4 * sizeof(unsigned long long)), "stats buffer");
+        p = buf;
+    }
 };
 
 void readStats(const unsigned long long* dev, qr_traversal_stats* out) {
@@ -800,11 +806,11 @@ double raycastImpl(const qr_bvh* bvh, const Ray* rays, int n, uint64_t* out, qr_
     if (n <= 0) return 0.0;
     hipStream_t s = currentStream();
     Stats st;
-    if (collectStats) zeroStats(st.d.p, s);
+    if (collectStats) zeroStats(st.p, s);
     EventSet<3> ev;
     ev.record(0, s);
     TraverseArgs a = argsFor(bvh->s, wide != 0);
-    a.stats = collectStats ? st.d.p : nullptr;
+    a.stats = collectStats ? st.p : nullptr;
     if (order != QR_ORDER_INPUT) {
         const float* base = reinterpret_cast<const float*>(rays) + (order == QR_ORDER_MORTON ? 0 : 3);
         const int mode = order == QR_ORDER_MORTON_DIRECTION ? kBoxUnit
@@ -820,7 +826,7 @@ double raycastImpl(const qr_bvh* bvh, const Ray* rays, int n, uint64_t* out, qr_
     ev.record(2, s);
     ev.sync(2);
     const double total = ev.ms(0, 2);
-    if (collectStats) readStats(st.d.p, &tRayStats);
+    if (collectStats) readStats(st.p, &tRayStats);
     tRayStats.sort_ms = order != QR_ORDER_INPUT ? ev.ms(0, 1) : 0.0;
     tRayStats.kernel_ms = ev.ms(1, 2);
     return total;
@@ -1215,11 +1221,11 @@ double pointQuery(const qr_bvh* bvh, const V3* pts, int n, int k, int outStride,
     if (n <= 0) return 0.0;
     hipStream_t s = currentStream();
     Stats st;
-    if (collectStats) zeroStats(st.d.p, s);
+    if (collectStats) zeroStats(st.p, s);
     EventSet<3> ev;
     ev.record(0, s);
     TraverseArgs a = argsFor(bvh->s, wide != 0);
-    a.stats = collectStats ? st.d.p : nullptr;
+    a.stats = collectStats ? st.p : nullptr;
     if (order == QR_ORDER_MORTON)
         a.order = reorder(*scratch, reinterpret_cast<const float*>(pts), 3, n,
                           scratch->bbValid ? kBoxGiven : kBoxMeasure, s);
@@ -1232,7 +1238,7 @@ double pointQuery(const qr_bvh* bvh, const V3* pts, int n, int k, int outStride,
     ev.record(2, s);
     ev.sync(2);
     const double total = ev.ms(0, 2);
-    if (collectStats) readStats(st.d.p, &tQueryStats);
+    if (collectStats) readStats(st.p, &tQueryStats);
     tQueryStats.sort_ms = order == QR_ORDER_MORTON ? ev.ms(0, 1) : 0.0;
     tQueryStats.kernel_ms = ev.ms(1, 2);
     return total;
@@ -1303,13 +1309,13 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
                 return;
             }
             Stats st;
-            if (collectStats) zeroStats(st.d.p, s);
+            if (collectStats) zeroStats(st.p, s);
             const float r2 = radius * radius;
             EventSet<3> ev;
             EventSet<4> kev;
             ev.record(0, s);
             TraverseArgs a = argsFor(bvh->s, wide != 0);
-            a.stats = collectStats ? st.d.p : nullptr;
+            a.stats = collectStats ? st.p : nullptr;
             if (order == QR_ORDER_MORTON)
                 a.order = reorder(*p, reinterpret_cast<const float*>(p->pts.p), 3, n,
                                   p->bbValid ? kBoxGiven : kBoxMeasure, s);
@@ -1346,7 +1352,7 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
             ev.record(2, s);
             ev.sync(2);
             const double total_ms = ev.ms(0, 2);
-            if (collectStats) readStats(st.d.p, &tQueryStats);
+            if (collectStats) readStats(st.p, &tQueryStats);
             tQueryStats.sort_ms = order == QR_ORDER_MORTON ? ev.ms(0, 1) : 0.0;
             tQueryStats.kernel_ms = kev.ms(0, 1) + kev.ms(2, 3);
             if (ms) *ms = total_ms;

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "qr_geom.h"
 
@@ -129,15 +130,35 @@ hipStream_t currentStream();
 
 // A fixed set of timing events, destroyed with the scope (so a query of the
 // elapsed time can never touch a destroyed event).
+// Per-host-thread pool of HIP events: a call's timing events are taken from it and
+// returned, so no query creates or destroys events.
+inline std::vector<hipEvent_t>& eventPool() {
+    static thread_local std::vector<hipEvent_t> pool;
+    return pool;
+}
+inline hipEvent_t takeEvent() {
+    auto& pool = eventPool();
+    if (!pool.empty()) {
+        const hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipCheck(hipEventCreate(&e), "event create");
+    return e;
+}
+inline void giveEvent(hipEvent_t e) { eventPool().push_back(e); }
+
 template <int N>
 struct EventSet {
     hipEvent_t e[N] = {};
+    // events come from a per-thread free list (created once, reused by every call)
     EventSet() {
-        for (auto& x : e) hipCheck(hipEventCreate(&x), "event create");
+        for (auto& x : e) x = takeEvent();
     }
     ~EventSet() {
         for (auto& x : e)
-            if (x) (void)hipEventDestroy(x);
+            if (x) giveEvent(x);
     }
     EventSet(const EventSet&) = delete;
     EventSet& operator=(const EventSet&) = delete;
