@@ -23,6 +23,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import statistics
 import sys
 import time
@@ -172,8 +173,9 @@ def main() -> int:
     alg = {
         "closest": touched + nq * 8,
         "knn": touched + nq * 8 * args.k,
-        # count + fill passes walk the same nodes; counts, offsets and keys written
-        "radius": 2 * touched + nq * 8 + radius_total * 8,
+        # one walk: counts and captured hits written, then copied into the CSR
+        # (offsets and keys); queries with more hits than the capture slot walk again
+        "radius": touched + nq * 8 + radius_total * 16,
     }[dom]
     achieved = alg / (per_kernel[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
@@ -531,8 +533,8 @@ def traffic_for(kernel, args):
         with open(path) as f:
             for ln in f:
                 if ln.startswith(key + " "):
-                    fields = dict(kv.split("=", 1) for kv in ln[len(key):].split())
-                    return int(float(fields["bytes"]))
+                    m = re.search(r"\bbytes=([0-9.eE+]+)", ln[len(key):])
+                    return int(float(m.group(1))) if m else None
     except OSError:
         pass
     return None
