@@ -1477,13 +1477,18 @@ static void launchCaptureT(const TraverseArgs& a, const V3* pts, int count, floa
         radiusQuadKernel<L, kRadiusCaptureMode, false><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, nullptr, capture, nullptr, maxCount, cap);
 }
 
-// 64 hits per query cover the radius workloads measured here with no second walk
-// (2M-triangle terrain, r = 2: at most 43); the buffer is held to 512 MB.
+// Hits captured per query by the single walk; a second walk serves only the queries
+// with more.  Up to 2,048 per query while the buffer stays within 256 MB (20k queries:
+// 1,638, which holds every segment of the 1M-point cloud at r = 1, at most 572), at
+// least 64 while 512 MB hold them (the 2M-triangle terrain at r = 2 has at most 43);
+// fewer than 8 turns the capture off.
 int radiusCaptureCap(int count) {
-    const long long budget = (512LL << 20) / sizeof(uint64_t);
-    long long cap = count > 0 ? budget / count : 64;
-    if (cap > 64) cap = 64;
-    return cap < 8 ? 0 : static_cast<int>(cap);
+    if (count <= 0) return 64;
+    const long long roomy = (256LL << 20) / static_cast<long long>(sizeof(uint64_t)) / count;
+    if (roomy >= 64) return roomy > 2048 ? 2048 : static_cast<int>(roomy);
+    const long long tight = (512LL << 20) / static_cast<long long>(sizeof(uint64_t)) / count;
+    if (tight >= 64) return 64;
+    return tight < 8 ? 0 : static_cast<int>(tight);
 }
 
 bool launchRadiusCapture(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts,
@@ -1522,24 +1527,26 @@ void launchRadiusFill(const TraverseArgs& a, const V3* pts, int count, float r2,
 namespace {
 // Moves every captured segment (queries with at most `cap` hits) into the CSR keys;
 // queries with more are written by the fill pass.
+// One wave per query copies its captured hits (queries with more than `cap` hits are
+// left to the fill walk).
 __global__ void radiusCompactKernel(const int* __restrict__ counts, const int* __restrict__ offsets,
                                     const uint64_t* __restrict__ capture, int count, int cap,
                                     uint64_t* __restrict__ keys) {
-    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-    const long long q = i / cap;
-    const int j = static_cast<int>(i % cap);
+    const long long q = static_cast<long long>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
     if (q >= count) return;
     const int c = counts[q];
-    if (c > cap || j >= c) return;
-    keys[offsets[q] + j] = capture[i];
+    if (c > cap) return;
+    const uint64_t* src = capture + q * cap;
+    uint64_t* dst = keys + offsets[q];
+    for (int j = threadIdx.x & 63; j < c; j += 64) dst[j] = src[j];
 }
 }  // namespace
 
 void radiusCompact(const int* counts, const int* offsets, const uint64_t* capture, int count, int cap,
                    uint64_t* keys, hipStream_t s) {
     if (count <= 0 || cap <= 0) return;
-    radiusCompactKernel<<<blocksFor(static_cast<long long>(count) * cap, kBlock), kBlock, 0, s>>>(
-        counts, offsets, capture, count, cap, keys);
+    radiusCompactKernel<<<blocksFor(count, kBlock / 64), kBlock, 0, s>>>(counts, offsets, capture, count, cap,
+                                                                        keys);
     hipCheck(hipGetLastError(), "radius compact");
 }
 
