@@ -1002,6 +1002,94 @@ __global__ __launch_bounds__(kBlock) void radiusQuadKernel(
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
+// Radius capture with one wave per query, for query sets too small to fill the
+// device four lanes per query.  The wave's 16 quads expand up to 16 nodes of the
+// query's walk per round (quad j takes the j-th entry from the top of a stack shared
+// in LDS, lane s tests entry s of it); admitted interior children are pushed in lane
+// order, hits are written at the query's cursor in lane order.  The walk order does
+// not matter: the segment sort orders the hits by (dist2, index).  Stack bound: a
+// round pops w entries and pushes at most 4w, and w is held to
+// (kWaveStack - kWaveHeadroom - sp) / 3 (at least 1), so sp stays below
+// kWaveStack - kWaveHeadroom except while popping one entry per round, which adds at
+// most 3 per 4-wide level -- within the headroom for trees up to 32 levels deep
+// (the quad kernels' stack assumes the same depth).
+constexpr int kWaveStack = 1024;
+constexpr int kWaveHeadroom = 3 * 32 + 4;
+
+template <typename L, bool kStats>
+__global__ __launch_bounds__(kBlock) void radiusWaveKernel(
+    const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
+    const int* __restrict__ primIds, const V3* __restrict__ points, const uint32_t* __restrict__ order,
+    int count, float r2, int* __restrict__ counts, uint64_t* __restrict__ capture,
+    unsigned long long* __restrict__ stats, int* __restrict__ maxCount, int cap) {
+    __shared__ int lds[kWaveStack * (kBlock / 64)];
+    const int lane = threadIdx.x & 63;
+    const int quad = lane >> 2;
+    const int slot = lane & 3;
+    const int qi = xcdBlock() * (kBlock / 64) + (threadIdx.x >> 6);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
+    int queryHits = 0;
+    if (qi < count) {  // the same in the whole wave
+        const int q = order ? static_cast<int>(order[qi]) : qi;
+        const V3 p = points[q];
+        LdsInt* stk = (LdsInt*)(lds + (threadIdx.x >> 6) * kWaveStack);
+        uint64_t* seg = capture + static_cast<long long>(q) * cap;
+        if (lane == 0) stk[0] = 0;
+        int sp = 1;
+        int found = 0;
+        while (sp > 0) {
+            int width = (kWaveStack - kWaveHeadroom - sp) / 3;
+            width = width < 1 ? 1 : (width > 16 ? 16 : width);
+            const int take = width < sp ? width : sp;
+            const int node = quad < take ? stk[sp - 1 - quad] : -1;
+            sp -= take;
+            if (kStats && slot == 0 && node >= 0) ++opened;
+            WideEntry e{};
+            int ch = kWideEmpty;
+            if (node >= 0) {
+                e = wide[node].c[slot];
+                ch = e.child;
+            }
+            const bool leaf = ch < 0 && ch != kWideEmpty;
+            typename L::Item it;
+            int pid = 0;
+            if (leaf) {
+                it = prims[-ch - 1];
+                pid = primIds[-ch - 1];
+            }
+            const bool adm = ch != kWideEmpty && boxDist2(p, entryBox(e)) <= r2;
+            if (adm) ++nodeTests;
+            const bool push = adm && !leaf;
+            const unsigned long long pm = __ballot(push);
+            const int pushes = __popcll(pm);
+            if (sp + pushes > kWaveStack) {  // deeper than the bound above allows
+                if (lane == 0) overflow = 1;
+                break;
+            }
+            if (push) stk[sp + __popcll(pm & below)] = ch;
+            sp += pushes;
+            bool hit = false;
+            float d2 = 0.0f;
+            if (adm && leaf) {
+                ++leafTests;
+                d2 = L::dist2(p, it);
+                hit = d2 <= r2;
+            }
+            const unsigned long long hm = __ballot(hit);
+            if (hit) {
+                const int at = found + __popcll(hm & below);
+                if (at < cap) seg[at] = (static_cast<uint64_t>(floatBits(d2)) << 32) | static_cast<uint32_t>(pid);
+            }
+            found += __popcll(hm);
+        }
+        if (lane == 0) counts[q] = found;
+        queryHits = found;
+    }
+    waveMaxAtomic(maxCount, queryHits);
+    flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
+}
+
 // ---- radius: count and fill passes ---------------------------------------------------
 // kFill = false counts matches per query; kFill = true writes the 64-bit keys
 // (dist2 bits << 32 | index) into the query's CSR segment (bounded by the next
@@ -1368,6 +1456,9 @@ void setTraversalMode(int mode) { gTraversalMode = mode; }
 // to 1M queries on the 2M-triangle terrain.  Closest (k = 1): four lanes per query
 // while the query set leaves most of the GPU idle one lane per query (fewer than 4
 // waves per SIMD: 256 CUs x 4 SIMDs x 4 waves x 64 lanes).
+// Radius capture one wave per query below this many queries, four lanes per query above.
+int gRadiusWaveBelow = 1 << 17;
+
 static bool useQuad(int count, int K) {
     if (gTraversalMode == 1) return false;
     if (gTraversalMode == 2) return true;
@@ -1471,7 +1562,13 @@ static void launchCaptureT(const TraverseArgs& a, const V3* pts, int count, floa
                            uint64_t* capture, int cap, int* maxCount, hipStream_t s) {
     const int qgrid = blocksFor(4LL * count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
-    if (a.stats)
+    if (count < gRadiusWaveBelow) {
+        const int wgrid = blocksFor(count, kBlock / 64);
+        if (a.stats)
+            radiusWaveKernel<L, true><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, capture, a.stats, maxCount, cap);
+        else
+            radiusWaveKernel<L, false><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, capture, nullptr, maxCount, cap);
+    } else if (a.stats)
         radiusQuadKernel<L, kRadiusCaptureMode, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, nullptr, capture, a.stats, maxCount, cap);
     else
         radiusQuadKernel<L, kRadiusCaptureMode, false><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, nullptr, capture, nullptr, maxCount, cap);
@@ -1525,8 +1622,6 @@ void launchRadiusFill(const TraverseArgs& a, const V3* pts, int count, float r2,
 }
 
 namespace {
-// Moves every captured segment (queries with at most `cap` hits) into the CSR keys;
-// queries with more are written by the fill pass.
 // One wave per query copies its captured hits (queries with more than `cap` hits are
 // left to the fill walk).
 __global__ void radiusCompactKernel(const int* __restrict__ counts, const int* __restrict__ offsets,
