@@ -1456,7 +1456,8 @@ void setTraversalMode(int mode) { gTraversalMode = mode; }
 // to 1M queries on the 2M-triangle terrain.  Closest (k = 1): four lanes per query
 // while the query set leaves most of the GPU idle one lane per query (fewer than 4
 // waves per SIMD: 256 CUs x 4 SIMDs x 4 waves x 64 lanes).
-// Radius capture one wave per query below this many queries, four lanes per query above.
+// Radius capture one wave per query below this many queries, four lanes per query
+// above (traversal mode 2 forces four lanes per query).
 int gRadiusWaveBelow = 1 << 17;
 
 static bool useQuad(int count, int K) {
@@ -1562,7 +1563,7 @@ static void launchCaptureT(const TraverseArgs& a, const V3* pts, int count, floa
                            uint64_t* capture, int cap, int* maxCount, hipStream_t s) {
     const int qgrid = blocksFor(4LL * count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
-    if (count < gRadiusWaveBelow) {
+    if (count < gRadiusWaveBelow && gTraversalMode == 0) {
         const int wgrid = blocksFor(count, kBlock / 64);
         if (a.stats)
             radiusWaveKernel<L, true><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, capture, a.stats, maxCount, cap);

@@ -341,9 +341,33 @@ def test_radius_modes_agree(A, orc, scenes, kind):
     b.build_wide()
     want = orc.radius_brute(kind, items, q, 2.5, threads=THREADS)
     try:
-        for mode in (1, 2):
+        for mode in (1, 2, 0):
             A.check(A.lib().qr_set_traversal_mode(mode))
             o, k, _, _ = b.radius(q, 2.5, order=A.ORDER_MORTON, wide=True)
             assert eq(o, want[0]) and eq(k, want[1]), mode
     finally:
         A.check(A.lib().qr_set_traversal_mode(0))
+
+
+def test_radius_wave_walk_wide_frontier(A, orc):
+    """The one-wave-per-query capture on walks whose frontier is the whole tree: a
+    radius holding every point (2,048-hit capture overflows, the fill walk takes
+    over) and one holding about 3,000; equal to the four-lane walk and the oracle."""
+    rng = np.random.default_rng(51)
+    pts = rng.uniform(0, 1, (50000, 3)).astype(np.float32)
+    pts[100:400] = pts[7]  # duplicates: equal keys, a deeper tree
+    b = A.Bvh(1, pts)
+    b.build_wide()
+    for r, nq in ((2.0, 40), (0.25, 1500)):
+        q = rng.uniform(-0.2, 1.2, (nq, 3)).astype(np.float32)
+        want = orc.radius_brute(1, pts, q, r, threads=THREADS)
+        seg = np.diff(want[0])
+        assert seg.max() > 2048 and (r < 1 or seg.min() == 50000), r
+        try:
+            for mode in (0, 2):
+                A.check(A.lib().qr_set_traversal_mode(mode))
+                o, k, _, st = b.radius(q, r, order=A.ORDER_MORTON, wide=True, stats=True)
+                assert eq(o, want[0]) and eq(k, want[1]), (r, mode)
+                assert st.stack_overflows == 0
+        finally:
+            A.check(A.lib().qr_set_traversal_mode(0))
