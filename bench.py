@@ -4,13 +4,15 @@
 Workload (the configuration the headline metric is quoted on): a 1024 x 1024-cell
 z-up terrain (2,097,152 FP32 triangles) with 100,000 query points uniform in the
 scene box grown by 4 on every side.  One step is one serving pass over the
-device-resident index: closest point, k-nearest (k = 16) and radius (r = 2.0)
-for all 100,000 queries, Morton-ordered, on 4-wide nodes.  value = queries
-answered per second over the K timed steps (300,000 per step), in millions.
+device-resident index (built with binned SAH, the reference's builder="sah";
+--builder lbvh serves from the LBVH tree): closest point, k-nearest (k = 16) and
+radius (r = 2.0) for all 100,000 queries, Morton-ordered, on 4-wide nodes.  value =
+queries answered per second over the K timed steps (300,000 per step), in millions.
+The same pass on the other tree is reported beside it (same result bits).
 
 Around the timed region the same run also measures and checks:
   * the LBVH build of the 2M-triangle scene (median of several builds) with its
-    HBM roofline, and the 4-wide collapse;
+    HBM roofline, the 4-wide collapse, and the SAH build;
   * a dirty refit after moving 1% of the grid vertices by N(0, 0.05);
   * closest-hit rays: 1,000,000 down rays on the 256 x 256-cell terrain;
   * bit-exact parity of every result set against the CPU oracle;
@@ -50,6 +52,8 @@ def parse():
     ap.add_argument("--queries", type=int, default=100_000)
     ap.add_argument("--k", type=int, default=16)
     ap.add_argument("--radius", type=float, default=2.0)
+    ap.add_argument("--builder", choices=["sah", "lbvh"], default="sah",
+                    help="builder of the index the timed serving pass runs on")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity legs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--profile", action="store_true", help="short run for rocprof: no CPU legs, no extras")
@@ -102,38 +106,66 @@ def main() -> int:
         f"tree {bt.tree_ms:.3f}, leaves+boxes {bt.refit_ms:.3f}); depth {info.depth}; "
         f"wide collapse {wide_ms:.3f} ms ({info.wide_count} nodes)")
 
+    # The index the serving pass runs on: binned SAH by default (the reference's
+    # builder="sah"; the results are the same bits on either tree, and on this terrain
+    # the SAH tree opens about a third of the LBVH tree's nodes per query).  Both
+    # trees stay resident; the LBVH tree is the one the build roofline and the refit
+    # are measured on, and its serving pass is reported beside the headline.
+    lbvh = bvh
+    sah = A.Bvh.from_device(A.PRIM_TRI, dtris.ptr.value, n, A.BUILDER_SAH)
+    sah.close()  # the first SAH build also sizes its workspaces; time the second
+    sah = A.Bvh.from_device(A.PRIM_TRI, dtris.ptr.value, n, A.BUILDER_SAH)
+    sah_build_ms = sah.timings.total_ms
+    sah_wide_ms = sah.build_wide()
+    log(f"SAH build: {sah_build_ms:.3f} ms (bin {sah.timings.sah_bin_ms:.3f}, split {sah.timings.sah_split_ms:.3f}, "
+        f"partition {sah.timings.sah_part_ms:.3f}); depth {sah.info().depth}; wide collapse {sah_wide_ms:.3f} ms")
+    serve = sah if args.builder == "sah" else lbvh
+
     pb1 = A.PointBuffer(queries, 1)
     pbk = A.PointBuffer(queries, args.k)
 
-    def step():
-        a = pb1.closest(bvh)
+    def step(tree):
+        a = pb1.closest(tree)
         sa = A.last_query_stats().kernel_ms
-        b = pbk.knn(bvh, args.k)
+        b = pbk.knn(tree, args.k)
         sb = A.last_query_stats().kernel_ms
-        c = pb1.radius(bvh, args.radius)
+        c = pb1.radius(tree, args.radius)
         sc = A.last_query_stats().kernel_ms
         return (a, b, c), (sa, sb, sc)
 
-    for _ in range(args.warmup):
-        step()
-    A.check(A.lib().qr_dev_synchronize())
-    call_ms = []
-    kern_ms = []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        c, k = step()
-        call_ms.append(c)
-        kern_ms.append(k)
-    A.check(A.lib().qr_dev_synchronize())
-    wall = time.perf_counter() - t_start
-    ms_per_step = 1000.0 * wall / args.steps
+    def timed(tree, steps, warmup):
+        for _ in range(warmup):
+            step(tree)
+        A.check(A.lib().qr_dev_synchronize())
+        call_ms, kern_ms = [], []
+        t_start = time.perf_counter()
+        for _ in range(steps):
+            c, k = step(tree)
+            call_ms.append(c)
+            kern_ms.append(k)
+        A.check(A.lib().qr_dev_synchronize())
+        wall = time.perf_counter() - t_start
+        names = ("closest", "knn", "radius")
+        return (wall, {name: statistics.median(c[i] for c in call_ms) for i, name in enumerate(names)},
+                {name: statistics.mean(k[i] for k in kern_ms) for i, name in enumerate(names)})
+
     q_per_step = 3 * queries.shape[0]
+    wall, per_op, per_kernel = timed(serve, args.steps, args.warmup)
+    ms_per_step = 1000.0 * wall / args.steps
     value = q_per_step / (wall / args.steps) / 1e6
-    per_op = {name: statistics.median(c[i] for c in call_ms) for i, name in
-              enumerate(("closest", "knn", "radius"))}
-    per_kernel = {name: statistics.mean(k[i] for k in kern_ms) for i, name in
-                  enumerate(("closest", "knn", "radius"))}
-    log(f"step: {ms_per_step:.3f} ms wall; per call (event ms) {per_op}; traversal kernels {per_kernel}")
+    log(f"step ({args.builder} index): {ms_per_step:.3f} ms wall; per call (event ms) {per_op}; "
+        f"traversal kernels {per_kernel}")
+    other = lbvh if serve is sah else sah
+    other_index = None
+    if not args.profile:  # (a profile covers the serving pass only)
+        o_steps = max(3, args.steps // 2)
+        o_wall, o_per_op, o_per_kernel = timed(other, o_steps, 2)
+        other_index = {"builder": "lbvh" if other is lbvh else "sah",
+                       "mq_per_s": round(q_per_step / (o_wall / o_steps) / 1e6, 3),
+                       "ms_per_step": round(1000.0 * o_wall / o_steps, 4),
+                       "per_op_kernel_ms": {k: round(v, 4) for k, v in o_per_kernel.items()}}
+        log(f"other index: {other_index}")
+    bvh = serve
 
     # traversal work per query (one extra counted pass, outside the timed region)
     work = {}
@@ -189,7 +221,7 @@ def main() -> int:
     build = {"ms": round(build_ms, 4), "morton_ms": round(bt.morton_ms, 4),
              "sort_ms": round(bt.sort_ms, 4), "tree_ms": round(bt.tree_ms, 4),
              "leaves_boxes_ms": round(bt.refit_ms, 4), "wide_ms": round(wide_ms, 4),
-             "depth": info.depth,
+             "depth": info.depth, "sah_ms": round(sah_build_ms, 4), "sah_wide_ms": round(sah_wide_ms, 4),
              "roofline": {"bound": "hbm", "achieved": round(build_gbs, 2), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
                           "alg_bytes": build_alg}}
@@ -213,6 +245,13 @@ def main() -> int:
         exp_c = cb.knn(queries, 0, threads)
         exp_k = cb.knn(queries, args.k, threads)
         exp_o, exp_keys = cb.radius(queries, args.radius, threads)
+        pb1.closest(other)
+        oth_c = pb1.read()
+        pbk.knn(other, args.k)
+        oth_k = pbk.read()
+        _, oth_o, oth_keys = pb1.radius(other, args.radius, keep=True)
+        parity["other_index_same_bits"] = bool(np.array_equal(oth_c, res_c) and np.array_equal(oth_k, res_k)
+                                               and np.array_equal(oth_o, offs) and np.array_equal(oth_keys, keys))
         parity["closest"] = bool(np.array_equal(res_c, exp_c))
         parity["knn"] = bool(np.array_equal(res_k, exp_k))
         parity["radius"] = bool(np.array_equal(offs, exp_o) and np.array_equal(keys, exp_keys))
@@ -224,9 +263,9 @@ def main() -> int:
 
         # ---- dirty refit after moving 1% of the vertices ---------------------------
         moved, dirty = S.displace_vertices(args.cells, tris, 0.01, 0.05, seed=4)
-        rt = bvh.refit(moved, dirty)
-        why = bvh.validate(moved)
-        pb1.closest(bvh)
+        rt = lbvh.refit(moved, dirty)
+        why = lbvh.validate(moved)
+        pb1.closest(lbvh)
         res_rc = pb1.read()
         cb2 = O.CpuBvh(0, moved)
         parity["refit_closest"] = bool(np.array_equal(res_rc, cb2.knn(queries, 0, threads)))
@@ -236,7 +275,7 @@ def main() -> int:
                           "mark_ms": round(rt.mark_ms, 4), "merge_ms": round(rt.merge_ms, 4),
                           "wide_ms": round(rt.wide_ms, 4), "total_ms": round(rt.total_ms, 4)}
         log(f"refit: {extra['refit']} validate={why!r}")
-        bvh.refit(tris)
+        lbvh.refit(tris)
         log("refit back to the original scene done")
 
         # ---- rays: 1M down rays on the 256 x 256 terrain ----------------------------
@@ -290,9 +329,11 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (seeded z-up sinusoid terrain, uniform queries)",
         "config": {"workload": "terrain 1024x1024 cells (2,097,152 tris), 100k queries, closest + "
-                               f"kNN{args.k} + radius {args.radius}, Morton order, 4-wide nodes",
+                               f"kNN{args.k} + radius {args.radius}, Morton order, 4-wide nodes, "
+                               f"{args.builder.upper()}-built index",
                    "triangles": n, "queries": nq, "k": args.k, "radius": args.radius,
-                   "radius_results": radius_total},
+                   "builder": args.builder, "radius_results": radius_total},
+        "other_index": other_index,
         "per_op_ms": {k: round(v, 4) for k, v in per_op.items()},
         "per_op_mqps": {k: round(nq / (v * 1e-3) / 1e6, 2) for k, v in per_op.items()},
         "per_op_kernel_ms": {k: round(v, 4) for k, v in per_kernel.items()},
@@ -317,7 +358,8 @@ def _median_ms(fn, reps=5):
 
 def leg_c3(A, S, O, threads):
     """Non-uniform scenes: city rays (LBVH and 32-bin SAH), 1M-point kNN8 / radius 1,
-    1M-Gaussian closest / kNN8; 20,000 uniform queries per scene."""
+    1M-Gaussian closest / kNN8 (each cloud on an LBVH and on a SAH tree); 20,000
+    uniform queries per scene."""
     out, ok = {}, True
     city = S.city(131072, 1)
     rays = S.cosine_down_rays(city, 1_000_000, 2)
@@ -339,31 +381,34 @@ def leg_c3(A, S, O, threads):
         b.close()
     out["city_triangles"] = int(city.shape[0])
     rng = np.random.default_rng(12)
-    for kind, items, label in ((1, S.point_cloud(1 << 20, 1), "points"), (2, S.gaussians(1 << 20, 1), "gaussians")):
-        q = rng.uniform(0.0, 64.0, (20_000, 3)).astype(np.float32)
-        b = A.Bvh(kind, items)
-        b.build_wide()
+    clouds = ((1, S.point_cloud(1 << 20, 1), "points"), (2, S.gaussians(1 << 20, 1), "gaussians"))
+    q = rng.uniform(0.0, 64.0, (20_000, 3)).astype(np.float32)
+    for kind, items, name in clouds:
         ref = O.CpuBvh(kind, items)
-        pb1, pb8 = A.PointBuffer(q, 1), A.PointBuffer(q, 8)
-        res = {"build_ms": round(b.timings.total_ms, 3)}
-        ms_c = _median_ms(lambda: pb1.closest(b))
-        same_c = bool(np.array_equal(pb1.read(), ref.knn(q, 0, threads)))
-        ms_k = _median_ms(lambda: pb8.knn(b, 8))
-        same_k = bool(np.array_equal(pb8.read(), ref.knn(q, 8, threads)))
-        res.update({"closest_mq_per_s": round(q.shape[0] / ms_c / 1e3, 2),
-                    "knn8_mq_per_s": round(q.shape[0] / ms_k / 1e3, 2), "parity_closest": same_c,
-                    "parity_knn8": same_k})
-        ok &= same_c and same_k
-        if kind == 1:
-            ms_r = _median_ms(lambda: pb1.radius(b, 1.0))
-            _, offs, keys = pb1.radius(b, 1.0, keep=True)
-            wo, wk = ref.radius(q, 1.0, threads)
-            same_r = bool(np.array_equal(offs, wo) and np.array_equal(keys, wk))
-            res.update({"radius1_mq_per_s": round(q.shape[0] / ms_r / 1e3, 2), "parity_radius": same_r})
-            ok &= same_r
-        out[label] = res
-        b.close()
+        want_c, want_k = ref.knn(q, 0, threads), ref.knn(q, 8, threads)
+        want_r = ref.radius(q, 1.0, threads) if kind == 1 else None
         del ref
+        for builder, label in ((A.BUILDER_LBVH, name), (A.BUILDER_SAH, name + "_sah")):
+            b = A.Bvh(kind, items, builder)
+            b.build_wide()
+            pb1, pb8 = A.PointBuffer(q, 1), A.PointBuffer(q, 8)
+            res = {"build_ms": round(b.timings.total_ms, 3)}
+            ms_c = _median_ms(lambda: pb1.closest(b))
+            same_c = bool(np.array_equal(pb1.read(), want_c))
+            ms_k = _median_ms(lambda: pb8.knn(b, 8))
+            same_k = bool(np.array_equal(pb8.read(), want_k))
+            res.update({"closest_mq_per_s": round(q.shape[0] / ms_c / 1e3, 2),
+                        "knn8_mq_per_s": round(q.shape[0] / ms_k / 1e3, 2), "parity_closest": same_c,
+                        "parity_knn8": same_k})
+            ok &= same_c and same_k
+            if want_r is not None:
+                ms_r = _median_ms(lambda: pb1.radius(b, 1.0))
+                _, offs, keys = pb1.radius(b, 1.0, keep=True)
+                same_r = bool(np.array_equal(offs, want_r[0]) and np.array_equal(keys, want_r[1]))
+                res.update({"radius1_mq_per_s": round(q.shape[0] / ms_r / 1e3, 2), "parity_radius": same_r})
+                ok &= same_r
+            out[label] = res
+            b.close()
     return out, bool(ok)
 
 
@@ -528,7 +573,8 @@ def traffic_for(kernel, args):
     same counters' reading for a copy of known size; the figures are kept in
     profiles/hbm_traffic.txt.  None when no measurement for this configuration exists."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
-    key = f"c2 cells={args.cells} queries={args.queries} k={args.k} radius={args.radius} kernel={kernel}"
+    key = (f"c2 cells={args.cells} queries={args.queries} k={args.k} radius={args.radius} "
+           f"builder={args.builder} kernel={kernel}")
     try:
         with open(path) as f:
             for ln in f:
