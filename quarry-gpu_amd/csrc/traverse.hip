@@ -1351,7 +1351,7 @@ __global__ void pointBoundsKernel(const float* __restrict__ base, int stride, in
 
 // fixedBox: use [-1,1]^3 (ray directions) instead of the measured bounds.
 __global__ void mortonKeysKernel(const float* __restrict__ base, int stride, int count,
-                                 const uint32_t* __restrict__ bb, int fixedBox,
+                                 const uint32_t* __restrict__ bb, int fixedBox, int drop,
                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[4][256];
     for (int i = threadIdx.x; i < 4 * 256; i += kBlock) (&h[0][0])[i] = 0u;
@@ -1363,9 +1363,9 @@ __global__ void mortonKeysKernel(const float* __restrict__ base, int stride, int
     __syncthreads();
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < count; i += kBlock * gridDim.x) {
         const float* p = base + static_cast<size_t>(i) * stride;
-        const uint32_t code = spread10b(quant10(p[0], lo[0], hi[0])) |
+        const uint32_t code = (spread10b(quant10(p[0], lo[0], hi[0])) |
                               (spread10b(quant10(p[1], lo[1], hi[1])) << 1) |
-                              (spread10b(quant10(p[2], lo[2], hi[2])) << 2);
+                              (spread10b(quant10(p[2], lo[2], hi[2])) << 2)) >> drop;
         keys[i] = code;
         atomicAdd(&h[0][code & 255u], 1u);
         atomicAdd(&h[1][(code >> 8) & 255u], 1u);
@@ -1397,8 +1397,20 @@ void zeroStats(unsigned long long* stats, hipStream_t s) {
 void mortonOrder(const float* base, int stride, int count, int boxMode, SortWorkspace& ws,
                  uint32_t* keys, uint32_t* keysSorted, uint32_t* orderOut, uint32_t* bb,
                  hipStream_t s) {
+    // The order only steers locality (results do not depend on it), so the code is
+    // cut to the whole 8-bit digits that give at least one cell per element: the top
+    // 16 bits below 65,536 elements, 24 below 16M, all 30 above -- one or two sort
+    // passes fewer than the full code.
+    int passes = 4, drop = 0;
+    if (count <= (1 << 16)) {
+        passes = 2;
+        drop = 14;
+    } else if (count <= (1 << 24)) {
+        passes = 3;
+        drop = 6;
+    }
     ws.reserve(static_cast<size_t>(count));
-    sortClear(ws, static_cast<size_t>(count), 4, s);
+    sortClear(ws, static_cast<size_t>(count), passes, s);
     int grid = blocksFor(count, kBlock * 4);
     if (grid < 1) grid = 1;
     if (grid > 1024) grid = 1024;
@@ -1406,10 +1418,10 @@ void mortonOrder(const float* base, int stride, int count, int boxMode, SortWork
         pointBoundsInitKernel<<<1, 64, 0, s>>>(bb);
         pointBoundsKernel<<<grid, kBlock, 0, s>>>(base, stride, count, bb);
     }
-    mortonKeysKernel<<<grid, kBlock, 0, s>>>(base, stride, count, bb, boxMode == kBoxUnit ? 1 : 0,
+    mortonKeysKernel<<<grid, kBlock, 0, s>>>(base, stride, count, bb, boxMode == kBoxUnit ? 1 : 0, drop,
                                              keys, ws.hist.p);
     hipCheck(hipGetLastError(), "morton keys");
-    sortPairs(ws, keys, nullptr, keysSorted, orderOut, static_cast<size_t>(count), 4, s);
+    sortPairs(ws, keys, nullptr, keysSorted, orderOut, static_cast<size_t>(count), passes, s);
 }
 
 template <bool kSave>
