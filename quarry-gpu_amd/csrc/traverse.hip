@@ -1090,6 +1090,169 @@ __global__ __launch_bounds__(kBlock) void radiusWaveKernel(
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
+// Closest / kNN with one wave per query (the radius capture's layout: 16 quads expand
+// up to 16 nodes of one walk per round from a shared LDS stack, lane s of a quad
+// tests entry s).  The k-best list is spread one entry per lane (lanes 0..K-1,
+// sorted by (dist2, index), (inf, INT_MAX) when empty); a round's leaf candidates
+// that beat the list's K-th are sorted across the wave (bitonic network) and merged
+// in (the elementwise smaller of the list and the reversed candidates holds the K
+// best, a bitonic merge sorts it).  Interior children are pushed in order of their
+// box distance, nearest on top, so the next round expands the nearest 16.  The
+// admission rule is the lane kernels' (n < K or bound <= K-th), so the list ends
+// as the K best by (dist2, index) whatever the walk order.
+template <typename V>
+__device__ __forceinline__ V waveXor(V v, int m) {
+    return __shfl_xor(v, m, 64);
+}
+
+// Bitonic sort of one (key, value) pair per lane, ascending by (key, value).
+__device__ __forceinline__ void waveSortPairs(float& k, int& v, int lane) {
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const float ok = waveXor(k, stride);
+            const int ov = waveXor(v, stride);
+            const bool up = (lane & size) == 0 || size == 64;
+            const bool low = (lane & stride) == 0;
+            const bool otherFirst = closer(ok, ov, k, v);
+            // the lower lane of an ascending pair keeps the smaller, the upper the larger
+            const bool take = (low == up) ? otherFirst : !otherFirst;
+            if (take) {
+                k = ok;
+                v = ov;
+            }
+        }
+    }
+}
+
+// Bitonic merge of a bitonic sequence (one pair per lane) into ascending order.
+__device__ __forceinline__ void waveMergePairs(float& k, int& v, int lane) {
+#pragma unroll
+    for (int stride = 32; stride > 0; stride >>= 1) {
+        const float ok = waveXor(k, stride);
+        const int ov = waveXor(v, stride);
+        const bool low = (lane & stride) == 0;
+        const bool otherFirst = closer(ok, ov, k, v);
+        if (low == otherFirst) {
+            k = ok;
+            v = ov;
+        }
+    }
+}
+
+template <typename L, int K, bool kStats>
+__global__ __launch_bounds__(kBlock) void nearestWaveKernel(
+    const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
+    const int* __restrict__ primIds, const V3* __restrict__ points, const uint32_t* __restrict__ order,
+    int count, int k, int outStride, ClosestResult* __restrict__ out, unsigned long long* __restrict__ stats,
+    int idBits) {
+    __shared__ int lds[kWaveStack * (kBlock / 64)];
+    const int lane = threadIdx.x & 63;
+    const int quad = lane >> 2;
+    const int slot = lane & 3;
+    const int qi = xcdBlock() * (kBlock / 64) + (threadIdx.x >> 6);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
+    if (qi < count) {  // the same in the whole wave
+        const int q = order ? static_cast<int>(order[qi]) : qi;
+        const V3 p = points[q];
+        LdsInt* stk = (LdsInt*)(lds + (threadIdx.x >> 6) * kWaveStack);
+        float ld = kInf;  // this lane's list entry (lanes 0..K-1)
+        int lid = 0x7fffffff;
+        float kd = kInf;  // the list's K-th (the same in every lane)
+        int kid = 0x7fffffff;
+        int n = 0;
+        auto admits = [&](float lb) { return n < K || lb <= kd; };
+        if (lane == 0) stk[0] = packEntry(0, 0.0f, idBits);
+        int sp = 1;
+        while (sp > 0) {
+            int width = (kWaveStack - kWaveHeadroom - sp) / 3;
+            width = width < 1 ? 1 : (width > 16 ? 16 : width);
+            const int take = width < sp ? width : sp;
+            int node = -1;
+            if (quad < take) {
+                const int v = stk[sp - 1 - quad];
+                if (admits(entryKey(v, idBits))) node = entryId(v, idBits);
+            }
+            sp -= take;
+            if (kStats && slot == 0 && node >= 0) ++opened;
+            WideEntry e{};
+            int ch = kWideEmpty;
+            if (node >= 0) {
+                e = wide[node].c[slot];
+                ch = e.child;
+            }
+            const bool leaf = ch < 0 && ch != kWideEmpty;
+            typename L::Item it;
+            int pid = 0;
+            if (leaf) {
+                it = prims[-ch - 1];
+                pid = primIds[-ch - 1];
+            }
+            float d = kInf;
+            bool adm = false;
+            if (ch != kWideEmpty) {
+                d = boxDist2(p, entryBox(e));
+                adm = admits(d);
+            }
+            if (adm) ++nodeTests;
+            float cd = kInf;
+            int cid = 0x7fffffff;
+            bool better = false;
+            if (adm && leaf) {
+                ++leafTests;
+                cd = L::dist2(p, it);
+                cid = pid;
+                better = n < K || closer(cd, cid, kd, kid);
+            }
+            const unsigned long long bm = __ballot(better);
+            if (bm != 0) {
+                float ck = better ? cd : kInf;
+                int cv = better ? cid : 0x7fffffff;
+                waveSortPairs(ck, cv, lane);
+                // the list (lanes >= K hold (inf, INT_MAX)) against the reversed candidates
+                const float rk = __shfl(ck, 63 - lane, 64);
+                const int rv = __shfl(cv, 63 - lane, 64);
+                float mk = ld;
+                int mv = lid;
+                if (closer(rk, rv, mk, mv)) {
+                    mk = rk;
+                    mv = rv;
+                }
+                waveMergePairs(mk, mv, lane);
+                ld = lane < K ? mk : kInf;
+                lid = lane < K ? mv : 0x7fffffff;
+                n = n + __popcll(bm);
+                n = n < K ? n : K;
+                kd = __shfl(ld, K - 1, 64);
+                kid = __shfl(lid, K - 1, 64);
+            }
+            // interior children that survive the updated bound, nearest on top
+            const bool push = adm && !leaf && admits(d);
+            const unsigned long long pm = __ballot(push);
+            const int pushes = __popcll(pm);
+            if (pushes > 0) {
+                if (sp + pushes > kWaveStack) {  // deeper than the stack bound allows
+                    if (lane == 0) overflow = 1;
+                    break;
+                }
+                float pk = push ? d : kInf;
+                int pv = push ? ch : 0x7fffffff;
+                waveSortPairs(pk, pv, lane);
+                if (lane < pushes) stk[sp + pushes - 1 - lane] = packEntry(pv, pk, idBits);
+                sp += pushes;
+            }
+        }
+        if (lane < k) {
+            ClosestResult r{0u, -1};
+            if (lane < n) r = {floatBits(ld), lid};
+            out[static_cast<size_t>(q) * outStride + lane] = r;
+        }
+    }
+    flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
+}
+
 // ---- radius: count and fill passes ---------------------------------------------------
 // kFill = false counts matches per query; kFill = true writes the 64-bit keys
 // (dist2 bits << 32 | index) into the query's CSR segment (bounded by the next
@@ -1472,9 +1635,25 @@ void setTraversalMode(int mode) { gTraversalMode = mode; }
 // above (traversal mode 2 forces four lanes per query).
 int gRadiusWaveBelow = 1 << 17;
 
+// Closest / kNN one wave per query (mode 0: below a leaf- and k-dependent count;
+// mode 3 forces it).  A small query set fills the device only with more lanes per
+// query, while past that the quad kernel needs fewer instructions per query.
+// Measured crossovers (kernel time, same bits): the 2M-triangle terrain on a SAH
+// tree -- closest about 55k queries, kNN16 about 85k (at 20k: 0.20 vs 0.43 ms and
+// 0.22 vs 0.58 ms); 1M Gaussians at 20k: closest 0.22 vs 0.31, kNN8 0.24 vs 0.66;
+// 1M points -- kNN8 about 25k, closest never (its leaf test is too cheap to pay for
+// the wave's sorting networks: 0.15 vs 0.11 ms at 20k).
+template <typename L, int K>
+static bool useWave(int count) {
+    if (gTraversalMode == 3) return true;
+    if (gTraversalMode != 0) return false;
+    const int below = L::kKind == 1 ? (K > 1 ? 24576 : 0) : 49152;
+    return count < below;
+}
+
 static bool useQuad(int count, int K) {
     if (gTraversalMode == 1) return false;
-    if (gTraversalMode == 2) return true;
+    if (gTraversalMode == 2 || gTraversalMode == 3) return true;
     return K > 1 || count < 256 * 4 * 4 * 64;
 }
 
@@ -1483,7 +1662,14 @@ static void launchNearestTK(const TraverseArgs& a, const V3* pts, int count, int
                             ClosestResult* out, hipStream_t s) {
     const int grid = blocksFor(count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
-    if (a.wide && useQuad(count, K)) {
+    if (a.wide && useWave<L, K>(count)) {
+        const int wgrid = blocksFor(count, kBlock / 64);
+        const int idBits = stackIdBitsDev(a.wideCount);
+        if (a.stats)
+            nearestWaveKernel<L, K, true><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats, idBits);
+        else
+            nearestWaveKernel<L, K, false><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, idBits);
+    } else if (a.wide && useQuad(count, K)) {
         const int qgrid = blocksFor(4LL * count, kBlock);
         const int idBits = stackIdBitsDev(a.wideCount);
         if (a.stats)
@@ -1575,7 +1761,7 @@ static void launchCaptureT(const TraverseArgs& a, const V3* pts, int count, floa
                            uint64_t* capture, int cap, int* maxCount, hipStream_t s) {
     const int qgrid = blocksFor(4LL * count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
-    if (count < gRadiusWaveBelow && gTraversalMode == 0) {
+    if ((count < gRadiusWaveBelow && gTraversalMode == 0) || gTraversalMode == 3) {
         const int wgrid = blocksFor(count, kBlock / 64);
         if (a.stats)
             radiusWaveKernel<L, true><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, capture, a.stats, maxCount, cap);

@@ -321,7 +321,7 @@ def test_traversal_modes_agree(A, orc, scenes, kind):
     b.build_wide()
     want = {k: orc.knn_brute(kind, items, q, k, threads=THREADS) for k in (0, 1, 5, 16, 32)}
     try:
-        for mode in (1, 2, 0):
+        for mode in (1, 2, 3, 0):
             A.check(A.lib().qr_set_traversal_mode(mode))
             for k, w in want.items():
                 got = b.closest(q, order=A.ORDER_MORTON, wide=True)[0] if k == 0 else \
@@ -341,7 +341,7 @@ def test_radius_modes_agree(A, orc, scenes, kind):
     b.build_wide()
     want = orc.radius_brute(kind, items, q, 2.5, threads=THREADS)
     try:
-        for mode in (1, 2, 0):
+        for mode in (1, 2, 3, 0):
             A.check(A.lib().qr_set_traversal_mode(mode))
             o, k, _, _ = b.radius(q, 2.5, order=A.ORDER_MORTON, wide=True)
             assert eq(o, want[0]) and eq(k, want[1]), mode
