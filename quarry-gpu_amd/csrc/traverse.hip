@@ -1096,10 +1096,11 @@ __global__ __launch_bounds__(kBlock) void radiusWaveKernel(
 // sorted by (dist2, index), (inf, INT_MAX) when empty); a round's leaf candidates
 // that beat the list's K-th are sorted across the wave (bitonic network) and merged
 // in (the elementwise smaller of the list and the reversed candidates holds the K
-// best, a bitonic merge sorts it).  Interior children are pushed in order of their
-// box distance, nearest on top, so the next round expands the nearest 16.  The
-// admission rule is the lane kernels' (n < K or bound <= K-th), so the list ends
-// as the K best by (dist2, index) whatever the walk order.
+// best, a bitonic merge sorts it).  Interior children are pushed nearest-parent
+// first and, within a parent, nearest on top, so the next round expands about the
+// nearest 16 without a sort (for Gaussians: sorted by box distance across the wave).  The admission rule is the lane kernels' (n < K or
+// bound <= K-th), so the list ends as the K best by (dist2, index) whatever the
+// walk order.
 template <typename V>
 __device__ __forceinline__ V waveXor(V v, int m) {
     return __shfl_xor(v, m, 64);
@@ -1147,6 +1148,9 @@ __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
     const int* __restrict__ primIds, const V3* __restrict__ points, const uint32_t* __restrict__ order,
     int count, int k, int outStride, ClosestResult* __restrict__ out, unsigned long long* __restrict__ stats,
     int idBits) {
+    // Gaussian leaves cost the most to test, so their walks pay for a sorted push
+    // order (fewer nodes opened); triangles and points push quad by quad.
+    constexpr bool kSortPushes = L::kKind == 2;
     __shared__ int lds[kWaveStack * (kBlock / 64)];
     const int lane = threadIdx.x & 63;
     const int quad = lane >> 2;
@@ -1228,7 +1232,9 @@ __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
                 kd = __shfl(ld, K - 1, 64);
                 kid = __shfl(lid, K - 1, 64);
             }
-            // interior children that survive the updated bound, nearest on top
+            // interior children that survive the updated bound: quad j popped the j-th
+            // nearest entry, so its children go above those of quads j+1.., each
+            // quad's nearest child on top of its group
             const bool push = adm && !leaf && admits(d);
             const unsigned long long pm = __ballot(push);
             const int pushes = __popcll(pm);
@@ -1237,10 +1243,23 @@ __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
                     if (lane == 0) overflow = 1;
                     break;
                 }
-                float pk = push ? d : kInf;
-                int pv = push ? ch : 0x7fffffff;
-                waveSortPairs(pk, pv, lane);
-                if (lane < pushes) stk[sp + pushes - 1 - lane] = packEntry(pv, pk, idBits);
+                if (kSortPushes) {  // all of the round's children in box-distance order
+                    float pk = push ? d : kInf;
+                    int pv = push ? ch : 0x7fffffff;
+                    waveSortPairs(pk, pv, lane);
+                    if (lane < pushes) stk[sp + pushes - 1 - lane] = packEntry(pv, pk, idBits);
+                } else {
+                    const float key = push ? d : kInf;
+                    const float k0 = quadGetF<0>(key), k1 = quadGetF<1>(key), k2 = quadGetF<2>(key), k3 = quadGetF<3>(key);
+                    int rank = 0;
+                    rank += (slot != 0) && (k0 < key || (k0 == key && 0 < slot));
+                    rank += (slot != 1) && (k1 < key || (k1 == key && 1 < slot));
+                    rank += (slot != 2) && (k2 < key || (k2 == key && 2 < slot));
+                    rank += (slot != 3) && (k3 < key || (k3 == key && 3 < slot));
+                    const int mine = __popcll((pm >> (4 * quad)) & 0xFull);
+                    const int deeper = quad < 15 ? __popcll(pm >> (4 * (quad + 1))) : 0;
+                    if (push) stk[sp + deeper + (mine - 1 - rank)] = packEntry(ch, d, idBits);
+                }
                 sp += pushes;
             }
         }
@@ -1638,16 +1657,19 @@ int gRadiusWaveBelow = 1 << 17;
 // Closest / kNN one wave per query (mode 0: below a leaf- and k-dependent count;
 // mode 3 forces it).  A small query set fills the device only with more lanes per
 // query, while past that the quad kernel needs fewer instructions per query.
-// Measured crossovers (kernel time, same bits): the 2M-triangle terrain on a SAH
-// tree -- closest about 55k queries, kNN16 about 85k (at 20k: 0.20 vs 0.43 ms and
-// 0.22 vs 0.58 ms); 1M Gaussians at 20k: closest 0.22 vs 0.31, kNN8 0.24 vs 0.66;
-// 1M points -- kNN8 about 25k, closest never (its leaf test is too cheap to pay for
-// the wave's sorting networks: 0.15 vs 0.11 ms at 20k).
+// Measured crossovers (kernel time against the quad kernel, same bits): the
+// 2M-triangle terrain on a SAH tree -- closest about 100k queries, kNN16 about 150k
+// (50k: 0.32 vs 0.47 ms and 0.39 vs 0.61 ms; 100k kNN16: 0.72 vs 0.80); 1M points
+// -- closest about 25k, kNN8 about 45k; 1M Gaussians -- closest about 35k, kNN8
+// beyond 80k (80k: 0.81 vs 1.08 ms).
 template <typename L, int K>
 static bool useWave(int count) {
     if (gTraversalMode == 3) return true;
     if (gTraversalMode != 0) return false;
-    const int below = L::kKind == 1 ? (K > 1 ? 24576 : 0) : 49152;
+    int below;
+    if (L::kKind == 0) below = K > 1 ? 131072 : 98304;
+    else if (L::kKind == 1) below = K > 1 ? 45056 : 24576;
+    else below = K > 1 ? 131072 : 32768;
     return count < below;
 }
 
