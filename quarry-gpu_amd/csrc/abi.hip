@@ -192,18 +192,20 @@ struct Stats {
     unsigned long long* p = nullptr;
     Stats() {
         static thread_local unsigned long long* buf = nullptr;
-        if (!buf) hipCheck(hipMalloc(reinterpret_cast<void**>(&buf), 4 * sizeof(unsigned long long)), "stats buffer");
+        if (!buf) hipCheck(hipMalloc(reinterpret_cast<void**>(&buf), kStatWords * sizeof(unsigned long long)), "stats buffer");
         p = buf;
     }
 };
 
 void readStats(const unsigned long long* dev, qr_traversal_stats* out) {
-    unsigned long long h[4] = {0, 0, 0, 0};
-    hipCheck(hipMemcpy(h, dev, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "stats");
-    out->node_tests = h[0];
-    out->leaf_tests = h[1];
-    out->stack_overflows = h[2];
-    out->nodes_opened = h[3];
+    unsigned long long h[kStatWords];
+    hipCheck(hipMemcpy(h, dev, sizeof(h), hipMemcpyDeviceToHost), "stats");
+    unsigned long long sum[4] = {0, 0, 0, 0};
+    for (int i = 0; i < kStatWords; ++i) sum[i % 4] += h[i];
+    out->node_tests = sum[0];
+    out->leaf_tests = sum[1];
+    out->stack_overflows = sum[2];
+    out->nodes_opened = sum[3];
 }
 
 TraverseArgs argsFor(const BvhState& b, bool wide) {
@@ -308,7 +310,7 @@ int qr_describe(char* buf, int len) {
 
 int qr_warmup(void) {
     return guard([&] {
-        DevBuf<unsigned long long> d(4);
+        DevBuf<unsigned long long> d(kStatWords);
         zeroStats(d.p, currentStream());
         hipCheck(hipStreamSynchronize(currentStream()), "warmup");
     });

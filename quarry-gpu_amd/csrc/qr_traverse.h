@@ -19,6 +19,12 @@ struct TraverseArgs {
     unsigned long long* stats = nullptr;  // optional [node tests, leaf tests, overflows]
 };
 
+// Traversal counters: kStatCopies copies of [node tests, leaf tests, stack
+// overflows, nodes opened], block b adding into copy b % kStatCopies (same-address
+// atomics serialize; one copy for every wave of a 100k-wave launch took
+// milliseconds), summed when read.
+constexpr int kStatCopies = 64;
+constexpr int kStatWords = 4 * kStatCopies;
 void zeroStats(unsigned long long* stats, hipStream_t s);
 void zeroInts(int* p, int n, hipStream_t s);
 

@@ -78,10 +78,11 @@ __device__ __forceinline__ void flushStats(unsigned long long* stats, int nodeTe
         d += __shfl_xor(d, off, 64);
     }
     if ((threadIdx.x & 63) == 0) {
-        if (a) atomicAdd(&stats[0], a);
-        if (b) atomicAdd(&stats[1], b);
-        if (c) atomicAdd(&stats[2], c);
-        if (d) atomicAdd(&stats[3], d);
+        unsigned long long* st = stats + 4 * (blockIdx.x % kStatCopies);
+        if (a) atomicAdd(&st[0], a);
+        if (b) atomicAdd(&st[1], b);
+        if (c) atomicAdd(&st[2], c);
+        if (d) atomicAdd(&st[3], d);
     }
 }
 
@@ -1573,7 +1574,7 @@ void zeroInts(int* p, int n, hipStream_t s) {
 }
 
 void zeroStats(unsigned long long* stats, hipStream_t s) {
-    zeroU64Kernel<<<1, kBlock, 0, s>>>(stats, 4);
+    zeroU64Kernel<<<blocksFor(kStatWords, kBlock), kBlock, 0, s>>>(stats, kStatWords);
 }
 
 void mortonOrder(const float* base, int stride, int count, int boxMode, SortWorkspace& ws,
