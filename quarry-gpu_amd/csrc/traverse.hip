@@ -1212,7 +1212,24 @@ __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
                 better = n < K || closer(cd, cid, kd, kid);
             }
             const unsigned long long bm = __ballot(better);
-            if (bm != 0) {
+            if (K == 1 && bm != 0) {  // the wave's smallest candidate replaces the entry
+                float ck = better ? cd : kInf;
+                int cv = better ? cid : 0x7fffffff;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const float ok = waveXor(ck, off);
+                    const int ov = waveXor(cv, off);
+                    if (closer(ok, ov, ck, cv)) {
+                        ck = ok;
+                        cv = ov;
+                    }
+                }
+                kd = ck;
+                kid = cv;
+                ld = lane == 0 ? ck : kInf;
+                lid = lane == 0 ? cv : 0x7fffffff;
+                n = 1;
+            } else if (bm != 0) {
                 float ck = better ? cd : kInf;
                 int cv = better ? cid : 0x7fffffff;
                 waveSortPairs(ck, cv, lane);
@@ -1659,18 +1676,18 @@ int gRadiusWaveBelow = 1 << 17;
 // mode 3 forces it).  A small query set fills the device only with more lanes per
 // query, while past that the quad kernel needs fewer instructions per query.
 // Measured crossovers (kernel time against the quad kernel, same bits): the
-// 2M-triangle terrain on a SAH tree -- closest about 100k queries, kNN16 about 150k
-// (50k: 0.32 vs 0.47 ms and 0.39 vs 0.61 ms; 100k kNN16: 0.72 vs 0.80); 1M points
-// -- closest about 25k, kNN8 about 45k; 1M Gaussians -- closest about 35k, kNN8
-// beyond 80k (80k: 0.81 vs 1.08 ms).
+// 2M-triangle terrain on a SAH tree -- closest about 200k queries, kNN16 about 150k
+// (100k: closest 0.45 vs 0.57 ms, kNN16 0.72 vs 0.80); 1M points -- closest about
+// 35k (20k: 0.08 vs 0.11 ms), kNN8 about 45k; 1M Gaussians -- closest about 45k,
+// kNN8 beyond 80k (80k: 0.81 vs 1.08 ms).
 template <typename L, int K>
 static bool useWave(int count) {
     if (gTraversalMode == 3) return true;
     if (gTraversalMode != 0) return false;
     int below;
-    if (L::kKind == 0) below = K > 1 ? 131072 : 98304;
-    else if (L::kKind == 1) below = K > 1 ? 45056 : 24576;
-    else below = K > 1 ? 131072 : 32768;
+    if (L::kKind == 0) below = K > 1 ? 131072 : 196608;
+    else if (L::kKind == 1) below = K > 1 ? 45056 : 32768;
+    else below = K > 1 ? 131072 : 45056;
     return count < below;
 }
 
