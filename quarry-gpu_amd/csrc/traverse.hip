@@ -1107,15 +1107,16 @@ __device__ __forceinline__ V waveXor(V v, int m) {
     return __shfl_xor(v, m, 64);
 }
 
-// Bitonic sort of one (key, value) pair per lane, ascending by (key, value).
-__device__ __forceinline__ void waveSortPairs(float& k, int& v, int lane) {
-#pragma unroll
-    for (int size = 2; size <= 64; size <<= 1) {
+// Bitonic sort of one (key, value) pair per lane, ascending by (key, value), over
+// aligned groups of `width` lanes (a power of two, at most 64; the default sorts
+// the whole wave).
+__device__ __forceinline__ void waveSortPairs(float& k, int& v, int lane, int width = 64) {
+    for (int size = 2; size <= width; size <<= 1) {
 #pragma unroll
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
             const float ok = waveXor(k, stride);
             const int ov = waveXor(v, stride);
-            const bool up = (lane & size) == 0 || size == 64;
+            const bool up = (lane & size) == 0 || size == width;
             const bool low = (lane & stride) == 0;
             const bool otherFirst = closer(ok, ov, k, v);
             // the lower lane of an ascending pair keeps the smaller, the upper the larger
@@ -1128,10 +1129,10 @@ __device__ __forceinline__ void waveSortPairs(float& k, int& v, int lane) {
     }
 }
 
-// Bitonic merge of a bitonic sequence (one pair per lane) into ascending order.
-__device__ __forceinline__ void waveMergePairs(float& k, int& v, int lane) {
-#pragma unroll
-    for (int stride = 32; stride > 0; stride >>= 1) {
+// Bitonic merge of a bitonic sequence (one pair per lane, aligned groups of `width`
+// lanes) into ascending order.
+__device__ __forceinline__ void waveMergePairs(float& k, int& v, int lane, int width = 64) {
+    for (int stride = width >> 1; stride > 0; stride >>= 1) {
         const float ok = waveXor(k, stride);
         const int ov = waveXor(v, stride);
         const bool low = (lane & stride) == 0;
@@ -1153,7 +1154,10 @@ __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
     // order (fewer nodes opened); triangles and points push quad by quad.
     constexpr bool kSortPushes = L::kKind == 2;
     __shared__ int lds[kWaveStack * (kBlock / 64)];
+    __shared__ int ldsCmp[2 * 64 * (kBlock / 64)];  // per wave: candidate keys, values
     const int lane = threadIdx.x & 63;
+    LdsInt* cmpK = (LdsInt*)(ldsCmp + (threadIdx.x >> 6) * 128);
+    LdsInt* cmpV = cmpK + 64;
     const int quad = lane >> 2;
     const int slot = lane & 3;
     const int qi = xcdBlock() * (kBlock / 64) + (threadIdx.x >> 6);
@@ -1230,19 +1234,30 @@ __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
                 lid = lane == 0 ? cv : 0x7fffffff;
                 n = 1;
             } else if (bm != 0) {
-                float ck = better ? cd : kInf;
-                int cv = better ? cid : 0x7fffffff;
-                waveSortPairs(ck, cv, lane);
-                // the list (lanes >= K hold (inf, INT_MAX)) against the reversed candidates
-                const float rk = __shfl(ck, 63 - lane, 64);
-                const int rv = __shfl(cv, 63 - lane, 64);
+                // the nb candidates move to lanes 0..nb-1 (through LDS), are sorted
+                // over the smallest power-of-two width m >= max(nb, K), and merged:
+                // the list (lanes >= K hold (inf, INT_MAX)) against the reversed
+                // candidates, elementwise, is a bitonic sequence holding the K best
+                const int nb = __popcll(bm);
+                int m = K;
+                while (m < nb) m <<= 1;
+                if (better) {
+                    const int at = __popcll(bm & below);
+                    cmpK[at] = __float_as_int(cd);
+                    cmpV[at] = cid;
+                }
+                float ck = lane < nb ? __int_as_float(cmpK[lane]) : kInf;
+                int cv = lane < nb ? cmpV[lane] : 0x7fffffff;
+                waveSortPairs(ck, cv, lane, m);
+                const float rk = __shfl(ck, m - 1 - (lane & (m - 1)), 64);
+                const int rv = __shfl(cv, m - 1 - (lane & (m - 1)), 64);
                 float mk = ld;
                 int mv = lid;
-                if (closer(rk, rv, mk, mv)) {
+                if (lane < m && closer(rk, rv, mk, mv)) {
                     mk = rk;
                     mv = rv;
                 }
-                waveMergePairs(mk, mv, lane);
+                waveMergePairs(mk, mv, lane, m);
                 ld = lane < K ? mk : kInf;
                 lid = lane < K ? mv : 0x7fffffff;
                 n = n + __popcll(bm);
