@@ -1613,11 +1613,11 @@ void mortonOrder(const float* base, int stride, int count, int boxMode, SortWork
                  uint32_t* keys, uint32_t* keysSorted, uint32_t* orderOut, uint32_t* bb,
                  hipStream_t s) {
     // The order only steers locality (results do not depend on it), so the code is
-    // cut to the whole 8-bit digits that give at least one cell per element: the top
-    // 16 bits below 65,536 elements, 24 below 16M, all 30 above -- one or two sort
-    // passes fewer than the full code.
+    // cut to whole 8-bit digits: the top 16 bits up to 262,144 elements (at most four
+    // per cell; the 100k-query terrain walks measured no slower than on the full
+    // code), 24 up to 16M, all 30 above -- one or two sort passes fewer.
     int passes = 4, drop = 0;
-    if (count <= (1 << 16)) {
+    if (count <= (1 << 18)) {
         passes = 2;
         drop = 14;
     } else if (count <= (1 << 24)) {
