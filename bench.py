@@ -178,6 +178,22 @@ def main() -> int:
         work[name] = (st.node_tests / nq, st.leaf_tests / nq, st.nodes_opened / nq)
     log("node/leaf tests, nodes opened per query: " +
         ", ".join(f"{k} {v[0]:.1f}/{v[1]:.1f}/{v[2]:.1f}" for k, v in work.items()))
+    # The same counts for a sequential nearest-first walk (the four-lane kernels,
+    # traversal mode 2): the work the algorithm needs, which the roofline counts.
+    # The serving kernels may open more nodes than that (a wave expands 16 of one
+    # query's nodes per round); their extra reads are not counted as achieved bytes.
+    seq = {}
+    A.check(A.lib().qr_set_traversal_mode(2))
+    try:
+        for name, fn in (("closest", lambda: pb1.closest(bvh, stats=True)),
+                         ("knn", lambda: pbk.knn(bvh, args.k, stats=True)),
+                         ("radius", lambda: pb1.radius(bvh, args.radius, stats=True))):
+            fn()
+            st = A.last_query_stats()
+            seq[name] = (st.node_tests / nq, st.leaf_tests / nq, st.nodes_opened / nq)
+    finally:
+        A.check(A.lib().qr_set_traversal_mode(0))
+    log("sequential walk: " + ", ".join(f"{k} {v[0]:.1f}/{v[1]:.1f}/{v[2]:.1f}" for k, v in seq.items()))
 
     # the same three queries on binary nodes (reported beside the wide pass)
     binary = {}
@@ -199,8 +215,8 @@ def main() -> int:
     # Algorithmic bytes per launch: the bytes the traversal has to read -- each
     # opened 4-wide node (128 B), each tested leaf (36 B triangle + 4 B prim id),
     # the query (12 B) and its order slot (4 B) -- plus the results it writes.
-    # Node and leaf counts come from the kernels' own counters above.
-    opened, leaves = work[dom][2] * nq, work[dom][1] * nq
+    # Node and leaf counts are the sequential walk's (kernel counters above).
+    opened, leaves = seq[dom][2] * nq, seq[dom][1] * nq
     touched = opened * 128 + leaves * (TRI_BYTES + 4) + nq * (12 + 4)
     alg = {
         "closest": touched + nq * 8,
@@ -339,6 +355,8 @@ def main() -> int:
         "per_op_kernel_ms": {k: round(v, 4) for k, v in per_kernel.items()},
         "tests_per_query": {k: {"node": round(v[0], 2), "leaf": round(v[1], 2),
                                 "opened": round(v[2], 2)} for k, v in work.items()},
+        "tests_per_query_sequential_walk": {k: {"node": round(v[0], 2), "leaf": round(v[1], 2),
+                                                "opened": round(v[2], 2)} for k, v in seq.items()},
         "binary_kernel_ms": binary,
         "roofline": roofline,
         "build": build,
