@@ -371,3 +371,26 @@ def test_radius_wave_walk_wide_frontier(A, orc):
                 assert st.stack_overflows == 0
         finally:
             A.check(A.lib().qr_set_traversal_mode(0))
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_wave_walk_on_identical_items(A, orc, scenes, kind):
+    """Every item identical: all box bounds tie, the whole tree is walked, and the k
+    best are the k smallest indices -- in every traversal shape."""
+    base = scenes.point_cloud(1, 3) if kind == 1 else scenes.gaussians(1, 3)
+    items = np.repeat(base, 3000, axis=0)
+    rng = np.random.default_rng(61)
+    q = rng.uniform(-2, 66, (300, 3)).astype(np.float32)
+    b = A.Bvh(kind, items)
+    b.build_wide()
+    want = {k: orc.knn_brute(kind, items, q, k, threads=THREADS) for k in (0, 7, 32)}
+    try:
+        for mode in (3, 2, 0):
+            A.check(A.lib().qr_set_traversal_mode(mode))
+            for k, w in want.items():
+                got, _, st = b.closest(q, wide=True, stats=True) if k == 0 else \
+                    b.knn(q, k, order=A.ORDER_MORTON, wide=True, stats=True)
+                assert eq(got, w), (mode, k)
+                assert st.stack_overflows == 0
+    finally:
+        A.check(A.lib().qr_set_traversal_mode(0))
