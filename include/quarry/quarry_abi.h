@@ -97,11 +97,14 @@ QR_API const char* qr_last_error(void);
 QR_API int qr_describe(char* buf, int len);
 /* warmupBuildGPU / warmupQueryGPU (src/gpu/gpu_bvh.h:122, :343) */
 QR_API int qr_warmup(void);
-/* Shape of the 4-wide traversal, process-wide: 0 automatic (by query count, leaf
- * kind and k: one wave per query for small sets, four lanes per query for sets too
- * small to fill the GPU one lane per query, one lane per query above), 1 one lane
- * per query, 2 four lanes per query, 3 one wave per query.  Results are identical
- * in every mode.  No reference counterpart (a tuning knob of this backend). */
+/* Shape of the 4-wide traversal, process-wide: 0 automatic (closest / kNN below
+ * 65,536 queries by query count, leaf kind and k; from 65,536 on, the first call
+ * for a tree, k class and query-count octave times one lane, four lanes and one
+ * wave per query on its own queries and keeps the fastest; radius one wave per
+ * query below 131,072 queries, four lanes above), 1 one lane per query, 2 four
+ * lanes per query, 3 one wave per query, 4 as 0 but closest / kNN tuned at every
+ * query count.  Results are identical in every mode.  No reference counterpart
+ * (a tuning knob of this backend). */
 QR_API int qr_set_traversal_mode(int mode);
 /* LBVH build: how many frontier nodes a workgroup (or group of workgroups) may hand
  * up to the next level, process-wide, 0..32 (default 32).  Units with more send

@@ -318,7 +318,7 @@ int qr_warmup(void) {
 
 int qr_set_traversal_mode(int mode) {
     return guard([&] {
-        need(mode >= 0 && mode <= 3, "traversal mode must be 0, 1, 2 or 3");
+        need(mode >= 0 && mode <= 4, "traversal mode must be 0, 1, 2, 3 or 4");
         setTraversalMode(mode);
     });
 }

@@ -24,6 +24,8 @@
 // on query order[i] and writes its result straight to slot order[i], so no
 // gather/scatter copies of queries or results are made.
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "qr_traverse.h"
@@ -1112,7 +1114,6 @@ __device__ __forceinline__ V waveXor(V v, int m) {
 // the whole wave).
 __device__ __forceinline__ void waveSortPairs(float& k, int& v, int lane, int width = 64) {
     for (int size = 2; size <= width; size <<= 1) {
-#pragma unroll
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
             const float ok = waveXor(k, stride);
             const int ov = waveXor(v, stride);
@@ -1673,7 +1674,9 @@ void launchRaysSave(const TraverseArgs& a, const Ray* rays, int count, uint64_t*
 }
 
 namespace {
-int gTraversalMode = 0;  // 0: automatic, 1: one lane per query, 2: four lanes per query
+// 0 automatic, 1 one lane per query, 2 four lanes per query, 3 one wave per query,
+// 4 closest / kNN tuned whatever the query count
+int gTraversalMode = 0;
 }
 
 void setTraversalMode(int mode) { gTraversalMode = mode; }
@@ -1687,9 +1690,10 @@ void setTraversalMode(int mode) { gTraversalMode = mode; }
 // above (traversal mode 2 forces four lanes per query).
 int gRadiusWaveBelow = 1 << 17;
 
-// Closest / kNN one wave per query (mode 0: below a leaf- and k-dependent count;
-// mode 3 forces it).  A small query set fills the device only with more lanes per
-// query, while past that the quad kernel needs fewer instructions per query.
+// Closest / kNN one wave per query below a leaf- and k-dependent count (automatic
+// mode, under kTuneFrom queries).  A small query set fills the device only with more
+// lanes per query, while past that the quad kernel needs fewer instructions per
+// query on short, even walks.
 // Measured crossovers (kernel time against the quad kernel, same bits): the
 // 2M-triangle terrain on a SAH tree -- closest about 200k queries, kNN16 about 150k
 // (100k: closest 0.45 vs 0.57 ms, kNN16 0.72 vs 0.80); 1M points -- closest about
@@ -1697,8 +1701,6 @@ int gRadiusWaveBelow = 1 << 17;
 // kNN8 beyond 80k (80k: 0.81 vs 1.08 ms).
 template <typename L, int K>
 static bool useWave(int count) {
-    if (gTraversalMode == 3) return true;
-    if (gTraversalMode != 0) return false;
     int below;
     if (L::kKind == 0) below = K > 1 ? 131072 : 196608;
     else if (L::kKind == 1) below = K > 1 ? 45056 : 32768;
@@ -1706,42 +1708,123 @@ static bool useWave(int count) {
     return count < below;
 }
 
-static bool useQuad(int count, int K) {
-    if (gTraversalMode == 1) return false;
-    if (gTraversalMode == 2 || gTraversalMode == 3) return true;
-    return K > 1 || count < 256 * 4 * 4 * 64;
+// Shapes of the closest / kNN walk over 4-wide nodes.
+enum NearestShape : int { kShapeLane = 0, kShapeQuad = 1, kShapeWave = 2 };
+
+template <typename L, int K>
+static void launchShape(int shape, const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
+                        ClosestResult* out, unsigned long long* stats, hipStream_t s) {
+    const auto* prims = static_cast<const typename L::Item*>(a.items);
+    const int idBits = stackIdBitsDev(a.wideCount);
+    if (shape == kShapeWave) {
+        const int grid = blocksFor(count, kBlock / 64);
+        if (stats)
+            nearestWaveKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, stats, idBits);
+        else
+            nearestWaveKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, idBits);
+    } else if (shape == kShapeQuad) {
+        const int grid = blocksFor(4LL * count, kBlock);
+        if (stats)
+            nearestQuadKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, stats, idBits);
+        else
+            nearestQuadKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, idBits);
+    } else {
+        const int grid = blocksFor(count, kBlock);
+        if (stats)
+            nearestWideKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, stats, idBits);
+        else
+            nearestWideKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, idBits);
+    }
+}
+
+// The shape for a set of `count` queries by the measured crossovers (above).
+template <typename L, int K>
+static int defaultShape(int count) {
+    if (useWave<L, K>(count)) return kShapeWave;
+    return (K > 1 || count < 256 * 4 * 4 * 64) ? kShapeQuad : kShapeLane;
+}
+
+// Tuned shapes.  The best shape depends on how long and how uneven the walks are,
+// which the query count alone does not tell: on the terrain a million closest
+// queries run fastest four lanes per query (3.2 ms, one wave per query 4.0), while
+// 262,144 queries in and around an icosphere -- long walks of very different
+// lengths -- run 5.5x faster one wave per query (1.6 ms against 8.6 one lane per
+// query).  From kTuneFrom queries on, the first call for a (tree, k class, query
+// count octave) times the three shapes on its own queries and keeps the fastest;
+// every shape writes the same bits, so the trials' output is the call's output.
+constexpr int kTuneFrom = 65536;
+
+struct TuneKey {
+    const void* tree;
+    int wideCount, kind, K, octave;
+    bool operator<(const TuneKey& o) const {
+        if (tree != o.tree) return tree < o.tree;
+        if (wideCount != o.wideCount) return wideCount < o.wideCount;
+        if (kind != o.kind) return kind < o.kind;
+        if (K != o.K) return K < o.K;
+        return octave < o.octave;
+    }
+};
+static std::mutex gTuneMutex;
+static std::map<TuneKey, int> gTuned;
+
+template <typename L, int K>
+static int tunedShape(const TraverseArgs& a, const V3* pts, int count, int k, int outStride, ClosestResult* out,
+                      hipStream_t s) {
+    int octave = 0;
+    while ((2LL << octave) <= count) ++octave;
+    const TuneKey key{a.wideNodes, a.wideCount, L::kKind, K, octave};
+    {
+        std::lock_guard<std::mutex> lock(gTuneMutex);
+        const auto it = gTuned.find(key);
+        if (it != gTuned.end()) return it->second;
+    }
+    EventSet<6> ev;
+    const int shapes[3] = {kShapeLane, kShapeQuad, kShapeWave};
+    for (int i = 0; i < 3; ++i) {
+        ev.record(2 * i, s);
+        launchShape<L, K>(shapes[i], a, pts, count, k, outStride, out, nullptr, s);
+        ev.record(2 * i + 1, s);
+    }
+    hipCheck(hipGetLastError(), "nearest tuning");
+    ev.sync(5);
+    int best = kShapeLane;
+    float bestMs = 0.0f;
+    for (int i = 0; i < 3; ++i) {
+        const float ms = ev.ms(2 * i, 2 * i + 1);
+        if (i == 0 || ms < bestMs) {
+            best = shapes[i];
+            bestMs = ms;
+        }
+    }
+    std::lock_guard<std::mutex> lock(gTuneMutex);
+    if (gTuned.size() >= 4096) gTuned.clear();  // trees come and go; keep the table small
+    gTuned[key] = best;
+    return best;
 }
 
 template <typename L, int K>
 static void launchNearestTK(const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
                             ClosestResult* out, hipStream_t s) {
-    const int grid = blocksFor(count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
-    if (a.wide && useWave<L, K>(count)) {
-        const int wgrid = blocksFor(count, kBlock / 64);
-        const int idBits = stackIdBitsDev(a.wideCount);
-        if (a.stats)
-            nearestWaveKernel<L, K, true><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats, idBits);
-        else
-            nearestWaveKernel<L, K, false><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, idBits);
-    } else if (a.wide && useQuad(count, K)) {
-        const int qgrid = blocksFor(4LL * count, kBlock);
-        const int idBits = stackIdBitsDev(a.wideCount);
-        if (a.stats)
-            nearestQuadKernel<L, K, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats, idBits);
-        else
-            nearestQuadKernel<L, K, false><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, idBits);
-    } else if (a.wide) {
-        if (a.stats)
-            nearestWideKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats, stackIdBitsDev(a.wideCount));
-        else
-            nearestWideKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, stackIdBitsDev(a.wideCount));
-    } else {
+    if (!a.wide) {
+        const int grid = blocksFor(count, kBlock);
         if (a.stats)
             nearestBinaryKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.nodes, prims, a.primIds, pts, a.order, count, k, outStride, out, a.stats, stackIdBitsDev(a.nodeCount));
         else
             nearestBinaryKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.nodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, stackIdBitsDev(a.nodeCount));
+        return;
     }
+    int shape;
+    switch (gTraversalMode) {
+        case 1: shape = kShapeLane; break;
+        case 2: shape = kShapeQuad; break;
+        case 3: shape = kShapeWave; break;
+        case 4: shape = tunedShape<L, K>(a, pts, count, k, outStride, out, s); break;
+        default:
+            shape = count < kTuneFrom ? defaultShape<L, K>(count) : tunedShape<L, K>(a, pts, count, k, outStride, out, s);
+    }
+    launchShape<L, K>(shape, a, pts, count, k, outStride, out, a.stats, s);
 }
 
 template <typename L>
@@ -1816,7 +1899,7 @@ static void launchCaptureT(const TraverseArgs& a, const V3* pts, int count, floa
                            uint64_t* capture, int cap, int* maxCount, hipStream_t s) {
     const int qgrid = blocksFor(4LL * count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
-    if ((count < gRadiusWaveBelow && gTraversalMode == 0) || gTraversalMode == 3) {
+    if ((count < gRadiusWaveBelow && (gTraversalMode == 0 || gTraversalMode == 4)) || gTraversalMode == 3) {
         const int wgrid = blocksFor(count, kBlock / 64);
         if (a.stats)
             radiusWaveKernel<L, true><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, capture, a.stats, maxCount, cap);

@@ -321,7 +321,7 @@ def test_traversal_modes_agree(A, orc, scenes, kind):
     b.build_wide()
     want = {k: orc.knn_brute(kind, items, q, k, threads=THREADS) for k in (0, 1, 5, 16, 32)}
     try:
-        for mode in (1, 2, 3, 0):
+        for mode in (1, 2, 3, 4, 0):
             A.check(A.lib().qr_set_traversal_mode(mode))
             for k, w in want.items():
                 got = b.closest(q, order=A.ORDER_MORTON, wide=True)[0] if k == 0 else \
@@ -385,7 +385,7 @@ def test_wave_walk_on_identical_items(A, orc, scenes, kind):
     b.build_wide()
     want = {k: orc.knn_brute(kind, items, q, k, threads=THREADS) for k in (0, 7, 32)}
     try:
-        for mode in (3, 2, 0):
+        for mode in (3, 2, 4, 0):
             A.check(A.lib().qr_set_traversal_mode(mode))
             for k, w in want.items():
                 got, _, st = b.closest(q, wide=True, stats=True) if k == 0 else \
@@ -394,3 +394,19 @@ def test_wave_walk_on_identical_items(A, orc, scenes, kind):
                 assert st.stack_overflows == 0
     finally:
         A.check(A.lib().qr_set_traversal_mode(0))
+
+
+def test_tuned_shape_on_large_sets(A, orc, scenes):
+    """From 65,536 queries the automatic mode tunes the shape on the first call (the
+    trials write the output); the first and later calls give the oracle's bits."""
+    tris = scenes.terrain(64, 3)
+    b = A.Bvh(0, tris)
+    b.build_wide()
+    q = np.random.default_rng(71).uniform(-4, 68, (70000, 3)).astype(np.float32)
+    q[:, 2] = np.random.default_rng(72).uniform(-12, 12, 70000).astype(np.float32)
+    for k in (0, 8):
+        want = orc.knn_brute(0, tris, q, k, threads=THREADS)
+        for call in range(2):
+            got = b.closest(q, order=A.ORDER_MORTON, wide=True)[0] if k == 0 else \
+                b.knn(q, k, order=A.ORDER_MORTON, wide=True)[0]
+            assert eq(got, want), (k, call)
