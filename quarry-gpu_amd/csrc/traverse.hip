@@ -1306,6 +1306,117 @@ __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
+// Closest-hit rays with one wave per ray (the layout of the wave kernels above: 16
+// quads expand up to 16 stacked nodes of one walk per round, lane s of a quad tests
+// entry s).  A round's hits reduce to the wave's smallest (t, then primitive index --
+// the lane kernels' rule for equal t), boxes are tested against the best t so far
+// with the lane kernels' slack, and popped entries whose entry distance (kept in the
+// stack entry, truncated: a lower bound) is beyond it are dropped.  Interior children
+// go on the stack nearest-parent first, each parent's nearest on top.
+template <bool kStats, bool kSave>
+__global__ __launch_bounds__(kBlock) void rayWaveKernel(const Wide4* __restrict__ wide,
+                                                        const Tri* __restrict__ tris,
+                                                        const Ray* __restrict__ rays,
+                                                        const uint32_t* __restrict__ order, int count,
+                                                        uint64_t* __restrict__ out,
+                                                        unsigned long long* __restrict__ stats,
+                                                        const int* __restrict__ primIds,
+                                                        int* __restrict__ hitPrim, int idBits) {
+    __shared__ int lds[kWaveStack * (kBlock / 64)];
+    const int lane = threadIdx.x & 63;
+    const int quad = lane >> 2;
+    const int slot = lane & 3;
+    const int qi = xcdBlock() * (kBlock / 64) + (threadIdx.x >> 6);
+    int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
+    if (qi < count) {  // the same in the whole wave
+        const int q = order ? static_cast<int>(order[qi]) : qi;
+        const Ray ray = rays[q];
+        float best = ray.tMax;
+        bool any = false;
+        int bestId = 0x7fffffff;
+        LdsInt* stk = (LdsInt*)(lds + (threadIdx.x >> 6) * kWaveStack);
+        if (lane == 0) stk[0] = packEntry(0, 0.0f, idBits);
+        int sp = 1;
+        while (sp > 0) {
+            int width = (kWaveStack - kWaveHeadroom - sp) / 3;
+            width = width < 1 ? 1 : (width > 16 ? 16 : width);
+            const int take = width < sp ? width : sp;
+            int node = -1;
+            if (quad < take) {
+                const int v = stk[sp - 1 - quad];
+                if (entryKey(v, idBits) <= best * (1.0f + 1e-6f) + 1e-6f) node = entryId(v, idBits);
+            }
+            sp -= take;
+            if (kStats && slot == 0 && node >= 0) ++opened;
+            WideEntry e{};
+            int ch = kWideEmpty;
+            if (node >= 0) {
+                e = wide[node].c[slot];
+                ch = e.child;
+            }
+            float tEnter = kInf;
+            bool adm = false;
+            if (ch != kWideEmpty) adm = rayBox(ray, entryBox(e), best, tEnter);
+            if (adm) ++nodeTests;
+            const bool leaf = ch < 0 && ch != kWideEmpty;
+            float ht = kInf;
+            int hid = 0x7fffffff;
+            if (adm && leaf) {
+                ++leafTests;
+                float t;
+                // the lane kernels' rule: a nearer hit, or an equal one once there is a hit
+                if (rayTri(ray, tris[-ch - 1], t) && (t < best || (any && t == best))) {
+                    ht = t;
+                    hid = primIds[-ch - 1];
+                }
+            }
+            if (__ballot(ht < kInf) != 0) {
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const float ot = waveXor(ht, off);
+                    const int oi = waveXor(hid, off);
+                    if (closer(ot, oi, ht, hid)) {
+                        ht = ot;
+                        hid = oi;
+                    }
+                }
+                if (ht < best) {
+                    best = ht;
+                    bestId = hid;
+                    any = true;
+                } else if (any && ht == best && hid < bestId) {
+                    bestId = hid;
+                }
+            }
+            const bool push = adm && !leaf && tEnter <= best * (1.0f + 1e-6f) + 1e-6f;
+            const unsigned long long pm = __ballot(push);
+            const int pushes = __popcll(pm);
+            if (pushes > 0) {
+                if (sp + pushes > kWaveStack) {  // deeper than the stack bound allows
+                    if (lane == 0) overflow = 1;
+                    break;
+                }
+                const float key = push ? tEnter : kInf;
+                const float k0 = quadGetF<0>(key), k1 = quadGetF<1>(key), k2 = quadGetF<2>(key), k3 = quadGetF<3>(key);
+                int rank = 0;
+                rank += (slot != 0) && (k0 < key || (k0 == key && 0 < slot));
+                rank += (slot != 1) && (k1 < key || (k1 == key && 1 < slot));
+                rank += (slot != 2) && (k2 < key || (k2 == key && 2 < slot));
+                rank += (slot != 3) && (k3 < key || (k3 == key && 3 < slot));
+                const int mine = __popcll((pm >> (4 * quad)) & 0xFull);
+                const int deeper = quad < 15 ? __popcll(pm >> (4 * (quad + 1))) : 0;
+                if (push) stk[sp + deeper + (mine - 1 - rank)] = packEntry(ch, tEnter, idBits);
+                sp += pushes;
+            }
+        }
+        if (lane == 0) {
+            out[q] = any ? ((uint64_t{1} << 32) | floatBits(best)) : 0ull;
+            if (kSave) hitPrim[q] = any ? bestId : -1;
+        }
+    }
+    flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
+}
+
 // ---- radius: count and fill passes ---------------------------------------------------
 // kFill = false counts matches per query; kFill = true writes the 64-bit keys
 // (dist2 bits << 32 | index) into the query's CSR segment (bounded by the next
@@ -1640,6 +1751,96 @@ void mortonOrder(const float* base, int stride, int count, int boxMode, SortWork
     sortPairs(ws, keys, nullptr, keysSorted, orderOut, static_cast<size_t>(count), passes, s);
 }
 
+// 0 automatic, 1 one lane per query, 2 four lanes per query, 3 one wave per query,
+// 4 closest / kNN and rays tuned whatever the query count
+int gTraversalMode = 0;
+
+// Shapes of a walk over 4-wide nodes.
+enum NearestShape : int { kShapeLane = 0, kShapeQuad = 1, kShapeWave = 2 };
+
+// Tuned shapes.  The best shape depends on how long and how uneven the walks are,
+// which the query count alone does not tell: on the terrain a million closest
+// queries run fastest four lanes per query (3.2 ms, one wave per query 4.0), while
+// 262,144 queries in and around an icosphere -- long walks of very different
+// lengths -- run 5.5x faster one wave per query (1.6 ms against 8.6 one lane per
+// query).  From kTuneFrom queries on, the first call for a (tree, k class, query
+// count octave) times the three shapes on its own queries and keeps the fastest;
+// every shape writes the same bits, so the trials' output is the call's output.
+constexpr int kTuneFrom = 65536;
+
+struct TuneKey {
+    const void* tree;
+    int wideCount, kind, K, octave;
+    bool operator<(const TuneKey& o) const {
+        if (tree != o.tree) return tree < o.tree;
+        if (wideCount != o.wideCount) return wideCount < o.wideCount;
+        if (kind != o.kind) return kind < o.kind;
+        if (K != o.K) return K < o.K;
+        return octave < o.octave;
+    }
+};
+static std::mutex gTuneMutex;
+static std::map<TuneKey, int> gTuned;
+
+template <bool kSave>
+static void launchRayShape(int shape, const TraverseArgs& a, const Ray* rays, int count, uint64_t* out,
+                           int* hitPrim, unsigned long long* stats, hipStream_t s) {
+    const Tri* tris = static_cast<const Tri*>(a.items);
+    if (shape == kShapeWave) {
+        const int grid = blocksFor(count, kBlock / 64);
+        const int idBits = stackIdBitsDev(a.wideCount);
+        if (stats)
+            rayWaveKernel<true, kSave><<<grid, kBlock, 0, s>>>(a.wideNodes, tris, rays, a.order, count, out, stats,
+                                                              a.primIds, hitPrim, idBits);
+        else
+            rayWaveKernel<false, kSave><<<grid, kBlock, 0, s>>>(a.wideNodes, tris, rays, a.order, count, out, nullptr,
+                                                               a.primIds, hitPrim, idBits);
+    } else {
+        const int grid = blocksFor(count, kBlock);
+        if (stats)
+            rayWideKernel<true, kSave><<<grid, kBlock, 0, s>>>(a.wideNodes, tris, rays, a.order, count, out, stats,
+                                                              a.primIds, hitPrim);
+        else
+            rayWideKernel<false, kSave><<<grid, kBlock, 0, s>>>(a.wideNodes, tris, rays, a.order, count, out,
+                                                               nullptr, a.primIds, hitPrim);
+    }
+}
+
+// Rays over 4-wide nodes run one lane or one wave per ray (same bits).  One lane per
+// ray unless mode 3 forces one wave or mode 4 tunes (the first call for a tree, save
+// flag and ray-count octave times both and keeps the faster): closest-hit walks are
+// short and prune well one lane at a time -- one wave per ray measured 1.3-5x slower
+// on the down rays, the city and the icosphere (0.44 vs 2.17 ms, 0.75 vs 2.52, 1.06
+// vs 3.04 for 1M rays).
+template <bool kSave>
+static int rayShape(const TraverseArgs& a, const Ray* rays, int count, uint64_t* out, int* hitPrim,
+                    hipStream_t s) {
+    if (gTraversalMode == 3) return kShapeWave;
+    if (gTraversalMode != 4) return kShapeLane;
+    int octave = 0;
+    while ((2LL << octave) <= count) ++octave;
+    const TuneKey key{a.wideNodes, a.wideCount, 100 + (kSave ? 1 : 0), 0, octave};
+    {
+        std::lock_guard<std::mutex> lock(gTuneMutex);
+        const auto it = gTuned.find(key);
+        if (it != gTuned.end()) return it->second;
+    }
+    EventSet<4> ev;
+    ev.record(0, s);
+    launchRayShape<kSave>(kShapeLane, a, rays, count, out, hitPrim, nullptr, s);
+    ev.record(1, s);
+    ev.record(2, s);
+    launchRayShape<kSave>(kShapeWave, a, rays, count, out, hitPrim, nullptr, s);
+    ev.record(3, s);
+    hipCheck(hipGetLastError(), "ray tuning");
+    ev.sync(3);
+    const int best = ev.ms(2, 3) < ev.ms(0, 1) ? kShapeWave : kShapeLane;
+    std::lock_guard<std::mutex> lock(gTuneMutex);
+    if (gTuned.size() >= 4096) gTuned.clear();
+    gTuned[key] = best;
+    return best;
+}
+
 template <bool kSave>
 static void launchRaysT(const TraverseArgs& a, const Ray* rays, int count, uint64_t* out, int* hitPrim,
                         hipStream_t s) {
@@ -1647,12 +1848,8 @@ static void launchRaysT(const TraverseArgs& a, const Ray* rays, int count, uint6
     if (grid == 0) return;
     const Tri* tris = static_cast<const Tri*>(a.items);
     if (a.wide) {
-        if (a.stats)
-            rayWideKernel<true, kSave><<<grid, kBlock, 0, s>>>(a.wideNodes, tris, rays, a.order, count, out, a.stats,
-                                                              a.primIds, hitPrim);
-        else
-            rayWideKernel<false, kSave><<<grid, kBlock, 0, s>>>(a.wideNodes, tris, rays, a.order, count, out,
-                                                               nullptr, a.primIds, hitPrim);
+        launchRayShape<kSave>(rayShape<kSave>(a, rays, count, out, hitPrim, s), a, rays, count, out, hitPrim,
+                              a.stats, s);
     } else {
         if (a.stats)
             rayBinaryKernel<true, kSave><<<grid, kBlock, 0, s>>>(a.nodes, tris, rays, a.order, count, out, a.stats,
@@ -1674,9 +1871,7 @@ void launchRaysSave(const TraverseArgs& a, const Ray* rays, int count, uint64_t*
 }
 
 namespace {
-// 0 automatic, 1 one lane per query, 2 four lanes per query, 3 one wave per query,
-// 4 closest / kNN tuned whatever the query count
-int gTraversalMode = 0;
+
 }
 
 void setTraversalMode(int mode) { gTraversalMode = mode; }
@@ -1707,9 +1902,6 @@ static bool useWave(int count) {
     else below = K > 1 ? 131072 : 45056;
     return count < below;
 }
-
-// Shapes of the closest / kNN walk over 4-wide nodes.
-enum NearestShape : int { kShapeLane = 0, kShapeQuad = 1, kShapeWave = 2 };
 
 template <typename L, int K>
 static void launchShape(int shape, const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
@@ -1743,30 +1935,6 @@ static int defaultShape(int count) {
     if (useWave<L, K>(count)) return kShapeWave;
     return (K > 1 || count < 256 * 4 * 4 * 64) ? kShapeQuad : kShapeLane;
 }
-
-// Tuned shapes.  The best shape depends on how long and how uneven the walks are,
-// which the query count alone does not tell: on the terrain a million closest
-// queries run fastest four lanes per query (3.2 ms, one wave per query 4.0), while
-// 262,144 queries in and around an icosphere -- long walks of very different
-// lengths -- run 5.5x faster one wave per query (1.6 ms against 8.6 one lane per
-// query).  From kTuneFrom queries on, the first call for a (tree, k class, query
-// count octave) times the three shapes on its own queries and keeps the fastest;
-// every shape writes the same bits, so the trials' output is the call's output.
-constexpr int kTuneFrom = 65536;
-
-struct TuneKey {
-    const void* tree;
-    int wideCount, kind, K, octave;
-    bool operator<(const TuneKey& o) const {
-        if (tree != o.tree) return tree < o.tree;
-        if (wideCount != o.wideCount) return wideCount < o.wideCount;
-        if (kind != o.kind) return kind < o.kind;
-        if (K != o.K) return K < o.K;
-        return octave < o.octave;
-    }
-};
-static std::mutex gTuneMutex;
-static std::map<TuneKey, int> gTuned;
 
 template <typename L, int K>
 static int tunedShape(const TraverseArgs& a, const V3* pts, int count, int k, int outStride, ClosestResult* out,

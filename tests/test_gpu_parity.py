@@ -410,3 +410,36 @@ def test_tuned_shape_on_large_sets(A, orc, scenes):
             got = b.closest(q, order=A.ORDER_MORTON, wide=True)[0] if k == 0 else \
                 b.knn(q, k, order=A.ORDER_MORTON, wide=True)[0]
             assert eq(got, want), (k, call)
+
+
+def test_ray_shapes_agree(A, orc, scenes):
+    """One lane and one wave per ray (forced, and tuned in mode 4) give the
+    oracle's hit flags and t bits; with the hit primitive saved, the same primitive
+    (the smallest index among equal-t hits)."""
+    import ctypes
+
+    V, F, tris = scenes.icosphere(levels=4, radius=10.0, noise=0.1, seed=2)
+    items = np.concatenate([tris.reshape(-1, 9), scenes.terrain(24, 5)]).astype(np.float32)
+    rays = scenes.sphere_rays(70000, seed=8)
+    b = A.Bvh(0, items)
+    b.build_wide()
+    want = orc.ray_brute(items, rays, threads=THREADS)
+    L = A.lib()
+    dr = A.DeviceArray(rays)
+    res = A.DeviceArray.empty(rays.shape[0], np.uint64)
+    hit = A.DeviceArray.empty(rays.shape[0], np.int32)
+    prims = {}
+    try:
+        for mode in (1, 3, 4, 0):
+            A.check(L.qr_set_traversal_mode(mode))
+            got = b.raycast(rays, order=A.ORDER_MORTON, wide=True)[0]
+            assert np.array_equal(got, want), mode
+            ms = ctypes.c_double()
+            A.check(L.qr_raycast_save_device(b.h, dr.ptr, rays.shape[0], res.ptr, hit.ptr, A.ORDER_INPUT, 1,
+                                             ctypes.byref(ms)))
+            assert np.array_equal(res.download(), want), mode
+            prims[mode] = hit.download()
+    finally:
+        A.check(L.qr_set_traversal_mode(0))
+    assert all(np.array_equal(p, prims[1]) for p in prims.values())
+    assert (prims[1] >= 0).sum() == int((want >> np.uint64(32)).sum())
