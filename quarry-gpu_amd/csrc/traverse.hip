@@ -1179,11 +1179,30 @@ __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
         while (sp > 0) {
             int width = (kWaveStack - kWaveHeadroom - sp) / 3;
             width = width < 1 ? 1 : (width > 16 ? 16 : width);
-            const int take = width < sp ? width : sp;
+            int take = width < sp ? width : sp;
             int node = -1;
             if (quad < take) {
                 const int v = stk[sp - 1 - quad];
                 if (admits(entryKey(v, idBits))) node = entryId(v, idBits);
+            }
+            if (__ballot(quad < take && node < 0) != 0) {
+                // the bound has overtaken entries the quads would take: drop those of
+                // the top 64 (order kept) and take again, so no quad idles on them
+                const int look = sp < 64 ? sp : 64;
+                int v = 0;
+                bool keep = false;
+                if (lane < look) {
+                    v = stk[sp - look + lane];
+                    keep = admits(entryKey(v, idBits));
+                }
+                const unsigned long long km = __ballot(keep);
+                if (keep) stk[sp - look + __popcll(km & below)] = v;
+                sp = sp - look + __popcll(km);
+                if (sp == 0) break;
+                width = (kWaveStack - kWaveHeadroom - sp) / 3;
+                width = width < 1 ? 1 : (width > 16 ? 16 : width);
+                take = width < sp ? width : sp;
+                node = quad < take ? entryId(stk[sp - 1 - quad], idBits) : -1;
             }
             sp -= take;
             if (kStats && slot == 0 && node >= 0) ++opened;
