@@ -96,7 +96,8 @@ struct qr_csr {
 // Pinned host words for the small device-to-host reads of a call (per host thread).
 int* hostWords() {
     static thread_local int* w = nullptr;
-    if (!w) hipCheck(hipHostMalloc(reinterpret_cast<void**>(&w), 4 * sizeof(int), hipHostMallocDefault), "pinned words");
+    if (!w) hipCheck(hipHostMalloc(reinterpret_cast<void**>(&w), (1 + kMaxCopies) * sizeof(int), hipHostMallocDefault),
+                     "pinned words");
     return w;
 }
 
@@ -1324,21 +1325,24 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
             ev.record(1, s);
             p->rCounts.reserve(static_cast<size_t>(n));
             p->rScratch.reserve(static_cast<size_t>(std::max(scanScratchInts(n), 4 + 2 * n)));
-            p->rMax.reserve(1);
+            p->rMax.reserve(kMaxCopies);
             const int cap = radiusCaptureCap(n);
             if (cap > 0) p->rCapture.reserve(static_cast<size_t>(n) * cap);
             int* counts = p->rCounts.p;
             int* scratch = p->rScratch.p;
-            zeroInts(p->rMax.p, 1, s);
+            zeroInts(p->rMax.p, kMaxCopies, s);
             kev.record(0, s);
             const bool captured = launchRadiusCapture(a, p->pts.p, n, r2, counts, p->rCapture.p, cap, p->rMax.p, s);
             kev.record(1, s);
             scanExclusive(counts, res->offsets.p, n, res->offsets.p + n, scratch, s);
             int* hw = hostWords();
             hipCheck(hipMemcpyAsync(hw, res->offsets.p + n, sizeof(int), hipMemcpyDeviceToHost, s), "D2H total");
-            hipCheck(hipMemcpyAsync(hw + 1, p->rMax.p, sizeof(int), hipMemcpyDeviceToHost, s), "D2H max");
+            hipCheck(hipMemcpyAsync(hw + 1, p->rMax.p, kMaxCopies * sizeof(int), hipMemcpyDeviceToHost, s),
+                     "D2H max");
             hipCheck(hipStreamSynchronize(s), "radius total sync");
-            const int total = hw[0], maxSeg = hw[1];
+            const int total = hw[0];
+            int maxSeg = 0;
+            for (int i = 1; i <= kMaxCopies; ++i) maxSeg = hw[i] > maxSeg ? hw[i] : maxSeg;
             res->total = total;
             res->keys.alloc(static_cast<size_t>(total > 0 ? total : 1), s);
             kev.record(2, s);

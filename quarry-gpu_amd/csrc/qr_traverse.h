@@ -51,7 +51,11 @@ void launchClosest(const TraverseArgs& a, const V3* pts, int count, ClosestResul
 // Result j of query q lands at out[q * outStride + j] (outStride >= k).
 void launchKnn(const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
                ClosestResult* out, hipStream_t s);
-// maxCount (may be null): atomically raised to the largest per-query count.
+// maxCount (may be null): kMaxCopies ints, each raised atomically to the largest
+// per-query count of the blocks b with b % kMaxCopies equal to its index (one word for
+// every wave of a 100k-wave launch serialized the atomics); the largest count is
+// the largest word.
+constexpr int kMaxCopies = 64;
 // Radius, one walk: counts hits per query and keeps the first `cap` of each query in
 // its slot of `capture` (count * cap keys); returns false when the tree or mode takes
 // the per-lane kernels, which only count (nothing is captured).

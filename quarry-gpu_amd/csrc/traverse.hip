@@ -59,7 +59,7 @@ __device__ __forceinline__ void waveMaxAtomic(int* dst, int v) {
         const int o = __shfl_xor(v, off, 64);
         v = o > v ? o : v;
     }
-    if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(dst, v);
+    if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(dst + blockIdx.x % kMaxCopies, v);
 }
 
 // ---- stats -------------------------------------------------------------------------
