@@ -1354,7 +1354,9 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
                     launchRadiusFill(a, p->pts.p, n, r2, counts, captured ? cap : 0, res->offsets.p, res->keys.p, s);
             }
             kev.record(3, s);
-            if (total > 0) segmentSort(res->offsets.p, res->keys.p, n, maxSeg, scratch, s);
+            // segments of at most 64 captured hits come sorted from the one-wave capture
+            const bool sorted = captured && maxSeg <= 64 && maxSeg <= cap && radiusCapturePresorts(a, n);
+            if (total > 0 && !sorted) segmentSort(res->offsets.p, res->keys.p, n, maxSeg, scratch, s);
             ev.record(2, s);
             ev.sync(2);
             const double total_ms = ev.ms(0, 2);

@@ -62,6 +62,9 @@ constexpr int kMaxCopies = 64;
 int radiusCaptureCap(int count);  // hits kept per query: 64, fewer for huge query sets
 bool launchRadiusCapture(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts,
                          uint64_t* capture, int cap, int* maxCount, hipStream_t s);
+// True when the capture runs one wave per query, which leaves every captured segment
+// of at most 64 hits sorted by (dist2, index).
+bool radiusCapturePresorts(const TraverseArgs& a, int count);
 // Writes the CSR segments of the queries the capture does not hold (all queries when
 // nothing was captured: cap 0); counts are those of the capture / count pass.
 void launchRadiusFill(const TraverseArgs& a, const V3* pts, int count, float r2, const int* counts,

@@ -1005,6 +1005,47 @@ __global__ __launch_bounds__(kBlock) void radiusQuadKernel(
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
 }
 
+template <typename V>
+__device__ __forceinline__ V waveXor(V v, int m) {
+    return __shfl_xor(v, m, 64);
+}
+
+// Bitonic sort of one (key, value) pair per lane, ascending by (key, value), over
+// aligned groups of `width` lanes (a power of two, at most 64; the default sorts
+// the whole wave).
+__device__ __forceinline__ void waveSortPairs(float& k, int& v, int lane, int width = 64) {
+    for (int size = 2; size <= width; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const float ok = waveXor(k, stride);
+            const int ov = waveXor(v, stride);
+            const bool up = (lane & size) == 0 || size == width;
+            const bool low = (lane & stride) == 0;
+            const bool otherFirst = closer(ok, ov, k, v);
+            // the lower lane of an ascending pair keeps the smaller, the upper the larger
+            const bool take = (low == up) ? otherFirst : !otherFirst;
+            if (take) {
+                k = ok;
+                v = ov;
+            }
+        }
+    }
+}
+
+// Bitonic merge of a bitonic sequence (one pair per lane, aligned groups of `width`
+// lanes) into ascending order.
+__device__ __forceinline__ void waveMergePairs(float& k, int& v, int lane, int width = 64) {
+    for (int stride = width >> 1; stride > 0; stride >>= 1) {
+        const float ok = waveXor(k, stride);
+        const int ov = waveXor(v, stride);
+        const bool low = (lane & stride) == 0;
+        const bool otherFirst = closer(ok, ov, k, v);
+        if (low == otherFirst) {
+            k = ok;
+            v = ov;
+        }
+    }
+}
+
 // Radius capture with one wave per query, for query sets too small to fill the
 // device four lanes per query.  The wave's 16 quads expand up to 16 nodes of the
 // query's walk per round (quad j takes the j-th entry from the top of a stack shared
@@ -1086,6 +1127,21 @@ __global__ __launch_bounds__(kBlock) void radiusWaveKernel(
             }
             found += __popcll(hm);
         }
+        if (found > 1 && found <= 64 && found <= cap) {
+            // a segment of at most 64 hits leaves sorted by (dist2, index): the order
+            // of the 64-bit keys, dist2 being non-negative
+            int w = 2;
+            while (w < found) w <<= 1;
+            float sk = kInf;
+            int sv = 0x7fffffff;
+            if (lane < found) {
+                const uint64_t key = seg[lane];
+                sk = bitsFloat(static_cast<uint32_t>(key >> 32));
+                sv = static_cast<int>(static_cast<uint32_t>(key));
+            }
+            waveSortPairs(sk, sv, lane, w);
+            if (lane < found) seg[lane] = (static_cast<uint64_t>(floatBits(sk)) << 32) | static_cast<uint32_t>(sv);
+        }
         if (lane == 0) counts[q] = found;
         queryHits = found;
     }
@@ -1104,47 +1160,6 @@ __global__ __launch_bounds__(kBlock) void radiusWaveKernel(
 // nearest 16 without a sort (for Gaussians: sorted by box distance across the wave).  The admission rule is the lane kernels' (n < K or
 // bound <= K-th), so the list ends as the K best by (dist2, index) whatever the
 // walk order.
-template <typename V>
-__device__ __forceinline__ V waveXor(V v, int m) {
-    return __shfl_xor(v, m, 64);
-}
-
-// Bitonic sort of one (key, value) pair per lane, ascending by (key, value), over
-// aligned groups of `width` lanes (a power of two, at most 64; the default sorts
-// the whole wave).
-__device__ __forceinline__ void waveSortPairs(float& k, int& v, int lane, int width = 64) {
-    for (int size = 2; size <= width; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            const float ok = waveXor(k, stride);
-            const int ov = waveXor(v, stride);
-            const bool up = (lane & size) == 0 || size == width;
-            const bool low = (lane & stride) == 0;
-            const bool otherFirst = closer(ok, ov, k, v);
-            // the lower lane of an ascending pair keeps the smaller, the upper the larger
-            const bool take = (low == up) ? otherFirst : !otherFirst;
-            if (take) {
-                k = ok;
-                v = ov;
-            }
-        }
-    }
-}
-
-// Bitonic merge of a bitonic sequence (one pair per lane, aligned groups of `width`
-// lanes) into ascending order.
-__device__ __forceinline__ void waveMergePairs(float& k, int& v, int lane, int width = 64) {
-    for (int stride = width >> 1; stride > 0; stride >>= 1) {
-        const float ok = waveXor(k, stride);
-        const int ov = waveXor(v, stride);
-        const bool low = (lane & stride) == 0;
-        const bool otherFirst = closer(ok, ov, k, v);
-        if (low == otherFirst) {
-            k = ok;
-            v = ov;
-        }
-    }
-}
-
 template <typename L, int K, bool kStats>
 __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
     const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
@@ -2110,6 +2125,11 @@ int radiusCaptureCap(int count) {
     const long long tight = (512LL << 20) / static_cast<long long>(sizeof(uint64_t)) / count;
     if (tight >= 64) return 64;
     return tight < 8 ? 0 : static_cast<int>(tight);
+}
+
+bool radiusCapturePresorts(const TraverseArgs& a, int count) {
+    return radiusOnQuads(a) && ((count < gRadiusWaveBelow && (gTraversalMode == 0 || gTraversalMode == 4)) ||
+                                gTraversalMode == 3);
 }
 
 bool launchRadiusCapture(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts,
