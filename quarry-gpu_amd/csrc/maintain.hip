@@ -282,15 +282,22 @@ __global__ void checkDepthKernel(const Node2* __restrict__ nodes, const int* __r
 
 }  // namespace
 
-void buildWide(BvhState& b, hipStream_t s) {
+void reserveWide(BvhState& b) {
     if (b.n < 2) throw ArgError("buildWideNodes: tree is empty");
-    computeDepth(b, s);
     const int internal = b.n - 1;
+    b.depthArr.reserve(static_cast<size_t>(b.nodeCount));
+    b.jump.reserve(static_cast<size_t>(b.nodeCount));
     b.wideId.reserve(static_cast<size_t>(internal));
     b.wideFlags.reserve(static_cast<size_t>(internal));
     b.wideScan.reserve(static_cast<size_t>(scanScratchInts(internal)) + 1);
     // at most one wide node per internal node: no count round trip before the emit
     b.wide.reserve(static_cast<size_t>(internal));
+}
+
+void buildWide(BvhState& b, hipStream_t s) {
+    reserveWide(b);
+    computeDepth(b, s);
+    const int internal = b.n - 1;
     wideFlagKernel<<<blocksFor(internal, kBlock), kBlock, 0, s>>>(b.depthArr.p, internal, b.wideFlags.p);
     int* total = b.wideScan.p + scanScratchInts(internal);
     scanExclusive(b.wideFlags.p, b.wideId.p, internal, total, b.wideScan.p, s);

@@ -118,6 +118,9 @@ void composeEdit(BvhState& b, const int* dDel, int nDel, const int* dIns, int nI
                  const void* dStaged, const int* dMoved, int nMoved, const void* dStagedMoved, void* dOut,
                  hipStream_t s);
 
+// Allocates the buffers of the 4-wide collapse (depth table, flags, scan, wide nodes)
+// so that buildWide itself allocates nothing more.
+void reserveWide(BvhState& b);
 void buildWide(BvhState& b, hipStream_t s);
 void reemitWide(BvhState& b, hipStream_t s);
 void ensurePrimSlot(BvhState& b, hipStream_t s);
