@@ -533,6 +533,24 @@ struct HostTask {
     float cLo[3], cHi[3];
 };
 
+// Pinned host staging for the level loop's tables (tasks, chunks, chunk index,
+// results, error flag).  A copy from pageable memory goes through a driver bounce
+// buffer and the loop makes five per level; from pinned memory they are plain DMA.
+// One block per host thread, grown on demand and kept for the process (no copy is
+// in flight when it grows: every level ends with a stream synchronize).
+char* pinnedStage(size_t bytes) {
+    static thread_local char* p = nullptr;
+    static thread_local size_t cap = 0;
+    if (bytes > cap) {
+        if (p) hipCheck(hipHostFree(p), "sah stage free");
+        p = nullptr;
+        cap = 0;
+        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&p), bytes, hipHostMallocDefault), "sah stage");
+        cap = bytes;
+    }
+    return p;
+}
+
 }  // namespace
 
 template <typename L>
@@ -568,6 +586,23 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     DevBuf<int> dChunkLeft(static_cast<size_t>(maxChunks));
     DevBuf<int> dChunkOff(static_cast<size_t>(maxChunks));
     DevBuf<uint32_t> dBins(static_cast<size_t>(maxBig) * kHist);
+    size_t stageBytes = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t at = stageBytes;
+        stageBytes += (bytes + 255) & ~static_cast<size_t>(255);
+        return at;
+    };
+    const size_t oTasks = carve(sizeof(SahTask) * static_cast<size_t>(maxTasks));
+    const size_t oRes = carve(sizeof(SahResult) * static_cast<size_t>(maxTasks));
+    const size_t oChunks = carve(sizeof(Chunk) * static_cast<size_t>(maxChunks));
+    const size_t oFirst = carve(sizeof(int) * static_cast<size_t>(maxBig + 1));
+    const size_t oErr = carve(sizeof(int) * 8);
+    char* stage = pinnedStage(stageBytes);
+    SahTask* hTasks = reinterpret_cast<SahTask*>(stage + oTasks);
+    SahResult* hRes = reinterpret_cast<SahResult*>(stage + oRes);
+    Chunk* hChunks = reinterpret_cast<Chunk*>(stage + oChunks);
+    int* hFirst = reinterpret_cast<int*>(stage + oFirst);
+    int* hErr = reinterpret_cast<int*>(stage + oErr);
 
     EventSet<4> ev;
     ev.record(0, s);
@@ -605,14 +640,11 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
 
     double binMs = 0.0, splitMs = 0.0, partMs = 0.0;
     int cur = 0;
-    std::vector<SahTask> hTasks;
-    std::vector<Chunk> hChunks;
-    std::vector<int> hFirst;
-    std::vector<SahResult> hRes;
     for (int lvl = 0; !level.empty(); ++lvl) {
         next.clear();
         const int nt = static_cast<int>(level.size());
-        hRes.assign(static_cast<size_t>(nt), SahResult{});
+        if (nt > maxTasks) throw DeviceError("sah: task table overflow");
+        std::fill(hRes, hRes + nt, SahResult{});
         if (lvl >= kMedianLevel) {
             for (int i = 0; i < nt; ++i) {
                 SahResult& r = hRes[static_cast<size_t>(i)];
@@ -628,7 +660,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
             std::stable_partition(level.begin(), level.end(), [](const HostTask& h) { return h.count > kChunk; });
             int nBig = 0;
             while (nBig < nt && level[static_cast<size_t>(nBig)].count > kChunk) ++nBig;
-            hTasks.resize(static_cast<size_t>(nt));
+            if (nBig > maxBig) throw DeviceError("sah: task table overflow");
             for (int i = 0; i < nt; ++i) {
                 const HostTask& h = level[static_cast<size_t>(i)];
                 SahTask& d = hTasks[static_cast<size_t>(i)];
@@ -641,25 +673,24 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
                 }
                 d.cLo[3] = d.cHi[3] = 0.0f;
             }
-            hChunks.clear();
-            hFirst.assign(static_cast<size_t>(nBig + 1), 0);
+            int nChunks = 0;
             for (int i = 0; i < nBig; ++i) {
-                hFirst[static_cast<size_t>(i)] = static_cast<int>(hChunks.size());
+                hFirst[i] = nChunks;
                 const HostTask& h = level[static_cast<size_t>(i)];
-                for (int off = 0; off < h.count; off += kChunk)
-                    hChunks.push_back({i, h.start + off, std::min(kChunk, h.count - off), 0});
+                for (int off = 0; off < h.count; off += kChunk) {
+                    if (nChunks >= maxChunks) throw DeviceError("sah: task table overflow");
+                    hChunks[nChunks++] = {i, h.start + off, std::min(kChunk, h.count - off), 0};
+                }
             }
-            hFirst[static_cast<size_t>(nBig)] = static_cast<int>(hChunks.size());
-            const int nChunks = static_cast<int>(hChunks.size());
-            if (nt > maxTasks || nBig > maxBig || nChunks > maxChunks) throw DeviceError("sah: task table overflow");
-            hipCheck(hipMemcpyAsync(dTasks.p, hTasks.data(), sizeof(SahTask) * static_cast<size_t>(nt),
+            hFirst[nBig] = nChunks;
+            hipCheck(hipMemcpyAsync(dTasks.p, hTasks, sizeof(SahTask) * static_cast<size_t>(nt),
                                     hipMemcpyHostToDevice, s), "sah tasks");
             EventSet<4> lev;
             lev.record(0, s);
             if (nBig > 0) {
-                hipCheck(hipMemcpyAsync(dChunks.p, hChunks.data(), sizeof(Chunk) * static_cast<size_t>(nChunks),
+                hipCheck(hipMemcpyAsync(dChunks.p, hChunks, sizeof(Chunk) * static_cast<size_t>(nChunks),
                                         hipMemcpyHostToDevice, s), "sah chunks");
-                hipCheck(hipMemcpyAsync(dFirst.p, hFirst.data(), sizeof(int) * static_cast<size_t>(nBig + 1),
+                hipCheck(hipMemcpyAsync(dFirst.p, hFirst, sizeof(int) * static_cast<size_t>(nBig + 1),
                                         hipMemcpyHostToDevice, s), "sah chunk index");
                 sahClearBinsKernel<<<blocksFor(static_cast<long long>(nBig) * kHist, kBlock), kBlock, 0, s>>>(dBins.p, nBig);
                 sahBinKernel<<<nChunks, kBlock, 0, s>>>(rec[cur].p, dTasks.p, dChunks.p, dBins.p);
@@ -677,12 +708,11 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
             }
             hipCheck(hipGetLastError(), "sah level");
             lev.record(3, s);
-            hipCheck(hipMemcpyAsync(hRes.data(), dRes.p, sizeof(SahResult) * static_cast<size_t>(nt),
+            hipCheck(hipMemcpyAsync(hRes, dRes.p, sizeof(SahResult) * static_cast<size_t>(nt),
                                     hipMemcpyDeviceToHost, s), "sah results");
-            int herr = 0;
-            hipCheck(hipMemcpyAsync(&herr, errDev, sizeof(int), hipMemcpyDeviceToHost, s), "sah check");
+            hipCheck(hipMemcpyAsync(hErr, errDev, sizeof(int), hipMemcpyDeviceToHost, s), "sah check");
             hipCheck(hipStreamSynchronize(s), "sah level sync");
-            if (herr != 0) throw DeviceError("sah: partition counts disagree with the bin histogram");
+            if (*hErr != 0) throw DeviceError("sah: partition counts disagree with the bin histogram");
             binMs += lev.ms(0, 1);
             splitMs += lev.ms(1, 2);
             partMs += lev.ms(2, 3);
