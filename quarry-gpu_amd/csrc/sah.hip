@@ -105,20 +105,73 @@ __device__ __forceinline__ void histInit(uint32_t* h) {
     }
 }
 
-__device__ __forceinline__ void histAdd(uint32_t* h, const PrimRec& r, const AxisMap& m) {
+// Bins of a contiguous range of records.  Each thread takes a run of consecutive
+// records and keeps, per axis, the values of its current bin in registers, flushing
+// them to the LDS histogram with 13 atomics only when the bin changes: neighbouring
+// records (a mesh in scene order, the halves of a partition) mostly share a bin,
+// and one-record-per-lane atomics on one shared bin serialize across the wave.
+// Count, min and max are exact and order-free, so the histogram is the same bits as
+// one histAdd per record.
+struct BinRun {
+    int bin;
+    uint32_t cnt;
+    uint32_t v[12];  // box lo xyz, box hi xyz, centroid lo xyz, centroid hi xyz
+};
+
+__device__ __forceinline__ void binRunFlush(uint32_t* h, int a, const BinRun& r) {
+    if (r.cnt == 0u) return;
+    atomicAdd(&h[hidx(a, r.bin, 0)], r.cnt);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        atomicMin(&h[hidx(a, r.bin, 1 + k)], r.v[k]);
+        atomicMax(&h[hidx(a, r.bin, 4 + k)], r.v[3 + k]);
+        atomicMin(&h[hidx(a, r.bin, 7 + k)], r.v[6 + k]);
+        atomicMax(&h[hidx(a, r.bin, 10 + k)], r.v[9 + k]);
+    }
+}
+
+__device__ __forceinline__ void histAddRange(uint32_t* h, const PrimRec* __restrict__ rec, int count,
+                                             const AxisMap& m) {
+    const int per = (count + kBlock - 1) / kBlock;
+    const int beg = threadIdx.x * per;
+    const int end = beg + per < count ? beg + per : count;
+    BinRun run[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (!m.use[a]) continue;
-        const int b = binOf(r.c[a], m.lo[a], m.scale[a]);
-        atomicAdd(&h[hidx(a, b, 0)], 1u);
+        run[a].bin = -1;
+        run[a].cnt = 0u;
+    }
+    for (int i = beg; i < end; ++i) {
+        const PrimRec r = rec[i];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            atomicMin(&h[hidx(a, b, 1 + k)], orderedBits(r.lo[k]));
-            atomicMax(&h[hidx(a, b, 4 + k)], orderedBits(r.hi[k]));
-            atomicMin(&h[hidx(a, b, 7 + k)], orderedBits(r.c[k]));
-            atomicMax(&h[hidx(a, b, 10 + k)], orderedBits(r.c[k]));
+        for (int a = 0; a < 3; ++a) {
+            if (!m.use[a]) continue;
+            const int b = binOf(r.c[a], m.lo[a], m.scale[a]);
+            BinRun& u = run[a];
+            if (b != u.bin) {
+                binRunFlush(h, a, u);
+                u.bin = b;
+                u.cnt = 0u;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    u.v[k] = u.v[6 + k] = 0xFFFFFFFFu;
+                    u.v[3 + k] = u.v[9 + k] = 0u;
+                }
+            }
+            ++u.cnt;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t lo = orderedBits(r.lo[k]), hi = orderedBits(r.hi[k]), c = orderedBits(r.c[k]);
+                u.v[k] = lo < u.v[k] ? lo : u.v[k];
+                u.v[3 + k] = hi > u.v[3 + k] ? hi : u.v[3 + k];
+                u.v[6 + k] = c < u.v[6 + k] ? c : u.v[6 + k];
+                u.v[9 + k] = c > u.v[9 + k] ? c : u.v[9 + k];
+            }
         }
     }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (m.use[a]) binRunFlush(h, a, run[a]);
 }
 
 __device__ __forceinline__ float areaOf(float lx, float ly, float lz, float hx, float hy, float hz) {
@@ -320,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void sahBinKernel(const PrimRec* __restrict
     const AxisMap m = axisMap(t);
     histInit(h);
     __syncthreads();
-    for (int i = threadIdx.x; i < ch.count; i += kBlock) histAdd(h, rec[ch.start + i], m);
+    histAddRange(h, rec + ch.start, ch.count, m);
     __syncthreads();
     uint32_t* g = bins + static_cast<size_t>(ch.task) * kHist;
     for (int i = threadIdx.x; i < kHist; i += kBlock) {
@@ -433,7 +486,7 @@ __global__ __launch_bounds__(kBlock) void sahLocalKernel(const PrimRec* __restri
     const AxisMap m = axisMap(t);
     histInit(h);
     __syncthreads();
-    for (int i = threadIdx.x; i < t.count; i += kBlock) histAdd(h, rec[t.start + i], m);
+    histAddRange(h, rec + t.start, t.count, m);
     __syncthreads();
     if (threadIdx.x < 64) chooseSplit(h, t, m, &sres);
     __syncthreads();
