@@ -120,8 +120,13 @@ def main() -> int:
     sah = A.Bvh.from_device(A.PRIM_TRI, dtris.ptr.value, n, A.BUILDER_SAH)
     sah_build_ms = sah.timings.total_ms
     sah_wide_ms = sah.build_wide()
-    log(f"SAH build: {sah_build_ms:.3f} ms (bin {sah.timings.sah_bin_ms:.3f}, split {sah.timings.sah_split_ms:.3f}, "
-        f"partition {sah.timings.sah_part_ms:.3f}); depth {sah.info().depth}; wide collapse {sah_wide_ms:.3f} ms")
+    st = sah.timings
+    sah_parts = {"prep_ms": round(st.morton_ms, 4), "levels_ms": round(st.tree_ms, 4),
+                 "bin_ms": round(st.sah_bin_ms, 4), "split_ms": round(st.sah_split_ms, 4),
+                 "partition_ms": round(st.sah_part_ms, 4), "leaves_boxes_ms": round(st.refit_ms, 4)}
+    log(f"SAH build: {sah_build_ms:.3f} ms (prep {st.morton_ms:.3f}, levels {st.tree_ms:.3f} of which bin "
+        f"{st.sah_bin_ms:.3f}, split {st.sah_split_ms:.3f}, partition {st.sah_part_ms:.3f}; leaves+boxes "
+        f"{st.refit_ms:.3f}); depth {sah.info().depth}; wide collapse {sah_wide_ms:.3f} ms")
     serve = sah if args.builder == "sah" else lbvh
 
     pb1 = A.PointBuffer(queries, 1)
@@ -240,7 +245,7 @@ def main() -> int:
     build = {"ms": round(build_ms, 4), "morton_ms": round(bt.morton_ms, 4),
              "sort_ms": round(bt.sort_ms, 4), "tree_ms": round(bt.tree_ms, 4),
              "leaves_boxes_ms": round(bt.refit_ms, 4), "wide_ms": round(wide_ms, 4),
-             "depth": info.depth, "sah_ms": round(sah_build_ms, 4), "sah_wide_ms": round(sah_wide_ms, 4),
+             "depth": info.depth, "sah_ms": round(sah_build_ms, 4), "sah_phases": sah_parts, "sah_wide_ms": round(sah_wide_ms, 4),
              "roofline": {"bound": "hbm", "achieved": round(build_gbs, 2), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
                           "alg_bytes": build_alg}}

@@ -337,7 +337,28 @@ __global__ __launch_bounds__(kBlock) void sahPrepKernel(const typename L::Item* 
         hi1 = fmaxf(hi1, __shfl_xor(hi1, off, 64));
         hi2 = fmaxf(hi2, __shfl_xor(hi2, off, 64));
     }
+    // the workgroup's waves meet in LDS: one set of six atomics per workgroup (all
+    // waves on six words serialize at the L2)
+    __shared__ float wb[kBlock / 64][6];
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        wb[w][0] = lo0;
+        wb[w][1] = lo1;
+        wb[w][2] = lo2;
+        wb[w][3] = hi0;
+        wb[w][4] = hi1;
+        wb[w][5] = hi2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; ++k) {
+            lo0 = fminf(lo0, wb[k][0]);
+            lo1 = fminf(lo1, wb[k][1]);
+            lo2 = fminf(lo2, wb[k][2]);
+            hi0 = fmaxf(hi0, wb[k][3]);
+            hi1 = fmaxf(hi1, wb[k][4]);
+            hi2 = fmaxf(hi2, wb[k][5]);
+        }
         atomicMin(&cb[0], orderedBits(lo0));
         atomicMin(&cb[1], orderedBits(lo1));
         atomicMin(&cb[2], orderedBits(lo2));
@@ -650,22 +671,23 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     const size_t oChunks = carve(sizeof(Chunk) * static_cast<size_t>(maxChunks));
     const size_t oFirst = carve(sizeof(int) * static_cast<size_t>(maxBig + 1));
     const size_t oErr = carve(sizeof(int) * 8);
+    const size_t oBounds = carve(sizeof(uint32_t) * 6);
     char* stage = pinnedStage(stageBytes);
     SahTask* hTasks = reinterpret_cast<SahTask*>(stage + oTasks);
     SahResult* hRes = reinterpret_cast<SahResult*>(stage + oRes);
     Chunk* hChunks = reinterpret_cast<Chunk*>(stage + oChunks);
     int* hFirst = reinterpret_cast<int*>(stage + oFirst);
     int* hErr = reinterpret_cast<int*>(stage + oErr);
+    uint32_t* cbh = reinterpret_cast<uint32_t*>(stage + oBounds);
 
     EventSet<4> ev;
     ev.record(0, s);
     sahResetKernel<<<blocksFor(n - 1 > 256 ? n - 1 : 256, kBlock), kBlock, 0, s>>>(cb, b.arrive.p, n - 1);
     int grid = blocksFor(n, kBlock * 4);
-    if (grid > 2048) grid = 2048;
+    if (grid > 512) grid = 512;  // two workgroups per CU stream the items; few bounds atomics
     sahPrepKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, rec[0].p, cb, b.parent.p);
     hipCheck(hipGetLastError(), "sah prep");
-    uint32_t cbh[6];
-    hipCheck(hipMemcpyAsync(cbh, cb, sizeof(cbh), hipMemcpyDeviceToHost, s), "sah bounds");
+    hipCheck(hipMemcpyAsync(cbh, cb, sizeof(uint32_t) * 6, hipMemcpyDeviceToHost, s), "sah bounds");
     ev.record(1, s);
     ev.sync(1);
 
