@@ -203,6 +203,10 @@ struct SortWorkspace {
 constexpr int kSortTileSmall = 2048;
 constexpr int kSortTileLarge = 4096;
 constexpr size_t kSortLargeFrom = size_t(1) << 20;
+// 8,192 keys (32 per thread) from 2M keys on: half the tiles again on the look-back
+// walk, two workgroups per CU by LDS.
+constexpr int kSortTileHuge = 8192;
+constexpr size_t kSortHugeFrom = size_t(1) << 21;
 
 int sortTiles(size_t n);
 

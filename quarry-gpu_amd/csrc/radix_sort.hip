@@ -331,7 +331,9 @@ __global__ void scanApplyKernel(const int* __restrict__ in, int n, const int* __
 
 }  // namespace
 
-static size_t sortTileKeys(size_t n) { return n < kSortLargeFrom ? kSortTileSmall : kSortTileLarge; }
+static size_t sortTileKeys(size_t n) {
+    return n < kSortLargeFrom ? kSortTileSmall : n < kSortHugeFrom ? kSortTileLarge : kSortTileHuge;
+}
 
 int sortTiles(size_t n) { return static_cast<int>((n + sortTileKeys(n) - 1) / sortTileKeys(n)); }
 
@@ -385,8 +387,12 @@ void sortPairs(SortWorkspace& ws, const uint32_t* keys, const uint32_t* vals, ui
             onesweepKernel<kSortTileSmall / kBlock><<<tiles, kBlock, 0, s>>>(
                 srcK, srcV, dstK, dstV, static_cast<long long>(n), 8 * p, ws.hist.p + p * 256, desc,
                 ws.tickets.p + p);
-        else
+        else if (sortTileKeys(n) == kSortTileLarge)
             onesweepKernel<kSortTileLarge / kBlock><<<tiles, kBlock, 0, s>>>(
+                srcK, srcV, dstK, dstV, static_cast<long long>(n), 8 * p, ws.hist.p + p * 256, desc,
+                ws.tickets.p + p);
+        else
+            onesweepKernel<kSortTileHuge / kBlock><<<tiles, kBlock, 0, s>>>(
                 srcK, srcV, dstK, dstV, static_cast<long long>(n), 8 * p, ws.hist.p + p * 256, desc,
                 ws.tickets.p + p);
         hipCheck(hipGetLastError(), "onesweep");

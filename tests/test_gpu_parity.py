@@ -43,7 +43,9 @@ def random_gaussians(rng, n, spread=30.0):
 
 # ---- build --------------------------------------------------------------------------
 
-@pytest.mark.parametrize("kind,n", [(0, 2), (0, 3), (0, 5000), (1, 7000), (2, 3000)])
+# (1, 2**21 + 12345): the sort's 8,192-key tiles (from 2M keys on), with a partial
+# last tile and many equal Morton codes ordered by index
+@pytest.mark.parametrize("kind,n", [(0, 2), (0, 3), (0, 5000), (1, 7000), (2, 3000), (1, (1 << 21) + 12345)])
 def test_node_bytes_match_cpu_lbvh(A, orc, scenes, kind, n):
     log(f"lbvh kind={kind} n={n}")
     rng = np.random.default_rng(n + kind)
