@@ -215,6 +215,12 @@ def main() -> int:
             ks.append(A.last_query_stats().kernel_ms)
         binary[name] = round(statistics.median(ks), 4)
     log(f"binary-node traversal kernels (ms): {binary}")
+    # the binary pass's results, compared with the oracle below
+    pb1.closest(bvh, wide=False)
+    bin_c = pb1.read()
+    pbk.knn(bvh, args.k, wide=False)
+    bin_k = pbk.read()
+    _, bin_o, bin_keys = pb1.radius(bvh, args.radius, wide=False, keep=True)
 
     # ---- roofline of the dominant kernel ------------------------------------------
     dom = max(per_kernel, key=per_kernel.get)
@@ -279,6 +285,8 @@ def main() -> int:
         parity["closest"] = bool(np.array_equal(res_c, exp_c))
         parity["knn"] = bool(np.array_equal(res_k, exp_k))
         parity["radius"] = bool(np.array_equal(offs, exp_o) and np.array_equal(keys, exp_keys))
+        parity["binary"] = bool(np.array_equal(bin_c, exp_c) and np.array_equal(bin_k, exp_k)
+                                and np.array_equal(bin_o, exp_o) and np.array_equal(bin_keys, exp_keys))
         fp = {"closest": hex(O.closest_hash(res_c)[0]), "knn": hex(O.knn_hash(res_k, args.k)[0]),
               "radius": hex(O.csr_hash(offs, keys)[0])}
         log(f"parity {parity} ({time.time() - t0:.1f}s); fingerprints {fp}")
@@ -286,20 +294,37 @@ def main() -> int:
         del cb
 
         # ---- dirty refit after moving 1% of the vertices ---------------------------
+        # Closest, kNN and radius on binary and 4-wide nodes of both trees after the
+        # dirty refit, against the CPU BVH of the moved scene.
         moved, dirty = S.displace_vertices(args.cells, tris, 0.01, 0.05, seed=4)
         rt = lbvh.refit(moved, dirty)
         why = lbvh.validate(moved)
-        pb1.closest(lbvh)
-        res_rc = pb1.read()
+        rt_sah = sah.refit(moved, dirty)
+        why_sah = sah.validate(moved)
         cb2 = O.CpuBvh(0, moved)
-        parity["refit_closest"] = bool(np.array_equal(res_rc, cb2.knn(queries, 0, threads)))
-        parity["refit_validate"] = why == ""
+        mv_c, mv_k = cb2.knn(queries, 0, threads), cb2.knn(queries, args.k, threads)
+        mv_o, mv_keys = cb2.radius(queries, args.radius, threads)
         del cb2
+        rc = rk = rr = True
+        for tree in (lbvh, sah):
+            for wide in (False, True):
+                pb1.closest(tree, wide=wide)
+                rc &= bool(np.array_equal(pb1.read(), mv_c))
+                pbk.knn(tree, args.k, wide=wide)
+                rk &= bool(np.array_equal(pbk.read(), mv_k))
+                _, o2, k2 = pb1.radius(tree, args.radius, wide=wide, keep=True)
+                rr &= bool(np.array_equal(o2, mv_o) and np.array_equal(k2, mv_keys))
+        parity["refit_closest"] = rc
+        parity["refit_knn"] = rk
+        parity["refit_radius"] = rr
+        parity["refit_validate"] = why == "" and why_sah == ""
         extra["refit"] = {"dirty_tris": int(rt.updated), "upload_ms": round(rt.upload_ms, 4),
                           "mark_ms": round(rt.mark_ms, 4), "merge_ms": round(rt.merge_ms, 4),
-                          "wide_ms": round(rt.wide_ms, 4), "total_ms": round(rt.total_ms, 4)}
-        log(f"refit: {extra['refit']} validate={why!r}")
+                          "wide_ms": round(rt.wide_ms, 4), "total_ms": round(rt.total_ms, 4),
+                          "sah_tree_total_ms": round(rt_sah.total_ms, 4)}
+        log(f"refit: {extra['refit']} validate={why!r} {why_sah!r}")
         lbvh.refit(tris)
+        sah.refit(tris)
         log("refit back to the original scene done")
 
         # ---- rays: 1M down rays on the 256 x 256 terrain ----------------------------
@@ -317,7 +342,8 @@ def main() -> int:
         parity["rays"] = bool(np.array_equal(got, exp_r))
         ray_ms = statistics.median(rms)
         extra["rays_c1"] = {"mrays_per_s": round(rays.shape[0] / (ray_ms * 1e-3) / 1e6, 2),
-                            "ms": round(ray_ms, 4), "hits": int(rs.hits), "hash": hex(rs.hash),
+                            "ms": round(ray_ms, 4), "hits": int(rs.hits), "sum_t": round(rs.sum_t, 6),
+                            "hash": hex(rs.hash),
                             "build_ms": round(b1.timings.total_ms, 4)}
         log(f"rays: {extra['rays_c1']} parity={parity['rays']}")
         del cbr, rb, b1

@@ -132,6 +132,29 @@ def ref():
     return _ref
 
 
+LEARN_LIB = os.path.join(HERE, "_ref", "liblbvh_learn.so")
+_learn = None
+
+
+def learn_lbvh(tris):
+    """The reference's LBVH walkthrough (learn/03_lbvh.cpp, compiled by `make ref`)
+    on triangles: (leaf order, left (n-1), right (n-1), depth), or None when that
+    build is absent."""
+    global _learn
+    if _learn is None:
+        if not os.path.exists(LEARN_LIB):
+            return None
+        L = ctypes.CDLL(LEARN_LIB)
+        L.learn_lbvh.argtypes = [_f32p, ctypes.c_int, _i32p, _i32p, _i32p]
+        L.learn_lbvh.restype = ctypes.c_int
+        _learn = L
+    t = _c32(tris).reshape(-1, 9)
+    n = t.shape[0]
+    order, left, right = (np.zeros(max(m, 1), np.int32) for m in (n, n - 1, n - 1))
+    depth = _learn.learn_lbvh(_ptr(t, _f32p), n, _ptr(order, _i32p), _ptr(left, _i32p), _ptr(right, _i32p))
+    return order[:n], left[:n - 1], right[:n - 1], depth
+
+
 def _c32(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float32)
 

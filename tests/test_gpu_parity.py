@@ -177,17 +177,33 @@ def test_gaussian_queries_vs_brute(A, orc, scenes):
     check_queries(A, orc, 2, g, q, 1.5, ks=(1, 8, 32))
 
 
-def test_c2_full_size(A, orc, scenes):
-    tris, q = scenes.workload_c2(seed=1)
-    log("C2: 2M triangles")
-    b = A.Bvh(0, tris)
-    b.build_wide()
-    ref = orc.CpuBvh(0, tris)
-    assert eq(b.closest(q, order=A.ORDER_MORTON, wide=True)[0], ref.knn(q, 0, THREADS))
-    assert eq(b.knn(q, 16, order=A.ORDER_MORTON, wide=True)[0], ref.knn(q, 16, THREADS))
-    o, k, _, _ = b.radius(q, 2.0, order=A.ORDER_MORTON, wide=True)
+def _c2_pass(A, b, ref, q, what):
+    """closest, kNN16 and radius 2.0 on binary and on 4-wide nodes, Morton order."""
+    want_c, want_k = ref.knn(q, 0, THREADS), ref.knn(q, 16, THREADS)
     wo, wk = ref.radius(q, 2.0, THREADS)
-    assert eq(o, wo) and eq(k, wk)
+    for wide in (False, True):
+        log(f"C2 {what}: wide={wide}")
+        assert eq(b.closest(q, order=A.ORDER_MORTON, wide=wide)[0], want_c), (what, "closest", wide)
+        assert eq(b.knn(q, 16, order=A.ORDER_MORTON, wide=wide)[0], want_k), (what, "knn16", wide)
+        o, k, _, _ = b.radius(q, 2.0, order=A.ORDER_MORTON, wide=wide)
+        assert eq(o, wo) and eq(k, wk), (what, "radius", wide)
+
+
+@pytest.mark.parametrize("builder", [0, 1])
+def test_c2_full_size(A, orc, scenes, builder):
+    """The whole 2M-triangle terrain workload: every result set of the query pass on
+    binary and 4-wide nodes, then again after moving 1% of the grid vertices by
+    N(0, 0.05) and a dirty refit, against the CPU BVH of the moved scene."""
+    tris, q = scenes.workload_c2(seed=1)
+    log(f"C2: 2M triangles, builder {builder}")
+    b = A.Bvh(0, tris, builder)
+    b.build_wide()
+    _c2_pass(A, b, orc.CpuBvh(0, tris), q, "as built")
+    moved, dirty = scenes.displace_vertices(1024, tris, 0.01, 0.05, seed=4)
+    t = b.refit(moved, dirty)
+    assert t.updated == len(np.unique(dirty))
+    assert b.validate(moved) == ""
+    _c2_pass(A, b, orc.CpuBvh(0, moved), q, "after the dirty refit")
 
 
 def test_golden_fixtures(A):

@@ -6,6 +6,7 @@ flips, parent boxes are the bitwise compare-select merge of the children,
 leaves hold their primitive's box, and the permutation sorts the Morton keys.
 """
 import numpy as np
+import pytest
 
 
 def morton_keys(cen, lo, hi):
@@ -68,3 +69,107 @@ def test_lbvh_structure(orc, scenes):
         assert np.array_equal(nodes[i, 3:6].view(np.uint32), want_hi.view(np.uint32))
     assert seen[0] == 0 and np.all(seen[1:] == 1)
     assert 10 <= depth <= 64
+
+
+def _links_depth(raw, n):
+    links = raw.view(np.int32).reshape(-1, 8)[:, 6:8]
+    return links[: n - 1, 0], links[: n - 1, 1]
+
+
+def _karras_rule(left, right, n):
+    """Karras 2012's numbering: the root is 0 and covers every slot; an internal
+    left child is numbered by its range's last slot, an internal right child by its
+    range's first slot; leaves are n - 1 + slot."""
+    ranges = {0: (0, n - 1)}
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        a, b = ranges[i]
+        l, r = int(left[i]), int(right[i])
+        # the children's ranges follow from the split: left [a, s], right [s + 1, b]
+        s = l - (n - 1) if l >= n - 1 else l       # left id is its range's last slot
+        assert a <= s < b
+        rs = r - (n - 1) if r >= n - 1 else r      # right id is its range's first slot
+        assert rs == s + 1, (i, a, b, l, r)
+        if l >= n - 1:
+            assert s == a                          # a leaf child covers one slot
+        else:
+            ranges[l] = (a, s)
+            stack.append(l)
+        if r >= n - 1:
+            assert rs == b
+        else:
+            ranges[r] = (rs, b)
+            stack.append(r)
+    assert len(ranges) == n - 1                    # every internal node reached once
+
+
+def _items(kind, pts, rng):
+    if kind == 0:
+        return np.concatenate([pts, pts, pts], 1).astype(np.float32)
+    if kind == 1:
+        return pts.astype(np.float32)
+    g = np.concatenate([pts, rng.normal(size=(pts.shape[0], 4)),
+                        np.exp(rng.uniform(-3, 0, (pts.shape[0], 3)))], 1)
+    return g.astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_karras_numbering_all_kinds(orc, scenes, kind):
+    rng = np.random.default_rng(90 + kind)
+    pts = rng.uniform(-20, 20, (3000, 3))
+    pts[500:700] = pts[3]                          # equal codes, ordered by index
+    items = _items(kind, pts, rng)
+    raw, ids, depth = orc.lbvh_nodes(kind, items)
+    n = items.shape[0]
+    left, right = _links_depth(raw, n)
+    _karras_rule(left, right, n)
+
+
+def _dyadic_points(rng, n):
+    """Points whose coordinates have few significant bits, so that the triangle
+    (p, p, p) has centroid ((p + p) + p) / 3 == p exactly in FP32."""
+    return (rng.integers(-4096, 4096, (n, 3)) / 64.0).astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def learn(orc):
+    if orc.learn_lbvh(np.zeros((2, 9), np.float32)) is None:
+        pytest.skip("the reference walkthrough build (oracle/_ref) is not present")
+    return orc.learn_lbvh
+
+
+def test_lbvh_links_equal_reference_walkthrough_triangles(orc, scenes, learn):
+    """Leaf order, child ids and depth of the CPU LBVH equal the reference's own
+    LBVH walkthrough (learn/03_lbvh.cpp) on terrain, random and duplicate triangles."""
+    rng = np.random.default_rng(77)
+    a = rng.uniform(-50, 50, (20000, 3))
+    rand = np.concatenate([a, a + rng.uniform(-5, 5, (20000, 3)), a + rng.uniform(-5, 5, (20000, 3))], 1)
+    dup = np.repeat(rand[:300], 23, axis=0)
+    for tris in (scenes.terrain(64, 3), rand.astype(np.float32), dup.astype(np.float32)):
+        n = tris.shape[0]
+        raw, ids, depth = orc.lbvh_nodes(0, tris)
+        order, left, right, d = learn(tris)
+        l2, r2 = _links_depth(raw, n)
+        assert np.array_equal(ids, order)
+        assert np.array_equal(l2, left) and np.array_equal(r2, right)
+        assert depth == d
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_lbvh_links_equal_reference_walkthrough_points_gaussians(orc, learn, kind):
+    """Points and Gaussian means take the triangle path's keys when each is posed as
+    the triangle (p, p, p), whose centroid is p exactly for these coordinates."""
+    rng = np.random.default_rng(80 + kind)
+    pts = _dyadic_points(rng, 15000)
+    pts[100:300] = pts[9]
+    tris = np.concatenate([pts, pts, pts], 1).astype(np.float32)
+    t = tris.reshape(-1, 3, 3)
+    assert np.array_equal(((t[:, 0] + t[:, 1]) + t[:, 2]) / np.float32(3.0), pts)
+    items = _items(kind, pts.astype(np.float64), rng)
+    raw, ids, depth = orc.lbvh_nodes(kind, items)
+    order, left, right, d = learn(tris)
+    l2, r2 = _links_depth(raw, pts.shape[0])
+    assert np.array_equal(ids, order)
+    assert np.array_equal(l2, left) and np.array_equal(r2, right)
+    assert depth == d
