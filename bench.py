@@ -119,8 +119,10 @@ def main() -> int:
     sah.close()  # the first SAH build also sizes its workspaces; time the second
     sah = A.Bvh.from_device(A.PRIM_TRI, dtris.ptr.value, n, A.BUILDER_SAH)
     sah_build_ms = sah.timings.total_ms
+    sah_work = A.last_sah_stats()
     sah_wide_ms = sah.build_wide()
     st = sah.timings
+    sah_levels_ms = st.tree_ms
     sah_parts = {"prep_ms": round(st.morton_ms, 4), "levels_ms": round(st.tree_ms, 4),
                  "bin_ms": round(st.sah_bin_ms, 4), "split_ms": round(st.sah_split_ms, 4),
                  "partition_ms": round(st.sah_part_ms, 4), "leaves_boxes_ms": round(st.refit_ms, 4)}
@@ -248,10 +250,23 @@ def main() -> int:
     # Build roofline: 5S + 232 bytes per primitive (S = 36 for triangles).
     build_alg = n * (5 * TRI_BYTES + 232)
     build_gbs = build_alg / (build_ms * 1e-3) / 1e9
+    # SAH split loop: (S + 8) bytes per slot binned and 16 per slot partitioned, summed
+    # over the levels the build ran (slot counts from the build's own level tables),
+    # over the levels' time.  The implementation moves 40-byte primitive records
+    # instead (40 per slot binned, 80 per slot partitioned): record_bytes.
+    sah_alg = sah_work.binned_slots * (TRI_BYTES + 8) + sah_work.partitioned_slots * 16
+    sah_gbs = sah_alg / (sah_levels_ms * 1e-3) / 1e9
+    sah_roof = {"bound": "hbm", "achieved": round(sah_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(sah_gbs / HBM_PEAK_GBS, 4), "alg_bytes": int(sah_alg), "levels": sah_work.levels,
+                "binned_slots": int(sah_work.binned_slots), "partitioned_slots": int(sah_work.partitioned_slots),
+                "serial_slots": int(sah_work.serial_slots),
+                "record_bytes": int(sah_work.binned_slots * 40 + sah_work.partitioned_slots * 80),
+                "levels_ms": round(sah_levels_ms, 4)}
     build = {"ms": round(build_ms, 4), "morton_ms": round(bt.morton_ms, 4),
              "sort_ms": round(bt.sort_ms, 4), "tree_ms": round(bt.tree_ms, 4),
              "leaves_boxes_ms": round(bt.refit_ms, 4), "wide_ms": round(wide_ms, 4),
-             "depth": info.depth, "sah_ms": round(sah_build_ms, 4), "sah_phases": sah_parts, "sah_wide_ms": round(sah_wide_ms, 4),
+             "depth": info.depth, "sah_ms": round(sah_build_ms, 4), "sah_phases": sah_parts,
+             "sah_roofline": sah_roof, "sah_wide_ms": round(sah_wide_ms, 4),
              "roofline": {"bound": "hbm", "achieved": round(build_gbs, 2), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
                           "alg_bytes": build_alg}}
@@ -318,13 +333,26 @@ def main() -> int:
         parity["refit_knn"] = rk
         parity["refit_radius"] = rr
         parity["refit_validate"] = why == "" and why_sah == ""
+        refit_roof = refit_roofline(A, lbvh, dirty, n, rt)
         extra["refit"] = {"dirty_tris": int(rt.updated), "upload_ms": round(rt.upload_ms, 4),
                           "mark_ms": round(rt.mark_ms, 4), "merge_ms": round(rt.merge_ms, 4),
                           "wide_ms": round(rt.wide_ms, 4), "total_ms": round(rt.total_ms, 4),
-                          "sah_tree_total_ms": round(rt_sah.total_ms, 4)}
+                          "sah_tree_total_ms": round(rt_sah.total_ms, 4), "roofline": refit_roof}
         log(f"refit: {extra['refit']} validate={why!r} {why_sah!r}")
         lbvh.refit(tris)
         sah.refit(tris)
+        # full refit from device-resident scene-order items (no upload): every leaf
+        # re-gathered and every internal box merged, 2S + 28 per leaf + 84 per node
+        full = {}
+        for name, tree in (("lbvh", lbvh), ("sah", sah)):
+            ms = [tree.refit_device(dtris.ptr.value, n) for _ in range(6)][1:]
+            fms = statistics.median(ms)
+            alg = n * (2 * TRI_BYTES + 28) + (n - 1) * 84
+            full[name] = {"ms": round(fms, 4), "alg_bytes": alg, "achieved": round(alg / (fms * 1e-3) / 1e9, 2),
+                          "frac": round(alg / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        extra["refit_full_device"] = full
+        parity["refit_full_validate"] = lbvh.validate(tris) == "" and sah.validate(tris) == ""
+        log(f"full refit (device-resident items): {full}")
         log("refit back to the original scene done")
 
         # ---- rays: 1M down rays on the 256 x 256 terrain ----------------------------
@@ -401,6 +429,41 @@ def main() -> int:
     out.update(extra)
     print(json.dumps(out), flush=True)
     return 0 if all(parity.values()) else 3
+
+
+def refit_roofline(A, tree, dirty, n, rt):
+    """Bytes of a dirty refit (2S + 28 per dirty leaf + 84 per internal node merged)
+    over the refit's device time after the upload.  The internal nodes merged are the
+    ancestors of the dirty leaves, counted here from the tree's node array; the path
+    is the library's rule: marked ancestors only while at most 1 leaf in 256 is
+    dirty, else the whole bottom-up pass (every leaf and internal node)."""
+    nodes, ids = tree.read()
+    links = nodes.view(np.int32).reshape(-1, 8)[:, 6:8]
+    parent = np.full(2 * n - 1, -1, np.int64)
+    internal = np.arange(n - 1)
+    parent[links[: n - 1, 0]] = internal
+    parent[links[: n - 1, 1]] = internal
+    slot = np.empty(n, np.int64)
+    slot[ids] = np.arange(n)
+    seen = np.zeros(n - 1, bool)
+    cur = np.unique(parent[n - 1 + slot[np.unique(dirty)]])
+    cur = cur[cur >= 0]
+    while cur.size:
+        cur = cur[~seen[cur]]
+        seen[cur] = True
+        cur = np.unique(parent[cur])
+        cur = cur[cur >= 0]
+    marked = int(seen.sum())
+    m = int(np.unique(dirty).size)
+    dense = m * 256 > n
+    alg = m * (2 * TRI_BYTES + 28) + marked * 84
+    done = m * (2 * TRI_BYTES + 8) + (n * (TRI_BYTES + 32) + (n - 1) * 84 if dense else marked * 84)
+    dev_ms = rt.mark_ms + rt.merge_ms
+    return {"path": "dense" if dense else "marked", "dirty_leaves": m, "marked_internal": int(marked),
+            "alg_bytes": int(alg), "path_bytes": int(done), "device_ms": round(dev_ms, 4),
+            "achieved": round(alg / (dev_ms * 1e-3) / 1e9, 2),
+            "frac": round(alg / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "path_achieved": round(done / (dev_ms * 1e-3) / 1e9, 2)}
 
 
 def _median_ms(fn, reps=5):
@@ -530,7 +593,70 @@ def leg_c4(A, S, O, threads):
            "adjoint_ms": round(t_bwd, 3),
            "grad_step_vs_two_plain_passes": round((t_fwd + t_bwd) / (2.0 * t_plain), 3),
            "forward_bit_exact": same_fwd, "adjoint_max_rel_err_vs_fp64": err}
-    return out, bool(same_fwd and err < 1e-4)
+    del b, dt, dq, dr, res, aux, rres, hit, gq, go, gd, gt
+    gauss, ok_g = leg_c4_gaussians(A, S, O, J, threads, timed)
+    out["gaussians"] = gauss
+    return out, bool(same_fwd and err < 1e-4 and ok_g)
+
+
+def leg_c4_gaussians(A, S, O, J, threads, timed):
+    """The differentiable closest-point pass on the 1,048,576-Gaussian scene (the
+    non-uniform scenes' Gaussians) with 262,144 uniform queries: plain forward,
+    forward with the saved Eberly roots, and the adjoint into query and Gaussian
+    (mean, quaternion, scale) gradients."""
+    import ctypes
+
+    L = A.lib()
+    g = S.gaussians(1 << 20, 1)
+    q = np.random.default_rng(21).uniform(0.0, 64.0, (262_144, 3)).astype(np.float32)
+    nq, ng = q.shape[0], g.shape[0]
+    dg, dq = A.DeviceArray(g), A.DeviceArray(q)
+    b = A.Bvh.from_device(2, dg.ptr.value, ng)
+    b.build_wide()
+    res = A.DeviceArray.empty((nq, 2), np.uint32)
+    aux = A.DeviceArray.empty(nq, np.int32)
+    ones = A.DeviceArray(np.ones(nq, np.float32))
+    gq = A.DeviceArray.empty((nq, 3), np.float32)
+    gg = A.DeviceArray(np.zeros((ng, 10), np.float32))
+    ms = ctypes.c_double()
+
+    def plain():
+        A.check(L.qr_closest_device(b.h, dq.ptr, nq, res.ptr, A.ORDER_MORTON, 1, ctypes.byref(ms)))
+
+    def forward_save():
+        plain()
+        A.check(L.qr_closest_save_aux(b.h, dq.ptr, nq, res.ptr, aux.ptr))
+
+    def backward():
+        A.check(L.qr_adjoint_closest(b.h, dq.ptr, nq, res.ptr, aux.ptr, ones.ptr, gq.ptr, gg.ptr))
+
+    t_plain = _median_ms(lambda: timed(plain))
+    t_fwd = _median_ms(lambda: timed(forward_save))
+    t_bwd = _median_ms(lambda: timed(backward))
+    plain()
+    plain_res = res.download()
+    forward_save()
+    same = bool(np.array_equal(res.download(), plain_res))
+    want = O.CpuBvh(2, g).knn(q[:20000], 0, threads)
+    same &= bool(np.array_equal(plain_res[:20000], want[:, 0, :]))
+    # gradients of the first 4,096 queries' loss against the FP64 CPU adjoint
+    m = 4096
+    gg0 = A.DeviceArray(np.zeros((ng, 10), np.float32))
+    A.check(L.qr_adjoint_closest(b.h, dq.ptr, m, res.ptr, aux.ptr, ones.ptr, gq.ptr, gg0.ptr))
+    prim = plain_res[:m, 1].view(np.int32)
+    cq, cg = J.gauss_adjoint(q[:m].astype(np.float64), g.astype(np.float64), prim, np.ones(m))
+    got_q, got_g = gq.download()[:m].astype(np.float64), gg0.download().astype(np.float64)
+    err = 0.0
+    for got, ref in ((got_q, cq), (got_g[:, 0:3], cg[:, 0:3]), (got_g[:, 3:7], cg[:, 3:7]),
+                     (got_g[:, 7:10], cg[:, 7:10])):
+        floor = 0.05 * float(np.sqrt(np.mean(ref[ref != 0] ** 2)))
+        err = max(err, float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), floor))))
+    out = {"gaussians": ng, "queries": nq, "plain_forward_ms": round(t_plain, 3),
+           "forward_with_save_ms": round(t_fwd, 3), "adjoint_ms": round(t_bwd, 3),
+           "grad_step_vs_two_plain_passes": round((t_fwd + t_bwd) / (2.0 * t_plain), 3),
+           "forward_bit_exact": same, "adjoint_max_rel_err_vs_fp64": err}
+    b.close()
+    return out, bool(same and err < 1e-4)
 
 
 def leg_c5(A, S, O, threads):
@@ -585,8 +711,13 @@ def leg_c5(A, S, O, threads):
     trees_equal = sum(int(bt.tree_hash(s) == trees[s].tree_hash()) for s in range(0, S_, 64))
     for tr in trees:
         tr.close()
+    bb_ms = statistics.median(builds)
+    bb_alg = int(poff[-1]) * (5 * TRI_BYTES + 232)
     out = {"scenes": S_, "triangles": int(poff[-1]), "queries": nq,
-           "batched_build_ms": round(statistics.median(builds), 3),
+           "batched_build_ms": round(bb_ms, 3),
+           "batched_build_roofline": {"bound": "hbm", "alg_bytes": bb_alg,
+                                      "achieved": round(bb_alg / (bb_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
+                                      "unit": "GB/s", "frac": round(bb_alg / (bb_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
            "batched_query_ms": round(ms_c + ms_k, 3),
            "batched_mq_per_s": round(nq / (ms_c + ms_k) / 1e3, 2),
            "plain_loop_build_ms": round(t_build, 1), "plain_loop_query_ms": round(t_query, 1),

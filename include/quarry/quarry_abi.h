@@ -99,12 +99,14 @@ QR_API int qr_describe(char* buf, int len);
 QR_API int qr_warmup(void);
 /* Shape of the 4-wide traversal, process-wide: 0 automatic (closest / kNN below
  * 65,536 queries by query count, leaf kind and k; from 65,536 on, the first call
- * for a tree, k class and query-count octave times one lane, four lanes and one
- * wave per query on its own queries and keeps the fastest; radius one wave per
- * query below 131,072 queries, four lanes above; rays one lane per ray), 1 one
+ * for a tree, k class and query-count octave times one lane, four lanes, 16 and 32
+ * lanes and one wave per query on its own queries and keeps the fastest; radius one
+ * wave per query below 131,072 queries, four lanes above; rays one lane per ray), 1 one
  * lane per query, 2 four lanes per query, 3 one wave per query or ray, 4 as 0 but
- * closest / kNN and rays tuned at every count.  Results are identical in every
- * mode.  No reference counterpart (a tuning knob of this backend). */
+ * closest / kNN and rays tuned at every count, 5 / 6 closest and kNN with 16 / 32
+ * lanes per query (four / two queries per wave; kNN with k > 16 takes 32 lanes),
+ * other queries as in mode 0.  Results are identical in every mode.  No reference
+ * counterpart (a tuning knob of this backend). */
 QR_API int qr_set_traversal_mode(int mode);
 /* LBVH build: how many frontier nodes a workgroup (or group of workgroups) may hand
  * up to the next level, process-wide, 0..32 (default 32).  Units with more send
@@ -228,6 +230,15 @@ QR_API int qr_radius_device_ptrs(const qr_csr* r, int** offsets_dev, uint64_t** 
 QR_API int qr_radius_destroy(qr_csr* r);
 /* lastQueryStatsGPU */
 QR_API int qr_last_query_stats(qr_traversal_stats* out);
+/* Work of the calling thread's last binned-SAH build (no reference counterpart: the
+ * measurement behind the SAH build's bandwidth figure): split-loop levels, primitive
+ * slots binned and partitioned summed over the levels, and slots finished by the
+ * serial tails (tasks of at most 64 primitives). */
+typedef struct qr_sah_stats {
+    int levels;
+    long long binned_slots, partitioned_slots, serial_slots;
+} qr_sah_stats;
+QR_API int qr_last_sah_stats(qr_sah_stats* out);
 /* Device-pointer variants: points_dev[n] (3 f32) -> results_dev[n * k] (8 B each). */
 QR_API int qr_closest_device(const qr_bvh* bvh, const float* points_dev, int n, void* results_dev,
                              int order, int wide, double* ms);

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/quarry/quarry_abi.h"
@@ -209,7 +210,9 @@ void readStats(const unsigned long long* dev, qr_traversal_stats* out) {
     out->nodes_opened = sum[3];
 }
 
-TraverseArgs argsFor(const BvhState& b, bool wide) {
+// arm: point the kernels at the overflow word first (allocates it once per device)
+TraverseArgs argsFor(const BvhState& b, bool wide, bool arm = true) {
+    if (arm) armOverflowWord();
     TraverseArgs a;
     a.kind = b.kind;
     a.wide = wide;
@@ -317,9 +320,19 @@ int qr_warmup(void) {
     });
 }
 
+int qr_last_sah_stats(qr_sah_stats* out) {
+    return guard([&] {
+        need(out != nullptr, "null argument");
+        out->levels = tSahStats.levels;
+        out->binned_slots = tSahStats.binnedSlots;
+        out->partitioned_slots = tSahStats.partitionedSlots;
+        out->serial_slots = tSahStats.serialSlots;
+    });
+}
+
 int qr_set_traversal_mode(int mode) {
     return guard([&] {
-        need(mode >= 0 && mode <= 4, "traversal mode must be 0, 1, 2, 3 or 4");
+        need(mode >= 0 && mode <= 6, "traversal mode must be 0 to 6");
         setTraversalMode(mode);
     });
 }
@@ -830,6 +843,7 @@ double raycastImpl(const qr_bvh* bvh, const Ray* rays, int n, uint64_t* out, qr_
     }
     ev.record(2, s);
     ev.sync(2);
+    checkOverflow();
     const double total = ev.ms(0, 2);
     if (collectStats) readStats(st.p, &tRayStats);
     tRayStats.sort_ms = order != QR_ORDER_INPUT ? ev.ms(0, 1) : 0.0;
@@ -1044,6 +1058,7 @@ int qr_batch_plan_create(qr_batch* b, const float* points, const int* qoffsets, 
         for (int i = 0; i < S; ++i) need(qoffsets[i + 1] >= qoffsets[i], "query offsets must not decrease");
         const int n = qoffsets[S];
         need(n == 0 || points != nullptr, "null points");
+        armOverflowWord();
         auto* p = new qr_batch_plan();
         try {
             p->batch = b;
@@ -1110,6 +1125,7 @@ int qr_batch_plan_run(qr_batch_plan* p, double* ms) {
         hipCheck(hipGraphLaunch(p->exec, p->stream), "graph launch");
         ev.record(1, p->stream);
         ev.sync(1);
+        checkOverflow();
         if (ms) *ms = ev.ms(0, 1);
     });
 }
@@ -1122,6 +1138,7 @@ int qr_batch_plan_run_eager(qr_batch_plan* p, double* ms) {
         enqueuePlan(*p, p->variant, p->morton, p->stream);
         ev.record(1, p->stream);
         ev.sync(1);
+        checkOverflow();
         if (ms) *ms = ev.ms(0, 1);
     });
 }
@@ -1219,7 +1236,7 @@ namespace {
 
 // Shared body of closest and kNN (k = 0 selects closest).
 double pointQuery(const qr_bvh* bvh, const V3* pts, int n, int k, int outStride, ClosestResult* out,
-                  qr_points* scratch, int collectStats, int order, int wide) {
+                  qr_points* work, int collectStats, int order, int wide) {
     checkTree(bvh, wide != 0);
     need(order == QR_ORDER_INPUT || order == QR_ORDER_MORTON, "query order must be input or morton");
     tQueryStats = qr_traversal_stats{};
@@ -1232,8 +1249,8 @@ double pointQuery(const qr_bvh* bvh, const V3* pts, int n, int k, int outStride,
     TraverseArgs a = argsFor(bvh->s, wide != 0);
     a.stats = collectStats ? st.p : nullptr;
     if (order == QR_ORDER_MORTON)
-        a.order = reorder(*scratch, reinterpret_cast<const float*>(pts), 3, n,
-                          scratch->bbValid ? kBoxGiven : kBoxMeasure, s);
+        a.order = reorder(*work, reinterpret_cast<const float*>(pts), 3, n,
+                          work->bbValid ? kBoxGiven : kBoxMeasure, s);
     ev.record(1, s);
     if (k == 0) {
         launchClosest(a, pts, n, out, s);
@@ -1242,6 +1259,7 @@ double pointQuery(const qr_bvh* bvh, const V3* pts, int n, int k, int outStride,
     }
     ev.record(2, s);
     ev.sync(2);
+    checkOverflow();
     const double total = ev.ms(0, 2);
     if (collectStats) readStats(st.p, &tQueryStats);
     tQueryStats.sort_ms = order == QR_ORDER_MORTON ? ev.ms(0, 1) : 0.0;
@@ -1272,9 +1290,9 @@ int qr_knn(const qr_bvh* bvh, qr_points* p, int k, int collectStats, int order, 
 int qr_closest_device(const qr_bvh* bvh, const float* pts, int n, void* results, int order, int wide,
                       double* ms) {
     return guard([&] {
-        qr_points scratch;
+        qr_points work;
         const double t = pointQuery(bvh, reinterpret_cast<const V3*>(pts), n, 0, 1,
-                                    static_cast<ClosestResult*>(results), &scratch, 0, order, wide);
+                                    static_cast<ClosestResult*>(results), &work, 0, order, wide);
         if (ms) *ms = t;
     });
 }
@@ -1283,9 +1301,9 @@ int qr_knn_device(const qr_bvh* bvh, const float* pts, int n, int k, void* resul
                   int wide, double* ms) {
     return guard([&] {
         need(k >= 1 && k <= 32, "kNearestGPU: k must be in [1, 32]");
-        qr_points scratch;
+        qr_points work;
         const double t = pointQuery(bvh, reinterpret_cast<const V3*>(pts), n, k, k,
-                                    static_cast<ClosestResult*>(results), &scratch, 0, order, wide);
+                                    static_cast<ClosestResult*>(results), &work, 0, order, wide);
         if (ms) *ms = t;
     });
 }
@@ -1361,6 +1379,7 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
             if (total > 0 && !sorted) segmentSort(res->offsets.p, res->keys.p, n, maxSeg, scratch, s);
             ev.record(2, s);
             ev.sync(2);
+            checkOverflow();
             const double total_ms = ev.ms(0, 2);
             if (collectStats) readStats(st.p, &tQueryStats);
             tQueryStats.sort_ms = order == QR_ORDER_MORTON ? ev.ms(0, 1) : 0.0;

@@ -23,8 +23,14 @@ constexpr int kBlock = 256;
 
 // ---- coherent (agent-scope) accessors for data produced inside one kernel ------
 // Boxes written by one workgroup are read by another that may sit on a different
-// XCD with its own L2.  sc1 loads/stores keep those accesses coherent; the
-// explicit vmcnt wait before the arrival atomic orders the box stores ahead of it.
+// XCD with its own L2.  Every value that crosses workgroups (boxes, heights, the
+// arrival counters) is an agent-scope atomic access, so each location is coherent;
+// the explicit vmcnt wait before the arrival atomic makes this thread's box stores
+// complete before the arrival is counted, and its "memory" clobber keeps the
+// compiler from moving them after it.  A release RMW plus an acquire fence on the
+// winning path would state the same order in the language's terms, but at agent
+// scope it also writes back the XCD's L2 on every arrival: measured on the 2M
+// terrain, the leaves + boxes pass of the SAH build went from 0.20 to 4.36 ms.
 __device__ __forceinline__ void stAgent(float* p, float v) {
     __hip_atomic_store(reinterpret_cast<uint32_t*>(p), floatBits(v), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);

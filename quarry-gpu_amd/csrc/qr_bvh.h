@@ -18,6 +18,15 @@ struct BuildTimingsC {
     double mortonMs, sortMs, treeMs, scatterMs, refitMs, sahBinMs, sahSplitMs, sahPartMs, totalMs;
 };
 
+// Work of the calling thread's last binned-SAH build: levels of the split loop,
+// primitive slots binned (levels that score SAH planes) and partitioned (every
+// level), and slots left to the serial tails (tasks of at most 64).
+struct SahBuildStats {
+    int levels = 0;
+    long long binnedSlots = 0, partitionedSlots = 0, serialSlots = 0;
+};
+extern thread_local SahBuildStats tSahStats;
+
 struct BvhState {
     int kind = kPrimTri;
     int builder = kBuildLbvh;

@@ -73,32 +73,38 @@ void hipCheck(hipError_t e, const char* what);
 
 inline int blocksFor(long long n, int block) { return static_cast<int>((n + block - 1) / block); }
 
-// Plain device allocation owned by a scope.
+// Plain device allocation owned by a scope (or a borrowed view of memory someone
+// else owns: borrow() makes release() forget the pointer instead of freeing it).
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool owned = true;
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), owned(o.owned) {
         o.p = nullptr;
         o.n = 0;
+        o.owned = true;
     }
     DevBuf& operator=(DevBuf&& o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             n = o.n;
+            owned = o.owned;
             o.p = nullptr;
             o.n = 0;
+            o.owned = true;
         }
         return *this;
     }
     ~DevBuf() { release(); }
     void alloc(size_t count) {
         release();
+        owned = true;
         if (count == 0) count = 1;
         hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
         if (e != hipSuccess) {
@@ -113,9 +119,17 @@ struct DevBuf {
         if (count > n || p == nullptr) alloc(count);
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        owned = true;
+    }
+    // A view of count elements at q, never freed here.
+    void borrow(T* q, size_t count) {
+        release();
+        p = q;
+        n = count;
+        owned = false;
     }
     T* release_ownership() {
         T* q = p;

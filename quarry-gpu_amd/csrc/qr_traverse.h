@@ -17,6 +17,9 @@ struct TraverseArgs {
     const int* primIds = nullptr;       // leaf slot -> scene index
     const uint32_t* order = nullptr;    // optional visiting order (lane i -> query order[i])
     unsigned long long* stats = nullptr;  // optional [node tests, leaf tests, overflows]
+    // false: the walk shape comes from an earlier tuning of the same tree and size
+    // class or from the measured crossovers, never from timing trials on this call
+    bool tune = true;
 };
 
 // Traversal counters: kStatCopies copies of [node tests, leaf tests, stack
@@ -96,6 +99,14 @@ struct BatchQueryArgs {
 enum BatchVariant : int { kBatchLds256 = 0, kBatchLds64 = 1, kBatchScratch64 = 2, kBatchVariants = 3 };
 void launchBatchNearest(const BatchQueryArgs& a, int variant, hipStream_t s);
 void launchBatchReset(int* counter, hipStream_t s);
+
+// Overflow reporting: armOverflowWord() points the traversal kernels at a pinned host
+// word of the current device (once per device; it allocates that word, so
+// allocation-free entry points do not call it); checkOverflow(), after a call's
+// kernels have completed, throws DeviceError if a walk overflowed its stack since
+// the last check, and clears the word.
+void armOverflowWord();
+void checkOverflow();
 
 // Traversal shape of the wide point queries: 0 automatic, 1 one lane per query,
 // 2 four lanes per query (process-wide).

@@ -627,6 +627,8 @@ char* pinnedStage(size_t bytes) {
 
 }  // namespace
 
+thread_local SahBuildStats tSahStats;
+
 template <typename L>
 static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStream_t s, BuildTimingsC* t) {
     using Item = typename L::Item;
@@ -714,10 +716,16 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     }
 
     double binMs = 0.0, splitMs = 0.0, partMs = 0.0;
+    SahBuildStats st{};
     int cur = 0;
     for (int lvl = 0; !level.empty(); ++lvl) {
         next.clear();
         const int nt = static_cast<int>(level.size());
+        long long slots = 0;
+        for (const HostTask& h : level) slots += h.count;
+        ++st.levels;
+        st.partitionedSlots += slots;
+        if (lvl < kMedianLevel) st.binnedSlots += slots;
         if (nt > maxTasks) throw DeviceError("sah: task table overflow");
         std::fill(hRes, hRes + nt, SahResult{});
         if (lvl >= kMedianLevel) {
@@ -810,6 +818,8 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
         if (lvl > 2 * kMedianLevel + 64) throw DeviceError("sah: level limit exceeded");
     }
 
+    for (const SerialTask& sk : serial) st.serialSlots += sk.count;
+    tSahStats = st;
     // node records decided on the host, then the serial tails
     DevBuf<NodeTriple> dTri(triples.size() + 1);
     DevBuf<SerialTask> dSerial(serial.size() + 1);

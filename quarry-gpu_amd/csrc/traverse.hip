@@ -64,9 +64,18 @@ __device__ __forceinline__ void waveMaxAtomic(int* dst, int v) {
 
 // ---- stats -------------------------------------------------------------------------
 // stats[0] node tests, [1] leaf tests, [2] stack overflows, [3] interior nodes opened.
+// Pinned host word that any walk whose stack overflowed sets to 1 (the walk stops
+// and its result is incomplete); the host checks it after every query call.  Null
+// until armOverflowWord() has run on the device.
+__device__ int* gOverflowWord = nullptr;
+
 template <bool kStats>
 __device__ __forceinline__ void flushStats(unsigned long long* stats, int nodeTests, int leafTests,
                                            int overflow, int opened) {
+    if (overflow) {  // an incomplete walk: tell the host, whatever the build
+        int* w = gOverflowWord;
+        if (w) __hip_atomic_store(w, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     if (!kStats) return;
     unsigned long long a = static_cast<unsigned long long>(nodeTests);
     unsigned long long b = static_cast<unsigned long long>(leafTests);
@@ -1160,7 +1169,7 @@ __global__ __launch_bounds__(kBlock) void radiusWaveKernel(
 // nearest 16 without a sort (for Gaussians: sorted by box distance across the wave).  The admission rule is the lane kernels' (n < K or
 // bound <= K-th), so the list ends as the K best by (dist2, index) whatever the
 // walk order.
-template <typename L, int K, bool kStats>
+template <typename L, int K, int G, bool kStats>
 __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
     const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
     const int* __restrict__ primIds, const V3* __restrict__ points, const uint32_t* __restrict__ order,
@@ -1169,53 +1178,61 @@ __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
     // Gaussian leaves cost the most to test, so their walks pay for a sorted push
     // order (fewer nodes opened); triangles and points push quad by quad.
     constexpr bool kSortPushes = L::kKind == 2;
+    static_assert(G == 16 || G == 32 || G == 64, "lanes per query");
+    static_assert(K <= G, "the k-best list takes one lane per entry");
+    constexpr int kPerWave = 64 / G;              // queries per wave
+    constexpr int kQuads = G / 4;                 // nodes a query expands per round
+    constexpr int kStack = kWaveStack / kPerWave;  // stack entries per query
     __shared__ int lds[kWaveStack * (kBlock / 64)];
     __shared__ int ldsCmp[2 * 64 * (kBlock / 64)];  // per wave: candidate keys, values
     const int lane = threadIdx.x & 63;
-    LdsInt* cmpK = (LdsInt*)(ldsCmp + (threadIdx.x >> 6) * 128);
+    const int gl = lane & (G - 1);   // lane within the query's group
+    const int gb = lane - gl;        // the group's first lane
+    const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull) << gb;
+    LdsInt* cmpK = (LdsInt*)(ldsCmp + (threadIdx.x >> 6) * 128) + gb;
     LdsInt* cmpV = cmpK + 64;
-    const int quad = lane >> 2;
+    const int quad = gl >> 2;
     const int slot = lane & 3;
-    const int qi = xcdBlock() * (kBlock / 64) + (threadIdx.x >> 6);
-    const unsigned long long below = (1ull << lane) - 1ull;
+    const int qi = (xcdBlock() * (kBlock / 64) + (threadIdx.x >> 6)) * kPerWave + lane / G;
+    const unsigned long long below = ((1ull << lane) - 1ull) & gmask;
     int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
-    if (qi < count) {  // the same in the whole wave
+    if (qi < count) {  // the same in the whole group
         const int q = order ? static_cast<int>(order[qi]) : qi;
         const V3 p = points[q];
-        LdsInt* stk = (LdsInt*)(lds + (threadIdx.x >> 6) * kWaveStack);
-        float ld = kInf;  // this lane's list entry (lanes 0..K-1)
+        LdsInt* stk = (LdsInt*)(lds + (threadIdx.x >> 6) * kWaveStack + (lane / G) * kStack);
+        float ld = kInf;  // this lane's list entry (group lanes 0..K-1)
         int lid = 0x7fffffff;
         float kd = kInf;  // the list's K-th (the same in every lane)
         int kid = 0x7fffffff;
         int n = 0;
         auto admits = [&](float lb) { return n < K || lb <= kd; };
-        if (lane == 0) stk[0] = packEntry(0, 0.0f, idBits);
+        if (gl == 0) stk[0] = packEntry(0, 0.0f, idBits);
         int sp = 1;
         while (sp > 0) {
-            int width = (kWaveStack - kWaveHeadroom - sp) / 3;
-            width = width < 1 ? 1 : (width > 16 ? 16 : width);
+            int width = (kStack - kWaveHeadroom - sp) / 3;
+            width = width < 1 ? 1 : (width > kQuads ? kQuads : width);
             int take = width < sp ? width : sp;
             int node = -1;
             if (quad < take) {
                 const int v = stk[sp - 1 - quad];
                 if (admits(entryKey(v, idBits))) node = entryId(v, idBits);
             }
-            if (__ballot(quad < take && node < 0) != 0) {
+            if ((__ballot(quad < take && node < 0) & gmask) != 0) {
                 // the bound has overtaken entries the quads would take: drop those of
-                // the top 64 (order kept) and take again, so no quad idles on them
-                const int look = sp < 64 ? sp : 64;
+                // the top G (order kept) and take again, so no quad idles on them
+                const int look = sp < G ? sp : G;
                 int v = 0;
                 bool keep = false;
-                if (lane < look) {
-                    v = stk[sp - look + lane];
+                if (gl < look) {
+                    v = stk[sp - look + gl];
                     keep = admits(entryKey(v, idBits));
                 }
-                const unsigned long long km = __ballot(keep);
+                const unsigned long long km = __ballot(keep) & gmask;
                 if (keep) stk[sp - look + __popcll(km & below)] = v;
                 sp = sp - look + __popcll(km);
                 if (sp == 0) break;
-                width = (kWaveStack - kWaveHeadroom - sp) / 3;
-                width = width < 1 ? 1 : (width > 16 ? 16 : width);
+                width = (kStack - kWaveHeadroom - sp) / 3;
+                width = width < 1 ? 1 : (width > kQuads ? kQuads : width);
                 take = width < sp ? width : sp;
                 node = quad < take ? entryId(stk[sp - 1 - quad], idBits) : -1;
             }
@@ -1250,12 +1267,12 @@ __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
                 cid = pid;
                 better = n < K || closer(cd, cid, kd, kid);
             }
-            const unsigned long long bm = __ballot(better);
-            if (K == 1 && bm != 0) {  // the wave's smallest candidate replaces the entry
+            const unsigned long long bm = __ballot(better) & gmask;
+            if (K == 1 && bm != 0) {  // the group's smallest candidate replaces the entry
                 float ck = better ? cd : kInf;
                 int cv = better ? cid : 0x7fffffff;
 #pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
+                for (int off = G / 2; off > 0; off >>= 1) {
                     const float ok = waveXor(ck, off);
                     const int ov = waveXor(cv, off);
                     if (closer(ok, ov, ck, cv)) {
@@ -1265,8 +1282,8 @@ __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
                 }
                 kd = ck;
                 kid = cv;
-                ld = lane == 0 ? ck : kInf;
-                lid = lane == 0 ? cv : 0x7fffffff;
+                ld = gl == 0 ? ck : kInf;
+                lid = gl == 0 ? cv : 0x7fffffff;
                 n = 1;
             } else if (bm != 0) {
                 // the nb candidates move to lanes 0..nb-1 (through LDS), are sorted
@@ -1281,41 +1298,41 @@ __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
                     cmpK[at] = __float_as_int(cd);
                     cmpV[at] = cid;
                 }
-                float ck = lane < nb ? __int_as_float(cmpK[lane]) : kInf;
-                int cv = lane < nb ? cmpV[lane] : 0x7fffffff;
+                float ck = gl < nb ? __int_as_float(cmpK[gl]) : kInf;
+                int cv = gl < nb ? cmpV[gl] : 0x7fffffff;
                 waveSortPairs(ck, cv, lane, m);
-                const float rk = __shfl(ck, m - 1 - (lane & (m - 1)), 64);
-                const int rv = __shfl(cv, m - 1 - (lane & (m - 1)), 64);
+                const float rk = __shfl(ck, gb + m - 1 - (gl & (m - 1)), 64);
+                const int rv = __shfl(cv, gb + m - 1 - (gl & (m - 1)), 64);
                 float mk = ld;
                 int mv = lid;
-                if (lane < m && closer(rk, rv, mk, mv)) {
+                if (gl < m && closer(rk, rv, mk, mv)) {
                     mk = rk;
                     mv = rv;
                 }
                 waveMergePairs(mk, mv, lane, m);
-                ld = lane < K ? mk : kInf;
-                lid = lane < K ? mv : 0x7fffffff;
+                ld = gl < K ? mk : kInf;
+                lid = gl < K ? mv : 0x7fffffff;
                 n = n + __popcll(bm);
                 n = n < K ? n : K;
-                kd = __shfl(ld, K - 1, 64);
-                kid = __shfl(lid, K - 1, 64);
+                kd = __shfl(ld, gb + K - 1, 64);
+                kid = __shfl(lid, gb + K - 1, 64);
             }
             // interior children that survive the updated bound: quad j popped the j-th
             // nearest entry, so its children go above those of quads j+1.., each
             // quad's nearest child on top of its group
             const bool push = adm && !leaf && admits(d);
-            const unsigned long long pm = __ballot(push);
+            const unsigned long long pm = __ballot(push) & gmask;
             const int pushes = __popcll(pm);
             if (pushes > 0) {
-                if (sp + pushes > kWaveStack) {  // deeper than the stack bound allows
-                    if (lane == 0) overflow = 1;
+                if (sp + pushes > kStack) {  // deeper than the stack bound allows
+                    if (gl == 0) overflow = 1;
                     break;
                 }
                 if (kSortPushes) {  // all of the round's children in box-distance order
                     float pk = push ? d : kInf;
                     int pv = push ? ch : 0x7fffffff;
-                    waveSortPairs(pk, pv, lane);
-                    if (lane < pushes) stk[sp + pushes - 1 - lane] = packEntry(pv, pk, idBits);
+                    waveSortPairs(pk, pv, lane, G);
+                    if (gl < pushes) stk[sp + pushes - 1 - gl] = packEntry(pv, pk, idBits);
                 } else {
                     const float key = push ? d : kInf;
                     const float k0 = quadGetF<0>(key), k1 = quadGetF<1>(key), k2 = quadGetF<2>(key), k3 = quadGetF<3>(key);
@@ -1324,17 +1341,17 @@ __global__ __launch_bounds__(kBlock) void nearestWaveKernel(
                     rank += (slot != 1) && (k1 < key || (k1 == key && 1 < slot));
                     rank += (slot != 2) && (k2 < key || (k2 == key && 2 < slot));
                     rank += (slot != 3) && (k3 < key || (k3 == key && 3 < slot));
-                    const int mine = __popcll((pm >> (4 * quad)) & 0xFull);
-                    const int deeper = quad < 15 ? __popcll(pm >> (4 * (quad + 1))) : 0;
+                    const int mine = __popcll((pm >> (gb + 4 * quad)) & 0xFull);
+                    const int deeper = quad < kQuads - 1 ? __popcll(pm >> (gb + 4 * (quad + 1))) : 0;
                     if (push) stk[sp + deeper + (mine - 1 - rank)] = packEntry(ch, d, idBits);
                 }
                 sp += pushes;
             }
         }
-        if (lane < k) {
+        if (gl < k) {
             ClosestResult r{0u, -1};
-            if (lane < n) r = {floatBits(ld), lid};
-            out[static_cast<size_t>(q) * outStride + lane] = r;
+            if (gl < n) r = {floatBits(ld), lid};
+            out[static_cast<size_t>(q) * outStride + gl] = r;
         }
     }
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
@@ -1790,7 +1807,8 @@ void mortonOrder(const float* base, int stride, int count, int boxMode, SortWork
 int gTraversalMode = 0;
 
 // Shapes of a walk over 4-wide nodes.
-enum NearestShape : int { kShapeLane = 0, kShapeQuad = 1, kShapeWave = 2 };
+// (Group16 / Group32: 16 or 32 lanes per query, four or two queries per wave.)
+enum NearestShape : int { kShapeLane = 0, kShapeQuad = 1, kShapeWave = 2, kShapeGroup16 = 3, kShapeGroup32 = 4 };
 
 // Tuned shapes.  The best shape depends on how long and how uneven the walks are,
 // which the query count alone does not tell: on the terrain a million closest
@@ -1910,6 +1928,42 @@ namespace {
 
 void setTraversalMode(int mode) { gTraversalMode = mode; }
 
+namespace {
+constexpr int kOverflowDevices = 64;
+std::mutex gOverflowMutex;
+int* gOverflowHost[kOverflowDevices] = {};
+
+int* overflowHost() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kOverflowDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(gOverflowMutex);
+    return gOverflowHost[dev];
+}
+}  // namespace
+
+void armOverflowWord() {
+    int dev = 0;
+    hipCheck(hipGetDevice(&dev), "get device");
+    if (dev < 0 || dev >= kOverflowDevices) return;
+    std::lock_guard<std::mutex> lock(gOverflowMutex);
+    if (gOverflowHost[dev]) return;
+    int* h = nullptr;
+    hipCheck(hipHostMalloc(reinterpret_cast<void**>(&h), 256, hipHostMallocDefault), "overflow word");
+    *h = 0;
+    int* d = nullptr;
+    hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&d), h, 0), "overflow word address");
+    hipCheck(hipMemcpyToSymbol(HIP_SYMBOL(gOverflowWord), &d, sizeof(d)), "overflow word symbol");
+    gOverflowHost[dev] = h;
+}
+
+void checkOverflow() {
+    int* h = overflowHost();
+    if (!h || __atomic_load_n(h, __ATOMIC_ACQUIRE) == 0) return;
+    __atomic_store_n(h, 0, __ATOMIC_RELEASE);
+    throw DeviceError("traversal stack overflow: the tree is deeper than the traversal stack holds, "
+                      "so some results are incomplete");
+}
+
 // kNN (k > 1): four lanes per query at every size -- the quad's spread k-best list
 // makes each offer a quarter of the per-lane work, and it measured faster from 20k
 // to 1M queries on the 2M-triangle terrain.  Closest (k = 1): four lanes per query
@@ -1937,17 +1991,35 @@ static bool useWave(int count) {
     return count < below;
 }
 
+// G lanes per query (64: one wave per query).
+template <typename L, int K, int G>
+static void launchGroup(const TraverseArgs& a, const V3* pts, int count, int k, int outStride, ClosestResult* out,
+                        unsigned long long* stats, hipStream_t s) {
+    const auto* prims = static_cast<const typename L::Item*>(a.items);
+    const int idBits = stackIdBitsDev(a.wideCount);
+    const int grid = blocksFor(count, (kBlock / 64) * (64 / G));
+    if (stats)
+        nearestWaveKernel<L, K, G, true><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, stats, idBits);
+    else
+        nearestWaveKernel<L, K, G, false><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, idBits);
+}
+
 template <typename L, int K>
 static void launchShape(int shape, const TraverseArgs& a, const V3* pts, int count, int k, int outStride,
                         ClosestResult* out, unsigned long long* stats, hipStream_t s) {
     const auto* prims = static_cast<const typename L::Item*>(a.items);
     const int idBits = stackIdBitsDev(a.wideCount);
     if (shape == kShapeWave) {
-        const int grid = blocksFor(count, kBlock / 64);
-        if (stats)
-            nearestWaveKernel<L, K, true><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, stats, idBits);
-        else
-            nearestWaveKernel<L, K, false><<<grid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, k, outStride, out, nullptr, idBits);
+        launchGroup<L, K, 64>(a, pts, count, k, outStride, out, stats, s);
+    } else if (shape == kShapeGroup16 || shape == kShapeGroup32) {
+        if constexpr (K <= 16) {
+            if (shape == kShapeGroup16)
+                launchGroup<L, K, 16>(a, pts, count, k, outStride, out, stats, s);
+            else
+                launchGroup<L, K, 32>(a, pts, count, k, outStride, out, stats, s);
+        } else {
+            launchGroup<L, K, 32>(a, pts, count, k, outStride, out, stats, s);  // 32 entries need 32 lanes
+        }
     } else if (shape == kShapeQuad) {
         const int grid = blocksFor(4LL * count, kBlock);
         if (stats)
@@ -1981,18 +2053,22 @@ static int tunedShape(const TraverseArgs& a, const V3* pts, int count, int k, in
         const auto it = gTuned.find(key);
         if (it != gTuned.end()) return it->second;
     }
-    EventSet<6> ev;
-    const int shapes[3] = {kShapeLane, kShapeQuad, kShapeWave};
-    for (int i = 0; i < 3; ++i) {
+    // every candidate shape once, after one untimed walk that brings the tree into
+    // the caches (so the first timed shape is not the only one reading it cold)
+    constexpr int kShapes = K <= 16 ? 5 : 4;
+    const int shapes[5] = {kShapeLane, kShapeQuad, kShapeWave, kShapeGroup32, kShapeGroup16};
+    EventSet<2 * 5> ev;
+    launchShape<L, K>(kShapeWave, a, pts, count, k, outStride, out, nullptr, s);
+    for (int i = 0; i < kShapes; ++i) {
         ev.record(2 * i, s);
         launchShape<L, K>(shapes[i], a, pts, count, k, outStride, out, nullptr, s);
         ev.record(2 * i + 1, s);
     }
     hipCheck(hipGetLastError(), "nearest tuning");
-    ev.sync(5);
+    ev.sync(2 * kShapes - 1);
     int best = kShapeLane;
     float bestMs = 0.0f;
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < kShapes; ++i) {
         const float ms = ev.ms(2 * i, 2 * i + 1);
         if (i == 0 || ms < bestMs) {
             best = shapes[i];
@@ -2003,6 +2079,21 @@ static int tunedShape(const TraverseArgs& a, const V3* pts, int count, int k, in
     if (gTuned.size() >= 4096) gTuned.clear();  // trees come and go; keep the table small
     gTuned[key] = best;
     return best;
+}
+
+// The shape an earlier tuning chose for this tree and size class, else the measured
+// crossovers' (no timing trials).
+template <typename L, int K>
+static int cachedShape(const TraverseArgs& a, int count) {
+    int octave = 0;
+    while ((2LL << octave) <= count) ++octave;
+    const TuneKey key{a.wideNodes, a.wideCount, L::kKind, K, octave};
+    {
+        std::lock_guard<std::mutex> lock(gTuneMutex);
+        const auto it = gTuned.find(key);
+        if (it != gTuned.end()) return it->second;
+    }
+    return defaultShape<L, K>(count);
 }
 
 template <typename L, int K>
@@ -2022,9 +2113,15 @@ static void launchNearestTK(const TraverseArgs& a, const V3* pts, int count, int
         case 1: shape = kShapeLane; break;
         case 2: shape = kShapeQuad; break;
         case 3: shape = kShapeWave; break;
-        case 4: shape = tunedShape<L, K>(a, pts, count, k, outStride, out, s); break;
+        case 5: shape = kShapeGroup16; break;
+        case 6: shape = kShapeGroup32; break;
+        case 4:
+            shape = a.tune ? tunedShape<L, K>(a, pts, count, k, outStride, out, s) : cachedShape<L, K>(a, count);
+            break;
         default:
-            shape = count < kTuneFrom ? defaultShape<L, K>(count) : tunedShape<L, K>(a, pts, count, k, outStride, out, s);
+            shape = count < kTuneFrom ? defaultShape<L, K>(count)
+                    : a.tune          ? tunedShape<L, K>(a, pts, count, k, outStride, out, s)
+                                      : cachedShape<L, K>(a, count);
     }
     launchShape<L, K>(shape, a, pts, count, k, outStride, out, a.stats, s);
 }

@@ -64,6 +64,11 @@ class RayStats(ctypes.Structure):
                 ("sum_t", ctypes.c_double)]
 
 
+class SahStats(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int), ("binned_slots", ctypes.c_longlong),
+                ("partitioned_slots", ctypes.c_longlong), ("serial_slots", ctypes.c_longlong)]
+
+
 class BvhInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "prim_count", "node_count", "depth", "wide_count", "prim_kind", "builder")] + [
@@ -127,6 +132,7 @@ def lib():
             "qr_last_query_stats": ([_vp], _ip),
             "qr_closest_device": ([_vp, _vp, _ip, _vp, _ip, _ip, _dp], _ip),
             "qr_knn_device": ([_vp, _vp, _ip, _ip, _vp, _ip, _ip, _dp], _ip),
+            "qr_last_sah_stats": ([ctypes.POINTER(SahStats)], _ip),
             "qr_bvh_update": ([_vp, _vp, _ip, _vp, _ip, _vp, _ip, _vp, _ip, _ip, _vp, _dp], _ip),
             "qr_rays_device_ptrs": ([_vp, _vp, _vp, _vp], _ip),
             "qr_points_device_ptrs": ([_vp, _vp, _vp], _ip),
@@ -231,6 +237,13 @@ class Bvh:
             d = np.ascontiguousarray(dirty, dtype=np.int32)
             check(lib().qr_bvh_refit(self.h, ptr(items), items.shape[0], ptr(d), d.shape[0], ctypes.byref(t), ctypes.byref(ms)))
         return t
+
+    def refit_device(self, dev_ptr: int, n: int) -> float:
+        """Full refit from device-resident scene-order items; returns device ms."""
+        t = RefitTimings()
+        ms = ctypes.c_double()
+        check(lib().qr_bvh_refit_device(self.h, ctypes.c_void_p(dev_ptr), n, ctypes.byref(t), ctypes.byref(ms)))
+        return ms.value
 
     def update(self, items: np.ndarray, deleted=None, moved=None, inserted=None,
                appended: int = 0) -> TopoTimings:
@@ -431,6 +444,12 @@ class RayBuffer:
 def last_query_stats() -> TraversalStats:
     st = TraversalStats()
     check(lib().qr_last_query_stats(ctypes.byref(st)))
+    return st
+
+
+def last_sah_stats() -> SahStats:
+    st = SahStats()
+    check(lib().qr_last_sah_stats(ctypes.byref(st)))
     return st
 
 

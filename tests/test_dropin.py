@@ -22,6 +22,9 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "gpu", "gpu_bvh.h")
 DROPIN = os.path.join(ROOT, "oracle", "_ref", "dropin")
+# the reference's examples/CMakeLists.txt built unchanged through find_package(quarry)
+# against this repository's installed CMake package (make -C oracle cmake_examples)
+CMAKE_EXAMPLES = os.path.join(ROOT, "oracle", "_ref", "cmake", "examples")
 REF_HEADER = "/root/reference/src/gpu/gpu_bvh.h"
 
 
@@ -56,8 +59,8 @@ def test_dropin_programs_built():
         assert os.access(os.path.join(DROPIN, prog), os.X_OK), prog
 
 
-def _run(prog: str, *args: str, timeout: int = 120) -> str:
-    path = os.path.join(DROPIN, prog)
+def _run(prog: str, *args: str, timeout: int = 120, where: str = DROPIN) -> str:
+    path = os.path.join(where, prog)
     if not os.access(path, os.X_OK):
         pytest.skip(f"{prog} not built (make -C oracle dropin needs the reference sources)")
     print(f"[dropin] {prog} {' '.join(args)}", flush=True)
@@ -82,6 +85,16 @@ def test_example_dynamic_scene():
     counts = re.findall(r"mismatch(?:es)? (\d+)/\d+", out)
     assert counts and all(c == "0" for c in counts), counts
     assert "validate failures 0" in out
+
+
+@pytest.mark.gpu
+def test_cmake_package_examples():
+    """The reference's examples as its own CMake project builds them (find_package(quarry)
+    + quarry::gpu), linked against the installed package's libquarry_hip."""
+    out = _run("quarry_example", where=CMAKE_EXAMPLES)
+    assert "MISMATCH" not in out and "matches cpu" in out
+    out = _run("dynamic_scene", where=CMAKE_EXAMPLES)
+    assert "checks ok" in out and "validate failures 0" in out
 
 
 FAST = ("--runs", "1", "--warmup-ms", "0")

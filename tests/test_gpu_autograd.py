@@ -221,3 +221,61 @@ def test_gradcheck_small(AG):
     d2.sum().backward()
     # point above the first triangle's interior: gradient 2 (q - p) with p = (1, 1, 0)
     assert torch.allclose(q.grad[0], torch.tensor([0.0, 0.0, 4.0], device=dev), atol=1e-6)
+
+
+def test_gaussian_gradients_million_scene(AG, orc, scenes):
+    """The 1,048,576-Gaussian scene (means uniform in a 64-unit cube, log-uniform
+    scales, random rotations) with 262,144 uniform queries: forward bits equal the
+    CPU BVH's, every gradient equals the FP64 CPU adjoint to 1e-4, and 1,024 sampled
+    parameters (768 of the Gaussians that hold a closest point, 256 query
+    coordinates) equal FP64 central differences of the loss."""
+    from oracle import adjoint as J
+
+    g = scenes.gaussians(1 << 20, 1)
+    rng = np.random.default_rng(21)
+    q = rng.uniform(0.0, 64.0, (262_144, 3)).astype(np.float32)
+    dev = torch.device("cuda")
+    m = torch.from_numpy(g[:, 0:3].copy()).to(dev).requires_grad_(True)
+    r = torch.from_numpy(g[:, 3:7].copy()).to(dev).requires_grad_(True)
+    s = torch.from_numpy(g[:, 7:10].copy()).to(dev).requires_grad_(True)
+    qt = torch.from_numpy(q).to(dev).requires_grad_(True)
+    print("forward + backward on 1M Gaussians, 262k queries", flush=True)
+    d2, idx = AG.gaussian_closest_dist2(qt, m, r, s)
+    d2.sum().backward()
+    want = orc.CpuBvh(2, g).knn(q, 0, THREADS)
+    assert np.array_equal(d2.detach().cpu().numpy().view(np.uint32), want[:, 0, 0])
+    prim = idx.cpu().numpy()
+    assert np.array_equal(prim, want[:, 0, 1].view(np.int32))
+    G64, q64 = g.astype(np.float64), q.astype(np.float64)
+    print("FP64 CPU adjoint", flush=True)
+    cq, cg = J.gauss_adjoint(q64, G64, prim, np.ones(len(q)))
+    grads = {"q": qt.grad.cpu().numpy().astype(np.float64), "m": m.grad.cpu().numpy().astype(np.float64),
+             "r": r.grad.cpu().numpy().astype(np.float64), "s": s.grad.cpu().numpy().astype(np.float64)}
+    for name, got, ref in (("q", grads["q"], cq), ("m", grads["m"], cg[:, 0:3]), ("r", grads["r"], cg[:, 3:7]),
+                           ("s", grads["s"], cg[:, 7:10])):
+        err = rel(got, ref, 0.05 * rms(ref[ref != 0]) if np.any(ref != 0) else 1.0)
+        print(name, "cpu-adjoint max rel err", float(err.max()), flush=True)
+        assert err.max() < TOL, name
+
+    # finite differences: a parameter of Gaussian j moves only the loss of the queries
+    # whose closest Gaussian is j, so the loss is summed over those
+    owners = {}
+    for qi, pj in enumerate(prim):
+        owners.setdefault(int(pj), []).append(qi)
+    held = np.array(sorted(owners), np.int64)
+    gflat = np.concatenate([grads["m"], grads["r"], grads["s"]], 1)
+    floor_g = {c: 0.05 * rms(gflat[held, c]) for c in range(10)}
+    stats = {"coarse": 0, "fine": 0}
+    rng = np.random.default_rng(22)
+    for j, c in zip(rng.choice(held, 768, replace=False), rng.integers(0, 10, 768)):
+        qs = np.array(owners[int(j)])
+        sub = G64[j:j + 1].copy()
+        fn = lambda x: J.gauss_dist2_fixed(q64[qs], x, np.zeros(len(qs), np.int64)).sum()  # noqa: E731
+        check_fd(gflat[j, c], fn, sub, int(c), floor_g[int(c)], stats)
+    floor_q = 0.05 * rms(grads["q"])
+    for qi, c in zip(rng.choice(len(q), 256, replace=False), rng.integers(0, 3, 256)):
+        sub = q64[qi:qi + 1].copy()
+        fn = lambda x: J.gauss_dist2_fixed(x, G64, prim[qi:qi + 1]).sum()  # noqa: E731,B023
+        check_fd(grads["q"][qi, c], fn, sub, int(c), floor_q, stats)
+    print("FD checks", stats, flush=True)
+    assert stats["coarse"] + stats["fine"] == 1024 and stats["fine"] <= 16

@@ -339,7 +339,7 @@ def test_traversal_modes_agree(A, orc, scenes, kind):
     b.build_wide()
     want = {k: orc.knn_brute(kind, items, q, k, threads=THREADS) for k in (0, 1, 5, 16, 32)}
     try:
-        for mode in (1, 2, 3, 4, 0):
+        for mode in (1, 2, 3, 4, 5, 6, 0):
             A.check(A.lib().qr_set_traversal_mode(mode))
             for k, w in want.items():
                 got = b.closest(q, order=A.ORDER_MORTON, wide=True)[0] if k == 0 else \
@@ -403,7 +403,7 @@ def test_wave_walk_on_identical_items(A, orc, scenes, kind):
     b.build_wide()
     want = {k: orc.knn_brute(kind, items, q, k, threads=THREADS) for k in (0, 7, 32)}
     try:
-        for mode in (3, 2, 4, 0):
+        for mode in (3, 2, 4, 5, 6, 0):
             A.check(A.lib().qr_set_traversal_mode(mode))
             for k, w in want.items():
                 got, _, st = b.closest(q, wide=True, stats=True) if k == 0 else \
