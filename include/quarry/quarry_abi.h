@@ -244,6 +244,15 @@ QR_API int qr_closest_device(const qr_bvh* bvh, const float* points_dev, int n, 
                              int order, int wide, double* ms);
 QR_API int qr_knn_device(const qr_bvh* bvh, const float* points_dev, int n, int k,
                          void* results_dev, int order, int wide, double* ms);
+/* qr_knn_device without device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (tight serving loops, code that
+ * must not call hipMalloc).  The Morton-order scratch lives in a fixed device array
+ * of about 23 MB per GPU, held until the process ends; n <= QR_NOALLOC_MAX_QUERIES
+ * (larger n returns QR_ERR_INVALID).  Synchronous, no timing.  Calls that overlap on
+ * one GPU share that array and race; results equal qr_knn_device's. */
+#define QR_NOALLOC_MAX_QUERIES 1048576
+QR_API int qr_knn_device_noalloc(const qr_bvh* bvh, const float* points_dev, int n, int k,
+                                 void* results_dev, int order, int wide);
 
 /* ---- batched mode ----------------------------------------------------------------
  * No reference counterpart (the reference builds and queries one scene per call,

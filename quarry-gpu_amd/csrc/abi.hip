@@ -1308,6 +1308,82 @@ int qr_knn_device(const qr_bvh* bvh, const float* pts, int n, int k, void* resul
     });
 }
 
+// ---- kNN without allocation ---------------------------------------------------------
+// The scratch of qr_knn_device's Morton order (codes, sorted codes, visiting order,
+// query bounds, and the radix sort's ping-pong buffers, digit histograms, look-back
+// descriptors and tile tickets) lives in one fixed device array, so the call makes no
+// device allocation.  Each GPU holds its own copy until the process ends.  Every call
+// on a GPU shares that GPU's copy: calls that overlap on one GPU race.
+namespace {
+
+constexpr size_t kNaQueries = QR_NOALLOC_MAX_QUERIES;
+constexpr size_t kNaKeys = 0;
+constexpr size_t kNaSorted = kNaKeys + kNaQueries;
+constexpr size_t kNaOrder = kNaSorted + kNaQueries;
+constexpr size_t kNaBounds = kNaOrder + kNaQueries;          // 6 words used, 64 kept
+constexpr size_t kNaAltKeys = kNaBounds + 64;
+constexpr size_t kNaAltVals = kNaAltKeys + kNaQueries;
+constexpr size_t kNaHist = kNaAltVals + kNaQueries;
+// descriptors for 4 passes at the smallest sort tile, as SortWorkspace::reserve sizes them
+constexpr size_t kNaDesc = kNaHist + static_cast<size_t>(kHistCopies) * kHistCopyWords;
+constexpr size_t kNaTickets = kNaDesc + 4 * ((kNaQueries + kSortTileSmall - 1) / kSortTileSmall) * 256;
+constexpr size_t kNaWords = kNaTickets + 64;
+static_assert(kNaQueries % 64 == 0, "every part stays 256-byte aligned");
+
+__device__ __align__(256) uint32_t scratch[kNaWords];
+
+constexpr int kNaDevices = 64;
+std::mutex gScratchMutex;
+uint32_t* gScratchDev[kNaDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device with that
+// device current (the array's host-side name is not a device address).
+uint32_t* scratchOnDevice() {
+    int dev = 0;
+    hipCheck(hipGetDevice(&dev), "get device");
+    need(dev >= 0 && dev < kNaDevices, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratchDev[dev]) {
+        void* ptr = nullptr;
+        hipCheck(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "scratch address");
+        gScratchDev[dev] = static_cast<uint32_t*>(ptr);
+    }
+    return gScratchDev[dev];
+}
+
+}  // namespace
+
+int qr_knn_device_noalloc(const qr_bvh* bvh, const float* pts, int n, int k, void* results, int order,
+                          int wide) {
+    return guard([&] {
+        need(k >= 1 && k <= 32, "kNearestGPU: k must be in [1, 32]");
+        need(n >= 0 && static_cast<size_t>(n) <= kNaQueries,
+             "qr_knn_device_noalloc: more queries than its fixed scratch holds (QR_NOALLOC_MAX_QUERIES)");
+        checkTree(bvh, wide != 0);
+        need(order == QR_ORDER_INPUT || order == QR_ORDER_MORTON, "query order must be input or morton");
+        tQueryStats = qr_traversal_stats{};
+        if (n == 0) return;
+        hipStream_t s = currentStream();
+        TraverseArgs a = argsFor(bvh->s, wide != 0, false);
+        a.tune = false;  // timing trials would take events; shapes give the same bits
+        if (order == QR_ORDER_MORTON) {
+            uint32_t* w = scratchOnDevice();
+            SortWorkspace ws;  // views into the fixed array, never freed
+            ws.keysAlt.borrow(w + kNaAltKeys, kNaQueries);
+            ws.valsAlt.borrow(w + kNaAltVals, kNaQueries);
+            ws.hist.borrow(w + kNaHist, static_cast<size_t>(kHistCopies) * kHistCopyWords);
+            ws.desc.borrow(w + kNaDesc, kNaTickets - kNaDesc);
+            ws.tickets.borrow(w + kNaTickets, 4);
+            ws.capacity = kNaQueries;
+            mortonOrder(pts, 3, n, kBoxMeasure, ws, w + kNaKeys, w + kNaSorted, w + kNaOrder, w + kNaBounds, s);
+            a.order = w + kNaOrder;
+        }
+        launchKnn(a, reinterpret_cast<const V3*>(pts), n, k, k, static_cast<ClosestResult*>(results), s);
+        hipCheck(hipStreamSynchronize(s), "knn");
+        checkOverflow();  // reported once another call has armed the device's word
+    });
+}
+
 int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, int order, int wide,
               qr_csr** out, double* ms) {
     return guard([&] {

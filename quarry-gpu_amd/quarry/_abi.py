@@ -132,6 +132,7 @@ def lib():
             "qr_last_query_stats": ([_vp], _ip),
             "qr_closest_device": ([_vp, _vp, _ip, _vp, _ip, _ip, _dp], _ip),
             "qr_knn_device": ([_vp, _vp, _ip, _ip, _vp, _ip, _ip, _dp], _ip),
+            "qr_knn_device_noalloc": ([_vp, _vp, _ip, _ip, _vp, _ip, _ip], _ip),
             "qr_last_sah_stats": ([ctypes.POINTER(SahStats)], _ip),
             "qr_bvh_update": ([_vp, _vp, _ip, _vp, _ip, _vp, _ip, _vp, _ip, _ip, _vp, _dp], _ip),
             "qr_rays_device_ptrs": ([_vp, _vp, _vp, _vp], _ip),
