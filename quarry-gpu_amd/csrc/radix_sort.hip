@@ -26,7 +26,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kStatAgg = 1u << 30;
 constexpr uint32_t kStatIncl = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
-constexpr int kLookBack = 32;  // predecessors read per look-back round
+constexpr int kLookBack = 8;  // predecessors read per look-back round (32 measured slower: 0.144 vs 0.120 ms for the 2M sort)
 
 __device__ __forceinline__ uint32_t loadAgent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -54,7 +54,7 @@ struct PrimRec {
 static_assert(sizeof(PrimRec) == 40, "PrimRec layout");
 
 struct SahTask {
-    int start, count, pad0, pad1;
+    int start, count, node, meta;
     float cLo[4], cHi[4];
 };
 
@@ -65,10 +65,6 @@ struct SahResult {
 
 struct Chunk {
     int task, start, count, pad;
-};
-
-struct NodeTriple {
-    int id, left, right, far;  // far: the other end of the node's leaf range
 };
 
 __device__ __forceinline__ int hidx(int a, int b, int v) { return (a * kBins + b) * kBinVals + v; }
@@ -179,39 +175,62 @@ __device__ __forceinline__ float areaOf(float lx, float ly, float lz, float hx, 
     return dx * dy + dy * dz + dz * dx;
 }
 
-// Plane choice by one wave (lanes 1..31 each score plane `lane` on all usable
-// axes); lane 0 writes the result.  h is the task's histogram (LDS or global).
+// Plane choice by one wave: lane b holds bin b (lanes 32..63 repeat lanes 0..31);
+// inclusive prefix and suffix unions of the bins' counts and boxes (five shuffle
+// steps each) give every plane p = 1..31 its left side (bins 0..p-1) and right side
+// (bins p..31) at lane p, which scores it on each usable axis; lane 0 writes the
+// result.  Counts and min / max are exact, so the costs are the same bits as summing
+// the bins plane by plane.  h is the task's histogram (LDS or global).
 __device__ void chooseSplit(const uint32_t* h, const SahTask& t, const AxisMap& m, SahResult* out) {
     const int lane = threadIdx.x & 63;
+    const int bin = lane & (kBins - 1);
     const int count = t.count;
     const int minSide = count / 8 > 1 ? count / 8 : 1;
     float best = kInf;
     int bestKey = 0x7fffffff;  // axis * 32 + plane
-    if (lane >= 1 && lane < kBins) {
-        for (int a = 0; a < 3; ++a) {
-            if (!m.use[a]) continue;
-            int nL = 0, nR = 0;
-            float l[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
-            float r[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
-            for (int b = 0; b < kBins; ++b) {
-                const int c = static_cast<int>(h[hidx(a, b, 0)]);
-                if (c == 0) continue;
-                float* dst = b < lane ? l : r;
-                if (b < lane) {
-                    nL += c;
-                } else {
-                    nR += c;
-                }
+    for (int a = 0; a < 3; ++a) {
+        if (!m.use[a]) continue;
+        const int c = static_cast<int>(h[hidx(a, bin, 0)]);
+        float v[6];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    dst[k] = fminf(dst[k], fromOrderedBits(h[hidx(a, b, 1 + k)]));
-                    dst[3 + k] = fmaxf(dst[3 + k], fromOrderedBits(h[hidx(a, b, 4 + k)]));
+        for (int k = 0; k < 3; ++k) {
+            v[k] = fromOrderedBits(h[hidx(a, bin, 1 + k)]);
+            v[3 + k] = fromOrderedBits(h[hidx(a, bin, 4 + k)]);
+        }
+        int pc = c, sc = c;
+        float pv[6], sv[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) pv[k] = sv[k] = v[k];
+#pragma unroll
+        for (int off = 1; off < kBins; off <<= 1) {
+            const int oc = __shfl_up(pc, off, kBins);
+            const int uc = __shfl_down(sc, off, kBins);
+            const bool up = bin >= off, down = bin + off < kBins;
+            pc += up ? oc : 0;
+            sc += down ? uc : 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const float op = __shfl_up(pv[k], off, kBins);
+                const float os = __shfl_down(sv[k], off, kBins);
+                if (k < 3) {
+                    pv[k] = up ? fminf(pv[k], op) : pv[k];
+                    sv[k] = down ? fminf(sv[k], os) : sv[k];
+                } else {
+                    pv[k] = up ? fmaxf(pv[k], op) : pv[k];
+                    sv[k] = down ? fmaxf(sv[k], os) : sv[k];
                 }
             }
-            if (nL < minSide || nR < minSide) continue;
+        }
+        // the left side of plane `bin` is the prefix through bin - 1
+        const int nL = __shfl_up(pc, 1, kBins);
+        float l[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) l[k] = __shfl_up(pv[k], 1, kBins);
+        const int nR = sc;
+        if (bin >= 1 && nL >= minSide && nR >= minSide) {
             const float cost = areaOf(l[0], l[1], l[2], l[3], l[4], l[5]) * static_cast<float>(nL) +
-                               areaOf(r[0], r[1], r[2], r[3], r[4], r[5]) * static_cast<float>(nR);
-            const int key = a * kBins + lane;
+                               areaOf(sv[0], sv[1], sv[2], sv[3], sv[4], sv[5]) * static_cast<float>(nR);
+            const int key = a * kBins + bin;
             if (cost < best || (cost == best && key < bestKey)) {
                 best = cost;
                 bestKey = key;
@@ -376,27 +395,184 @@ __global__ void sahResetKernel(uint32_t* __restrict__ cb, int* __restrict__ arri
     for (int j = i; j < internal; j += kBlock * gridDim.x) arrive[j] = 0;
 }
 
-__global__ void sahClearBinsKernel(uint32_t* __restrict__ bins, int tasks) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= tasks * kHist) return;
-    const int v = (i % kHist) % kBinVals;
-    bins[i] = v == 0 ? 0u : ((v >= 1 && v <= 3) || (v >= 7 && v <= 9)) ? orderedBits(kInf) : orderedBits(-kInf);
+// ---- device-driven split loop ------------------------------------------------------------
+// Control words of the loop (device memory): the big-task counts of the two level
+// buffers, the local and serial list lengths, the current level's chunk count, an
+// error flag and the work counters.
+enum : int {
+    kCtlBig0 = 0,       // tasks in level buffer 0 (1: buffer 1)
+    kCtlLocal = 2,      // tasks of kSerial+1 .. kChunk primitives, for the subtree kernel
+    kCtlSerial = 3,     // ranges of at most kSerial primitives, for the serial kernel
+    kCtlChunks = 4,     // chunks of the current level's big tasks
+    kCtlError = 5,      // set when a task table overflows or a split is empty
+    kCtlBinned = 6,     // primitive slots binned (summed over levels)
+    kCtlParted = 7,     // primitive slots partitioned
+    kCtlSerialSlots = 8,
+    kCtlMaxLevel = 9,   // deepest level that split a task
+    kCtlWords = 16,
+};
+
+struct SerialTask {
+    int start, count, node, buf;
+};
+
+// SahTask.meta: bit 0 = the record buffer holding the task's range, bits 1.. = level.
+__device__ __forceinline__ int metaOf(int buf, int level) { return buf | (level << 1); }
+
+// Where the split loop's tables live (all sized for the worst case of n primitives:
+// big tasks of one level and local tasks are disjoint ranges of more than kChunk /
+// kSerial primitives, serial ranges are disjoint and non-empty).
+struct SahTables {
+    PrimRec* rec[2];
+    SahTask* big[2];
+    SahTask* local;
+    SerialTask* serial;
+    Chunk* chunks;
+    int* first;
+    int* chunkLeft;
+    int* chunkOff;
+    uint32_t* bins;
+    SahResult* results;
+    int* ctl;
+    int maxBig, maxLocal, maxSerial, maxChunks;
+};
+
+// Records the split of task (start, count, node) after its first nL positions: the
+// node's children in the LBVH numbering, parent links and the node's far end.
+__device__ __forceinline__ void writeSplit(int n, int start, int count, int node, int nL, Node2* __restrict__ nodes,
+                                           int* __restrict__ parent, int* __restrict__ farEnd, int* lId, int* rId) {
+    const int g = start + nL - 1;
+    const int e = start + count - 1;
+    const int l = (g == start) ? n - 1 + g : g;
+    const int r = (g + 1 == e) ? n - 1 + e : g + 1;
+    nodes[node].left = l;
+    nodes[node].right = r;
+    parent[l] = node;
+    parent[r] = node;
+    farEnd[node] = node == start ? e : start;
+    *lId = l;
+    *rId = r;
+}
+
+// The tables' two-entry arrays indexed by a selection, not a dynamic subscript (the
+// tables are a kernel argument; a subscript would copy them to scratch).
+__device__ __forceinline__ PrimRec* recOf(const SahTables& T, int buf) { return buf ? T.rec[1] : T.rec[0]; }
+__device__ __forceinline__ SahTask* bigOf(const SahTables& T, int cur) { return cur ? T.big[1] : T.big[0]; }
+
+__device__ __forceinline__ void pushSerial(const SahTables& T, int start, int count, int node, int buf) {
+    const int at = atomicAdd(&T.ctl[kCtlSerial], 1);
+    if (at < T.maxSerial) {
+        T.serial[at] = {start, count, node, buf};
+    } else {
+        atomicOr(&T.ctl[kCtlError], 1);
+    }
+}
+
+__device__ __forceinline__ SahTask makeTask(int start, int count, int node, int meta, const float* lo, const float* hi) {
+    SahTask t;
+    t.start = start;
+    t.count = count;
+    t.node = node;
+    t.meta = meta;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        t.cLo[k] = lo[k];
+        t.cHi[k] = hi[k];
+    }
+    t.cLo[3] = t.cHi[3] = 0.0f;
+    return t;
+}
+
+// A child of a big task: serial range, local task or next-level big task.
+__device__ __forceinline__ void routeChild(const SahTables& T, int nextBuf, int start, int count, int node, int buf,
+                                           int level, const float* lo, const float* hi) {
+    if (count <= kSerial) {
+        pushSerial(T, start, count, node, buf);
+    } else if (count <= kChunk) {
+        const int at = atomicAdd(&T.ctl[kCtlLocal], 1);
+        if (at < T.maxLocal) {
+            T.local[at] = makeTask(start, count, node, metaOf(buf, level), lo, hi);
+        } else {
+            atomicOr(&T.ctl[kCtlError], 1);
+        }
+    } else {
+        const int at = atomicAdd(&T.ctl[kCtlBig0 + nextBuf], 1);
+        if (at < T.maxBig) {
+            bigOf(T, nextBuf)[at] = makeTask(start, count, node, metaOf(buf, level), lo, hi);
+        } else {
+            atomicOr(&T.ctl[kCtlError], 1);
+        }
+    }
+}
+
+// The root: one task over [0, n) with the centroid bounds of the prep kernel.
+__global__ void sahRootKernel(SahTables T, const uint32_t* __restrict__ cb, int n) {
+    if (threadIdx.x != 0) return;
+    for (int i = 0; i < kCtlWords; ++i) T.ctl[i] = 0;
+    float lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = fromOrderedBits(cb[a]);
+        hi[a] = fromOrderedBits(cb[3 + a]);
+    }
+    routeChild(T, 0, 0, n, 0, 0, 0, lo, hi);
+}
+
+// Level plan (one workgroup): the current big tasks' chunk list (chunks of one task
+// consecutive, first[task] .. first[task + 1]); clears the count of the buffer this
+// level's children go to.
+__global__ __launch_bounds__(kBlock) void sahPlanKernel(SahTables T, int cur, int median) {
+    __shared__ int lds[4];
+    const int nt = T.ctl[kCtlBig0 + cur];
+    int carry = 0;
+    for (int base = 0; base < nt && !median; base += kBlock) {
+        const int i = base + threadIdx.x;
+        int c = 0;
+        SahTask t{};
+        if (i < nt) {
+            t = bigOf(T, cur)[i];
+            c = (t.count + kChunk - 1) / kChunk;
+        }
+        int total = 0;
+        const int ex = carry + blockExclusive(c, lds, &total);
+        if (i < nt) {
+            T.first[i] = ex;
+            for (int k = 0; k < c && ex + k < T.maxChunks; ++k)
+                T.chunks[ex + k] = {i, t.start + k * kChunk, min(kChunk, t.count - k * kChunk), 0};
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) {
+        if (carry > T.maxChunks) {
+            atomicOr(&T.ctl[kCtlError], 1);
+            carry = 0;
+        }
+        if (!median) T.first[nt] = carry;
+        T.ctl[kCtlChunks] = median ? 0 : carry;
+        T.ctl[kCtlBig0 + (1 - cur)] = 0;
+    }
+}
+
+__global__ void sahClearBinsKernel(SahTables T, int cur) {
+    const long long words = static_cast<long long>(T.ctl[kCtlBig0 + cur]) * kHist;
+    for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < words;
+         i += static_cast<long long>(gridDim.x) * kBlock) {
+        const int v = static_cast<int>(i % kHist) % kBinVals;
+        T.bins[i] = v == 0 ? 0u : ((v >= 1 && v <= 3) || (v >= 7 && v <= 9)) ? orderedBits(kInf) : orderedBits(-kInf);
+    }
 }
 
 // One workgroup per chunk of a large task: LDS histogram, then one global flush.
-__global__ __launch_bounds__(kBlock) void sahBinKernel(const PrimRec* __restrict__ rec,
-                                                       const SahTask* __restrict__ tasks,
-                                                       const Chunk* __restrict__ chunks,
-                                                       uint32_t* __restrict__ bins) {
+__global__ __launch_bounds__(kBlock) void sahBinKernel(SahTables T, int cur) {
+    if (static_cast<int>(blockIdx.x) >= T.ctl[kCtlChunks]) return;
     __shared__ uint32_t h[kHist];
-    const Chunk ch = chunks[blockIdx.x];
-    const SahTask t = tasks[ch.task];
+    const Chunk ch = T.chunks[blockIdx.x];
+    const SahTask t = bigOf(T, cur)[ch.task];
     const AxisMap m = axisMap(t);
     histInit(h);
     __syncthreads();
-    histAddRange(h, rec + ch.start, ch.count, m);
+    histAddRange(h, recOf(T, t.meta & 1) + ch.start, ch.count, m);
     __syncthreads();
-    uint32_t* g = bins + static_cast<size_t>(ch.task) * kHist;
+    uint32_t* g = T.bins + static_cast<size_t>(ch.task) * kHist;
     for (int i = threadIdx.x; i < kHist; i += kBlock) {
         const int v = i % kBinVals;
         if (h[i - v] == 0u) continue;  // empty bin
@@ -409,58 +585,69 @@ __global__ __launch_bounds__(kBlock) void sahBinKernel(const PrimRec* __restrict
             atomicMax(&g[i], x);
         }
     }
+    if (threadIdx.x == 0) atomicAdd(&T.ctl[kCtlBinned], ch.count);
+}
+
+__device__ __forceinline__ SahResult medianResult(const SahTask& t) {
+    SahResult res;
+    res.axis = -1;
+    res.plane = 0;
+    res.nL = t.count / 2;
+    res.pad = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        res.lLo[k] = res.rLo[k] = t.cLo[k];
+        res.lHi[k] = res.rHi[k] = t.cHi[k];
+    }
+    return res;
 }
 
 // One wave per large task.
-__global__ __launch_bounds__(64) void sahSplitKernel(const uint32_t* __restrict__ bins,
-                                                     const SahTask* __restrict__ tasks,
-                                                     SahResult* __restrict__ results) {
-    const SahTask t = tasks[blockIdx.x];
-    chooseSplit(bins + static_cast<size_t>(blockIdx.x) * kHist, t, axisMap(t), &results[blockIdx.x]);
+__global__ __launch_bounds__(64) void sahSplitKernel(SahTables T, int cur, int median) {
+    if (static_cast<int>(blockIdx.x) >= T.ctl[kCtlBig0 + cur]) return;
+    const SahTask t = bigOf(T, cur)[blockIdx.x];
+    if (median) {
+        if (threadIdx.x == 0) T.results[blockIdx.x] = medianResult(t);
+        return;
+    }
+    chooseSplit(T.bins + static_cast<size_t>(blockIdx.x) * kHist, t, axisMap(t), &T.results[blockIdx.x]);
 }
 
-__global__ __launch_bounds__(kBlock) void sahPartCountKernel(const PrimRec* __restrict__ rec,
-                                                             const SahTask* __restrict__ tasks,
-                                                             const SahResult* __restrict__ results,
-                                                             const Chunk* __restrict__ chunks,
-                                                             int* __restrict__ chunkLeft) {
+__global__ __launch_bounds__(kBlock) void sahPartCountKernel(SahTables T, int cur) {
+    if (static_cast<int>(blockIdx.x) >= T.ctl[kCtlChunks]) return;
     __shared__ int lds[4];
-    const Chunk ch = chunks[blockIdx.x];
-    const SahTask t = tasks[ch.task];
-    const SahResult s = results[ch.task];
+    const Chunk ch = T.chunks[blockIdx.x];
+    const SahTask t = bigOf(T, cur)[ch.task];
+    const SahResult s = T.results[ch.task];
     const AxisMap m = axisMap(t);
+    const PrimRec* rec = recOf(T, t.meta & 1);
     int c = 0;
     for (int i = threadIdx.x; i < ch.count; i += kBlock) c += goesLeft(rec[ch.start + i], ch.start + i, t, s, m) ? 1 : 0;
     int total = 0;
     blockExclusive(c, lds, &total);
-    if (threadIdx.x == 0) chunkLeft[blockIdx.x] = total;
+    if (threadIdx.x == 0) T.chunkLeft[blockIdx.x] = total;
 }
 
-// One thread per large task: exclusive offsets of its chunks' left counts
-// (chunks of one task are consecutive, first[task] .. first[task+1]).
-__global__ void sahChunkScanKernel(const int* __restrict__ first, int tasks, const int* __restrict__ chunkLeft,
-                                   int* __restrict__ chunkOff) {
+// One thread per large task: exclusive offsets of its chunks' left counts.
+__global__ void sahChunkScanKernel(SahTables T, int cur) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= tasks) return;
+    if (t >= T.ctl[kCtlBig0 + cur] || T.ctl[kCtlChunks] == 0) return;
     int acc = 0;
-    for (int c = first[t]; c < first[t + 1]; ++c) {
-        chunkOff[c] = acc;
-        acc += chunkLeft[c];
+    for (int c = T.first[t]; c < T.first[t + 1]; ++c) {
+        T.chunkOff[c] = acc;
+        acc += T.chunkLeft[c];
     }
 }
 
-__global__ __launch_bounds__(kBlock) void sahPartScatterKernel(const PrimRec* __restrict__ rec,
-                                                               PrimRec* __restrict__ out,
-                                                               const SahTask* __restrict__ tasks,
-                                                               const SahResult* __restrict__ results,
-                                                               const Chunk* __restrict__ chunks,
-                                                               const int* __restrict__ chunkOff,
-                                                               int* __restrict__ err) {
+__global__ __launch_bounds__(kBlock) void sahPartScatterKernel(SahTables T, int cur) {
+    if (static_cast<int>(blockIdx.x) >= T.ctl[kCtlChunks]) return;
     __shared__ int lds[4];
-    const Chunk ch = chunks[blockIdx.x];
-    const SahTask t = tasks[ch.task];
-    const SahResult s = results[ch.task];
+    const Chunk ch = T.chunks[blockIdx.x];
+    const SahTask t = bigOf(T, cur)[ch.task];
+    const SahResult s = T.results[ch.task];
     const AxisMap m = axisMap(t);
+    const PrimRec* rec = recOf(T, t.meta & 1);
+    PrimRec* out = recOf(T, 1 - (t.meta & 1));
     const int base = threadIdx.x * kPerThread;
     unsigned flags = 0u;
     int c = 0;
@@ -473,9 +660,9 @@ __global__ __launch_bounds__(kBlock) void sahPartScatterKernel(const PrimRec* __
     }
     int total = 0;
     int lrank = blockExclusive(c, lds, &total);
-    const int leftBase = t.start + chunkOff[blockIdx.x];
+    const int leftBase = t.start + T.chunkOff[blockIdx.x];
     const int chunkPos = ch.start - t.start;
-    const int rightBase = t.start + s.nL + (chunkPos - chunkOff[blockIdx.x]);
+    const int rightBase = t.start + s.nL + (chunkPos - T.chunkOff[blockIdx.x]);
     int rrank = base - lrank;
     const int leftEnd = t.start + s.nL, rightEnd = t.start + t.count;
     for (int j = 0; j < kPerThread; ++j) {
@@ -487,142 +674,193 @@ __global__ __launch_bounds__(kBlock) void sahPartScatterKernel(const PrimRec* __
         if (ok) {
             out[dst] = r;
         } else {
-            atomicAdd(err, 1);  // counts disagree with the histogram: reported, never written
+            atomicOr(&T.ctl[kCtlError], 2);  // counts disagree with the histogram: reported, never written
         }
     }
+    if (threadIdx.x == 0) atomicAdd(&T.ctl[kCtlParted], ch.count);
 }
 
-// One workgroup per task of kSerial+1 .. kChunk primitives: histogram, plane and
-// stable partition inside the workgroup.
-__global__ __launch_bounds__(kBlock) void sahLocalKernel(const PrimRec* __restrict__ rec,
-                                                         PrimRec* __restrict__ out,
-                                                         const SahTask* __restrict__ tasks,
-                                                         SahResult* __restrict__ results, int firstTask,
-                                                         int* __restrict__ err) {
+// One thread per large task: its node, and its children routed by size.
+__global__ void sahAssignKernel(SahTables T, int cur, int median, int n, Node2* __restrict__ nodes,
+                                int* __restrict__ parent, int* __restrict__ farEnd) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= T.ctl[kCtlBig0 + cur]) return;
+    const SahTask t = bigOf(T, cur)[i];
+    const SahResult r = T.results[i];
+    int nL = r.nL;
+    if (nL < 1 || nL >= t.count) {
+        atomicOr(&T.ctl[kCtlError], 4);
+        return;
+    }
+    const int level = t.meta >> 1;
+    const int buf = median ? (t.meta & 1) : 1 - (t.meta & 1);
+    atomicMax(&T.ctl[kCtlMaxLevel], level);
+    int l, rr;
+    writeSplit(n, t.start, t.count, t.node, nL, nodes, parent, farEnd, &l, &rr);
+    routeChild(T, 1 - cur, t.start, nL, l, buf, level + 1, r.lLo, r.lHi);
+    routeChild(T, 1 - cur, t.start + nL, t.count - nL, rr, buf, level + 1, r.rLo, r.rHi);
+}
+
+// One workgroup per local task (kSerial+1 .. kChunk primitives): the whole subtree
+// down to ranges of kSerial, from an LDS stack -- histogram, plane choice and stable
+// partition per task inside the workgroup, no return to the host between levels.
+constexpr int kSubtreeStack = 64;  // a chain of count/8 sides from 4096 to 64 is 32 deep
+__global__ __launch_bounds__(kBlock) void sahSubtreeKernel(SahTables T, int n, Node2* __restrict__ nodes,
+                                                           int* __restrict__ parent, int* __restrict__ farEnd) {
     __shared__ uint32_t h[kHist];
     __shared__ SahResult sres;
+    __shared__ SahTask stk[kSubtreeStack];
+    __shared__ SahTask curTask;
+    __shared__ int ssp;
     __shared__ int lds[4];
-    const int ti = firstTask + blockIdx.x;
-    const SahTask t = tasks[ti];
-    const AxisMap m = axisMap(t);
-    histInit(h);
-    __syncthreads();
-    histAddRange(h, rec + t.start, t.count, m);
-    __syncthreads();
-    if (threadIdx.x < 64) chooseSplit(h, t, m, &sres);
-    __syncthreads();
-    const SahResult s = sres;
-    if (threadIdx.x == 0) results[ti] = s;
-    const int base = threadIdx.x * kPerThread;
-    unsigned flags = 0u;
-    int c = 0;
-    for (int j = 0; j < kPerThread; ++j) {
-        const int i = base + j;
-        if (i < t.count && goesLeft(rec[t.start + i], t.start + i, t, s, m)) {
-            flags |= 1u << j;
-            ++c;
-        }
+    if (threadIdx.x == 0) {
+        stk[0] = T.local[blockIdx.x];
+        ssp = 1;
     }
-    int total = 0;
-    int lrank = blockExclusive(c, lds, &total);
-    int rrank = base - lrank;
-    for (int j = 0; j < kPerThread; ++j) {
-        const int i = base + j;
-        if (i >= t.count) break;
-        const PrimRec r = rec[t.start + i];
-        const bool left = (flags & (1u << j)) != 0;
-        const int k = left ? lrank++ : rrank++;
-        if (left ? k < s.nL : k < t.count - s.nL) {
-            out[t.start + (left ? 0 : s.nL) + k] = r;
-        } else {
-            atomicAdd(err, 1);  // counts disagree with the histogram: reported, never written
+    __syncthreads();
+    int binned = 0, parted = 0, deepest = 0;
+    while (true) {
+        if (threadIdx.x == 0) {
+            if (ssp > 0) {
+                curTask = stk[--ssp];
+            } else {
+                curTask.count = -1;
+            }
         }
+        __syncthreads();
+        const SahTask t = curTask;
+        if (t.count < 0) break;
+        const int buf = t.meta & 1, level = t.meta >> 1;
+        const bool median = level >= kMedianLevel;
+        const PrimRec* src = recOf(T, buf);
+        PrimRec* dst = recOf(T, 1 - buf);
+        const AxisMap m = axisMap(t);
+        if (!median) {
+            histInit(h);
+            __syncthreads();
+            histAddRange(h, src + t.start, t.count, m);
+            __syncthreads();
+            if (threadIdx.x < 64) chooseSplit(h, t, m, &sres);
+        } else if (threadIdx.x == 0) {
+            sres = medianResult(t);
+        }
+        __syncthreads();
+        const SahResult s = sres;
+        if (!median) {
+            const int base = threadIdx.x * kPerThread;
+            unsigned flags = 0u;
+            int c = 0;
+            for (int j = 0; j < kPerThread; ++j) {
+                const int i = base + j;
+                if (i < t.count && goesLeft(src[t.start + i], t.start + i, t, s, m)) {
+                    flags |= 1u << j;
+                    ++c;
+                }
+            }
+            int total = 0;
+            int lrank = blockExclusive(c, lds, &total);
+            int rrank = base - lrank;
+            for (int j = 0; j < kPerThread; ++j) {
+                const int i = base + j;
+                if (i >= t.count) break;
+                const PrimRec r = src[t.start + i];
+                const bool left = (flags & (1u << j)) != 0;
+                const int k = left ? lrank++ : rrank++;
+                if (left ? k < s.nL : k < t.count - s.nL) {
+                    dst[t.start + (left ? 0 : s.nL) + k] = r;
+                } else {
+                    atomicOr(&T.ctl[kCtlError], 2);  // counts disagree with the histogram
+                }
+            }
+            binned += t.count;
+            parted += t.count;
+        }
+        if (threadIdx.x == 0) {
+            deepest = level > deepest ? level : deepest;
+            const int nL = s.nL;
+            if (nL < 1 || nL >= t.count) {
+                atomicOr(&T.ctl[kCtlError], 4);
+            } else {
+                const int childBuf = median ? buf : 1 - buf;
+                int l, r;
+                writeSplit(n, t.start, t.count, t.node, nL, nodes, parent, farEnd, &l, &r);
+                const int cs[2] = {t.start, t.start + nL}, cc[2] = {nL, t.count - nL}, ci[2] = {l, r};
+                for (int side = 1; side >= 0; --side) {
+                    if (cc[side] <= kSerial) {
+                        pushSerial(T, cs[side], cc[side], ci[side], childBuf);
+                    } else if (ssp < kSubtreeStack) {
+                        stk[ssp++] = makeTask(cs[side], cc[side], ci[side], metaOf(childBuf, level + 1),
+                                              side ? s.rLo : s.lLo, side ? s.rHi : s.lHi);
+                    } else {
+                        atomicOr(&T.ctl[kCtlError], 1);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&T.ctl[kCtlBinned], binned);
+        atomicAdd(&T.ctl[kCtlParted], parted);
+        atomicMax(&T.ctl[kCtlMaxLevel], deepest);
     }
 }
 
-// Internal nodes decided on the host (levels of large and local tasks).
-__global__ void sahWriteNodesKernel(const NodeTriple* __restrict__ tri, int count, Node2* __restrict__ nodes,
-                                    int* __restrict__ parent, int* __restrict__ farEnd) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= count) return;
-    const NodeTriple t = tri[i];
-    nodes[t.id].left = t.left;
-    nodes[t.id].right = t.right;
-    parent[t.left] = t.id;
-    parent[t.right] = t.id;
-    farEnd[t.id] = t.far;
-}
-
-struct SerialTask {
-    int start, count, node, buf;
-};
-
-// One thread per final range of at most kSerial primitives: its leaf order (the
-// scene indices, read from whichever record buffer holds the range) and, for two
-// or more primitives, a balanced subtree of count/2 splits.
-__global__ void sahSerialKernel(const PrimRec* __restrict__ recA, const PrimRec* __restrict__ recB,
-                                const SerialTask* __restrict__ tasks, int count, int n,
-                                uint32_t* __restrict__ order, Node2* __restrict__ nodes,
+// One thread per final range of at most kSerial primitives (grid-stride over the
+// serial list): its leaf order (the scene indices, read from the record buffer that
+// holds the range) and, for two or more primitives, a balanced subtree of count/2
+// splits.
+__global__ void sahSerialKernel(SahTables T, int n, uint32_t* __restrict__ order, Node2* __restrict__ nodes,
                                 int* __restrict__ parent, int* __restrict__ farEnd) {
-    const int ti = blockIdx.x * kBlock + threadIdx.x;
-    if (ti >= count) return;
-    const SerialTask t = tasks[ti];
-    const PrimRec* src = t.buf ? recB : recA;
-    for (int i = 0; i < t.count; ++i) order[t.start + i] = static_cast<uint32_t>(src[t.start + i].id);
-    if (t.count < 2) return;
-    int stS[8], stE[8], stId[8];
-    int sp = 0;
-    stS[0] = t.start;
-    stE[0] = t.start + t.count - 1;
-    stId[0] = t.node;
-    sp = 1;
-    while (sp > 0) {
-        --sp;
-        const int s = stS[sp], e = stE[sp], id = stId[sp];
-        const int g = s + (e - s + 1) / 2 - 1;
-        const int l = (g == s) ? n - 1 + s : g;
-        const int r = (g + 1 == e) ? n - 1 + e : g + 1;
-        nodes[id].left = l;
-        nodes[id].right = r;
-        parent[l] = id;
-        parent[r] = id;
-        farEnd[id] = id == s ? e : s;
-        if (g > s) {
-            stS[sp] = s;
-            stE[sp] = g;
-            stId[sp] = g;
-            ++sp;
-        }
-        if (e > g + 1) {
-            stS[sp] = g + 1;
-            stE[sp] = e;
-            stId[sp] = g + 1;
-            ++sp;
+    const int count = T.ctl[kCtlSerial] < T.maxSerial ? T.ctl[kCtlSerial] : T.maxSerial;
+    int slots = 0;
+    for (int ti = blockIdx.x * kBlock + threadIdx.x; ti < count; ti += kBlock * gridDim.x) {
+        const SerialTask t = T.serial[ti];
+        const PrimRec* src = recOf(T, t.buf);
+        slots += t.count;
+        for (int i = 0; i < t.count; ++i) order[t.start + i] = static_cast<uint32_t>(src[t.start + i].id);
+        if (t.count < 2) continue;
+        int stS[8], stE[8], stId[8];
+        int sp = 0;
+        stS[0] = t.start;
+        stE[0] = t.start + t.count - 1;
+        stId[0] = t.node;
+        sp = 1;
+        while (sp > 0) {
+            --sp;
+            const int s = stS[sp], e = stE[sp], id = stId[sp];
+            const int g = s + (e - s + 1) / 2 - 1;
+            const int l = (g == s) ? n - 1 + s : g;
+            const int r = (g + 1 == e) ? n - 1 + e : g + 1;
+            nodes[id].left = l;
+            nodes[id].right = r;
+            parent[l] = id;
+            parent[r] = id;
+            farEnd[id] = id == s ? e : s;
+            if (g > s) {
+                stS[sp] = s;
+                stE[sp] = g;
+                stId[sp] = g;
+                ++sp;
+            }
+            if (e > g + 1) {
+                stS[sp] = g + 1;
+                stE[sp] = e;
+                stId[sp] = g + 1;
+                ++sp;
+            }
         }
     }
+    if (slots) atomicAdd(&T.ctl[kCtlSerialSlots], slots);
 }
 
-struct HostTask {
-    int start, count, node;
-    float cLo[3], cHi[3];
-};
-
-// Pinned host staging for the level loop's tables (tasks, chunks, chunk index,
-// results, error flag).  A copy from pageable memory goes through a driver bounce
-// buffer and the loop makes five per level; from pinned memory they are plain DMA.
-// One block per host thread, grown on demand and kept for the process (no copy is
-// in flight when it grows: every level ends with a stream synchronize).
-char* pinnedStage(size_t bytes) {
-    static thread_local char* p = nullptr;
-    static thread_local size_t cap = 0;
-    if (bytes > cap) {
-        if (p) hipCheck(hipHostFree(p), "sah stage free");
-        p = nullptr;
-        cap = 0;
-        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&p), bytes, hipHostMallocDefault), "sah stage");
-        cap = bytes;
-    }
-    return p;
+// Pinned host words for the loop's control reads (per host thread, kept for the
+// process): a read into pageable memory would go through a driver bounce buffer.
+int* pinnedWords() {
+    static thread_local int* w = nullptr;
+    if (!w) hipCheck(hipHostMalloc(reinterpret_cast<void**>(&w), kCtlWords * sizeof(int), hipHostMallocDefault),
+                     "sah control words");
+    return w;
 }
 
 }  // namespace
@@ -645,215 +883,152 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     b.sortWs.reserve(static_cast<size_t>(n));
     b.counters.reserve(16);
     uint32_t* cb = reinterpret_cast<uint32_t*>(b.counters.p);
-    int* errDev = b.counters.p + 8;
 
     DevBuf<PrimRec> rec[2];
     rec[0].alloc(static_cast<size_t>(n));
     rec[1].alloc(static_cast<size_t>(n));
     DevBuf<uint32_t> order(static_cast<size_t>(n));
 
-    const int maxTasks = n / (kSerial + 1) + 2;
-    const int maxBig = n / (kChunk + 1) + 2;
-    const int maxChunks = n / kChunk + maxBig + 2;
-    DevBuf<SahTask> dTasks(static_cast<size_t>(maxTasks));
-    DevBuf<SahResult> dRes(static_cast<size_t>(maxTasks));
-    DevBuf<Chunk> dChunks(static_cast<size_t>(maxChunks));
-    DevBuf<int> dFirst(static_cast<size_t>(maxBig + 1));
-    DevBuf<int> dChunkLeft(static_cast<size_t>(maxChunks));
-    DevBuf<int> dChunkOff(static_cast<size_t>(maxChunks));
-    DevBuf<uint32_t> dBins(static_cast<size_t>(maxBig) * kHist);
-    size_t stageBytes = 0;
-    auto carve = [&](size_t bytes) {
-        const size_t at = stageBytes;
-        stageBytes += (bytes + 255) & ~static_cast<size_t>(255);
-        return at;
-    };
-    const size_t oTasks = carve(sizeof(SahTask) * static_cast<size_t>(maxTasks));
-    const size_t oRes = carve(sizeof(SahResult) * static_cast<size_t>(maxTasks));
-    const size_t oChunks = carve(sizeof(Chunk) * static_cast<size_t>(maxChunks));
-    const size_t oFirst = carve(sizeof(int) * static_cast<size_t>(maxBig + 1));
-    const size_t oErr = carve(sizeof(int) * 8);
-    const size_t oBounds = carve(sizeof(uint32_t) * 6);
-    char* stage = pinnedStage(stageBytes);
-    SahTask* hTasks = reinterpret_cast<SahTask*>(stage + oTasks);
-    SahResult* hRes = reinterpret_cast<SahResult*>(stage + oRes);
-    Chunk* hChunks = reinterpret_cast<Chunk*>(stage + oChunks);
-    int* hFirst = reinterpret_cast<int*>(stage + oFirst);
-    int* hErr = reinterpret_cast<int*>(stage + oErr);
-    uint32_t* cbh = reinterpret_cast<uint32_t*>(stage + oBounds);
+    SahTables T{};
+    T.maxBig = n / (kChunk + 1) + 2;
+    T.maxLocal = n / (kSerial + 1) + 2;
+    T.maxSerial = n + 2;
+    T.maxChunks = n / kChunk + T.maxBig + 2;
+    DevBuf<SahTask> dBig(2 * static_cast<size_t>(T.maxBig)), dLocal(static_cast<size_t>(T.maxLocal));
+    DevBuf<SerialTask> dSerial(static_cast<size_t>(T.maxSerial));
+    DevBuf<Chunk> dChunks(static_cast<size_t>(T.maxChunks));
+    DevBuf<int> dFirst(static_cast<size_t>(T.maxBig) + 1), dChunkLeft(static_cast<size_t>(T.maxChunks)),
+        dChunkOff(static_cast<size_t>(T.maxChunks)), dCtl(kCtlWords);
+    DevBuf<uint32_t> dBins(static_cast<size_t>(T.maxBig) * kHist);
+    DevBuf<SahResult> dRes(static_cast<size_t>(T.maxBig));
+    T.rec[0] = rec[0].p;
+    T.rec[1] = rec[1].p;
+    T.big[0] = dBig.p;
+    T.big[1] = dBig.p + T.maxBig;
+    T.local = dLocal.p;
+    T.serial = dSerial.p;
+    T.chunks = dChunks.p;
+    T.first = dFirst.p;
+    T.chunkLeft = dChunkLeft.p;
+    T.chunkOff = dChunkOff.p;
+    T.bins = dBins.p;
+    T.results = dRes.p;
+    T.ctl = dCtl.p;
+    int* hw = pinnedWords();
 
-    EventSet<4> ev;
+    EventSet<6> ev;
     ev.record(0, s);
     sahResetKernel<<<blocksFor(n - 1 > 256 ? n - 1 : 256, kBlock), kBlock, 0, s>>>(cb, b.arrive.p, n - 1);
     int grid = blocksFor(n, kBlock * 4);
     if (grid > 512) grid = 512;  // two workgroups per CU stream the items; few bounds atomics
     sahPrepKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, rec[0].p, cb, b.parent.p);
+    sahRootKernel<<<1, 64, 0, s>>>(T, cb, n);
     hipCheck(hipGetLastError(), "sah prep");
-    hipCheck(hipMemcpyAsync(cbh, cb, sizeof(uint32_t) * 6, hipMemcpyDeviceToHost, s), "sah bounds");
     ev.record(1, s);
-    ev.sync(1);
 
-    std::vector<NodeTriple> triples;
-    triples.reserve(static_cast<size_t>(n / kSerial + 16) * 2);
-    std::vector<SerialTask> serial;
-    std::vector<HostTask> level, next;
-    auto pushChild = [&](int start, int count, int node, const float* lo, const float* hi, int buf) {
-        if (count <= kSerial) {
-            serial.push_back({start, count, node, buf});
-        } else {
-            HostTask h{start, count, node, {lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}};
-            next.push_back(h);
-        }
+    // Large tasks, level by level, enqueued in batches with no host work between the
+    // levels of a batch; after each batch one read of the next level's task count
+    // says whether to go on.  The first batch covers the levels a balanced split
+    // would need to bring every task under kChunk.
+    std::vector<hipEvent_t> lev;
+    auto mark = [&](hipStream_t st) {
+        const hipEvent_t e = takeEvent();
+        hipCheck(hipEventRecord(e, st), "event record");
+        lev.push_back(e);
     };
-    {
-        float lo[3], hi[3];
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = fromOrderedBits(cbh[a]);
-            hi[a] = fromOrderedBits(cbh[3 + a]);
-        }
-        pushChild(0, n, 0, lo, hi, 0);
-        level.swap(next);
-    }
-
-    double binMs = 0.0, splitMs = 0.0, partMs = 0.0;
-    SahBuildStats st{};
-    int cur = 0;
-    for (int lvl = 0; !level.empty(); ++lvl) {
-        next.clear();
-        const int nt = static_cast<int>(level.size());
-        long long slots = 0;
-        for (const HostTask& h : level) slots += h.count;
-        ++st.levels;
-        st.partitionedSlots += slots;
-        if (lvl < kMedianLevel) st.binnedSlots += slots;
-        if (nt > maxTasks) throw DeviceError("sah: task table overflow");
-        std::fill(hRes, hRes + nt, SahResult{});
-        if (lvl >= kMedianLevel) {
-            for (int i = 0; i < nt; ++i) {
-                SahResult& r = hRes[static_cast<size_t>(i)];
-                r.axis = -1;
-                r.nL = level[static_cast<size_t>(i)].count / 2;
-                for (int k = 0; k < 3; ++k) {
-                    r.lLo[k] = r.rLo[k] = level[static_cast<size_t>(i)].cLo[k];
-                    r.lHi[k] = r.rHi[k] = level[static_cast<size_t>(i)].cHi[k];
-                }
-            }
-        } else {
-            // large tasks first, then local ones
-            std::stable_partition(level.begin(), level.end(), [](const HostTask& h) { return h.count > kChunk; });
-            int nBig = 0;
-            while (nBig < nt && level[static_cast<size_t>(nBig)].count > kChunk) ++nBig;
-            if (nBig > maxBig) throw DeviceError("sah: task table overflow");
-            for (int i = 0; i < nt; ++i) {
-                const HostTask& h = level[static_cast<size_t>(i)];
-                SahTask& d = hTasks[static_cast<size_t>(i)];
-                d.start = h.start;
-                d.count = h.count;
-                d.pad0 = d.pad1 = 0;
-                for (int k = 0; k < 3; ++k) {
-                    d.cLo[k] = h.cLo[k];
-                    d.cHi[k] = h.cHi[k];
-                }
-                d.cLo[3] = d.cHi[3] = 0.0f;
-            }
-            int nChunks = 0;
-            for (int i = 0; i < nBig; ++i) {
-                hFirst[i] = nChunks;
-                const HostTask& h = level[static_cast<size_t>(i)];
-                for (int off = 0; off < h.count; off += kChunk) {
-                    if (nChunks >= maxChunks) throw DeviceError("sah: task table overflow");
-                    hChunks[nChunks++] = {i, h.start + off, std::min(kChunk, h.count - off), 0};
-                }
-            }
-            hFirst[nBig] = nChunks;
-            hipCheck(hipMemcpyAsync(dTasks.p, hTasks, sizeof(SahTask) * static_cast<size_t>(nt),
-                                    hipMemcpyHostToDevice, s), "sah tasks");
-            EventSet<4> lev;
-            lev.record(0, s);
-            if (nBig > 0) {
-                hipCheck(hipMemcpyAsync(dChunks.p, hChunks, sizeof(Chunk) * static_cast<size_t>(nChunks),
-                                        hipMemcpyHostToDevice, s), "sah chunks");
-                hipCheck(hipMemcpyAsync(dFirst.p, hFirst, sizeof(int) * static_cast<size_t>(nBig + 1),
-                                        hipMemcpyHostToDevice, s), "sah chunk index");
-                sahClearBinsKernel<<<blocksFor(static_cast<long long>(nBig) * kHist, kBlock), kBlock, 0, s>>>(dBins.p, nBig);
-                sahBinKernel<<<nChunks, kBlock, 0, s>>>(rec[cur].p, dTasks.p, dChunks.p, dBins.p);
-            }
-            if (nt > nBig)
-                sahLocalKernel<<<nt - nBig, kBlock, 0, s>>>(rec[cur].p, rec[1 - cur].p, dTasks.p, dRes.p, nBig, errDev);
-            lev.record(1, s);
-            if (nBig > 0) sahSplitKernel<<<nBig, 64, 0, s>>>(dBins.p, dTasks.p, dRes.p);
-            lev.record(2, s);
-            if (nBig > 0) {
-                sahPartCountKernel<<<nChunks, kBlock, 0, s>>>(rec[cur].p, dTasks.p, dRes.p, dChunks.p, dChunkLeft.p);
-                sahChunkScanKernel<<<blocksFor(nBig, kBlock), kBlock, 0, s>>>(dFirst.p, nBig, dChunkLeft.p, dChunkOff.p);
-                sahPartScatterKernel<<<nChunks, kBlock, 0, s>>>(rec[cur].p, rec[1 - cur].p, dTasks.p, dRes.p,
-                                                               dChunks.p, dChunkOff.p, errDev);
+    int level = 0;
+    if (n > kChunk) {
+        int batch = 1;
+        while ((static_cast<long long>(kChunk) << batch) < n) ++batch;
+        const int chunkGrid = T.maxChunks, taskGrid = blocksFor(T.maxBig, kBlock);
+        const int clearGrid = std::min(blocksFor(static_cast<long long>(T.maxBig) * kHist, kBlock), 1024);
+        while (true) {
+            for (int j = 0; j < batch; ++j, ++level) {
+                const int cur = level & 1;
+                const int median = level >= kMedianLevel ? 1 : 0;
+                mark(s);
+                sahPlanKernel<<<1, kBlock, 0, s>>>(T, cur, median);
+                sahClearBinsKernel<<<clearGrid, kBlock, 0, s>>>(T, cur);
+                sahBinKernel<<<chunkGrid, kBlock, 0, s>>>(T, cur);
+                mark(s);
+                sahSplitKernel<<<T.maxBig, 64, 0, s>>>(T, cur, median);
+                mark(s);
+                sahPartCountKernel<<<chunkGrid, kBlock, 0, s>>>(T, cur);
+                sahChunkScanKernel<<<taskGrid, kBlock, 0, s>>>(T, cur);
+                sahPartScatterKernel<<<chunkGrid, kBlock, 0, s>>>(T, cur);
+                sahAssignKernel<<<taskGrid, kBlock, 0, s>>>(T, cur, median, n, b.nodes.p, b.parent.p, b.farEnd.p);
+                mark(s);
             }
             hipCheck(hipGetLastError(), "sah level");
-            lev.record(3, s);
-            hipCheck(hipMemcpyAsync(hRes, dRes.p, sizeof(SahResult) * static_cast<size_t>(nt),
-                                    hipMemcpyDeviceToHost, s), "sah results");
-            hipCheck(hipMemcpyAsync(hErr, errDev, sizeof(int), hipMemcpyDeviceToHost, s), "sah check");
+            hipCheck(hipMemcpyAsync(hw, T.ctl, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, s), "sah control");
             hipCheck(hipStreamSynchronize(s), "sah level sync");
-            if (*hErr != 0) throw DeviceError("sah: partition counts disagree with the bin histogram");
-            binMs += lev.ms(0, 1);
-            splitMs += lev.ms(1, 2);
-            partMs += lev.ms(2, 3);
-            cur = 1 - cur;
+            if (hw[kCtlError] != 0) break;
+            if (hw[kCtlBig0 + (level & 1)] == 0) break;
+            if (level > 2 * kMedianLevel + 64) throw DeviceError("sah: level limit exceeded");
+            batch = 2;
         }
-        for (int i = 0; i < nt; ++i) {
-            const HostTask& h = level[static_cast<size_t>(i)];
-            const SahResult& r = hRes[static_cast<size_t>(i)];
-            int nL = r.nL;
-            if (nL < 1 || nL >= h.count) throw DeviceError("sah: empty side in a split");
-            const int g = h.start + nL - 1;
-            const int e = h.start + h.count - 1;
-            const int left = (g == h.start) ? n - 1 + g : g;
-            const int right = (g + 1 == e) ? n - 1 + e : g + 1;
-            triples.push_back({h.node, left, right, h.node == h.start ? e : h.start});
-            pushChild(h.start, nL, left, r.lLo, r.lHi, cur);
-            pushChild(g + 1, h.count - nL, right, r.rLo, r.rHi, cur);
-        }
-        level.swap(next);
-        if (lvl > 2 * kMedianLevel + 64) throw DeviceError("sah: level limit exceeded");
+    } else {
+        hipCheck(hipMemcpyAsync(hw, T.ctl, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, s), "sah control");
+        hipCheck(hipStreamSynchronize(s), "sah sync");
     }
-
-    for (const SerialTask& sk : serial) st.serialSlots += sk.count;
-    tSahStats = st;
-    // node records decided on the host, then the serial tails
-    DevBuf<NodeTriple> dTri(triples.size() + 1);
-    DevBuf<SerialTask> dSerial(serial.size() + 1);
-    if (!triples.empty()) {
-        hipCheck(hipMemcpyAsync(dTri.p, triples.data(), sizeof(NodeTriple) * triples.size(), hipMemcpyHostToDevice, s),
-                 "sah nodes");
-        sahWriteNodesKernel<<<blocksFor(static_cast<long long>(triples.size()), kBlock), kBlock, 0, s>>>(
-            dTri.p, static_cast<int>(triples.size()), b.nodes.p, b.parent.p, b.farEnd.p);
-    }
-    hipCheck(hipMemcpyAsync(dSerial.p, serial.data(), sizeof(SerialTask) * serial.size(), hipMemcpyHostToDevice, s),
-             "sah serial");
-    sahSerialKernel<<<blocksFor(static_cast<long long>(serial.size()), kBlock), kBlock, 0, s>>>(
-        rec[0].p, rec[1].p, dSerial.p, static_cast<int>(serial.size()), n, order.p, b.nodes.p, b.parent.p,
-        b.farEnd.p);
-    hipCheck(hipGetLastError(), "sah serial");
+    auto failed = [&]() {
+        for (hipEvent_t e : lev) giveEvent(e);
+        if (hw[kCtlError] & 1) throw DeviceError("sah: task table overflow");
+        if (hw[kCtlError] & 2) throw DeviceError("sah: partition counts disagree with the bin histogram");
+        throw DeviceError("sah: empty side in a split");
+    };
+    if (hw[kCtlError] != 0) failed();
     ev.record(2, s);
+    // local subtrees (the host knows their number from the last read), then the
+    // serial tails; the topology is checked complete before the bottom-up pass
+    // climbs it
+    if (hw[kCtlLocal] > 0)
+        sahSubtreeKernel<<<std::min(hw[kCtlLocal], T.maxLocal), kBlock, 0, s>>>(T, n, b.nodes.p, b.parent.p,
+                                                                            b.farEnd.p);
+    ev.record(3, s);
+    sahSerialKernel<<<std::min(blocksFor(T.maxSerial, kBlock), 1024), kBlock, 0, s>>>(T, n, order.p, b.nodes.p,
+                                                                                     b.parent.p, b.farEnd.p);
+    hipCheck(hipGetLastError(), "sah subtrees");
+    hipCheck(hipMemcpyAsync(hw, T.ctl, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, s), "sah control");
+    ev.record(4, s);
+    ev.sync(4);
+    if (hw[kCtlError] != 0) failed();
 
     finishTopology(b, dSrc, order.p, s);
+    hipCheck(hipMemcpyAsync(hw, T.ctl, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, s), "sah control");
     int depth = 0;
     hipCheck(hipMemcpyAsync(&depth, b.height.p, sizeof(int), hipMemcpyDeviceToHost, s), "depth");
-    ev.record(3, s);
-    ev.sync(3);
+    ev.record(5, s);
+    ev.sync(5);
     b.depth = depth;
+    SahBuildStats st;
+    st.levels = hw[kCtlMaxLevel] + 1;
+    st.binnedSlots = static_cast<unsigned int>(hw[kCtlBinned]);
+    st.partitionedSlots = static_cast<unsigned int>(hw[kCtlParted]);
+    st.serialSlots = static_cast<unsigned int>(hw[kCtlSerialSlots]);
+    tSahStats = st;
     if (t) {
+        double binMs = 0.0, splitMs = 0.0, partMs = 0.0;
+        for (size_t i = 0; i + 3 < lev.size(); i += 4) {
+            float v = 0.0f;
+            hipCheck(hipEventElapsedTime(&v, lev[i], lev[i + 1]), "event elapsed");
+            binMs += v;
+            hipCheck(hipEventElapsedTime(&v, lev[i + 1], lev[i + 2]), "event elapsed");
+            splitMs += v;
+            hipCheck(hipEventElapsedTime(&v, lev[i + 2], lev[i + 3]), "event elapsed");
+            partMs += v;
+        }
         t->mortonMs = ev.ms(0, 1);
         t->sortMs = 0.0;
-        t->treeMs = ev.ms(1, 2);
-        t->scatterMs = 0.0;
-        t->refitMs = ev.ms(2, 3);
+        t->treeMs = ev.ms(1, 4);
+        t->scatterMs = ev.ms(2, 4);  // local subtrees and serial tails
+        t->refitMs = ev.ms(4, 5);
         t->sahBinMs = binMs;
         t->sahSplitMs = splitMs;
         t->sahPartMs = partMs;
-        t->totalMs = ev.ms(0, 3);
+        t->totalMs = ev.ms(0, 5);
     }
+    for (hipEvent_t e : lev) giveEvent(e);
     b.primSlotValid = false;
     b.depthValid = false;
     b.wideValid = false;
