@@ -278,6 +278,13 @@ def main() -> int:
     extra = {}
     parity = {}
     cpu_baseline = None
+    if args.profile:
+        # the refit kernels for the counter runs: the 1% dirty refit (dense path) and
+        # a full refit from device-resident items, then back to the original scene
+        moved, dirty = S.displace_vertices(args.cells, tris, 0.01, 0.05, seed=4)
+        lbvh.refit(moved, dirty)
+        lbvh.refit_device(dtris.ptr.value, n)
+        log("profile: dirty refit and full device refit done")
     if not args.profile:
         # ---- parity of the query pass against the CPU oracle -----------------------
         import oracle as O

@@ -274,10 +274,12 @@ QR_API int qr_batch_read_scene(const qr_batch* b, int scene, void* nodes_host, i
 /* FNV-1a over one scene's node bytes: equals qr_bvh_tree_hash of that scene built alone. */
 QR_API int qr_batch_tree_hash(const qr_batch* b, int scene, uint64_t* out);
 /* Query plan over device points (scene-major, qoffsets_host[scenes + 1]); k = 0: closest,
- * 1..32: kNN.  tune = 1 times every kernel variant (stack in LDS with 256 or 64 lanes
- * per workgroup, or in scratch) with and without per-scene Morton order and keeps the
- * fastest; tune = 0 takes variant 0 with Morton order; tune = 2 + 2 v + m forces
- * variant v with Morton order m.  The chosen pipeline is captured in a hipGraph.
+ * 1..32: kNN.  Kernel variants v: 0..3 walk the binary nodes, 4..7 the 4-wide nodes
+ * (collapsed for every scene on first use), each with the traversal stack in LDS with
+ * 256 or 64 lanes per workgroup (v % 4 = 0, 1), in scratch (2), or its top 8 entries in
+ * registers (3).  tune = 1 times every variant with and without per-scene Morton order
+ * and keeps the fastest; tune = 0 takes variant 0 with Morton order; tune = 2 + 2 v + m
+ * forces variant v with Morton order m.  The chosen pipeline is captured in a hipGraph.
  * The points must stay valid while the plan lives. */
 QR_API int qr_batch_plan_create(qr_batch* b, const float* points_dev, const int* qoffsets_host, int k,
                                 int tune, qr_batch_plan** out);

@@ -924,6 +924,8 @@ void enqueuePlan(qr_batch_plan& p, int variant, bool morton, hipStream_t s) {
     BatchQueryArgs a;
     a.kind = b.kind;
     a.nodes = b.nodes.p;
+    a.wide = b.wide.p;
+    a.wideRoot = b.wideRoot.p;
     a.items = b.items.p;
     a.primIds = b.primIds.p;
     a.primOff = b.primOff.p;
@@ -1087,8 +1089,10 @@ int qr_batch_plan_create(qr_batch* b, const float* points, const int* qoffsets, 
                 need(tune < 2 + 2 * kBatchVariants, "unknown forced batch configuration");
                 bestV = (tune - 2) / 2;
                 bestM = ((tune - 2) % 2) != 0;
+                if (batchVariantWide(bestV)) buildBatchWide(b->s, p->stream);
                 best = timePlan(*p, bestV, bestM);
             } else if (tune == 1) {
+                buildBatchWide(b->s, p->stream);  // the 4-wide variants are candidates too
                 for (int v = 0; v < kBatchVariants; ++v)
                     for (int m = 0; m < 2; ++m) {
                         const double ms = timePlan(*p, v, m != 0);

@@ -316,6 +316,142 @@ void buildWide(BvhState& b, hipStream_t s) {
     b.wideValid = true;
 }
 
+// ---- 4-wide collapse of a batch ------------------------------------------------------
+// Scene sc's binary nodes sit at 2 primOff[sc] - sc (local ids, root 0).  Depths come
+// from pointer jumping over global node indices (a root jumps to itself), the wide
+// nodes of all scenes are numbered by one scan over the flags of every node, and each
+// scene's wide nodes keep local child indices (global index minus the scene's root
+// index), so a scene's walk runs on wide + wideRoot[sc] as on a plain tree.
+namespace {
+
+// Scene of a global node index: the last sc with 2 primOff[sc] - sc <= g.
+__device__ __forceinline__ int nodeScene(const int* __restrict__ primOff, int scenes, int g) {
+    int lo = 0, hi = scenes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (2 * primOff[mid] - mid <= g) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return lo;
+}
+
+__global__ void batchJumpInitKernel(const int* __restrict__ parent, const int* __restrict__ primOff, int scenes,
+                                    int nodeCount, unsigned long long* __restrict__ jump) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= nodeCount) return;
+    const int sc = nodeScene(primOff, scenes, g);
+    const int off = 2 * primOff[sc] - sc;
+    const int p = parent[g];
+    const int anc = p < 0 ? g : off + p;
+    const int dist = p < 0 ? 0 : 1;
+    jump[g] = (static_cast<unsigned long long>(static_cast<uint32_t>(dist)) << 32) | static_cast<uint32_t>(anc);
+}
+
+// Every step moves to a strict ancestor, so the loop ends at the scene's root.
+__global__ void batchJumpKernel(int nodeCount, unsigned long long* __restrict__ jump, int* __restrict__ depth) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= nodeCount) return;
+    const unsigned long long j0 = __hip_atomic_load(&jump[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int anc = static_cast<int>(j0 & 0xffffffffu);
+    int dist = static_cast<int>(j0 >> 32);
+    while (true) {
+        const unsigned long long pj = __hip_atomic_load(&jump[anc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int a2 = static_cast<int>(pj & 0xffffffffu);
+        if (a2 == anc) break;
+        dist += static_cast<int>(pj >> 32);
+        anc = a2;
+        const unsigned long long mine = (static_cast<unsigned long long>(static_cast<uint32_t>(dist)) << 32) |
+                                        static_cast<uint32_t>(anc);
+        __hip_atomic_store(&jump[g], mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    depth[g] = dist;
+}
+
+// flags[g] = 1 for an internal node (local id below the scene's n - 1) at even depth.
+__global__ void batchWideFlagKernel(const int* __restrict__ depth, const int* __restrict__ primOff, int scenes,
+                                    int nodeCount, int* __restrict__ flags) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= nodeCount) return;
+    const int sc = nodeScene(primOff, scenes, g);
+    const int local = g - (2 * primOff[sc] - sc);
+    const int ns = primOff[sc + 1] - primOff[sc];
+    flags[g] = (local < ns - 1 && (depth[g] & 1) == 0) ? 1 : 0;
+}
+
+__global__ void batchWideRootKernel(const int* __restrict__ primOff, int scenes, const int* __restrict__ wideId,
+                                    const int* __restrict__ total, int* __restrict__ wideRoot) {
+    const int sc = blockIdx.x * kBlock + threadIdx.x;
+    if (sc < scenes) wideRoot[sc] = wideId[2 * primOff[sc] - sc];
+    if (sc == scenes) wideRoot[sc] = *total;
+}
+
+// Four lanes per node, as wideEmitKernel; interior entries hold scene-local wide ids.
+__global__ __launch_bounds__(kBlock) void batchWideEmitKernel(const Node2* __restrict__ nodes,
+                                                              const int* __restrict__ primOff, int scenes,
+                                                              int nodeCount, const int* __restrict__ flags,
+                                                              const int* __restrict__ wideId,
+                                                              const int* __restrict__ wideRoot,
+                                                              Wide4* __restrict__ wide) {
+    const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const int g = static_cast<int>(t >> 2);
+    const int slot = static_cast<int>(t & 3);
+    if (g >= nodeCount || flags[g] == 0) return;
+    const int sc = nodeScene(primOff, scenes, g);
+    const int off = 2 * primOff[sc] - sc;
+    const int ns = primOff[sc + 1] - primOff[sc];
+    const Node2* tn = nodes + off;
+    const Node2 nd = nodes[g];
+    const int a0 = nd.left >= ns - 1 ? 1 : 2;
+    const int a1 = nd.right >= ns - 1 ? 1 : 2;
+    WideEntry e;
+    clearEntry(e);
+    int src = -1;
+    if (slot < a0) {
+        src = a0 == 1 ? nd.left : (slot == 0 ? tn[nd.left].left : tn[nd.left].right);
+    } else if (slot - a0 < a1) {
+        const int j = slot - a0;
+        src = a1 == 1 ? nd.right : (j == 0 ? tn[nd.right].left : tn[nd.right].right);
+    }
+    if (src >= 0) {
+        const Node2 sn = tn[src];
+        setEntry(e, sn, src >= ns - 1 ? sn.left : wideId[off + src] - wideRoot[sc]);
+    }
+    wide[wideId[g]].c[slot] = e;
+}
+
+}  // namespace
+
+void buildBatchWide(BatchState& b, hipStream_t s) {
+    if (b.wideValid) return;
+    const int nc = 2 * b.total - b.scenes;
+    DevBuf<unsigned long long> jump(static_cast<size_t>(nc));
+    DevBuf<int> depth(static_cast<size_t>(nc)), flags(static_cast<size_t>(nc)),
+        scan(static_cast<size_t>(scanScratchInts(nc)) + 1);
+    b.wideId.alloc(static_cast<size_t>(nc));
+    b.wideRoot.alloc(static_cast<size_t>(b.scenes) + 1);
+    const int grid = blocksFor(nc, kBlock);
+    batchJumpInitKernel<<<grid, kBlock, 0, s>>>(b.parent.p, b.primOff.p, b.scenes, nc, jump.p);
+    batchJumpKernel<<<grid, kBlock, 0, s>>>(nc, jump.p, depth.p);
+    batchWideFlagKernel<<<grid, kBlock, 0, s>>>(depth.p, b.primOff.p, b.scenes, nc, flags.p);
+    int* total = scan.p + scanScratchInts(nc);
+    scanExclusive(flags.p, b.wideId.p, nc, total, scan.p, s);
+    batchWideRootKernel<<<blocksFor(b.scenes + 1, kBlock), kBlock, 0, s>>>(b.primOff.p, b.scenes, b.wideId.p,
+                                                                          total, b.wideRoot.p);
+    int count = 0;
+    hipCheck(hipMemcpyAsync(&count, total, sizeof(int), hipMemcpyDeviceToHost, s), "batch wide count");
+    hipCheck(hipStreamSynchronize(s), "batch wide count sync");
+    if (count < b.scenes) throw DeviceError("batch wide: fewer wide nodes than scenes");
+    b.wide.alloc(static_cast<size_t>(count));
+    batchWideEmitKernel<<<blocksFor(4LL * nc, kBlock), kBlock, 0, s>>>(b.nodes.p, b.primOff.p, b.scenes, nc,
+                                                                       flags.p, b.wideId.p, b.wideRoot.p, b.wide.p);
+    hipCheck(hipGetLastError(), "batch wide");
+    hipCheck(hipStreamSynchronize(s), "batch wide sync");
+    b.wideValid = true;
+}
+
 // Re-emits every wide node from the current binary boxes (after a refit).
 void reemitWide(BvhState& b, hipStream_t s) {
     if (!b.wideValid) return;

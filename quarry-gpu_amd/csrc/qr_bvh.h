@@ -86,8 +86,15 @@ struct BatchState {
     DevBuf<int> primIds;             // per slot: scene-local index
     DevBuf<int> parent, arrive, height, depth, sceneId;
     SortWorkspace ws;
+    // 4-wide nodes of every scene (buildBatchWide): scene sc's wide nodes start at
+    // wideRoot[sc] (its root) and hold scene-local wide ids; wideRoot[scenes] = total
+    bool wideValid = false;
+    DevBuf<Wide4> wide;
+    DevBuf<int> wideId, wideRoot;
 };
 
+// 4-wide collapse of every scene of a batch (once; synchronous).
+void buildBatchWide(BatchState& b, hipStream_t s);
 // Segmented LBVH of all scenes; b.primOff / primOffH / scenes / total must be set.
 void buildBatchDevice(BatchState& b, int kind, const void* dItems, hipStream_t s, BuildTimingsC* t);
 // sceneId[i] = scene of item i, for scene-major items with offsets primOff.

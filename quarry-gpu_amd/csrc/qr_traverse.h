@@ -83,6 +83,8 @@ void segmentSort(const int* offsets, uint64_t* keys, int count, int maxSeg, int*
 struct BatchQueryArgs {
     int kind = 0;
     const Node2* nodes = nullptr;   // all trees (scene s from 2 primOff[s] - s)
+    const Wide4* wide = nullptr;    // 4-wide nodes of all trees (scene s from wideRoot[s]), wide variants
+    const int* wideRoot = nullptr;  // scenes + 1
     const void* items = nullptr;    // leaf-order items of all trees
     const int* primIds = nullptr;   // per slot: scene-local index
     const int* primOff = nullptr;   // scenes + 1
@@ -95,8 +97,15 @@ struct BatchQueryArgs {
     ClosestResult* out = nullptr;   // per query: outStride records, scene-local indices
     int* counter = nullptr;         // work-queue head (reset before each launch)
 };
-// Stack placement / workgroup size variants the autotuner chooses from.
-enum BatchVariant : int { kBatchLds256 = 0, kBatchLds64 = 1, kBatchScratch64 = 2, kBatchVariants = 3 };
+// Variants the autotuner chooses from: binary or 4-wide nodes; the stack in LDS
+// (256- or 64-lane workgroups, spilling to scratch), in scratch, or its top entries in
+// registers.
+enum BatchVariant : int {
+    kBatchLds256 = 0, kBatchLds64 = 1, kBatchScratch64 = 2, kBatchReg64 = 3,
+    kBatchWideLds256 = 4, kBatchWideLds64 = 5, kBatchWideScratch64 = 6, kBatchWideReg64 = 7,
+    kBatchVariants = 8
+};
+inline bool batchVariantWide(int v) { return v >= kBatchWideLds256; }
 void launchBatchNearest(const BatchQueryArgs& a, int variant, hipStream_t s);
 void launchBatchReset(int* counter, hipStream_t s);
 

@@ -41,7 +41,6 @@ constexpr int kChunk = 4096;
 constexpr int kSerial = 64;
 constexpr int kMedianLevel = 32;
 constexpr int kBlock = 256;
-constexpr int kPerThread = kChunk / kBlock;
 constexpr int kBinVals = 13;  // count, box lo xyz, box hi xyz, centroid lo xyz, centroid hi xyz
 constexpr int kHist = 3 * kBins * kBinVals;
 
@@ -314,6 +313,63 @@ __device__ __forceinline__ int blockExclusive(int v, int* lds, int* total) {
     __syncthreads();
     *total = all;
     return base + x - v;
+}
+
+// Stable partition of the records [begin, begin + cnt) of task t by split s into
+// dst: the range's left records go to leftDst onwards, its right records to rightDst
+// onwards, each side in record order.  Wave w takes the w-th quarter of the range and
+// reads it lane-strided (coalesced): one ballot pass counts its lefts, the waves'
+// counts meet in LDS (one barrier), and a second pass writes every record at its
+// rank.  Called by the whole workgroup.  A destination outside the task's side
+// (counts that disagree with the histogram) is reported through err, never written.
+__device__ void partitionRange(const PrimRec* __restrict__ src, PrimRec* __restrict__ dst, int begin, int cnt,
+                               int leftDst, int rightDst, const SahTask& t, const SahResult& s, const AxisMap& m,
+                               int* ldsWave, int* err) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr int kWaves = kBlock / 64;
+    const int per = (cnt + kWaves - 1) / kWaves;
+    const int wBeg = w * per < cnt ? w * per : cnt;
+    const int wEnd = wBeg + per < cnt ? wBeg + per : cnt;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int lefts = 0;
+    for (int r = wBeg; r < wEnd; r += 64) {
+        const int i = r + lane;
+        const bool left = i < wEnd && goesLeft(src[begin + i], begin + i, t, s, m);
+        lefts += __popcll(__ballot(left));
+    }
+    if (lane == 0) ldsWave[w] = lefts;
+    __syncthreads();
+    int leftBefore = 0;
+    for (int v = 0; v < w; ++v) leftBefore += ldsWave[v];
+    __syncthreads();  // ldsWave is free for the caller's next use
+    int runL = leftDst + leftBefore;
+    int runR = rightDst + (wBeg - leftBefore);
+    const int leftEnd = t.start + s.nL, rightEnd = t.start + t.count;
+    bool bad = false;
+    for (int r = wBeg; r < wEnd; r += 64) {
+        const int i = r + lane;
+        const bool valid = i < wEnd;
+        PrimRec rec{};
+        bool left = false;
+        if (valid) {
+            rec = src[begin + i];
+            left = goesLeft(rec, begin + i, t, s, m);
+        }
+        const unsigned long long lm = __ballot(valid && left);
+        const unsigned long long rm = __ballot(valid && !left);
+        if (valid) {
+            const int d = left ? runL + __popcll(lm & below) : runR + __popcll(rm & below);
+            const bool ok = left ? (d >= t.start && d < leftEnd) : (d >= leftEnd && d < rightEnd);
+            if (ok) {
+                dst[d] = rec;
+            } else {
+                bad = true;
+            }
+        }
+        runL += __popcll(lm);
+        runR += __popcll(rm);
+    }
+    if (bad) atomicOr(err, 2);
 }
 
 // ---- kernels ------------------------------------------------------------------------------
@@ -646,37 +702,10 @@ __global__ __launch_bounds__(kBlock) void sahPartScatterKernel(SahTables T, int 
     const SahTask t = bigOf(T, cur)[ch.task];
     const SahResult s = T.results[ch.task];
     const AxisMap m = axisMap(t);
-    const PrimRec* rec = recOf(T, t.meta & 1);
-    PrimRec* out = recOf(T, 1 - (t.meta & 1));
-    const int base = threadIdx.x * kPerThread;
-    unsigned flags = 0u;
-    int c = 0;
-    for (int j = 0; j < kPerThread; ++j) {
-        const int i = base + j;
-        if (i < ch.count && goesLeft(rec[ch.start + i], ch.start + i, t, s, m)) {
-            flags |= 1u << j;
-            ++c;
-        }
-    }
-    int total = 0;
-    int lrank = blockExclusive(c, lds, &total);
-    const int leftBase = t.start + T.chunkOff[blockIdx.x];
+    const int chunkLeft = T.chunkOff[blockIdx.x];  // lefts of the task's earlier chunks
     const int chunkPos = ch.start - t.start;
-    const int rightBase = t.start + s.nL + (chunkPos - T.chunkOff[blockIdx.x]);
-    int rrank = base - lrank;
-    const int leftEnd = t.start + s.nL, rightEnd = t.start + t.count;
-    for (int j = 0; j < kPerThread; ++j) {
-        const int i = base + j;
-        if (i >= ch.count) break;
-        const PrimRec r = rec[ch.start + i];
-        const int dst = (flags & (1u << j)) ? leftBase + lrank++ : rightBase + rrank++;
-        const bool ok = (flags & (1u << j)) ? dst < leftEnd : (dst >= leftEnd && dst < rightEnd);
-        if (ok) {
-            out[dst] = r;
-        } else {
-            atomicOr(&T.ctl[kCtlError], 2);  // counts disagree with the histogram: reported, never written
-        }
-    }
+    partitionRange(recOf(T, t.meta & 1), recOf(T, 1 - (t.meta & 1)), ch.start, ch.count, t.start + chunkLeft,
+                   t.start + s.nL + (chunkPos - chunkLeft), t, s, m, lds, &T.ctl[kCtlError]);
     if (threadIdx.x == 0) atomicAdd(&T.ctl[kCtlParted], ch.count);
 }
 
@@ -705,6 +734,20 @@ __global__ void sahAssignKernel(SahTables T, int cur, int median, int n, Node2* 
 // down to ranges of kSerial, from an LDS stack -- histogram, plane choice and stable
 // partition per task inside the workgroup, no return to the host between levels.
 constexpr int kSubtreeStack = 64;  // a chain of count/8 sides from 4096 to 64 is 32 deep
+constexpr int kSerialBatch = 64;
+
+// Appends count buffered serial ranges to the global list with one counter bump.
+__device__ __forceinline__ void flushSerial(const SahTables& T, const SerialTask* buf, int count) {
+    if (count == 0) return;
+    const int at = atomicAdd(&T.ctl[kCtlSerial], count);
+    for (int i = 0; i < count; ++i) {
+        if (at + i < T.maxSerial) {
+            T.serial[at + i] = buf[i];
+        } else {
+            atomicOr(&T.ctl[kCtlError], 1);
+        }
+    }
+}
 __global__ __launch_bounds__(kBlock) void sahSubtreeKernel(SahTables T, int n, Node2* __restrict__ nodes,
                                                            int* __restrict__ parent, int* __restrict__ farEnd) {
     __shared__ uint32_t h[kHist];
@@ -713,6 +756,8 @@ __global__ __launch_bounds__(kBlock) void sahSubtreeKernel(SahTables T, int n, N
     __shared__ SahTask curTask;
     __shared__ int ssp;
     __shared__ int lds[4];
+    __shared__ SerialTask serBuf[kSerialBatch];
+    int serN = 0;  // thread 0's count of buffered serial ranges
     if (threadIdx.x == 0) {
         stk[0] = T.local[blockIdx.x];
         ssp = 1;
@@ -747,31 +792,7 @@ __global__ __launch_bounds__(kBlock) void sahSubtreeKernel(SahTables T, int n, N
         __syncthreads();
         const SahResult s = sres;
         if (!median) {
-            const int base = threadIdx.x * kPerThread;
-            unsigned flags = 0u;
-            int c = 0;
-            for (int j = 0; j < kPerThread; ++j) {
-                const int i = base + j;
-                if (i < t.count && goesLeft(src[t.start + i], t.start + i, t, s, m)) {
-                    flags |= 1u << j;
-                    ++c;
-                }
-            }
-            int total = 0;
-            int lrank = blockExclusive(c, lds, &total);
-            int rrank = base - lrank;
-            for (int j = 0; j < kPerThread; ++j) {
-                const int i = base + j;
-                if (i >= t.count) break;
-                const PrimRec r = src[t.start + i];
-                const bool left = (flags & (1u << j)) != 0;
-                const int k = left ? lrank++ : rrank++;
-                if (left ? k < s.nL : k < t.count - s.nL) {
-                    dst[t.start + (left ? 0 : s.nL) + k] = r;
-                } else {
-                    atomicOr(&T.ctl[kCtlError], 2);  // counts disagree with the histogram
-                }
-            }
+            partitionRange(src, dst, t.start, t.count, t.start, t.start + s.nL, t, s, m, lds, &T.ctl[kCtlError]);
             binned += t.count;
             parted += t.count;
         }
@@ -787,7 +808,13 @@ __global__ __launch_bounds__(kBlock) void sahSubtreeKernel(SahTables T, int n, N
                 const int cs[2] = {t.start, t.start + nL}, cc[2] = {nL, t.count - nL}, ci[2] = {l, r};
                 for (int side = 1; side >= 0; --side) {
                     if (cc[side] <= kSerial) {
-                        pushSerial(T, cs[side], cc[side], ci[side], childBuf);
+                        // serial ranges collect in LDS and go out in batches: one global
+                        // counter bump per batch instead of one per range
+                        if (serN == kSerialBatch) {
+                            flushSerial(T, serBuf, serN);
+                            serN = 0;
+                        }
+                        serBuf[serN++] = {cs[side], cc[side], ci[side], childBuf};
                     } else if (ssp < kSubtreeStack) {
                         stk[ssp++] = makeTask(cs[side], cc[side], ci[side], metaOf(childBuf, level + 1),
                                               side ? s.rLo : s.lLo, side ? s.rHi : s.lHi);
@@ -800,6 +827,7 @@ __global__ __launch_bounds__(kBlock) void sahSubtreeKernel(SahTables T, int n, N
         __syncthreads();
     }
     if (threadIdx.x == 0) {
+        flushSerial(T, serBuf, serN);
         atomicAdd(&T.ctl[kCtlBinned], binned);
         atomicAdd(&T.ctl[kCtlParted], parted);
         atomicMax(&T.ctl[kCtlMaxLevel], deepest);

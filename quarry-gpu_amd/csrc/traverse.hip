@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "qr_traverse.h"
@@ -149,6 +150,47 @@ struct SpillArea {
             (out) = *(spill).at((st).sp - (st).kDepth);       \
         }                                                     \
     } while (0)
+
+// Stack operations shared by the walks: the LDS/scratch stack above, and a stack
+// whose top R entries live in registers (a shift register: static indices only, so
+// it stays in VGPRs) with deeper entries in the lane's scratch spill area.
+template <typename Stack>
+__device__ __forceinline__ void stackPush(Stack& st, SpillArea& spill, int v) {
+    QR_PUSH(st, spill, v);
+}
+template <typename Stack>
+__device__ __forceinline__ int stackPop(Stack& st, SpillArea& spill) {
+    int v;
+    QR_POP(st, spill, v);
+    return v;
+}
+
+template <int R>
+struct RegStack {
+    int r[R];  // r[0]: top
+    int sp;    // entries; those beyond R sit in the spill area, deepest first
+    __device__ __forceinline__ void init(int*) { sp = 0; }
+    __device__ __forceinline__ bool empty() const { return sp == 0; }
+    __device__ __forceinline__ bool room(int n) const { return sp + n <= R + kSpill; }
+};
+template <int R>
+__device__ __forceinline__ void stackPush(RegStack<R>& st, SpillArea& spill, int v) {
+    if (st.sp >= R) *spill.at(st.sp - R) = st.r[R - 1];
+#pragma unroll
+    for (int k = R - 1; k > 0; --k) st.r[k] = st.r[k - 1];
+    st.r[0] = v;
+    ++st.sp;
+}
+template <int R>
+__device__ __forceinline__ int stackPop(RegStack<R>& st, SpillArea& spill) {
+    const int v = st.r[0];
+#pragma unroll
+    for (int k = 0; k < R - 1; ++k) st.r[k] = st.r[k + 1];
+    --st.sp;
+    if (st.sp >= R) st.r[R - 1] = *spill.at(st.sp - R);
+    return v;
+}
+constexpr int kRegStack = 8;
 
 // Compare-exchange on (key, payload) pairs: ascending by key.
 __device__ __forceinline__ void cx(float& ka, int& va, float& kb, int& vb) {
@@ -506,7 +548,7 @@ __device__ __forceinline__ void nearestBinaryWalk(const Node2* __restrict__ node
                     overflow = 1;
                     break;
                 }
-                QR_PUSH(st, spill, packEntry(c1, k1, idBits));
+                stackPush(st, spill, packEntry(c1, k1, idBits));
             }
             id = c0;
         } else if (in1) {
@@ -514,8 +556,7 @@ __device__ __forceinline__ void nearestBinaryWalk(const Node2* __restrict__ node
         } else {
             id = -1;
             while (!st.empty()) {
-                int v;
-                QR_POP(st, spill, v);
+                const int v = stackPop(st, spill);
                 if (list.admits(entryKey(v, idBits))) {
                     id = entryId(v, idBits);
                     break;
@@ -551,16 +592,18 @@ __global__ __launch_bounds__(kBlock) void nearestBinaryKernel(
 // ---- batched scenes: persistent work-queue traversal ------------------------------------
 // Queries of all scenes form one queue (scene-major, optionally Morton-ordered inside
 // each scene).  Each wave takes the next 64 queries with one atomic and walks, per
-// lane, the tree of that query's scene (scene s: items from primOff[s], nodes from
-// 2 primOff[s] - s); the grid is sized to the resident occupancy and every wave
-// leaves when the queue is exhausted.  kD = 0 keeps the whole stack in scratch.
-template <typename L, int K, int kBT, int kD>
+// lane, the tree of that query's scene (scene s: items from primOff[s]; binary nodes
+// from 2 primOff[s] - s, or 4-wide nodes from wideRoot[s]); the grid is sized to the
+// resident occupancy and every wave leaves when the queue is exhausted.  Stack
+// placement: kD entries per lane in LDS with the rest in scratch (kD = 0: all in
+// scratch), or, with kReg, the top kRegStack entries in registers.
+template <typename L, int K, int kBT, int kD, bool kWide, bool kReg>
 __global__ __launch_bounds__(kBT) void batchNearestKernel(
-    const Node2* __restrict__ nodes, const typename L::Item* __restrict__ prims,
-    const int* __restrict__ primIds, const int* __restrict__ primOff, const V3* __restrict__ points,
-    const int* __restrict__ qScene, const uint32_t* __restrict__ order, int count, int k, int outStride,
-    ClosestResult* __restrict__ out, int* __restrict__ counter) {
-    __shared__ int lds[(kD > 0 ? kD : 1) * kBT];
+    const Node2* __restrict__ nodes, const Wide4* __restrict__ wide, const int* __restrict__ wideRoot,
+    const typename L::Item* __restrict__ prims, const int* __restrict__ primIds, const int* __restrict__ primOff,
+    const V3* __restrict__ points, const int* __restrict__ qScene, const uint32_t* __restrict__ order, int count,
+    int k, int outStride, ClosestResult* __restrict__ out, int* __restrict__ counter) {
+    __shared__ int lds[(kD > 0 && !kReg ? kD : 1) * kBT];
     const int lane = threadIdx.x & 63;
     while (true) {
         int base = 0;
@@ -574,14 +617,24 @@ __global__ __launch_bounds__(kBT) void batchNearestKernel(
             const int ps = primOff[s];
             KnnList<K> list;
             list.init();
-            LaneStackT<kBT, kD> st;
+            typename std::conditional<kReg, RegStack<kRegStack>, LaneStackT<kBT, kD>>::type st;
             SpillArea spill;
             st.init(lds);
             int a = 0, b = 0, c = 0, d = 0;
-            // node ids of the scene's tree stay below 2 n_s - 1
-            const int idBits = stackIdBitsDev(2 * (primOff[s + 1] - ps) - 1);
-            nearestBinaryWalk<L, K, false>(nodes + (2 * ps - s), prims + ps, primIds + ps, points[q], list,
-                                           st, spill, a, b, c, d, idBits);
+            if (kWide) {
+                const int wr = wideRoot[s];
+                // wide ids of the scene's tree stay below its wide-node count
+                nearestWideWalk<L, K, false>(wide + wr, prims + ps, primIds + ps, points[q], list, st, spill, a, b,
+                                             c, d, stackIdBitsDev(wideRoot[s + 1] - wr));
+            } else {
+                // node ids of the scene's tree stay below 2 n_s - 1
+                nearestBinaryWalk<L, K, false>(nodes + (2 * ps - s), prims + ps, primIds + ps, points[q], list, st,
+                                               spill, a, b, c, d, stackIdBitsDev(2 * (primOff[s + 1] - ps) - 1));
+            }
+            if (c) {
+                int* w = gOverflowWord;
+                if (w) __hip_atomic_store(w, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
             writeList<L, K>(list, k, out + static_cast<size_t>(q) * outStride);
         }
     }
@@ -589,6 +642,76 @@ __global__ __launch_bounds__(kBT) void batchNearestKernel(
 
 __global__ void batchResetKernel(int* __restrict__ counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
+}
+
+// Closest / kNN walk of one query over a 4-wide tree whose node ids, leaf slots and
+// prim ids are local to `wide`, `prims` and `primIds` (root at wide[0]).
+template <typename L, int K, bool kStats, typename Stack>
+__device__ __forceinline__ void nearestWideWalk(const Wide4* __restrict__ wide,
+                                                const typename L::Item* __restrict__ prims,
+                                                const int* __restrict__ primIds, V3 p, KnnList<K>& list,
+                                                Stack& st, SpillArea& spill, int& nodeTests, int& leafTests,
+                                                int& overflow, int& opened, int idBits) {
+    int id = 0;
+    while (id >= 0) {
+        if (kStats) ++opened;
+        WideEntry e[4];
+        loadWide(wide, id, e);
+        float key[4];
+        int c[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            c[s] = e[s].child;
+            key[s] = kInf;
+            if (c[s] != kWideEmpty) {
+                const float d = boxDist2(p, entryBox(e[s]));
+                if (list.admits(d)) {
+                    key[s] = d;
+                    ++nodeTests;
+                } else {
+                    c[s] = kWideEmpty;
+                }
+            }
+        }
+        sort4(key, c);
+        // leaves now, nearest first, against the live bound
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (c[s] != kWideEmpty && c[s] < 0 && list.admits(key[s])) {
+                const int slot = -c[s] - 1;
+                ++leafTests;
+                list.offer(L::dist2(p, prims[slot]), primIds[slot]);
+            }
+        }
+        // nearest surviving interior child next; the others pushed far first with
+        // their distance keys; pops skip entries the k-th distance has overtaken
+        int first = -1;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (c[s] >= 0 && list.admits(key[s]) && first < 0) first = s;
+        }
+        if (!st.room(3)) {
+            overflow = 1;
+            break;
+        }
+#pragma unroll
+        for (int s = 3; s >= 0; --s) {
+            if (c[s] >= 0 && list.admits(key[s]) && s != first) stackPush(st, spill, packEntry(c[s], key[s], idBits));
+        }
+        id = -1;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            if (s == first) id = c[s];
+        if (first < 0) {
+            while (!st.empty()) {
+                const int v = stackPop(st, spill);
+                if (list.admits(entryKey(v, idBits))) {
+                    id = entryId(v, idBits);
+                    break;
+                }
+            }
+        }
+    }
 }
 
 template <typename L, int K, bool kStats>
@@ -602,73 +725,13 @@ __global__ __launch_bounds__(kBlock) void nearestWideKernel(
     int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
     if (i < count) {
         const int q = order ? static_cast<int>(order[i]) : i;
-        const V3 p = points[q];
         KnnList<K> list;
         list.init();
         LaneStack st;
         SpillArea spill;
         st.init(lds);
-        int id = 0;
-        while (id >= 0) {
-            if (kStats) ++opened;
-            WideEntry e[4];
-            loadWide(wide, id, e);
-            float key[4];
-            int c[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                c[s] = e[s].child;
-                key[s] = kInf;
-                if (c[s] != kWideEmpty) {
-                    const float d = boxDist2(p, entryBox(e[s]));
-                    if (list.admits(d)) {
-                        key[s] = d;
-                        ++nodeTests;
-                    } else {
-                        c[s] = kWideEmpty;
-                    }
-                }
-            }
-            sort4(key, c);
-            // leaves now, nearest first, against the live bound
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                if (c[s] != kWideEmpty && c[s] < 0 && list.admits(key[s])) {
-                    const int slot = -c[s] - 1;
-                    ++leafTests;
-                    list.offer(L::dist2(p, prims[slot]), primIds[slot]);
-                }
-            }
-            // nearest surviving interior child next; the others pushed far first with
-            // their distance keys; pops skip entries the k-th distance has overtaken
-            int first = -1;
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                if (c[s] >= 0 && list.admits(key[s]) && first < 0) first = s;
-            }
-            if (!st.room(3)) {
-                overflow = 1;
-                break;
-            }
-#pragma unroll
-            for (int s = 3; s >= 0; --s) {
-                if (c[s] >= 0 && list.admits(key[s]) && s != first) QR_PUSH(st, spill, packEntry(c[s], key[s], idBits));
-            }
-            id = -1;
-#pragma unroll
-            for (int s = 0; s < 4; ++s)
-                if (s == first) id = c[s];
-            if (first < 0) {
-                while (!st.empty()) {
-                    int v;
-                    QR_POP(st, spill, v);
-                    if (list.admits(entryKey(v, idBits))) {
-                        id = entryId(v, idBits);
-                        break;
-                    }
-                }
-            }
-        }
+        nearestWideWalk<L, K, kStats>(wide, prims, primIds, points[q], list, st, spill, nodeTests, leafTests,
+                                      overflow, opened, idBits);
         writeList<L, K>(list, k, out + static_cast<size_t>(q) * outStride);
     }
     flushStats<kStats>(stats, nodeTests, leafTests, overflow, opened);
@@ -2359,9 +2422,9 @@ int residentCus() {
     return cus;
 }
 
-template <typename L, int K, int kBT, int kD>
+template <typename L, int K, int kBT, int kD, bool kWide, bool kReg>
 void launchBatchTKV(const BatchQueryArgs& a, hipStream_t s) {
-    auto kern = batchNearestKernel<L, K, kBT, kD>;
+    auto kern = batchNearestKernel<L, K, kBT, kD, kWide, kReg>;
     static int perCu = 0;
     if (perCu == 0) {
         hipCheck(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void*>(kern), kBT, 0),
@@ -2373,16 +2436,21 @@ void launchBatchTKV(const BatchQueryArgs& a, hipStream_t s) {
     const int need = (waves * 64 + kBT - 1) / kBT;
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
-    kern<<<grid, kBT, 0, s>>>(a.nodes, static_cast<const typename L::Item*>(a.items), a.primIds, a.primOff,
-                              a.points, a.qScene, a.order, a.count, a.k, a.outStride, a.out, a.counter);
+    kern<<<grid, kBT, 0, s>>>(a.nodes, a.wide, a.wideRoot, static_cast<const typename L::Item*>(a.items), a.primIds,
+                              a.primOff, a.points, a.qScene, a.order, a.count, a.k, a.outStride, a.out, a.counter);
 }
 
 template <typename L, int K>
 void launchBatchTK(const BatchQueryArgs& a, int variant, hipStream_t s) {
     switch (variant) {
-        case kBatchLds256: launchBatchTKV<L, K, 256, kLdsDepth>(a, s); break;
-        case kBatchLds64: launchBatchTKV<L, K, 64, kLdsDepth>(a, s); break;
-        default: launchBatchTKV<L, K, 64, 0>(a, s); break;
+        case kBatchLds256: launchBatchTKV<L, K, 256, kLdsDepth, false, false>(a, s); break;
+        case kBatchLds64: launchBatchTKV<L, K, 64, kLdsDepth, false, false>(a, s); break;
+        case kBatchScratch64: launchBatchTKV<L, K, 64, 0, false, false>(a, s); break;
+        case kBatchReg64: launchBatchTKV<L, K, 64, 0, false, true>(a, s); break;
+        case kBatchWideLds256: launchBatchTKV<L, K, 256, kLdsDepth, true, false>(a, s); break;
+        case kBatchWideLds64: launchBatchTKV<L, K, 64, kLdsDepth, true, false>(a, s); break;
+        case kBatchWideScratch64: launchBatchTKV<L, K, 64, 0, true, false>(a, s); break;
+        default: launchBatchTKV<L, K, 64, 0, true, true>(a, s); break;
     }
 }
 

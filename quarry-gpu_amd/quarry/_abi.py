@@ -502,6 +502,12 @@ class DeviceArray:
             pass
 
 
+# Batched traversal variants (qr_batch_plan_config): binary or 4-wide nodes x the
+# traversal stack in LDS (256- / 64-lane workgroups), scratch, or registers.
+BATCH_VARIANTS = ("lds256", "lds64", "scratch64", "reg64", "wide_lds256", "wide_lds64", "wide_scratch64",
+                  "wide_reg64")
+
+
 class Batch:
     """Owning wrapper of a qr_batch: many scenes of one kind built together."""
 
@@ -584,7 +590,7 @@ class BatchPlan:
     def config(self):
         v, m, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         check(lib().qr_batch_plan_config(self.h, ctypes.byref(v), ctypes.byref(m), ctypes.byref(t)))
-        return {"variant": ("lds256", "lds64", "scratch64")[v.value], "morton": bool(m.value),
+        return {"variant": BATCH_VARIANTS[v.value], "morton": bool(m.value),
                 "tuned_ms": t.value}
 
     def close(self) -> None:

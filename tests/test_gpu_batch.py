@@ -75,17 +75,44 @@ def test_batch_queries_equal_plain(A, orc, work, k):
         assert np.array_equal(got[qoff[s]:qoff[s + 1]], ref), s
 
 
-def test_batch_every_variant(A, work):
-    """Every configuration the autotuner can pick gives the plain results."""
+@pytest.mark.parametrize("k", [0, 8, 32])
+def test_batch_every_variant(A, work, k):
+    """Every configuration the autotuner can pick -- binary or 4-wide nodes, the stack
+    in LDS, scratch or registers, with and without Morton order -- gives the plain
+    results."""
     items, poff, cp, coff, kp, koff = work
+    pts, qoff = (cp, coff) if k == 0 else (kp, koff)
     b = A.Batch(A.PRIM_TRI, items, poff)
-    want = _plain_results(A, items, poff, kp, koff, 8)
-    for v in range(3):
+    want = _plain_results(A, items, poff, pts, qoff, k)
+    for v in range(len(A.BATCH_VARIANTS)):
         for m in (False, True):
-            plan = A.BatchPlan(b, kp, koff, 8, tune=(v, m))
-            assert plan.config()["variant"] == ("lds256", "lds64", "scratch64")[v]
+            print(f"batch variant {A.BATCH_VARIANTS[v]} morton={m} k={k}", flush=True)
+            plan = A.BatchPlan(b, pts, qoff, k, tune=(v, m))
+            assert plan.config()["variant"] == A.BATCH_VARIANTS[v]
             plan.run()
             assert np.array_equal(plan.read(), want), (v, m)
+
+
+def test_batch_wide_deep_trees(A, orc, scenes):
+    """4-wide batched walks on scenes whose trees are deep (duplicate items: every
+    scene's tree holds long index-ordered chains) and on two-primitive scenes."""
+    rng = np.random.default_rng(17)
+    parts = [np.repeat(rng.uniform(-5, 5, (40, 3)), 60, axis=0).astype(np.float32), rng.uniform(-5, 5, (2, 3)),
+             scenes.point_cloud(5000, 3), rng.uniform(-5, 5, (3, 3))]
+    parts = [p.astype(np.float32) for p in parts]
+    poff = np.zeros(len(parts) + 1, np.int32)
+    np.cumsum([p.shape[0] for p in parts], out=poff[1:])
+    qs = [rng.uniform(-6, 6, (200, 3)).astype(np.float32) for _ in parts]
+    qoff = np.zeros(len(qs) + 1, np.int32)
+    np.cumsum([q.shape[0] for q in qs], out=qoff[1:])
+    b = A.Batch(1, np.concatenate(parts), poff)
+    for v in (4, 7, 3):
+        plan = A.BatchPlan(b, np.concatenate(qs), qoff, 16, tune=(v, True))
+        plan.run()
+        got = plan.read()
+        for s in range(len(parts)):
+            ref = orc.knn_brute(1, parts[s], qs[s], 16, threads=THREADS)
+            assert np.array_equal(got[qoff[s]:qoff[s + 1]], ref), (v, s)
 
 
 @pytest.mark.parametrize("kind", [1, 2])
