@@ -102,6 +102,10 @@ struct GpuTopoEdit {
     int appended = 0;
 };
 
+// The fields keep the reference's names; an edit that changes the primitive count is
+// served by a full rebuild of the device-composed scene here (fullRebuild = true,
+// surgeryRoots = 1, rebuiltLeaves = n, planMs = composition, surgeryMs = rebuild,
+// relayoutMs = mergeMs = 0); a pure-move edit is a dirty refit (refitMs).
 struct GpuTopoUpdateTimings {
     double uploadMs = 0.0;
     double planMs = 0.0;

@@ -155,7 +155,13 @@ QR_API int qr_bvh_refit(qr_bvh* bvh, const void* items_host, int n, const int* d
 QR_API int qr_bvh_refit_device(qr_bvh* bvh, const void* items_dev, int n,
                                qr_refit_timings* timings, double* ms);
 
-/* GpuTopoUpdateTimings (src/gpu/gpu_bvh.h:190-206) */
+/* GpuTopoUpdateTimings (src/gpu/gpu_bvh.h:190-206).  The fields keep the reference's
+ * names, but an edit that changes the primitive count is not a subtree surgery here:
+ * the new scene is composed on the device from the tree's items and the uploaded
+ * ones, and the whole tree is rebuilt with its builder.  Such an update reports
+ * full_rebuild = 1, surgery_roots = 1 (the root), rebuilt_leaves = n, plan_ms = the
+ * composition, surgery_ms = the rebuild, relayout_ms = merge_ms = 0.  A pure-move
+ * edit takes the dirty refit (refit_ms) and reports full_rebuild = 0. */
 typedef struct {
     double upload_ms, plan_ms, surgery_ms, relayout_ms, merge_ms, wide_ms, total_ms;
     int deleted, moved, inserted, appended, surgery_roots, rebuilt_leaves, new_depth;

@@ -125,9 +125,10 @@ __device__ __forceinline__ void binRunFlush(uint32_t* h, int a, const BinRun& r)
     }
 }
 
+template <int kBT = kBlock>
 __device__ __forceinline__ void histAddRange(uint32_t* h, const PrimRec* __restrict__ rec, int count,
                                              const AxisMap& m) {
-    const int per = (count + kBlock - 1) / kBlock;
+    const int per = (count + kBT - 1) / kBT;
     const int beg = threadIdx.x * per;
     const int end = beg + per < count ? beg + per : count;
     BinRun run[3];
@@ -322,11 +323,12 @@ __device__ __forceinline__ int blockExclusive(int v, int* lds, int* total) {
 // counts meet in LDS (one barrier), and a second pass writes every record at its
 // rank.  Called by the whole workgroup.  A destination outside the task's side
 // (counts that disagree with the histogram) is reported through err, never written.
+template <int kBT = kBlock>
 __device__ void partitionRange(const PrimRec* __restrict__ src, PrimRec* __restrict__ dst, int begin, int cnt,
                                int leftDst, int rightDst, const SahTask& t, const SahResult& s, const AxisMap& m,
                                int* ldsWave, int* err) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    constexpr int kWaves = kBlock / 64;
+    constexpr int kWaves = kBT / 64;
     const int per = (cnt + kWaves - 1) / kWaves;
     const int wBeg = w * per < cnt ? w * per : cnt;
     const int wEnd = wBeg + per < cnt ? wBeg + per : cnt;
@@ -465,6 +467,7 @@ enum : int {
     kCtlParted = 7,     // primitive slots partitioned
     kCtlSerialSlots = 8,
     kCtlMaxLevel = 9,   // deepest level that split a task
+    kCtlSmall = 10,     // tasks of kSerial+1 .. kSmallTask primitives from the top subtree tier
     kCtlWords = 16,
 };
 
@@ -482,10 +485,11 @@ struct SahTables {
     PrimRec* rec[2];
     SahTask* big[2];
     SahTask* local;
+    SahTask* small;
     SerialTask* serial;
     Chunk* chunks;
     int* first;
-    int* chunkLeft;
+    int* chunkBins;  // per chunk: its primitives' counts per axis and bin (3 x kBins)
     int* chunkOff;
     uint32_t* bins;
     SahResult* results;
@@ -539,11 +543,21 @@ __device__ __forceinline__ SahTask makeTask(int start, int count, int node, int 
     return t;
 }
 
-// A child of a big task: serial range, local task or next-level big task.
+// Local tasks of at most this many primitives go to the one-wave subtree tier.
+constexpr int kSmallTask = 512;
+
+// A child of a big task: serial range, small or local task, or next-level big task.
 __device__ __forceinline__ void routeChild(const SahTables& T, int nextBuf, int start, int count, int node, int buf,
                                            int level, const float* lo, const float* hi) {
     if (count <= kSerial) {
         pushSerial(T, start, count, node, buf);
+    } else if (count <= kSmallTask) {
+        const int at = atomicAdd(&T.ctl[kCtlSmall], 1);
+        if (at < T.maxLocal) {
+            T.small[at] = makeTask(start, count, node, metaOf(buf, level), lo, hi);
+        } else {
+            atomicOr(&T.ctl[kCtlError], 1);
+        }
     } else if (count <= kChunk) {
         const int at = atomicAdd(&T.ctl[kCtlLocal], 1);
         if (at < T.maxLocal) {
@@ -629,6 +643,9 @@ __global__ __launch_bounds__(kBlock) void sahBinKernel(SahTables T, int cur) {
     histAddRange(h, recOf(T, t.meta & 1) + ch.start, ch.count, m);
     __syncthreads();
     uint32_t* g = T.bins + static_cast<size_t>(ch.task) * kHist;
+    // the chunk's own bin counts: its left count for any plane, without a re-read
+    for (int i = threadIdx.x; i < 3 * kBins; i += kBlock)
+        T.chunkBins[static_cast<size_t>(blockIdx.x) * 3 * kBins + i] = static_cast<int>(h[i * kBinVals]);
     for (int i = threadIdx.x; i < kHist; i += kBlock) {
         const int v = i % kBinVals;
         if (h[i - v] == 0u) continue;  // empty bin
@@ -669,30 +686,29 @@ __global__ __launch_bounds__(64) void sahSplitKernel(SahTables T, int cur, int m
     chooseSplit(T.bins + static_cast<size_t>(blockIdx.x) * kHist, t, axisMap(t), &T.results[blockIdx.x]);
 }
 
-__global__ __launch_bounds__(kBlock) void sahPartCountKernel(SahTables T, int cur) {
-    if (static_cast<int>(blockIdx.x) >= T.ctl[kCtlChunks]) return;
-    __shared__ int lds[4];
-    const Chunk ch = T.chunks[blockIdx.x];
-    const SahTask t = bigOf(T, cur)[ch.task];
-    const SahResult s = T.results[ch.task];
-    const AxisMap m = axisMap(t);
-    const PrimRec* rec = recOf(T, t.meta & 1);
-    int c = 0;
-    for (int i = threadIdx.x; i < ch.count; i += kBlock) c += goesLeft(rec[ch.start + i], ch.start + i, t, s, m) ? 1 : 0;
-    int total = 0;
-    blockExclusive(c, lds, &total);
-    if (threadIdx.x == 0) T.chunkLeft[blockIdx.x] = total;
-}
-
-// One thread per large task: exclusive offsets of its chunks' left counts.
+// One thread per large task: exclusive offsets of its chunks' left counts, from the
+// chunks' bin counts (the bins left of the chosen plane on its axis) or, for a split
+// by position, from the chunk's place in the task.
 __global__ void sahChunkScanKernel(SahTables T, int cur) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= T.ctl[kCtlBig0 + cur] || T.ctl[kCtlChunks] == 0) return;
+    const int ti = blockIdx.x * kBlock + threadIdx.x;
+    if (ti >= T.ctl[kCtlBig0 + cur] || T.ctl[kCtlChunks] == 0) return;
+    const SahTask t = bigOf(T, cur)[ti];
+    const SahResult r = T.results[ti];
     int acc = 0;
-    for (int c = T.first[t]; c < T.first[t + 1]; ++c) {
+    for (int c = T.first[ti]; c < T.first[ti + 1]; ++c) {
         T.chunkOff[c] = acc;
-        acc += T.chunkLeft[c];
+        int left = 0;
+        if (r.axis < 0) {
+            const Chunk ch = T.chunks[c];
+            const int before = t.start + r.nL - ch.start;
+            left = before < 0 ? 0 : (before > ch.count ? ch.count : before);
+        } else {
+            const int* cnt = T.chunkBins + static_cast<size_t>(c) * 3 * kBins + r.axis * kBins;
+            for (int b = 0; b < r.plane; ++b) left += cnt[b];
+        }
+        acc += left;
     }
+    if (acc != r.nL) atomicOr(&T.ctl[kCtlError], 2);  // the chunks disagree with the task's histogram
 }
 
 __global__ __launch_bounds__(kBlock) void sahPartScatterKernel(SahTables T, int cur) {
@@ -730,9 +746,14 @@ __global__ void sahAssignKernel(SahTables T, int cur, int median, int n, Node2* 
     routeChild(T, 1 - cur, t.start + nL, t.count - nL, rr, buf, level + 1, r.rLo, r.rHi);
 }
 
-// One workgroup per local task (kSerial+1 .. kChunk primitives): the whole subtree
-// down to ranges of kSerial, from an LDS stack -- histogram, plane choice and stable
-// partition per task inside the workgroup, no return to the host between levels.
+// Local tasks (kSerial+1 .. kChunk primitives) build their whole subtrees without a
+// return to the host, in two tiers.  The top tier takes one local task per 256-lane
+// workgroup and splits it from an LDS stack (histogram, plane choice and stable
+// partition per task inside the workgroup) until its pieces have at most kSmallTask
+// primitives; those go to the small list.  The bottom tier takes one small task per
+// one-wave workgroup (many per CU, no cross-wave barriers) down to ranges of kSerial:
+// the many small tasks of a subtree's lower levels run side by side instead of one
+// after another in the workgroup that split their ancestors.
 constexpr int kSubtreeStack = 64;  // a chain of count/8 sides from 4096 to 64 is 32 deep
 constexpr int kSerialBatch = 64;
 
@@ -748,8 +769,9 @@ __device__ __forceinline__ void flushSerial(const SahTables& T, const SerialTask
         }
     }
 }
-__global__ __launch_bounds__(kBlock) void sahSubtreeKernel(SahTables T, int n, Node2* __restrict__ nodes,
-                                                           int* __restrict__ parent, int* __restrict__ farEnd) {
+template <int kBT, bool kTop>
+__global__ __launch_bounds__(kBT) void sahSubtreeKernel(SahTables T, int n, Node2* __restrict__ nodes,
+                                                        int* __restrict__ parent, int* __restrict__ farEnd) {
     __shared__ uint32_t h[kHist];
     __shared__ SahResult sres;
     __shared__ SahTask stk[kSubtreeStack];
@@ -758,12 +780,16 @@ __global__ __launch_bounds__(kBlock) void sahSubtreeKernel(SahTables T, int n, N
     __shared__ int lds[4];
     __shared__ SerialTask serBuf[kSerialBatch];
     int serN = 0;  // thread 0's count of buffered serial ranges
+    int binned = 0, parted = 0, deepest = 0;
+    // top tier: task blockIdx.x of the local list; bottom tier: small tasks
+    // blockIdx.x, + gridDim.x, ... (the grid is not sized by the list's length)
+    const int listLen = kTop ? T.ctl[kCtlLocal] : T.ctl[kCtlSmall];
+    for (int task = blockIdx.x; task < listLen; task += (kTop ? listLen : gridDim.x)) {
     if (threadIdx.x == 0) {
-        stk[0] = T.local[blockIdx.x];
+        stk[0] = kTop ? T.local[task] : T.small[task];
         ssp = 1;
     }
     __syncthreads();
-    int binned = 0, parted = 0, deepest = 0;
     while (true) {
         if (threadIdx.x == 0) {
             if (ssp > 0) {
@@ -783,7 +809,7 @@ __global__ __launch_bounds__(kBlock) void sahSubtreeKernel(SahTables T, int n, N
         if (!median) {
             histInit(h);
             __syncthreads();
-            histAddRange(h, src + t.start, t.count, m);
+            histAddRange<kBT>(h, src + t.start, t.count, m);
             __syncthreads();
             if (threadIdx.x < 64) chooseSplit(h, t, m, &sres);
         } else if (threadIdx.x == 0) {
@@ -792,7 +818,8 @@ __global__ __launch_bounds__(kBlock) void sahSubtreeKernel(SahTables T, int n, N
         __syncthreads();
         const SahResult s = sres;
         if (!median) {
-            partitionRange(src, dst, t.start, t.count, t.start, t.start + s.nL, t, s, m, lds, &T.ctl[kCtlError]);
+            partitionRange<kBT>(src, dst, t.start, t.count, t.start, t.start + s.nL, t, s, m, lds,
+                                &T.ctl[kCtlError]);
             binned += t.count;
             parted += t.count;
         }
@@ -815,6 +842,14 @@ __global__ __launch_bounds__(kBlock) void sahSubtreeKernel(SahTables T, int n, N
                             serN = 0;
                         }
                         serBuf[serN++] = {cs[side], cc[side], ci[side], childBuf};
+                    } else if (kTop && cc[side] <= kSmallTask) {
+                        const int at = atomicAdd(&T.ctl[kCtlSmall], 1);
+                        if (at < T.maxLocal) {
+                            T.small[at] = makeTask(cs[side], cc[side], ci[side], metaOf(childBuf, level + 1),
+                                                   side ? s.rLo : s.lLo, side ? s.rHi : s.lHi);
+                        } else {
+                            atomicOr(&T.ctl[kCtlError], 1);
+                        }
                     } else if (ssp < kSubtreeStack) {
                         stk[ssp++] = makeTask(cs[side], cc[side], ci[side], metaOf(childBuf, level + 1),
                                               side ? s.rLo : s.lLo, side ? s.rHi : s.lHi);
@@ -825,6 +860,7 @@ __global__ __launch_bounds__(kBlock) void sahSubtreeKernel(SahTables T, int n, N
             }
         }
         __syncthreads();
+    }
     }
     if (threadIdx.x == 0) {
         flushSerial(T, serBuf, serN);
@@ -922,10 +958,11 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     T.maxLocal = n / (kSerial + 1) + 2;
     T.maxSerial = n + 2;
     T.maxChunks = n / kChunk + T.maxBig + 2;
-    DevBuf<SahTask> dBig(2 * static_cast<size_t>(T.maxBig)), dLocal(static_cast<size_t>(T.maxLocal));
+    DevBuf<SahTask> dBig(2 * static_cast<size_t>(T.maxBig)), dLocal(static_cast<size_t>(T.maxLocal)),
+        dSmall(static_cast<size_t>(T.maxLocal));
     DevBuf<SerialTask> dSerial(static_cast<size_t>(T.maxSerial));
     DevBuf<Chunk> dChunks(static_cast<size_t>(T.maxChunks));
-    DevBuf<int> dFirst(static_cast<size_t>(T.maxBig) + 1), dChunkLeft(static_cast<size_t>(T.maxChunks)),
+    DevBuf<int> dFirst(static_cast<size_t>(T.maxBig) + 1), dChunkBins(static_cast<size_t>(T.maxChunks) * 3 * kBins),
         dChunkOff(static_cast<size_t>(T.maxChunks)), dCtl(kCtlWords);
     DevBuf<uint32_t> dBins(static_cast<size_t>(T.maxBig) * kHist);
     DevBuf<SahResult> dRes(static_cast<size_t>(T.maxBig));
@@ -934,10 +971,11 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     T.big[0] = dBig.p;
     T.big[1] = dBig.p + T.maxBig;
     T.local = dLocal.p;
+    T.small = dSmall.p;
     T.serial = dSerial.p;
     T.chunks = dChunks.p;
     T.first = dFirst.p;
-    T.chunkLeft = dChunkLeft.p;
+    T.chunkBins = dChunkBins.p;
     T.chunkOff = dChunkOff.p;
     T.bins = dBins.p;
     T.results = dRes.p;
@@ -981,7 +1019,6 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
                 mark(s);
                 sahSplitKernel<<<T.maxBig, 64, 0, s>>>(T, cur, median);
                 mark(s);
-                sahPartCountKernel<<<chunkGrid, kBlock, 0, s>>>(T, cur);
                 sahChunkScanKernel<<<taskGrid, kBlock, 0, s>>>(T, cur);
                 sahPartScatterKernel<<<chunkGrid, kBlock, 0, s>>>(T, cur);
                 sahAssignKernel<<<taskGrid, kBlock, 0, s>>>(T, cur, median, n, b.nodes.p, b.parent.p, b.farEnd.p);
@@ -1011,8 +1048,14 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     // serial tails; the topology is checked complete before the bottom-up pass
     // climbs it
     if (hw[kCtlLocal] > 0)
-        sahSubtreeKernel<<<std::min(hw[kCtlLocal], T.maxLocal), kBlock, 0, s>>>(T, n, b.nodes.p, b.parent.p,
-                                                                            b.farEnd.p);
+        sahSubtreeKernel<kBlock, true><<<std::min(hw[kCtlLocal], T.maxLocal), kBlock, 0, s>>>(
+            T, n, b.nodes.p, b.parent.p, b.farEnd.p);
+    if (hw[kCtlLocal] > 0 || hw[kCtlSmall] > 0) {
+        // the top tier adds small tasks: their count is known only on the device, so a
+        // fixed grid strides the list
+        const int smallGrid = std::min(T.maxLocal, 8192);
+        sahSubtreeKernel<64, false><<<smallGrid, 64, 0, s>>>(T, n, b.nodes.p, b.parent.p, b.farEnd.p);
+    }
     ev.record(3, s);
     sahSerialKernel<<<std::min(blocksFor(T.maxSerial, kBlock), 1024), kBlock, 0, s>>>(T, n, order.p, b.nodes.p,
                                                                                      b.parent.p, b.farEnd.p);
