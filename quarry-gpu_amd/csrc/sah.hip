@@ -246,9 +246,10 @@ __device__ void chooseSplit(const uint32_t* h, const SahTask& t, const AxisMap& 
             bestKey = ok;
         }
     }
-    if (lane != 0) return;
     SahResult res;
+    res.pad = 0;
     if (bestKey == 0x7fffffff) {
+        if (lane != 0) return;
         res.axis = -1;
         res.plane = 0;
         res.nL = count / 2;
@@ -257,43 +258,66 @@ __device__ void chooseSplit(const uint32_t* h, const SahTask& t, const AxisMap& 
             res.lLo[k] = res.rLo[k] = t.cLo[k];
             res.lHi[k] = res.rHi[k] = t.cHi[k];
         }
-    } else {
-        const int a = bestKey / kBins, plane = bestKey % kBins;
-        res.axis = a;
-        res.plane = plane;
-        int nL = 0;
-        float lc[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
-        float rc[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
-        for (int b = 0; b < kBins; ++b) {
-            const int c = static_cast<int>(h[hidx(a, b, 0)]);
-            if (c == 0) continue;
-            float* dst = b < plane ? lc : rc;
-            if (b < plane) nL += c;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                dst[k] = fminf(dst[k], fromOrderedBits(h[hidx(a, b, 7 + k)]));
-                dst[3 + k] = fmaxf(dst[3 + k], fromOrderedBits(h[hidx(a, b, 10 + k)]));
-            }
-        }
-        res.nL = nL;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            res.lLo[k] = lc[k];
-            res.lHi[k] = lc[3 + k];
-            res.rLo[k] = rc[k];
-            res.rHi[k] = rc[3 + k];
-        }
-        res.lLo[3] = res.lHi[3] = res.rLo[3] = res.rHi[3] = 0.0f;
+        *out = res;
+        return;
     }
-    res.pad = 0;
+    // the chosen plane's two sides: counts and centroid boxes of the bins left and
+    // right of it, reduced across the 32 bin lanes (a min / max of ordered bits is
+    // exact and order-free; empty bins take the identities)
+    const int a = bestKey / kBins, plane = bestKey % kBins;
+    const uint32_t c = h[hidx(a, bin, 0)];
+    const bool isLeft = bin < plane;
+    int nL = isLeft ? static_cast<int>(c) : 0;
+    uint32_t v[12];  // left min xyz, left max xyz, right min xyz, right max xyz
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t lo = c ? h[hidx(a, bin, 7 + k)] : 0xFFFFFFFFu;
+        const uint32_t hi = c ? h[hidx(a, bin, 10 + k)] : 0u;
+        v[k] = isLeft ? lo : 0xFFFFFFFFu;
+        v[3 + k] = isLeft ? hi : 0u;
+        v[6 + k] = isLeft ? 0xFFFFFFFFu : lo;
+        v[9 + k] = isLeft ? 0u : hi;
+    }
+#pragma unroll
+    for (int off = kBins / 2; off > 0; off >>= 1) {
+        nL += __shfl_xor(nL, off, 64);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const uint32_t o = __shfl_xor(v[k], off, 64);
+            const bool isMin = k < 3 || (k >= 6 && k < 9);
+            v[k] = isMin ? (o < v[k] ? o : v[k]) : (o > v[k] ? o : v[k]);
+        }
+    }
+    if (lane != 0) return;
+    res.axis = a;
+    res.plane = plane;
+    res.nL = nL;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        res.lLo[k] = fromOrderedBits(v[k]);
+        res.lHi[k] = fromOrderedBits(v[3 + k]);
+        res.rLo[k] = fromOrderedBits(v[6 + k]);
+        res.rHi[k] = fromOrderedBits(v[9 + k]);
+    }
+    res.lLo[3] = res.lHi[3] = res.rLo[3] = res.rHi[3] = 0.0f;
     *out = res;
 }
 
-__device__ __forceinline__ bool goesLeft(const PrimRec& r, int pos, const SahTask& t, const SahResult& s,
-                                         const AxisMap& m) {
+// Coordinate `axis` of a record held in registers, by selection (a dynamic subscript
+// into a register copy would go through scratch).
+__device__ __forceinline__ float axisCoord(const PrimRec& r, int axis) {
+    return axis == 0 ? r.c[0] : axis == 1 ? r.c[1] : r.c[2];
+}
+
+__device__ __forceinline__ float axisSel(const float* v, int axis) {
+    return axis == 0 ? v[0] : axis == 1 ? v[1] : v[2];
+}
+
+// Side test of one record for split s of task t (by plane, or by position for a
+// split without a plane); c is the record's coordinate on the split axis.
+__device__ __forceinline__ bool goesLeft(float c, int pos, const SahTask& t, const SahResult& s, const AxisMap& m) {
     if (s.axis < 0) return pos - t.start < s.nL;
-    const int a = s.axis;
-    return binOf(r.c[a], m.lo[a], m.scale[a]) < s.plane;
+    return binOf(c, axisSel(m.lo, s.axis), axisSel(m.scale, s.axis)) < s.plane;
 }
 
 // Block-wide exclusive scan of one int per thread (kBlock threads); returns the
@@ -336,7 +360,7 @@ __device__ void partitionRange(const PrimRec* __restrict__ src, PrimRec* __restr
     int lefts = 0;
     for (int r = wBeg; r < wEnd; r += 64) {
         const int i = r + lane;
-        const bool left = i < wEnd && goesLeft(src[begin + i], begin + i, t, s, m);
+        const bool left = i < wEnd && goesLeft(axisCoord(src[begin + i], s.axis), begin + i, t, s, m);
         lefts += __popcll(__ballot(left));
     }
     if (lane == 0) ldsWave[w] = lefts;
@@ -355,7 +379,7 @@ __device__ void partitionRange(const PrimRec* __restrict__ src, PrimRec* __restr
         bool left = false;
         if (valid) {
             rec = src[begin + i];
-            left = goesLeft(rec, begin + i, t, s, m);
+            left = goesLeft(axisCoord(rec, s.axis), begin + i, t, s, m);
         }
         const unsigned long long lm = __ballot(valid && left);
         const unsigned long long rm = __ballot(valid && !left);
@@ -528,6 +552,22 @@ __device__ __forceinline__ void pushSerial(const SahTables& T, int start, int co
     }
 }
 
+// A split's child task: side 0 takes the left centroid box, side 1 the right.
+__device__ __forceinline__ SahTask childTask(int start, int count, int node, int meta, const SahResult& s, int side) {
+    SahTask t;
+    t.start = start;
+    t.count = count;
+    t.node = node;
+    t.meta = meta;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        t.cLo[k] = side ? s.rLo[k] : s.lLo[k];
+        t.cHi[k] = side ? s.rHi[k] : s.lHi[k];
+    }
+    t.cLo[3] = t.cHi[3] = 0.0f;
+    return t;
+}
+
 __device__ __forceinline__ SahTask makeTask(int start, int count, int node, int meta, const float* lo, const float* hi) {
     SahTask t;
     t.start = start;
@@ -589,27 +629,42 @@ __global__ void sahRootKernel(SahTables T, const uint32_t* __restrict__ cb, int 
 
 // Level plan (one workgroup): the current big tasks' chunk list (chunks of one task
 // consecutive, first[task] .. first[task + 1]); clears the count of the buffer this
-// level's children go to.
+// level's children go to.  Per group of kBlock tasks, an exclusive scan of their chunk
+// counts, then every thread writes chunks, each finding its task by binary search of
+// the group's chunk offsets in LDS.
 __global__ __launch_bounds__(kBlock) void sahPlanKernel(SahTables T, int cur, int median) {
     __shared__ int lds[4];
+    __shared__ int sEx[kBlock], sStart[kBlock], sCount[kBlock];
     const int nt = T.ctl[kCtlBig0 + cur];
     int carry = 0;
     for (int base = 0; base < nt && !median; base += kBlock) {
         const int i = base + threadIdx.x;
         int c = 0;
-        SahTask t{};
         if (i < nt) {
-            t = bigOf(T, cur)[i];
+            const SahTask t = bigOf(T, cur)[i];
             c = (t.count + kChunk - 1) / kChunk;
+            sStart[threadIdx.x] = t.start;
+            sCount[threadIdx.x] = t.count;
         }
         int total = 0;
-        const int ex = carry + blockExclusive(c, lds, &total);
+        const int ex = blockExclusive(c, lds, &total);
         if (i < nt) {
-            T.first[i] = ex;
-            for (int k = 0; k < c && ex + k < T.maxChunks; ++k)
-                T.chunks[ex + k] = {i, t.start + k * kChunk, min(kChunk, t.count - k * kChunk), 0};
+            T.first[i] = carry + ex;
+            sEx[threadIdx.x] = ex;
+        }
+        __syncthreads();
+        const int g = min(kBlock, nt - base);
+        for (int k = threadIdx.x; k < total && carry + k < T.maxChunks; k += kBlock) {
+            int lo = 0, hi = g - 1;  // the last task of the group whose first chunk is <= k
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (sEx[mid] <= k) lo = mid; else hi = mid - 1;
+            }
+            const int j = k - sEx[lo];
+            T.chunks[carry + k] = {base + lo, sStart[lo] + j * kChunk, min(kChunk, sCount[lo] - j * kChunk), 0};
         }
         carry += total;
+        __syncthreads();
     }
     if (threadIdx.x == 0) {
         if (carry > T.maxChunks) {
@@ -686,29 +741,40 @@ __global__ __launch_bounds__(64) void sahSplitKernel(SahTables T, int cur, int m
     chooseSplit(T.bins + static_cast<size_t>(blockIdx.x) * kHist, t, axisMap(t), &T.results[blockIdx.x]);
 }
 
-// One thread per large task: exclusive offsets of its chunks' left counts, from the
+// One wave per large task: exclusive offsets of its chunks' left counts, from the
 // chunks' bin counts (the bins left of the chosen plane on its axis) or, for a split
-// by position, from the chunk's place in the task.
-__global__ void sahChunkScanKernel(SahTables T, int cur) {
-    const int ti = blockIdx.x * kBlock + threadIdx.x;
+// by position, from the chunk's place in the task; 64 chunks per step, a wave scan.
+__global__ __launch_bounds__(64) void sahChunkScanKernel(SahTables T, int cur) {
+    const int ti = blockIdx.x;
     if (ti >= T.ctl[kCtlBig0 + cur] || T.ctl[kCtlChunks] == 0) return;
+    const int lane = threadIdx.x;
     const SahTask t = bigOf(T, cur)[ti];
     const SahResult r = T.results[ti];
+    const int c0 = T.first[ti], c1 = T.first[ti + 1];
     int acc = 0;
-    for (int c = T.first[ti]; c < T.first[ti + 1]; ++c) {
-        T.chunkOff[c] = acc;
+    for (int base = c0; base < c1; base += 64) {
+        const int c = base + lane;
         int left = 0;
-        if (r.axis < 0) {
-            const Chunk ch = T.chunks[c];
-            const int before = t.start + r.nL - ch.start;
-            left = before < 0 ? 0 : (before > ch.count ? ch.count : before);
-        } else {
-            const int* cnt = T.chunkBins + static_cast<size_t>(c) * 3 * kBins + r.axis * kBins;
-            for (int b = 0; b < r.plane; ++b) left += cnt[b];
+        if (c < c1) {
+            if (r.axis < 0) {
+                const Chunk ch = T.chunks[c];
+                const int before = t.start + r.nL - ch.start;
+                left = before < 0 ? 0 : (before > ch.count ? ch.count : before);
+            } else {
+                const int* cnt = T.chunkBins + static_cast<size_t>(c) * 3 * kBins + r.axis * kBins;
+                for (int b = 0; b < r.plane; ++b) left += cnt[b];
+            }
         }
-        acc += left;
+        int x = left;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (c < c1) T.chunkOff[c] = acc + x - left;
+        acc += __shfl(x, 63, 64);
     }
-    if (acc != r.nL) atomicOr(&T.ctl[kCtlError], 2);  // the chunks disagree with the task's histogram
+    if (lane == 0 && acc != r.nL) atomicOr(&T.ctl[kCtlError], 2);  // the chunks disagree with the task's histogram
 }
 
 __global__ __launch_bounds__(kBlock) void sahPartScatterKernel(SahTables T, int cur) {
@@ -750,10 +816,11 @@ __global__ void sahAssignKernel(SahTables T, int cur, int median, int n, Node2* 
 // return to the host, in two tiers.  The top tier takes one local task per 256-lane
 // workgroup and splits it from an LDS stack (histogram, plane choice and stable
 // partition per task inside the workgroup) until its pieces have at most kSmallTask
-// primitives; those go to the small list.  The bottom tier takes one small task per
-// one-wave workgroup (many per CU, no cross-wave barriers) down to ranges of kSerial:
-// the many small tasks of a subtree's lower levels run side by side instead of one
-// after another in the workgroup that split their ancestors.
+// primitives; those go to the small list.  The bottom tier (sahSmallKernel) takes one
+// small task per one-wave workgroup (many per CU, no cross-wave barriers) and splits
+// it in LDS down to ranges of kSerial: the many small tasks of a subtree's lower
+// levels run side by side instead of one after another in the workgroup that split
+// their ancestors.
 constexpr int kSubtreeStack = 64;  // a chain of count/8 sides from 4096 to 64 is 32 deep
 constexpr int kSerialBatch = 64;
 
@@ -769,7 +836,7 @@ __device__ __forceinline__ void flushSerial(const SahTables& T, const SerialTask
         }
     }
 }
-template <int kBT, bool kTop>
+template <int kBT>
 __global__ __launch_bounds__(kBT) void sahSubtreeKernel(SahTables T, int n, Node2* __restrict__ nodes,
                                                         int* __restrict__ parent, int* __restrict__ farEnd) {
     __shared__ uint32_t h[kHist];
@@ -781,12 +848,11 @@ __global__ __launch_bounds__(kBT) void sahSubtreeKernel(SahTables T, int n, Node
     __shared__ SerialTask serBuf[kSerialBatch];
     int serN = 0;  // thread 0's count of buffered serial ranges
     int binned = 0, parted = 0, deepest = 0;
-    // top tier: task blockIdx.x of the local list; bottom tier: small tasks
-    // blockIdx.x, + gridDim.x, ... (the grid is not sized by the list's length)
-    const int listLen = kTop ? T.ctl[kCtlLocal] : T.ctl[kCtlSmall];
-    for (int task = blockIdx.x; task < listLen; task += (kTop ? listLen : gridDim.x)) {
+    // local tasks blockIdx.x, + gridDim.x, ...
+    const int listLen = T.ctl[kCtlLocal];
+    for (int task = blockIdx.x; task < listLen; task += gridDim.x) {
     if (threadIdx.x == 0) {
-        stk[0] = kTop ? T.local[task] : T.small[task];
+        stk[0] = T.local[task];
         ssp = 1;
     }
     __syncthreads();
@@ -833,6 +899,7 @@ __global__ __launch_bounds__(kBT) void sahSubtreeKernel(SahTables T, int n, Node
                 int l, r;
                 writeSplit(n, t.start, t.count, t.node, nL, nodes, parent, farEnd, &l, &r);
                 const int cs[2] = {t.start, t.start + nL}, cc[2] = {nL, t.count - nL}, ci[2] = {l, r};
+#pragma unroll
                 for (int side = 1; side >= 0; --side) {
                     if (cc[side] <= kSerial) {
                         // serial ranges collect in LDS and go out in batches: one global
@@ -842,17 +909,15 @@ __global__ __launch_bounds__(kBT) void sahSubtreeKernel(SahTables T, int n, Node
                             serN = 0;
                         }
                         serBuf[serN++] = {cs[side], cc[side], ci[side], childBuf};
-                    } else if (kTop && cc[side] <= kSmallTask) {
+                    } else if (cc[side] <= kSmallTask) {
                         const int at = atomicAdd(&T.ctl[kCtlSmall], 1);
                         if (at < T.maxLocal) {
-                            T.small[at] = makeTask(cs[side], cc[side], ci[side], metaOf(childBuf, level + 1),
-                                                   side ? s.rLo : s.lLo, side ? s.rHi : s.lHi);
+                            T.small[at] = childTask(cs[side], cc[side], ci[side], metaOf(childBuf, level + 1), s, side);
                         } else {
                             atomicOr(&T.ctl[kCtlError], 1);
                         }
                     } else if (ssp < kSubtreeStack) {
-                        stk[ssp++] = makeTask(cs[side], cc[side], ci[side], metaOf(childBuf, level + 1),
-                                              side ? s.rLo : s.lLo, side ? s.rHi : s.lHi);
+                        stk[ssp++] = childTask(cs[side], cc[side], ci[side], metaOf(childBuf, level + 1), s, side);
                     } else {
                         atomicOr(&T.ctl[kCtlError], 1);
                     }
@@ -861,6 +926,218 @@ __global__ __launch_bounds__(kBT) void sahSubtreeKernel(SahTables T, int n, Node
         }
         __syncthreads();
     }
+    }
+    if (threadIdx.x == 0) {
+        flushSerial(T, serBuf, serN);
+        atomicAdd(&T.ctl[kCtlBinned], binned);
+        atomicAdd(&T.ctl[kCtlParted], parted);
+        atomicMax(&T.ctl[kCtlMaxLevel], deepest);
+    }
+}
+
+// Bottom tier: one small task (kSerial+1 .. kSmallTask primitives) per one-wave
+// workgroup, grid-strided over the small list.  The task's records are read into LDS
+// once; its subtree is then split entirely in LDS -- histograms read the records
+// through a position -> record index, and each stable partition moves 16-bit indices
+// between two LDS index buffers -- so the lower levels cost no HBM traffic.  A range
+// that reaches kSerial or fewer primitives writes its leaf order (scene ids) at once
+// and goes to the serial list marked kOrderWritten for its balanced splits.
+constexpr int kOrderWritten = 2;  // SerialTask.buf: the range's leaf order is already in place
+constexpr int kSmallBT = 128;     // threads per small task
+
+template <int kBT>
+__device__ __forceinline__ void histAddLds(uint32_t* h, const PrimRec* recL, const unsigned short* idx, int count,
+                                           const AxisMap& m) {
+    const int per = (count + kBT - 1) / kBT;
+    const int beg = threadIdx.x * per;
+    const int end = beg + per < count ? beg + per : count;
+    BinRun run[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        run[a].bin = -1;
+        run[a].cnt = 0u;
+    }
+    for (int i = beg; i < end; ++i) {
+        const PrimRec r = recL[idx[i]];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (!m.use[a]) continue;
+            const int b = binOf(r.c[a], m.lo[a], m.scale[a]);
+            BinRun& u = run[a];
+            if (b != u.bin) {
+                binRunFlush(h, a, u);
+                u.bin = b;
+                u.cnt = 0u;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    u.v[k] = u.v[6 + k] = 0xFFFFFFFFu;
+                    u.v[3 + k] = u.v[9 + k] = 0u;
+                }
+            }
+            ++u.cnt;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t lo = orderedBits(r.lo[k]), hi = orderedBits(r.hi[k]), c = orderedBits(r.c[k]);
+                u.v[k] = lo < u.v[k] ? lo : u.v[k];
+                u.v[3 + k] = hi > u.v[3 + k] ? hi : u.v[3 + k];
+                u.v[6 + k] = c < u.v[6 + k] ? c : u.v[6 + k];
+                u.v[9 + k] = c > u.v[9 + k] ? c : u.v[9 + k];
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (m.use[a]) binRunFlush(h, a, run[a]);
+}
+
+template <int kBT>
+__global__ __launch_bounds__(kBT) void sahSmallKernel(SahTables T, int n, uint32_t* __restrict__ order,
+                                                      Node2* __restrict__ nodes, int* __restrict__ parent,
+                                                      int* __restrict__ farEnd) {
+    constexpr int kWaves = kBT / 64;
+    __shared__ PrimRec recL[kSmallTask];
+    __shared__ unsigned short idxL[2][kSmallTask];
+    __shared__ uint32_t h[kHist];
+    __shared__ SahResult sres;
+    __shared__ SahTask stk[kSubtreeStack];
+    __shared__ SerialTask serBuf[kSerialBatch];
+    __shared__ int waveLefts[kWaves];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int serN = 0;  // thread 0's count of buffered serial ranges
+    int binned = 0, parted = 0, deepest = 0;
+    int* err = &T.ctl[kCtlError];
+    const int listLen = T.ctl[kCtlSmall];
+    for (int task = blockIdx.x; task < listLen; task += gridDim.x) {
+        const SahTask root = T.small[task];
+        if (root.count > kSmallTask) {  // the list holds tasks of at most kSmallTask primitives
+            if (threadIdx.x == 0) atomicOr(err, 4);
+            continue;
+        }
+        const int base = root.start;
+        const PrimRec* src = recOf(T, root.meta & 1) + base;
+        for (int i = threadIdx.x; i < root.count; i += kBT) {
+            recL[i] = src[i];
+            idxL[0][i] = static_cast<unsigned short>(i);
+        }
+        int sp = 1;  // uniform: every thread keeps the stack depth
+        if (threadIdx.x == 0)
+            stk[0] = makeTask(root.start, root.count, root.node, metaOf(0, root.meta >> 1), root.cLo, root.cHi);
+        __syncthreads();
+        while (sp > 0) {
+            --sp;
+            const SahTask t = stk[sp];
+            const int buf = t.meta & 1, level = t.meta >> 1;
+            const bool median = level >= kMedianLevel;
+            const int ls = t.start - base;  // the range's first position in LDS
+            const unsigned short* idxS = idxL[buf] + ls;
+            const AxisMap m = axisMap(t);
+            if (!median) {
+                histInit(h);
+                __syncthreads();
+                histAddLds<kBT>(h, recL, idxS, t.count, m);
+                __syncthreads();
+                if (threadIdx.x < 64) chooseSplit(h, t, m, &sres);
+            } else if (threadIdx.x == 0) {
+                sres = medianResult(t);
+            }
+            __syncthreads();
+            const SahResult s = sres;
+            const int nL = s.nL;
+            if (nL < 1 || nL >= t.count) {
+                if (threadIdx.x == 0) atomicOr(err, 4);
+                __syncthreads();
+                continue;
+            }
+            const int childBuf = median ? buf : 1 - buf;
+            if (!median) {
+                // stable partition of the range's indices into the other buffer: wave w
+                // takes the w-th part of the range (lefts of the earlier parts counted
+                // first when there are several waves)
+                unsigned short* idxD = idxL[childBuf] + ls;
+                const int axis = s.axis < 0 ? 0 : s.axis;
+                const int per = (t.count + kWaves - 1) / kWaves;
+                const int wBeg = w * per < t.count ? w * per : t.count;
+                const int wEnd = wBeg + per < t.count ? wBeg + per : t.count;
+                int leftBefore = 0;
+                if constexpr (kWaves > 1) {
+                    int lefts = 0;
+                    for (int r = wBeg; r < wEnd; r += 64) {
+                        const int i = r + lane;
+                        const bool left = i < wEnd && goesLeft(recL[idxS[i]].c[axis], t.start + i, t, s, m);
+                        lefts += __popcll(__ballot(left));
+                    }
+                    if (lane == 0) waveLefts[w] = lefts;
+                    __syncthreads();
+                    for (int v = 0; v < w; ++v) leftBefore += waveLefts[v];
+                }
+                int runL = leftBefore, runR = nL + (wBeg - leftBefore);
+                bool bad = false;
+                for (int r = wBeg; r < wEnd; r += 64) {
+                    const int i = r + lane;
+                    const bool valid = i < wEnd;
+                    unsigned short id = 0;
+                    bool left = false;
+                    if (valid) {
+                        id = idxS[i];
+                        left = goesLeft(recL[id].c[axis], t.start + i, t, s, m);
+                    }
+                    const unsigned long long lm = __ballot(valid && left);
+                    const unsigned long long rm = __ballot(valid && !left);
+                    if (valid) {
+                        const int d = left ? runL + __popcll(lm & below) : runR + __popcll(rm & below);
+                        const bool ok = left ? (d >= 0 && d < nL) : (d >= nL && d < t.count);
+                        if (ok) {
+                            idxD[d] = id;
+                        } else {
+                            bad = true;
+                        }
+                    }
+                    runL += __popcll(lm);
+                    runR += __popcll(rm);
+                }
+                if (bad) atomicOr(err, 2);
+                binned += t.count;
+                parted += t.count;
+                __syncthreads();
+            }
+            deepest = level > deepest ? level : deepest;
+            // every thread computes the split's ids; thread 0 writes the node
+            const int g = t.start + nL - 1, e = t.start + t.count - 1;
+            const int l = (g == t.start) ? n - 1 + g : g;
+            const int r = (g + 1 == e) ? n - 1 + e : g + 1;
+            if (threadIdx.x == 0) {
+                nodes[t.node].left = l;
+                nodes[t.node].right = r;
+                parent[l] = t.node;
+                parent[r] = t.node;
+                farEnd[t.node] = t.node == t.start ? e : t.start;
+            }
+            const int cs[2] = {t.start, t.start + nL}, cc[2] = {nL, t.count - nL}, ci[2] = {l, r};
+#pragma unroll
+            for (int side = 1; side >= 0; --side) {
+                if (cc[side] <= kSerial) {
+                    // the range's leaf order, one thread per position (at most kSerial)
+                    if (threadIdx.x < cc[side])
+                        order[cs[side] + threadIdx.x] =
+                            static_cast<uint32_t>(recL[idxL[childBuf][cs[side] - base + threadIdx.x]].id);
+                    if (threadIdx.x == 0) {
+                        if (serN == kSerialBatch) {
+                            flushSerial(T, serBuf, serN);
+                            serN = 0;
+                        }
+                        serBuf[serN++] = {cs[side], cc[side], ci[side], kOrderWritten};
+                    }
+                } else if (sp < kSubtreeStack) {
+                    if (threadIdx.x == 0)
+                        stk[sp] = childTask(cs[side], cc[side], ci[side], metaOf(childBuf, level + 1), s, side);
+                    ++sp;
+                } else if (threadIdx.x == 0) {
+                    atomicOr(err, 1);
+                }
+            }
+            __syncthreads();
+        }
     }
     if (threadIdx.x == 0) {
         flushSerial(T, serBuf, serN);
@@ -882,7 +1159,8 @@ __global__ void sahSerialKernel(SahTables T, int n, uint32_t* __restrict__ order
         const SerialTask t = T.serial[ti];
         const PrimRec* src = recOf(T, t.buf);
         slots += t.count;
-        for (int i = 0; i < t.count; ++i) order[t.start + i] = static_cast<uint32_t>(src[t.start + i].id);
+        if (t.buf != kOrderWritten)
+            for (int i = 0; i < t.count; ++i) order[t.start + i] = static_cast<uint32_t>(src[t.start + i].id);
         if (t.count < 2) continue;
         int stS[8], stE[8], stId[8];
         int sp = 0;
@@ -1019,7 +1297,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
                 mark(s);
                 sahSplitKernel<<<T.maxBig, 64, 0, s>>>(T, cur, median);
                 mark(s);
-                sahChunkScanKernel<<<taskGrid, kBlock, 0, s>>>(T, cur);
+                sahChunkScanKernel<<<T.maxBig, 64, 0, s>>>(T, cur);
                 sahPartScatterKernel<<<chunkGrid, kBlock, 0, s>>>(T, cur);
                 sahAssignKernel<<<taskGrid, kBlock, 0, s>>>(T, cur, median, n, b.nodes.p, b.parent.p, b.farEnd.p);
                 mark(s);
@@ -1048,13 +1326,13 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     // serial tails; the topology is checked complete before the bottom-up pass
     // climbs it
     if (hw[kCtlLocal] > 0)
-        sahSubtreeKernel<kBlock, true><<<std::min(hw[kCtlLocal], T.maxLocal), kBlock, 0, s>>>(
+        sahSubtreeKernel<kBlock><<<std::min(hw[kCtlLocal], T.maxLocal), kBlock, 0, s>>>(
             T, n, b.nodes.p, b.parent.p, b.farEnd.p);
     if (hw[kCtlLocal] > 0 || hw[kCtlSmall] > 0) {
         // the top tier adds small tasks: their count is known only on the device, so a
         // fixed grid strides the list
         const int smallGrid = std::min(T.maxLocal, 8192);
-        sahSubtreeKernel<64, false><<<smallGrid, 64, 0, s>>>(T, n, b.nodes.p, b.parent.p, b.farEnd.p);
+        sahSmallKernel<kSmallBT><<<smallGrid, kSmallBT, 0, s>>>(T, n, order.p, b.nodes.p, b.parent.p, b.farEnd.p);
     }
     ev.record(3, s);
     sahSerialKernel<<<std::min(blocksFor(T.maxSerial, kBlock), 1024), kBlock, 0, s>>>(T, n, order.p, b.nodes.p,
