@@ -545,20 +545,29 @@ int qr_bvh_update(qr_bvh* bvh, const void* items, int n, const int* deleted, int
             lists.push_back(0);
             EventSet<5> ev;
             ev.record(0, s);
-            DevBuf<unsigned char> dStaged(staged.size());
-            DevBuf<int> dLists(lists.size());
-            hipCheck(hipMemcpyAsync(dStaged.p, staged.data(), staged.size(), hipMemcpyHostToDevice, s), "H2D edit");
-            hipCheck(hipMemcpyAsync(dLists.p, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice, s),
+            // the staged items and index lists share the tree's edit buffer; the new
+            // scene is composed into its stage buffer and the tree rebuilt in place,
+            // so repeated edits of one tree reuse its device memory
+            const size_t listOff = (staged.size() + 255) & ~size_t(255);
+            b.editBuf.reserve(listOff + lists.size() * sizeof(int));
+            unsigned char* dStaged = b.editBuf.p;
+            int* dLists = reinterpret_cast<int*>(b.editBuf.p + listOff);
+            hipCheck(hipMemcpyAsync(dStaged, staged.data(), staged.size(), hipMemcpyHostToDevice, s), "H2D edit");
+            hipCheck(hipMemcpyAsync(dLists, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice, s),
                      "H2D edit lists");
             ev.record(1, s);
-            DevBuf<unsigned char> composed(static_cast<size_t>(n) * isz);
-            composeEdit(b, dLists.p, nDeleted, dLists.p + nDeleted, nInserted, static_cast<int>(tailBeg), n,
-                        dStaged.p, dLists.p + nDeleted + nInserted, nMoved,
-                        dStaged.p + static_cast<size_t>(nStaged) * isz, composed.p, s);
+            b.stage.reserve(static_cast<size_t>(n) * isz);
+            composeEdit(b, dLists, nDeleted, dLists + nDeleted, nInserted, static_cast<int>(tailBeg), n, dStaged,
+                        dLists + nDeleted + nInserted, nMoved, dStaged + static_cast<size_t>(nStaged) * isz, b.stage.p,
+                        s);
             ev.record(2, s);
-            BvhState fresh;
-            buildInto(fresh, b.kind, b.builder, composed.p, n, nullptr);
-            b = std::move(fresh);
+            try {
+                buildInto(b, b.kind, b.builder, b.stage.p, n, nullptr);
+            } catch (...) {
+                b.n = 0;  // the old topology is partly overwritten: the tree is no longer usable
+                b.nodeCount = 0;
+                throw;
+            }
             ev.record(3, s);
             if (hadWide) buildWide(b, s);
             ev.record(4, s);

@@ -1103,14 +1103,14 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     using Item = typename L::Item;
     b.n = n;
     b.nodeCount = 2 * n - 1;
-    b.nodes.alloc(static_cast<size_t>(b.nodeCount));
-    b.items.alloc(static_cast<size_t>(n) * sizeof(Item));
-    b.primIds.alloc(static_cast<size_t>(n));
+    b.nodes.reserve(static_cast<size_t>(b.nodeCount));
+    b.items.reserve(static_cast<size_t>(n) * sizeof(Item));
+    b.primIds.reserve(static_cast<size_t>(n));
     b.primSlot.reserve(static_cast<size_t>(n));  // filled on first use (refits, adjoints)
-    b.parent.alloc(static_cast<size_t>(b.nodeCount));
-    b.arrive.alloc(static_cast<size_t>(n - 1));
-    b.height.alloc(static_cast<size_t>(n - 1));
-    b.farEnd.alloc(static_cast<size_t>(n - 1));
+    b.parent.reserve(static_cast<size_t>(b.nodeCount));
+    b.arrive.reserve(static_cast<size_t>(n - 1));
+    b.height.reserve(static_cast<size_t>(n - 1));
+    b.farEnd.reserve(static_cast<size_t>(n - 1));
     b.sortWs.reserve(static_cast<size_t>(n));
     b.counters.reserve(kBoundsWords);
     int stageEntries = 0, countInts = 0;
@@ -1118,9 +1118,12 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     b.emitStage.reserve(static_cast<size_t>(stageEntries) * (sizeof(FrontierEntry) / sizeof(int)));
     b.emitCounts.reserve(static_cast<size_t>(countInts));
 
-    DevBuf<uint32_t> codes(static_cast<size_t>(n));
-    DevBuf<uint32_t> codesSorted(static_cast<size_t>(n));
-    DevBuf<uint32_t> order(static_cast<size_t>(n));
+    b.keys.reserve(static_cast<size_t>(n));
+    b.keysSorted.reserve(static_cast<size_t>(n));
+    b.order.reserve(static_cast<size_t>(n));
+    uint32_t* codes = b.keys.p;
+    uint32_t* codesSorted = b.keysSorted.p;
+    uint32_t* order = b.order.p;
     uint32_t* cb = reinterpret_cast<uint32_t*>(b.counters.p);
 
     EventSet<6> ev;
@@ -1138,11 +1141,11 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     int grid = blocksFor(n, kBlock * 4 * kUnroll);
     if (grid > 512) grid = 512;
     centroidBoundsKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb);
-    mortonHistKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb, codes.p, b.sortWs.hist.p);
+    mortonHistKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb, codes, b.sortWs.hist.p);
     hipCheck(hipGetLastError(), "morton");
     ev.record(1, s);
 
-    sortPairs(b.sortWs, codes.p, nullptr, codesSorted.p, order.p, static_cast<size_t>(n), 4, s);
+    sortPairs(b.sortWs, codes, nullptr, codesSorted, order, static_cast<size_t>(n), 4, s);
     ev.record(2, s);
 
     // hierarchy, leaf gather and boxes: the windows, then one merge kernel per level
@@ -1150,7 +1153,7 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     ev.record(3, s);
     auto* stage = reinterpret_cast<FrontierEntry*>(b.emitStage.p);
     lbvhEmitKernel<L><<<lv.units[0], kBlock, 0, s>>>(
-        dSrc, codesSorted.p, order.p, n, reinterpret_cast<Item*>(b.items.p), b.primIds.p, b.nodes.p,
+        dSrc, codesSorted, order, n, reinterpret_cast<Item*>(b.items.p), b.primIds.p, b.nodes.p,
         b.parent.p, b.arrive.p, b.height.p, b.farEnd.p, stage + lv.stageOff[0], b.emitCounts.p + lv.countOff[0],
         gFrontierCap);
     hipCheck(hipGetLastError(), "lbvh emit");
@@ -1159,7 +1162,7 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
         lbvhMergeKernel<<<lv.units[level], kMergeBlock, 0, s>>>(
             stage + lv.stageOff[level - 1], b.emitCounts.p + lv.countOff[level - 1], lv.units[level - 1],
             top ? nullptr : stage + lv.stageOff[level], top ? nullptr : b.emitCounts.p + lv.countOff[level], top,
-            codesSorted.p, order.p, n, b.nodes.p, b.parent.p, b.arrive.p, b.height.p, b.farEnd.p, gFrontierCap);
+            codesSorted, order, n, b.nodes.p, b.parent.p, b.arrive.p, b.height.p, b.farEnd.p, gFrontierCap);
         hipCheck(hipGetLastError(), "lbvh merge");
     }
     ev.record(4, s);

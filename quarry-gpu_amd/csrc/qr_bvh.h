@@ -69,6 +69,14 @@ struct BvhState {
 
     // scene-order primitives kept for validate / refit-by-device-pointer paths
     SortWorkspace sortWs;
+
+    // build work buffers, kept with the tree so that a rebuild or topology update of
+    // the same tree allocates nothing once they are large enough: Morton codes, sorted
+    // codes and leaf order (LBVH), and one arena for the SAH builder's records and
+    // task tables
+    DevBuf<uint32_t> keys, keysSorted, order;
+    DevBuf<unsigned char> sahArena;
+    DevBuf<unsigned char> editBuf;  // topology edit: staged items and index lists
 };
 
 // Many independent trees built at once (the batched mode).  Scene s owns items

@@ -114,9 +114,15 @@ struct DevBuf {
         }
         n = count;
     }
-    // Grows to at least count elements; contents are not preserved.
+    // Grows to at least count elements; contents are not preserved.  A buffer that
+    // grows again takes a quarter more than asked, so that a tree growing by small
+    // edits reallocates rarely.
     void reserve(size_t count) {
-        if (count > n || p == nullptr) alloc(count);
+        if (p == nullptr) {
+            alloc(count);
+        } else if (count > n) {
+            alloc(count + count / 4);
+        }
     }
     void release() {
         if (p && owned) (void)hipFree(p);

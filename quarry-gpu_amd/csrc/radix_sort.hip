@@ -339,7 +339,8 @@ int sortTiles(size_t n) { return static_cast<int>((n + sortTileKeys(n) - 1) / so
 
 void SortWorkspace::reserve(size_t n) {
     if (n <= capacity && keysAlt.p) return;
-    const size_t cap = n < 4096 ? 4096 : n;
+    size_t cap = n < 4096 ? 4096 : n;
+    if (keysAlt.p) cap += cap / 4;  // growing again: a quarter more, as DevBuf::reserve
     keysAlt.alloc(cap);
     valsAlt.alloc(cap);
     hist.alloc(static_cast<size_t>(kHistCopies) * kHistCopyWords);

@@ -6,9 +6,10 @@
 //   * tasks above kChunk primitives are cut into kChunk-position chunks; each
 //     workgroup builds its chunk's histogram (3 axes x 32 bins x {count, box,
 //     centroid box}) with LDS atomics and flushes it once with global atomics;
-//     one wave per task then scores the 3 x 31 planes, and a stable two-pass
-//     partition (chunk counts -> per-task chunk offsets -> block scan) moves the
-//     primitive records;
+//     one wave per task then scores the 3 x 31 planes, and a one-pass stable
+//     partition moves the primitive records (the bin pass also counts each
+//     quarter-chunk part's bins, a per-task scan of the parts' left counts gives
+//     every partition wave its destinations);
 //   * tasks of kSerial+1 .. kChunk primitives are handled by one workgroup each:
 //     histogram, plane choice and stable partition never leave the workgroup;
 //   * ranges of kSerial or fewer primitives get balanced count/2 splits in their
@@ -28,6 +29,8 @@
 // path unchanged.  The tree is deterministic: bin boxes are min/max (order-free)
 // and the partition is stable.
 #include <algorithm>
+#include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "qr_bvh.h"
@@ -43,6 +46,8 @@ constexpr int kMedianLevel = 32;
 constexpr int kBlock = 256;
 constexpr int kBinVals = 13;  // count, box lo xyz, box hi xyz, centroid lo xyz, centroid hi xyz
 constexpr int kHist = 3 * kBins * kBinVals;
+constexpr int kParts = 4;                   // a chunk's parts: one partition wave each
+constexpr int kPartLen = kChunk / kParts;   // records per part
 
 struct PrimRec {
     float c[3];
@@ -113,61 +118,92 @@ struct BinRun {
     uint32_t v[12];  // box lo xyz, box hi xyz, centroid lo xyz, centroid hi xyz
 };
 
-__device__ __forceinline__ void binRunFlush(uint32_t* h, int a, const BinRun& r) {
+// Adds a finished run to the histogram (and its count to part[a * kBins + bin] when
+// part is given: the counts of the current part of a chunk).
+// kCent = false leaves the centroid bounds out (a caller that takes its children's
+// centroid boxes from the records themselves).
+template <bool kCent = true>
+__device__ __forceinline__ void binRunFlush(uint32_t* h, int a, const BinRun& r, uint32_t* part = nullptr) {
     if (r.cnt == 0u) return;
     atomicAdd(&h[hidx(a, r.bin, 0)], r.cnt);
+    if (part) atomicAdd(&part[a * kBins + r.bin], r.cnt);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         atomicMin(&h[hidx(a, r.bin, 1 + k)], r.v[k]);
         atomicMax(&h[hidx(a, r.bin, 4 + k)], r.v[3 + k]);
-        atomicMin(&h[hidx(a, r.bin, 7 + k)], r.v[6 + k]);
-        atomicMax(&h[hidx(a, r.bin, 10 + k)], r.v[9 + k]);
+        if (kCent) {
+            atomicMin(&h[hidx(a, r.bin, 7 + k)], r.v[6 + k]);
+            atomicMax(&h[hidx(a, r.bin, 10 + k)], r.v[9 + k]);
+        }
     }
 }
 
-template <int kBT = kBlock>
+__device__ __forceinline__ void binRunsInit(BinRun* run) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        run[a].bin = -1;
+        run[a].cnt = 0u;
+    }
+}
+
+// One record into the thread's runs (a run is flushed when its bin changes); one
+// instantiation per axis, so that the runs and the axis map stay in registers.
+template <int a, bool kCent>
+__device__ __forceinline__ void binRunAddAxis(uint32_t* h, BinRun& u, const PrimRec& r, const AxisMap& m,
+                                              uint32_t* part) {
+    if (!m.use[a]) return;
+    const int b = binOf(r.c[a], m.lo[a], m.scale[a]);
+    if (b != u.bin) {
+        binRunFlush<kCent>(h, a, u, part);
+        u.bin = b;
+        u.cnt = 0u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            u.v[k] = u.v[6 + k] = 0xFFFFFFFFu;
+            u.v[3 + k] = u.v[9 + k] = 0u;
+        }
+    }
+    ++u.cnt;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t lo = orderedBits(r.lo[k]), hi = orderedBits(r.hi[k]), c = orderedBits(r.c[k]);
+        u.v[k] = lo < u.v[k] ? lo : u.v[k];
+        u.v[3 + k] = hi > u.v[3 + k] ? hi : u.v[3 + k];
+        u.v[6 + k] = c < u.v[6 + k] ? c : u.v[6 + k];
+        u.v[9 + k] = c > u.v[9 + k] ? c : u.v[9 + k];
+    }
+}
+
+// r is taken by value: loaded once, not re-read after the flushes' LDS atomics.
+template <bool kCent = true>
+__device__ __forceinline__ void binRunAdd(uint32_t* h, BinRun* run, const PrimRec r, const AxisMap& m,
+                                          uint32_t* part = nullptr) {
+    binRunAddAxis<0, kCent>(h, run[0], r, m, part);
+    binRunAddAxis<1, kCent>(h, run[1], r, m, part);
+    binRunAddAxis<2, kCent>(h, run[2], r, m, part);
+}
+
+template <bool kCent = true>
+__device__ __forceinline__ void binRunsFlush(uint32_t* h, const BinRun* run, const AxisMap& m,
+                                             uint32_t* part = nullptr) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (m.use[a]) binRunFlush<kCent>(h, a, run[a], part);
+}
+
+// Bins a contiguous range of records: each of the kBT threads takes a run of
+// consecutive records (neighbouring records mostly share a bin, so the runs flush
+// rarely).
+template <int kBT>
 __device__ __forceinline__ void histAddRange(uint32_t* h, const PrimRec* __restrict__ rec, int count,
                                              const AxisMap& m) {
     const int per = (count + kBT - 1) / kBT;
     const int beg = threadIdx.x * per;
     const int end = beg + per < count ? beg + per : count;
     BinRun run[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        run[a].bin = -1;
-        run[a].cnt = 0u;
-    }
-    for (int i = beg; i < end; ++i) {
-        const PrimRec r = rec[i];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (!m.use[a]) continue;
-            const int b = binOf(r.c[a], m.lo[a], m.scale[a]);
-            BinRun& u = run[a];
-            if (b != u.bin) {
-                binRunFlush(h, a, u);
-                u.bin = b;
-                u.cnt = 0u;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    u.v[k] = u.v[6 + k] = 0xFFFFFFFFu;
-                    u.v[3 + k] = u.v[9 + k] = 0u;
-                }
-            }
-            ++u.cnt;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const uint32_t lo = orderedBits(r.lo[k]), hi = orderedBits(r.hi[k]), c = orderedBits(r.c[k]);
-                u.v[k] = lo < u.v[k] ? lo : u.v[k];
-                u.v[3 + k] = hi > u.v[3 + k] ? hi : u.v[3 + k];
-                u.v[6 + k] = c < u.v[6 + k] ? c : u.v[6 + k];
-                u.v[9 + k] = c > u.v[9 + k] ? c : u.v[9 + k];
-            }
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-        if (m.use[a]) binRunFlush(h, a, run[a]);
+    binRunsInit(run);
+    for (int i = beg; i < end; ++i) binRunAdd(h, run, rec[i], m);
+    binRunsFlush(h, run, m);
 }
 
 __device__ __forceinline__ float areaOf(float lx, float ly, float lz, float hx, float hy, float hz) {
@@ -175,12 +211,107 @@ __device__ __forceinline__ float areaOf(float lx, float ly, float lz, float hx, 
     return dx * dy + dy * dz + dz * dx;
 }
 
-// Plane choice by one wave: lane b holds bin b (lanes 32..63 repeat lanes 0..31);
-// inclusive prefix and suffix unions of the bins' counts and boxes (five shuffle
-// steps each) give every plane p = 1..31 its left side (bins 0..p-1) and right side
-// (bins p..31) at lane p, which scores it on each usable axis; lane 0 writes the
-// result.  Counts and min / max are exact, so the costs are the same bits as summing
-// the bins plane by plane.  h is the task's histogram (LDS or global).
+// ---- cross-lane steps within each 32-lane half of a wave (DPP) -------------------------
+// Lanes 32..63 of the plane-choice wave repeat lanes 0..31, so every step works on
+// the two halves alike and a half's result can be read from lanes 0..31.
+// kCtrl is a DPP control: row_shr:k 0x110+k (lane i reads lane i-k of its 16-lane
+// row), row_shl:k 0x100+k (lane i+k), wave_shr:1 0x138 (lane i-1 across the wave),
+// quad_perm [1,0,3,2] 0xB1 and [2,3,0,1] 0x4E, row_half_mirror 0x141, row_mirror
+// 0x140.  A lane whose source lies outside its row keeps `id`.
+template <int kCtrl>
+__device__ __forceinline__ int dppI(int x, int id) {
+    return __builtin_amdgcn_update_dpp(id, x, kCtrl, 0xF, 0xF, false);
+}
+template <int kCtrl>
+__device__ __forceinline__ float dppF(float x, float id) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(id), __float_as_int(x), kCtrl, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float readlaneF(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+struct MinF {
+    static constexpr float id = kInf;
+    __device__ static float op(float a, float b) { return fminf(a, b); }
+};
+struct MaxF {
+    static constexpr float id = -kInf;
+    __device__ static float op(float a, float b) { return fmaxf(a, b); }
+};
+
+// Inclusive prefix over the lanes of each half: four row shifts, then rows 1 and 3
+// take the total of the row below (lane 15 of the half; the halves are equal).
+template <typename Op>
+__device__ __forceinline__ float prefix32(float x, int lane) {
+    x = Op::op(x, dppF<0x111>(x, Op::id));
+    x = Op::op(x, dppF<0x112>(x, Op::id));
+    x = Op::op(x, dppF<0x114>(x, Op::id));
+    x = Op::op(x, dppF<0x118>(x, Op::id));
+    const float low = readlaneF(x, 15);
+    return (lane & 16) ? Op::op(x, low) : x;
+}
+__device__ __forceinline__ int prefixAdd32(int x, int lane) {
+    x += dppI<0x111>(x, 0);
+    x += dppI<0x112>(x, 0);
+    x += dppI<0x114>(x, 0);
+    x += dppI<0x118>(x, 0);
+    const int low = __builtin_amdgcn_readlane(x, 15);
+    return (lane & 16) ? x + low : x;
+}
+// Inclusive suffix (the lane and the lanes above it in its half).
+template <typename Op>
+__device__ __forceinline__ float suffix32(float x, int lane) {
+    x = Op::op(x, dppF<0x101>(x, Op::id));
+    x = Op::op(x, dppF<0x102>(x, Op::id));
+    x = Op::op(x, dppF<0x104>(x, Op::id));
+    x = Op::op(x, dppF<0x108>(x, Op::id));
+    const float high = readlaneF(x, 16);
+    return (lane & 16) ? x : Op::op(x, high);
+}
+// Reduction over a half, the same value returned to every lane.
+template <typename Op>
+__device__ __forceinline__ uint32_t reduce32(uint32_t x) {
+    int v = static_cast<int>(x);
+    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0xB1>(v, 0))));
+    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0x4E>(v, 0))));
+    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0x141>(v, 0))));
+    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0x140>(v, 0))));
+    return Op::op(static_cast<uint32_t>(__builtin_amdgcn_readlane(v, 0)),
+                  static_cast<uint32_t>(__builtin_amdgcn_readlane(v, 16)));
+}
+// Reduction over a whole wave (lanes 0..63 all distinct), the same value returned to
+// every lane.
+template <typename Op>
+__device__ __forceinline__ uint32_t reduce64(uint32_t x) {
+    int v = static_cast<int>(x);
+    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0xB1>(v, 0))));
+    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0x4E>(v, 0))));
+    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0x141>(v, 0))));
+    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0x140>(v, 0))));
+    const uint32_t r0 = Op::op(static_cast<uint32_t>(__builtin_amdgcn_readlane(v, 0)),
+                               static_cast<uint32_t>(__builtin_amdgcn_readlane(v, 16)));
+    const uint32_t r1 = Op::op(static_cast<uint32_t>(__builtin_amdgcn_readlane(v, 32)),
+                               static_cast<uint32_t>(__builtin_amdgcn_readlane(v, 48)));
+    return Op::op(r0, r1);
+}
+struct MinU {
+    __device__ static uint32_t op(uint32_t a, uint32_t b) { return a < b ? a : b; }
+};
+struct MaxU {
+    __device__ static uint32_t op(uint32_t a, uint32_t b) { return a > b ? a : b; }
+};
+struct AddU {
+    __device__ static uint32_t op(uint32_t a, uint32_t b) { return a + b; }
+};
+
+// Plane choice by one wave (called by all 64 lanes): lane b holds bin b (lanes
+// 32..63 repeat lanes 0..31); inclusive prefix and suffix unions of the bins' counts
+// and boxes give every plane p = 1..31 its left side (bins 0..p-1, the prefix one
+// lane down) and right side (bins p..31) at lane p, which scores it on each usable
+// axis.  Counts and min / max are exact and order-free, so the costs are the same
+// bits as summing the bins plane by plane.  The (cost, axis * 32 + plane) minimum
+// is reduced across the lanes, then the chosen plane's side counts and centroid
+// boxes; lane 0 writes the result.  h is the task's histogram (LDS or global).
 __device__ void chooseSplit(const uint32_t* h, const SahTask& t, const AxisMap& m, SahResult* out) {
     const int lane = threadIdx.x & 63;
     const int bin = lane & (kBins - 1);
@@ -188,6 +319,7 @@ __device__ void chooseSplit(const uint32_t* h, const SahTask& t, const AxisMap& 
     const int minSide = count / 8 > 1 ? count / 8 : 1;
     float best = kInf;
     int bestKey = 0x7fffffff;  // axis * 32 + plane
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (!m.use[a]) continue;
         const int c = static_cast<int>(h[hidx(a, bin, 0)]);
@@ -197,39 +329,21 @@ __device__ void chooseSplit(const uint32_t* h, const SahTask& t, const AxisMap& 
             v[k] = fromOrderedBits(h[hidx(a, bin, 1 + k)]);
             v[3 + k] = fromOrderedBits(h[hidx(a, bin, 4 + k)]);
         }
-        int pc = c, sc = c;
-        float pv[6], sv[6];
+        const int pc = prefixAdd32(c, lane);
+        const int total = __builtin_amdgcn_readlane(pc, 31);
+        const int nL = dppI<0x138>(pc, 0);  // the prefix through bin - 1 (wave_shr:1)
+        const int nR = total - nL;
+        float l[6], r[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) pv[k] = sv[k] = v[k];
-#pragma unroll
-        for (int off = 1; off < kBins; off <<= 1) {
-            const int oc = __shfl_up(pc, off, kBins);
-            const int uc = __shfl_down(sc, off, kBins);
-            const bool up = bin >= off, down = bin + off < kBins;
-            pc += up ? oc : 0;
-            sc += down ? uc : 0;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                const float op = __shfl_up(pv[k], off, kBins);
-                const float os = __shfl_down(sv[k], off, kBins);
-                if (k < 3) {
-                    pv[k] = up ? fminf(pv[k], op) : pv[k];
-                    sv[k] = down ? fminf(sv[k], os) : sv[k];
-                } else {
-                    pv[k] = up ? fmaxf(pv[k], op) : pv[k];
-                    sv[k] = down ? fmaxf(sv[k], os) : sv[k];
-                }
-            }
+        for (int k = 0; k < 3; ++k) {
+            l[k] = dppF<0x138>(prefix32<MinF>(v[k], lane), kInf);
+            l[3 + k] = dppF<0x138>(prefix32<MaxF>(v[3 + k], lane), -kInf);
+            r[k] = suffix32<MinF>(v[k], lane);
+            r[3 + k] = suffix32<MaxF>(v[3 + k], lane);
         }
-        // the left side of plane `bin` is the prefix through bin - 1
-        const int nL = __shfl_up(pc, 1, kBins);
-        float l[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) l[k] = __shfl_up(pv[k], 1, kBins);
-        const int nR = sc;
         if (bin >= 1 && nL >= minSide && nR >= minSide) {
             const float cost = areaOf(l[0], l[1], l[2], l[3], l[4], l[5]) * static_cast<float>(nL) +
-                               areaOf(sv[0], sv[1], sv[2], sv[3], sv[4], sv[5]) * static_cast<float>(nR);
+                               areaOf(r[0], r[1], r[2], r[3], r[4], r[5]) * static_cast<float>(nR);
             const int key = a * kBins + bin;
             if (cost < best || (cost == best && key < bestKey)) {
                 best = cost;
@@ -237,14 +351,23 @@ __device__ void chooseSplit(const uint32_t* h, const SahTask& t, const AxisMap& 
             }
         }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const float ob = __shfl_xor(best, off, 64);
-        const int ok = __shfl_xor(bestKey, off, 64);
-        if (ob < best || (ob == best && ok < bestKey)) {
-            best = ob;
-            bestKey = ok;
-        }
+    {
+        // (cost, key) minimum over the half, then over the two rows' results
+        auto take = [&](float oc, int ok) {
+            if (oc < best || (oc == best && ok < bestKey)) {
+                best = oc;
+                bestKey = ok;
+            }
+        };
+        take(dppF<0xB1>(best, kInf), dppI<0xB1>(bestKey, 0x7fffffff));
+        take(dppF<0x4E>(best, kInf), dppI<0x4E>(bestKey, 0x7fffffff));
+        take(dppF<0x141>(best, kInf), dppI<0x141>(bestKey, 0x7fffffff));
+        take(dppF<0x140>(best, kInf), dppI<0x140>(bestKey, 0x7fffffff));
+        const float c0 = readlaneF(best, 0), c1 = readlaneF(best, 16);
+        const int k0 = __builtin_amdgcn_readlane(bestKey, 0), k1 = __builtin_amdgcn_readlane(bestKey, 16);
+        best = c0;
+        bestKey = k0;
+        take(c1, k1);
     }
     SahResult res;
     res.pad = 0;
@@ -262,36 +385,26 @@ __device__ void chooseSplit(const uint32_t* h, const SahTask& t, const AxisMap& 
         return;
     }
     // the chosen plane's two sides: counts and centroid boxes of the bins left and
-    // right of it, reduced across the 32 bin lanes (a min / max of ordered bits is
-    // exact and order-free; empty bins take the identities)
+    // right of it (a min / max of ordered bits is exact and order-free; empty bins
+    // take the identities)
     const int a = bestKey / kBins, plane = bestKey % kBins;
     const uint32_t c = h[hidx(a, bin, 0)];
     const bool isLeft = bin < plane;
-    int nL = isLeft ? static_cast<int>(c) : 0;
+    const uint32_t nL = reduce32<AddU>(isLeft ? c : 0u);
     uint32_t v[12];  // left min xyz, left max xyz, right min xyz, right max xyz
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const uint32_t lo = c ? h[hidx(a, bin, 7 + k)] : 0xFFFFFFFFu;
         const uint32_t hi = c ? h[hidx(a, bin, 10 + k)] : 0u;
-        v[k] = isLeft ? lo : 0xFFFFFFFFu;
-        v[3 + k] = isLeft ? hi : 0u;
-        v[6 + k] = isLeft ? 0xFFFFFFFFu : lo;
-        v[9 + k] = isLeft ? 0u : hi;
-    }
-#pragma unroll
-    for (int off = kBins / 2; off > 0; off >>= 1) {
-        nL += __shfl_xor(nL, off, 64);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            const uint32_t o = __shfl_xor(v[k], off, 64);
-            const bool isMin = k < 3 || (k >= 6 && k < 9);
-            v[k] = isMin ? (o < v[k] ? o : v[k]) : (o > v[k] ? o : v[k]);
-        }
+        v[k] = reduce32<MinU>(isLeft ? lo : 0xFFFFFFFFu);
+        v[3 + k] = reduce32<MaxU>(isLeft ? hi : 0u);
+        v[6 + k] = reduce32<MinU>(isLeft ? 0xFFFFFFFFu : lo);
+        v[9 + k] = reduce32<MaxU>(isLeft ? 0u : hi);
     }
     if (lane != 0) return;
     res.axis = a;
     res.plane = plane;
-    res.nL = nL;
+    res.nL = static_cast<int>(nL);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         res.lLo[k] = fromOrderedBits(v[k]);
@@ -303,12 +416,8 @@ __device__ void chooseSplit(const uint32_t* h, const SahTask& t, const AxisMap& 
     *out = res;
 }
 
-// Coordinate `axis` of a record held in registers, by selection (a dynamic subscript
-// into a register copy would go through scratch).
-__device__ __forceinline__ float axisCoord(const PrimRec& r, int axis) {
-    return axis == 0 ? r.c[0] : axis == 1 ? r.c[1] : r.c[2];
-}
-
+// Element `axis` of a three-element array held in registers, by selection (a dynamic
+// subscript into a register copy would go through scratch).
 __device__ __forceinline__ float axisSel(const float* v, int axis) {
     return axis == 0 ? v[0] : axis == 1 ? v[1] : v[2];
 }
@@ -348,10 +457,11 @@ __device__ __forceinline__ int blockExclusive(int v, int* lds, int* total) {
 // rank.  Called by the whole workgroup.  A destination outside the task's side
 // (counts that disagree with the histogram) is reported through err, never written.
 template <int kBT = kBlock>
-__device__ void partitionRange(const PrimRec* __restrict__ src, PrimRec* __restrict__ dst, int begin, int cnt,
+__device__ __forceinline__ void partitionRange(const PrimRec* __restrict__ src, PrimRec* __restrict__ dst, int begin, int cnt,
                                int leftDst, int rightDst, const SahTask& t, const SahResult& s, const AxisMap& m,
                                int* ldsWave, int* err) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ax = s.axis < 0 ? 0 : s.axis;  // the split axis (any for a split by position)
     constexpr int kWaves = kBT / 64;
     const int per = (cnt + kWaves - 1) / kWaves;
     const int wBeg = w * per < cnt ? w * per : cnt;
@@ -360,7 +470,7 @@ __device__ void partitionRange(const PrimRec* __restrict__ src, PrimRec* __restr
     int lefts = 0;
     for (int r = wBeg; r < wEnd; r += 64) {
         const int i = r + lane;
-        const bool left = i < wEnd && goesLeft(axisCoord(src[begin + i], s.axis), begin + i, t, s, m);
+        const bool left = i < wEnd && goesLeft(src[begin + i].c[ax], begin + i, t, s, m);
         lefts += __popcll(__ballot(left));
     }
     if (lane == 0) ldsWave[w] = lefts;
@@ -375,11 +485,19 @@ __device__ void partitionRange(const PrimRec* __restrict__ src, PrimRec* __restr
     for (int r = wBeg; r < wEnd; r += 64) {
         const int i = r + lane;
         const bool valid = i < wEnd;
-        PrimRec rec{};
+        // the record as five 8-byte words (plain values: a struct copy here would be
+        // kept in scratch once the split coordinate is read out of it by index)
+        uint2 w0{}, w1{}, w2{}, w3{}, w4{};
         bool left = false;
         if (valid) {
-            rec = src[begin + i];
-            left = goesLeft(axisCoord(rec, s.axis), begin + i, t, s, m);
+            const uint2* sw = reinterpret_cast<const uint2*>(src + begin + i);
+            w0 = sw[0];
+            w1 = sw[1];
+            w2 = sw[2];
+            w3 = sw[3];
+            w4 = sw[4];
+            const float c = ax == 0 ? __uint_as_float(w0.x) : ax == 1 ? __uint_as_float(w0.y) : __uint_as_float(w1.x);
+            left = goesLeft(c, begin + i, t, s, m);
         }
         const unsigned long long lm = __ballot(valid && left);
         const unsigned long long rm = __ballot(valid && !left);
@@ -387,7 +505,62 @@ __device__ void partitionRange(const PrimRec* __restrict__ src, PrimRec* __restr
             const int d = left ? runL + __popcll(lm & below) : runR + __popcll(rm & below);
             const bool ok = left ? (d >= t.start && d < leftEnd) : (d >= leftEnd && d < rightEnd);
             if (ok) {
-                dst[d] = rec;
+                uint2* dw = reinterpret_cast<uint2*>(dst + d);
+                dw[0] = w0;
+                dw[1] = w1;
+                dw[2] = w2;
+                dw[3] = w3;
+                dw[4] = w4;
+            } else {
+                bad = true;
+            }
+        }
+        runL += __popcll(lm);
+        runR += __popcll(rm);
+    }
+    if (bad) atomicOr(err, 2);
+}
+
+// Stable partition of the records [begin, begin + cnt) by one wave whose left and
+// right destinations are known (from the counts of the range's bins): one pass,
+// lane-strided reads, ballot ranks.  Destinations outside the task's sides are
+// reported through err, never written.
+__device__ __forceinline__ void partitionWaveRange(const PrimRec* __restrict__ src, PrimRec* __restrict__ dst,
+                                                   int begin, int cnt, int leftDst, int rightDst, const SahTask& t,
+                                                   const SahResult& s, const AxisMap& m, int* err) {
+    const int lane = threadIdx.x & 63;
+    const int ax = s.axis < 0 ? 0 : s.axis;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int leftEnd = t.start + s.nL, rightEnd = t.start + t.count;
+    int runL = leftDst, runR = rightDst;
+    bool bad = false;
+    for (int r = 0; r < cnt; r += 64) {
+        const int i = r + lane;
+        const bool valid = i < cnt;
+        uint2 w0{}, w1{}, w2{}, w3{}, w4{};
+        bool left = false;
+        if (valid) {
+            const uint2* sw = reinterpret_cast<const uint2*>(src + begin + i);
+            w0 = sw[0];
+            w1 = sw[1];
+            w2 = sw[2];
+            w3 = sw[3];
+            w4 = sw[4];
+            const float c = ax == 0 ? __uint_as_float(w0.x) : ax == 1 ? __uint_as_float(w0.y) : __uint_as_float(w1.x);
+            left = goesLeft(c, begin + i, t, s, m);
+        }
+        const unsigned long long lm = __ballot(valid && left);
+        const unsigned long long rm = __ballot(valid && !left);
+        if (valid) {
+            const int d = left ? runL + __popcll(lm & below) : runR + __popcll(rm & below);
+            const bool ok = left ? (d >= t.start && d < leftEnd) : (d >= leftEnd && d < rightEnd);
+            if (ok) {
+                uint2* dw = reinterpret_cast<uint2*>(dst + d);
+                dw[0] = w0;
+                dw[1] = w1;
+                dw[2] = w2;
+                dw[3] = w3;
+                dw[4] = w4;
             } else {
                 bad = true;
             }
@@ -513,8 +686,8 @@ struct SahTables {
     SerialTask* serial;
     Chunk* chunks;
     int* first;
-    int* chunkBins;  // per chunk: its primitives' counts per axis and bin (3 x kBins)
-    int* chunkOff;
+    int* chunkBins;  // per chunk part: its primitives' counts per axis and bin (3 x kBins)
+    int* chunkOff;   // per chunk part: lefts of the task's earlier parts
     uint32_t* bins;
     SahResult* results;
     int* ctl;
@@ -687,20 +860,40 @@ __global__ void sahClearBinsKernel(SahTables T, int cur) {
 }
 
 // One workgroup per chunk of a large task: LDS histogram, then one global flush.
+// Wave w bins the chunk's part w (kPartLen records); besides the task's histogram it
+// records each part's bin counts, from which the chunk scan gives every part its
+// partition offsets without a counting pass.
 __global__ __launch_bounds__(kBlock) void sahBinKernel(SahTables T, int cur) {
+    static_assert(kChunk == kParts * kPartLen, "a chunk is its parts");
+    static_assert(kBlock / 64 == kParts, "one wave per part");
     if (static_cast<int>(blockIdx.x) >= T.ctl[kCtlChunks]) return;
     __shared__ uint32_t h[kHist];
+    __shared__ uint32_t part[kParts][3 * kBins];
     const Chunk ch = T.chunks[blockIdx.x];
     const SahTask t = bigOf(T, cur)[ch.task];
     const AxisMap m = axisMap(t);
     histInit(h);
+    for (int i = threadIdx.x; i < kParts * 3 * kBins; i += kBlock) (&part[0][0])[i] = 0u;
     __syncthreads();
-    histAddRange(h, recOf(T, t.meta & 1) + ch.start, ch.count, m);
+    // wave w bins part w, each lane a run of consecutive records
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int pBeg = w * kPartLen;
+    const int pCnt = ch.count - pBeg < kPartLen ? ch.count - pBeg : kPartLen;
+    if (pCnt > 0) {
+        const PrimRec* src = recOf(T, t.meta & 1) + ch.start + pBeg;
+        const int per = (pCnt + 63) / 64;
+        const int beg = lane * per;
+        const int end = beg + per < pCnt ? beg + per : pCnt;
+        BinRun run[3];
+        binRunsInit(run);
+        for (int i = beg; i < end; ++i) binRunAdd(h, run, src[i], m, part[w]);
+        binRunsFlush(h, run, m, part[w]);
+    }
     __syncthreads();
     uint32_t* g = T.bins + static_cast<size_t>(ch.task) * kHist;
-    // the chunk's own bin counts: its left count for any plane, without a re-read
-    for (int i = threadIdx.x; i < 3 * kBins; i += kBlock)
-        T.chunkBins[static_cast<size_t>(blockIdx.x) * 3 * kBins + i] = static_cast<int>(h[i * kBinVals]);
+    // the parts' bin counts: each part's left count for any plane, without a re-read
+    for (int i = threadIdx.x; i < kParts * 3 * kBins; i += kBlock)
+        T.chunkBins[static_cast<size_t>(blockIdx.x) * kParts * 3 * kBins + i] = static_cast<int>((&part[0][0])[i]);
     for (int i = threadIdx.x; i < kHist; i += kBlock) {
         const int v = i % kBinVals;
         if (h[i - v] == 0u) continue;  // empty bin
@@ -741,53 +934,66 @@ __global__ __launch_bounds__(64) void sahSplitKernel(SahTables T, int cur, int m
     chooseSplit(T.bins + static_cast<size_t>(blockIdx.x) * kHist, t, axisMap(t), &T.results[blockIdx.x]);
 }
 
-// One wave per large task: exclusive offsets of its chunks' left counts, from the
-// chunks' bin counts (the bins left of the chosen plane on its axis) or, for a split
-// by position, from the chunk's place in the task; 64 chunks per step, a wave scan.
-__global__ __launch_bounds__(64) void sahChunkScanKernel(SahTables T, int cur) {
+// One workgroup per large task: exclusive offsets of its chunk parts' left counts
+// (part e = kParts * chunk + p), from the parts' bin counts (the bins left of the
+// chosen plane on its axis, read as eight 16-byte words) or, for a split by
+// position, from the part's place in the task; kBlock parts per step, a block scan.
+__global__ __launch_bounds__(kBlock) void sahChunkScanKernel(SahTables T, int cur) {
+    __shared__ int lds[4];
     const int ti = blockIdx.x;
     if (ti >= T.ctl[kCtlBig0 + cur] || T.ctl[kCtlChunks] == 0) return;
-    const int lane = threadIdx.x;
     const SahTask t = bigOf(T, cur)[ti];
     const SahResult r = T.results[ti];
-    const int c0 = T.first[ti], c1 = T.first[ti + 1];
+    const int e0 = T.first[ti] * kParts, e1 = T.first[ti + 1] * kParts;
     int acc = 0;
-    for (int base = c0; base < c1; base += 64) {
-        const int c = base + lane;
+    for (int base = e0; base < e1; base += kBlock) {
+        const int e = base + threadIdx.x;
         int left = 0;
-        if (c < c1) {
+        if (e < e1) {
             if (r.axis < 0) {
-                const Chunk ch = T.chunks[c];
-                const int before = t.start + r.nL - ch.start;
-                left = before < 0 ? 0 : (before > ch.count ? ch.count : before);
+                const Chunk ch = T.chunks[e / kParts];
+                const int pBeg = (e % kParts) * kPartLen;
+                int pCnt = ch.count - pBeg;
+                pCnt = pCnt < 0 ? 0 : (pCnt > kPartLen ? kPartLen : pCnt);
+                const int before = t.start + r.nL - (ch.start + pBeg);
+                left = before < 0 ? 0 : (before > pCnt ? pCnt : before);
             } else {
-                const int* cnt = T.chunkBins + static_cast<size_t>(c) * 3 * kBins + r.axis * kBins;
-                for (int b = 0; b < r.plane; ++b) left += cnt[b];
+                const int4* cnt =
+                    reinterpret_cast<const int4*>(T.chunkBins + static_cast<size_t>(e) * 3 * kBins + r.axis * kBins);
+#pragma unroll
+                for (int q = 0; q < kBins / 4; ++q) {
+                    const int4 v = cnt[q];
+                    left += (4 * q < r.plane ? v.x : 0) + (4 * q + 1 < r.plane ? v.y : 0) +
+                            (4 * q + 2 < r.plane ? v.z : 0) + (4 * q + 3 < r.plane ? v.w : 0);
+                }
             }
         }
-        int x = left;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int y = __shfl_up(x, off, 64);
-            if (lane >= off) x += y;
-        }
-        if (c < c1) T.chunkOff[c] = acc + x - left;
-        acc += __shfl(x, 63, 64);
+        int total = 0;
+        const int ex = blockExclusive(left, lds, &total);
+        if (e < e1) T.chunkOff[e] = acc + ex;
+        acc += total;
     }
-    if (lane == 0 && acc != r.nL) atomicOr(&T.ctl[kCtlError], 2);  // the chunks disagree with the task's histogram
+    if (threadIdx.x == 0 && acc != r.nL) atomicOr(&T.ctl[kCtlError], 2);  // the parts disagree with the task's histogram
 }
 
+// One workgroup per chunk, one wave per part: each part's records go to their
+// sides at the offsets the chunk scan gave it, in one pass.
 __global__ __launch_bounds__(kBlock) void sahPartScatterKernel(SahTables T, int cur) {
     if (static_cast<int>(blockIdx.x) >= T.ctl[kCtlChunks]) return;
-    __shared__ int lds[4];
     const Chunk ch = T.chunks[blockIdx.x];
     const SahTask t = bigOf(T, cur)[ch.task];
     const SahResult s = T.results[ch.task];
     const AxisMap m = axisMap(t);
-    const int chunkLeft = T.chunkOff[blockIdx.x];  // lefts of the task's earlier chunks
-    const int chunkPos = ch.start - t.start;
-    partitionRange(recOf(T, t.meta & 1), recOf(T, 1 - (t.meta & 1)), ch.start, ch.count, t.start + chunkLeft,
-                   t.start + s.nL + (chunkPos - chunkLeft), t, s, m, lds, &T.ctl[kCtlError]);
+    const int w = threadIdx.x >> 6;
+    const int pBeg = w * kPartLen;
+    const int pCnt = ch.count - pBeg < kPartLen ? ch.count - pBeg : kPartLen;
+    if (pCnt > 0) {
+        const int e = blockIdx.x * kParts + w;
+        const int partLeft = T.chunkOff[e];  // lefts of the task's earlier parts
+        const int partPos = ch.start + pBeg - t.start;
+        partitionWaveRange(recOf(T, t.meta & 1), recOf(T, 1 - (t.meta & 1)), ch.start + pBeg, pCnt,
+                           t.start + partLeft, t.start + s.nL + (partPos - partLeft), t, s, m, &T.ctl[kCtlError]);
+    }
     if (threadIdx.x == 0) atomicAdd(&T.ctl[kCtlParted], ch.count);
 }
 
@@ -935,13 +1141,16 @@ __global__ __launch_bounds__(kBT) void sahSubtreeKernel(SahTables T, int n, Node
     }
 }
 
-// Bottom tier: one small task (kSerial+1 .. kSmallTask primitives) per one-wave
-// workgroup, grid-strided over the small list.  The task's records are read into LDS
-// once; its subtree is then split entirely in LDS -- histograms read the records
-// through a position -> record index, and each stable partition moves 16-bit indices
-// between two LDS index buffers -- so the lower levels cost no HBM traffic.  A range
-// that reaches kSerial or fewer primitives writes its leaf order (scene ids) at once
-// and goes to the serial list marked kOrderWritten for its balanced splits.
+// Bottom tier: one small task (kSerial+1 .. kSmallTask primitives) per workgroup of
+// kSmallBT threads, grid-strided over the small list.  The task's records are read
+// into LDS once; its subtree is then split entirely in LDS -- histograms read the
+// records through a position -> record index, and each stable partition moves
+// 16-bit indices between two LDS index buffers -- so the lower levels cost no HBM
+// traffic.  The histograms keep no centroid bounds: after a split by plane the
+// children's centroid boxes are reduced from their records (the same min / max the
+// bins would give).  A range that reaches kSerial or fewer primitives writes its
+// leaf order (scene ids) at once and goes to the serial list marked kOrderWritten
+// for its balanced splits.
 constexpr int kOrderWritten = 2;  // SerialTask.buf: the range's leaf order is already in place
 constexpr int kSmallBT = 128;     // threads per small task
 
@@ -952,42 +1161,9 @@ __device__ __forceinline__ void histAddLds(uint32_t* h, const PrimRec* recL, con
     const int beg = threadIdx.x * per;
     const int end = beg + per < count ? beg + per : count;
     BinRun run[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        run[a].bin = -1;
-        run[a].cnt = 0u;
-    }
-    for (int i = beg; i < end; ++i) {
-        const PrimRec r = recL[idx[i]];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (!m.use[a]) continue;
-            const int b = binOf(r.c[a], m.lo[a], m.scale[a]);
-            BinRun& u = run[a];
-            if (b != u.bin) {
-                binRunFlush(h, a, u);
-                u.bin = b;
-                u.cnt = 0u;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    u.v[k] = u.v[6 + k] = 0xFFFFFFFFu;
-                    u.v[3 + k] = u.v[9 + k] = 0u;
-                }
-            }
-            ++u.cnt;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const uint32_t lo = orderedBits(r.lo[k]), hi = orderedBits(r.hi[k]), c = orderedBits(r.c[k]);
-                u.v[k] = lo < u.v[k] ? lo : u.v[k];
-                u.v[3 + k] = hi > u.v[3 + k] ? hi : u.v[3 + k];
-                u.v[6 + k] = c < u.v[6 + k] ? c : u.v[6 + k];
-                u.v[9 + k] = c > u.v[9 + k] ? c : u.v[9 + k];
-            }
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-        if (m.use[a]) binRunFlush(h, a, run[a]);
+    binRunsInit(run);
+    for (int i = beg; i < end; ++i) binRunAdd<false>(h, run, recL[idx[i]], m);
+    binRunsFlush<false>(h, run, m);
 }
 
 template <int kBT>
@@ -1002,6 +1178,7 @@ __global__ __launch_bounds__(kBT) void sahSmallKernel(SahTables T, int n, uint32
     __shared__ SahTask stk[kSubtreeStack];
     __shared__ SerialTask serBuf[kSerialBatch];
     __shared__ int waveLefts[kWaves];
+    __shared__ uint32_t waveBox[kWaves][12];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const unsigned long long below = (1ull << lane) - 1ull;
     int serN = 0;  // thread 0's count of buffered serial ranges
@@ -1101,6 +1278,56 @@ __global__ __launch_bounds__(kBT) void sahSmallKernel(SahTables T, int n, uint32
                 parted += t.count;
                 __syncthreads();
             }
+            SahResult sc = s;  // s with the children's centroid boxes
+            if (!median && s.axis >= 0) {
+                // left and right centroid boxes as ordered bits: min xyz, max xyz each
+                uint32_t bx[12];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    bx[k] = bx[6 + k] = 0xFFFFFFFFu;
+                    bx[3 + k] = bx[9 + k] = 0u;
+                }
+                const unsigned short* idxD = idxL[childBuf] + ls;
+                for (int j = threadIdx.x; j < t.count; j += kBT) {
+                    const PrimRec r = recL[idxD[j]];
+                    const bool isLeft = j < nL;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const uint32_t u = orderedBits(r.c[k]);
+                        const uint32_t lo = isLeft ? bx[k] : bx[6 + k], hi = isLeft ? bx[3 + k] : bx[9 + k];
+                        const uint32_t nlo = u < lo ? u : lo, nhi = u > hi ? u : hi;
+                        bx[k] = isLeft ? nlo : bx[k];
+                        bx[6 + k] = isLeft ? bx[6 + k] : nlo;
+                        bx[3 + k] = isLeft ? nhi : bx[3 + k];
+                        bx[9 + k] = isLeft ? bx[9 + k] : nhi;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 12; ++k) {
+                    const bool isMin = k < 3 || (k >= 6 && k < 9);
+                    bx[k] = isMin ? reduce64<MinU>(bx[k]) : reduce64<MaxU>(bx[k]);
+                }
+                if (lane == 0) {
+#pragma unroll
+                    for (int k = 0; k < 12; ++k) waveBox[w][k] = bx[k];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < 12; ++k) {
+                    const bool isMin = k < 3 || (k >= 6 && k < 9);
+                    uint32_t x = waveBox[0][k];
+                    for (int v = 1; v < kWaves; ++v) x = isMin ? MinU::op(x, waveBox[v][k]) : MaxU::op(x, waveBox[v][k]);
+                    bx[k] = x;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    sc.lLo[k] = fromOrderedBits(bx[k]);
+                    sc.lHi[k] = fromOrderedBits(bx[3 + k]);
+                    sc.rLo[k] = fromOrderedBits(bx[6 + k]);
+                    sc.rHi[k] = fromOrderedBits(bx[9 + k]);
+                }
+                __syncthreads();  // waveBox is read before the next split writes it
+            }
             deepest = level > deepest ? level : deepest;
             // every thread computes the split's ids; thread 0 writes the node
             const int g = t.start + nL - 1, e = t.start + t.count - 1;
@@ -1130,7 +1357,7 @@ __global__ __launch_bounds__(kBT) void sahSmallKernel(SahTables T, int n, uint32
                     }
                 } else if (sp < kSubtreeStack) {
                     if (threadIdx.x == 0)
-                        stk[sp] = childTask(cs[side], cc[side], ci[side], metaOf(childBuf, level + 1), s, side);
+                        stk[sp] = childTask(cs[side], cc[side], ci[side], metaOf(childBuf, level + 1), sc, side);
                     ++sp;
                 } else if (threadIdx.x == 0) {
                     atomicOr(err, 1);
@@ -1214,50 +1441,54 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     using Item = typename L::Item;
     b.n = n;
     b.nodeCount = 2 * n - 1;
-    b.nodes.alloc(static_cast<size_t>(b.nodeCount));
-    b.items.alloc(static_cast<size_t>(n) * sizeof(Item));
-    b.primIds.alloc(static_cast<size_t>(n));
+    b.nodes.reserve(static_cast<size_t>(b.nodeCount));
+    b.items.reserve(static_cast<size_t>(n) * sizeof(Item));
+    b.primIds.reserve(static_cast<size_t>(n));
     b.primSlot.reserve(static_cast<size_t>(n));  // filled on first use (refits, adjoints)
-    b.parent.alloc(static_cast<size_t>(b.nodeCount));
-    b.arrive.alloc(static_cast<size_t>(n - 1));
-    b.height.alloc(static_cast<size_t>(n - 1));
-    b.farEnd.alloc(static_cast<size_t>(n - 1));
+    b.parent.reserve(static_cast<size_t>(b.nodeCount));
+    b.arrive.reserve(static_cast<size_t>(n - 1));
+    b.height.reserve(static_cast<size_t>(n - 1));
+    b.farEnd.reserve(static_cast<size_t>(n - 1));
     b.sortWs.reserve(static_cast<size_t>(n));
     b.counters.reserve(16);
     uint32_t* cb = reinterpret_cast<uint32_t*>(b.counters.p);
-
-    DevBuf<PrimRec> rec[2];
-    rec[0].alloc(static_cast<size_t>(n));
-    rec[1].alloc(static_cast<size_t>(n));
-    DevBuf<uint32_t> order(static_cast<size_t>(n));
 
     SahTables T{};
     T.maxBig = n / (kChunk + 1) + 2;
     T.maxLocal = n / (kSerial + 1) + 2;
     T.maxSerial = n + 2;
     T.maxChunks = n / kChunk + T.maxBig + 2;
-    DevBuf<SahTask> dBig(2 * static_cast<size_t>(T.maxBig)), dLocal(static_cast<size_t>(T.maxLocal)),
-        dSmall(static_cast<size_t>(T.maxLocal));
-    DevBuf<SerialTask> dSerial(static_cast<size_t>(T.maxSerial));
-    DevBuf<Chunk> dChunks(static_cast<size_t>(T.maxChunks));
-    DevBuf<int> dFirst(static_cast<size_t>(T.maxBig) + 1), dChunkBins(static_cast<size_t>(T.maxChunks) * 3 * kBins),
-        dChunkOff(static_cast<size_t>(T.maxChunks)), dCtl(kCtlWords);
-    DevBuf<uint32_t> dBins(static_cast<size_t>(T.maxBig) * kHist);
-    DevBuf<SahResult> dRes(static_cast<size_t>(T.maxBig));
-    T.rec[0] = rec[0].p;
-    T.rec[1] = rec[1].p;
-    T.big[0] = dBig.p;
-    T.big[1] = dBig.p + T.maxBig;
-    T.local = dLocal.p;
-    T.small = dSmall.p;
-    T.serial = dSerial.p;
-    T.chunks = dChunks.p;
-    T.first = dFirst.p;
-    T.chunkBins = dChunkBins.p;
-    T.chunkOff = dChunkOff.p;
-    T.bins = dBins.p;
-    T.results = dRes.p;
-    T.ctl = dCtl.p;
+    // the records, leaf order and task tables, carved from the tree's arena (a dry
+    // run sizes it)
+    uint32_t* order = nullptr;
+    auto carve = [&](unsigned char* base) {
+        size_t off = 0;
+        auto take = [&](auto*& ptr, size_t count) {
+            off = (off + 255) & ~size_t(255);
+            ptr = reinterpret_cast<std::remove_reference_t<decltype(ptr)>>(reinterpret_cast<uintptr_t>(base) + off);
+            off += count * sizeof(*ptr);
+        };
+        SahTask* big = nullptr;
+        take(T.rec[0], static_cast<size_t>(n));
+        take(T.rec[1], static_cast<size_t>(n));
+        take(order, static_cast<size_t>(n));
+        take(big, 2 * static_cast<size_t>(T.maxBig));
+        take(T.local, static_cast<size_t>(T.maxLocal));
+        take(T.small, static_cast<size_t>(T.maxLocal));
+        take(T.serial, static_cast<size_t>(T.maxSerial));
+        take(T.chunks, static_cast<size_t>(T.maxChunks));
+        take(T.first, static_cast<size_t>(T.maxBig) + 1);
+        take(T.chunkBins, static_cast<size_t>(T.maxChunks) * kParts * 3 * kBins);
+        take(T.chunkOff, static_cast<size_t>(T.maxChunks) * kParts);
+        take(T.ctl, static_cast<size_t>(kCtlWords));
+        take(T.bins, static_cast<size_t>(T.maxBig) * kHist);
+        take(T.results, static_cast<size_t>(T.maxBig));
+        T.big[0] = big;
+        T.big[1] = reinterpret_cast<SahTask*>(reinterpret_cast<uintptr_t>(big) + T.maxBig * sizeof(SahTask));
+        return off;
+    };
+    b.sahArena.reserve(carve(nullptr));
+    carve(b.sahArena.p);
     int* hw = pinnedWords();
 
     EventSet<6> ev;
@@ -1265,7 +1496,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     sahResetKernel<<<blocksFor(n - 1 > 256 ? n - 1 : 256, kBlock), kBlock, 0, s>>>(cb, b.arrive.p, n - 1);
     int grid = blocksFor(n, kBlock * 4);
     if (grid > 512) grid = 512;  // two workgroups per CU stream the items; few bounds atomics
-    sahPrepKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, rec[0].p, cb, b.parent.p);
+    sahPrepKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, T.rec[0], cb, b.parent.p);
     sahRootKernel<<<1, 64, 0, s>>>(T, cb, n);
     hipCheck(hipGetLastError(), "sah prep");
     ev.record(1, s);
@@ -1297,7 +1528,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
                 mark(s);
                 sahSplitKernel<<<T.maxBig, 64, 0, s>>>(T, cur, median);
                 mark(s);
-                sahChunkScanKernel<<<T.maxBig, 64, 0, s>>>(T, cur);
+                sahChunkScanKernel<<<T.maxBig, kBlock, 0, s>>>(T, cur);
                 sahPartScatterKernel<<<chunkGrid, kBlock, 0, s>>>(T, cur);
                 sahAssignKernel<<<taskGrid, kBlock, 0, s>>>(T, cur, median, n, b.nodes.p, b.parent.p, b.farEnd.p);
                 mark(s);
@@ -1332,10 +1563,10 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
         // the top tier adds small tasks: their count is known only on the device, so a
         // fixed grid strides the list
         const int smallGrid = std::min(T.maxLocal, 8192);
-        sahSmallKernel<kSmallBT><<<smallGrid, kSmallBT, 0, s>>>(T, n, order.p, b.nodes.p, b.parent.p, b.farEnd.p);
+        sahSmallKernel<kSmallBT><<<smallGrid, kSmallBT, 0, s>>>(T, n, order, b.nodes.p, b.parent.p, b.farEnd.p);
     }
     ev.record(3, s);
-    sahSerialKernel<<<std::min(blocksFor(T.maxSerial, kBlock), 1024), kBlock, 0, s>>>(T, n, order.p, b.nodes.p,
+    sahSerialKernel<<<std::min(blocksFor(T.maxSerial, kBlock), 1024), kBlock, 0, s>>>(T, n, order, b.nodes.p,
                                                                                      b.parent.p, b.farEnd.p);
     hipCheck(hipGetLastError(), "sah subtrees");
     hipCheck(hipMemcpyAsync(hw, T.ctl, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, s), "sah control");
@@ -1343,7 +1574,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     ev.sync(4);
     if (hw[kCtlError] != 0) failed();
 
-    finishTopology(b, dSrc, order.p, s);
+    finishTopology(b, dSrc, order, s);
     hipCheck(hipMemcpyAsync(hw, T.ctl, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, s), "sah control");
     int depth = 0;
     hipCheck(hipMemcpyAsync(&depth, b.height.p, sizeof(int), hipMemcpyDeviceToHost, s), "depth");
