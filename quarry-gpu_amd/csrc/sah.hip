@@ -850,13 +850,14 @@ __global__ __launch_bounds__(kBlock) void sahPlanKernel(SahTables T, int cur, in
     }
 }
 
-__global__ void sahClearBinsKernel(SahTables T, int cur) {
-    const long long words = static_cast<long long>(T.ctl[kCtlBig0 + cur]) * kHist;
-    for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < words;
-         i += static_cast<long long>(gridDim.x) * kBlock) {
-        const int v = static_cast<int>(i % kHist) % kBinVals;
-        T.bins[i] = v == 0 ? 0u : ((v >= 1 && v <= 3) || (v >= 7 && v <= 9)) ? orderedBits(kInf) : orderedBits(-kInf);
-    }
+// The histogram buffer of a level's tasks (two buffers, by level parity).
+__device__ __forceinline__ uint32_t* binsOf(const SahTables& T, int cur) {
+    return T.bins + (cur ? static_cast<size_t>(T.maxBig) * kHist : size_t(0));
+}
+
+__device__ __forceinline__ uint32_t binInit(int i) {
+    const int v = i % kBinVals;
+    return v == 0 ? 0u : ((v >= 1 && v <= 3) || (v >= 7 && v <= 9)) ? orderedBits(kInf) : orderedBits(-kInf);
 }
 
 // One workgroup per chunk of a large task: LDS histogram, then one global flush.
@@ -890,7 +891,7 @@ __global__ __launch_bounds__(kBlock) void sahBinKernel(SahTables T, int cur) {
         binRunsFlush(h, run, m, part[w]);
     }
     __syncthreads();
-    uint32_t* g = T.bins + static_cast<size_t>(ch.task) * kHist;
+    uint32_t* g = binsOf(T, cur) + static_cast<size_t>(ch.task) * kHist;
     // the parts' bin counts: each part's left count for any plane, without a re-read
     for (int i = threadIdx.x; i < kParts * 3 * kBins; i += kBlock)
         T.chunkBins[static_cast<size_t>(blockIdx.x) * kParts * 3 * kBins + i] = static_cast<int>((&part[0][0])[i]);
@@ -923,57 +924,89 @@ __device__ __forceinline__ SahResult medianResult(const SahTask& t) {
     return res;
 }
 
-// One wave per large task.
-__global__ __launch_bounds__(64) void sahSplitKernel(SahTables T, int cur, int median) {
-    if (static_cast<int>(blockIdx.x) >= T.ctl[kCtlBig0 + cur]) return;
-    const SahTask t = bigOf(T, cur)[blockIdx.x];
-    if (median) {
-        if (threadIdx.x == 0) T.results[blockIdx.x] = medianResult(t);
-        return;
-    }
-    chooseSplit(T.bins + static_cast<size_t>(blockIdx.x) * kHist, t, axisMap(t), &T.results[blockIdx.x]);
+// Both histogram buffers to their empty state, once per build; afterwards each
+// task's split leaves its histogram empty again for the level two on.
+__global__ void sahClearBinsKernel(SahTables T) {
+    const long long words = 2LL * T.maxBig * kHist;
+    for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < words;
+         i += static_cast<long long>(gridDim.x) * kBlock)
+        T.bins[i] = binInit(static_cast<int>(i % kHist));
 }
 
-// One workgroup per large task: exclusive offsets of its chunk parts' left counts
-// (part e = kParts * chunk + p), from the parts' bin counts (the bins left of the
-// chosen plane on its axis, read as eight 16-byte words) or, for a split by
-// position, from the part's place in the task; kBlock parts per step, a block scan.
-__global__ __launch_bounds__(kBlock) void sahChunkScanKernel(SahTables T, int cur) {
+// One workgroup per large task: the plane choice (wave 0), then the exclusive
+// offsets of the task's chunk parts' left counts (part e = kParts * chunk + p, from
+// the parts' bin counts -- the bins left of the chosen plane on its axis, read as
+// eight 16-byte words -- or, for a split by position, from the part's place in the
+// task; kBlock parts per step, a block scan), the task's node and its children
+// routed by size.  The task's histogram is left empty for the level two on.
+__global__ __launch_bounds__(kBlock) void sahSplitKernel(SahTables T, int cur, int median, int n,
+                                                         Node2* __restrict__ nodes, int* __restrict__ parent,
+                                                         int* __restrict__ farEnd) {
     __shared__ int lds[4];
+    __shared__ SahResult sres;
     const int ti = blockIdx.x;
-    if (ti >= T.ctl[kCtlBig0 + cur] || T.ctl[kCtlChunks] == 0) return;
+    if (ti >= T.ctl[kCtlBig0 + cur]) return;
     const SahTask t = bigOf(T, cur)[ti];
-    const SahResult r = T.results[ti];
-    const int e0 = T.first[ti] * kParts, e1 = T.first[ti + 1] * kParts;
-    int acc = 0;
-    for (int base = e0; base < e1; base += kBlock) {
-        const int e = base + threadIdx.x;
-        int left = 0;
-        if (e < e1) {
-            if (r.axis < 0) {
-                const Chunk ch = T.chunks[e / kParts];
-                const int pBeg = (e % kParts) * kPartLen;
-                int pCnt = ch.count - pBeg;
-                pCnt = pCnt < 0 ? 0 : (pCnt > kPartLen ? kPartLen : pCnt);
-                const int before = t.start + r.nL - (ch.start + pBeg);
-                left = before < 0 ? 0 : (before > pCnt ? pCnt : before);
-            } else {
-                const int4* cnt =
-                    reinterpret_cast<const int4*>(T.chunkBins + static_cast<size_t>(e) * 3 * kBins + r.axis * kBins);
+    uint32_t* h = binsOf(T, cur) + static_cast<size_t>(ti) * kHist;
+    if (median) {
+        if (threadIdx.x == 0) sres = medianResult(t);
+    } else if (threadIdx.x < 64) {
+        chooseSplit(h, t, axisMap(t), &sres);
+    }
+    __syncthreads();
+    const SahResult r = sres;
+    if (threadIdx.x == 0) T.results[ti] = r;
+    if (!median)
+        for (int i = threadIdx.x; i < kHist; i += kBlock) h[i] = binInit(i);
+
+    if (T.ctl[kCtlChunks] != 0) {
+        const int e0 = T.first[ti] * kParts, e1 = T.first[ti + 1] * kParts;
+        int acc = 0;
+        for (int base = e0; base < e1; base += kBlock) {
+            const int e = base + threadIdx.x;
+            int left = 0;
+            if (e < e1) {
+                if (r.axis < 0) {
+                    const Chunk ch = T.chunks[e / kParts];
+                    const int pBeg = (e % kParts) * kPartLen;
+                    int pCnt = ch.count - pBeg;
+                    pCnt = pCnt < 0 ? 0 : (pCnt > kPartLen ? kPartLen : pCnt);
+                    const int before = t.start + r.nL - (ch.start + pBeg);
+                    left = before < 0 ? 0 : (before > pCnt ? pCnt : before);
+                } else {
+                    const int4* cnt = reinterpret_cast<const int4*>(T.chunkBins + static_cast<size_t>(e) * 3 * kBins +
+                                                                    r.axis * kBins);
 #pragma unroll
-                for (int q = 0; q < kBins / 4; ++q) {
-                    const int4 v = cnt[q];
-                    left += (4 * q < r.plane ? v.x : 0) + (4 * q + 1 < r.plane ? v.y : 0) +
-                            (4 * q + 2 < r.plane ? v.z : 0) + (4 * q + 3 < r.plane ? v.w : 0);
+                    for (int q = 0; q < kBins / 4; ++q) {
+                        const int4 v = cnt[q];
+                        left += (4 * q < r.plane ? v.x : 0) + (4 * q + 1 < r.plane ? v.y : 0) +
+                                (4 * q + 2 < r.plane ? v.z : 0) + (4 * q + 3 < r.plane ? v.w : 0);
+                    }
                 }
             }
+            int total = 0;
+            const int ex = blockExclusive(left, lds, &total);
+            if (e < e1) T.chunkOff[e] = acc + ex;
+            acc += total;
         }
-        int total = 0;
-        const int ex = blockExclusive(left, lds, &total);
-        if (e < e1) T.chunkOff[e] = acc + ex;
-        acc += total;
+        // the parts disagree with the task's histogram
+        if (threadIdx.x == 0 && acc != r.nL) atomicOr(&T.ctl[kCtlError], 2);
     }
-    if (threadIdx.x == 0 && acc != r.nL) atomicOr(&T.ctl[kCtlError], 2);  // the parts disagree with the task's histogram
+
+    if (threadIdx.x == 0) {
+        const int nL = r.nL;
+        if (nL < 1 || nL >= t.count) {
+            atomicOr(&T.ctl[kCtlError], 4);
+            return;
+        }
+        const int level = t.meta >> 1;
+        const int buf = median ? (t.meta & 1) : 1 - (t.meta & 1);
+        atomicMax(&T.ctl[kCtlMaxLevel], level);
+        int l, rr;
+        writeSplit(n, t.start, t.count, t.node, nL, nodes, parent, farEnd, &l, &rr);
+        routeChild(T, 1 - cur, t.start, nL, l, buf, level + 1, r.lLo, r.lHi);
+        routeChild(T, 1 - cur, t.start + nL, t.count - nL, rr, buf, level + 1, r.rLo, r.rHi);
+    }
 }
 
 // One workgroup per chunk, one wave per part: each part's records go to their
@@ -995,27 +1028,6 @@ __global__ __launch_bounds__(kBlock) void sahPartScatterKernel(SahTables T, int 
                            t.start + partLeft, t.start + s.nL + (partPos - partLeft), t, s, m, &T.ctl[kCtlError]);
     }
     if (threadIdx.x == 0) atomicAdd(&T.ctl[kCtlParted], ch.count);
-}
-
-// One thread per large task: its node, and its children routed by size.
-__global__ void sahAssignKernel(SahTables T, int cur, int median, int n, Node2* __restrict__ nodes,
-                                int* __restrict__ parent, int* __restrict__ farEnd) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= T.ctl[kCtlBig0 + cur]) return;
-    const SahTask t = bigOf(T, cur)[i];
-    const SahResult r = T.results[i];
-    int nL = r.nL;
-    if (nL < 1 || nL >= t.count) {
-        atomicOr(&T.ctl[kCtlError], 4);
-        return;
-    }
-    const int level = t.meta >> 1;
-    const int buf = median ? (t.meta & 1) : 1 - (t.meta & 1);
-    atomicMax(&T.ctl[kCtlMaxLevel], level);
-    int l, rr;
-    writeSplit(n, t.start, t.count, t.node, nL, nodes, parent, farEnd, &l, &rr);
-    routeChild(T, 1 - cur, t.start, nL, l, buf, level + 1, r.lLo, r.lHi);
-    routeChild(T, 1 - cur, t.start + nL, t.count - nL, rr, buf, level + 1, r.rLo, r.rHi);
 }
 
 // Local tasks (kSerial+1 .. kChunk primitives) build their whole subtrees without a
@@ -1481,7 +1493,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
         take(T.chunkBins, static_cast<size_t>(T.maxChunks) * kParts * 3 * kBins);
         take(T.chunkOff, static_cast<size_t>(T.maxChunks) * kParts);
         take(T.ctl, static_cast<size_t>(kCtlWords));
-        take(T.bins, static_cast<size_t>(T.maxBig) * kHist);
+        take(T.bins, 2 * static_cast<size_t>(T.maxBig) * kHist);
         take(T.results, static_cast<size_t>(T.maxBig));
         T.big[0] = big;
         T.big[1] = reinterpret_cast<SahTask*>(reinterpret_cast<uintptr_t>(big) + T.maxBig * sizeof(SahTask));
@@ -1515,22 +1527,22 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     if (n > kChunk) {
         int batch = 1;
         while ((static_cast<long long>(kChunk) << batch) < n) ++batch;
-        const int chunkGrid = T.maxChunks, taskGrid = blocksFor(T.maxBig, kBlock);
-        const int clearGrid = std::min(blocksFor(static_cast<long long>(T.maxBig) * kHist, kBlock), 1024);
+        const int chunkGrid = T.maxChunks;
+        const int clearGrid = std::min(blocksFor(2LL * T.maxBig * kHist, kBlock), 1024);
+        sahClearBinsKernel<<<clearGrid, kBlock, 0, s>>>(T);
         while (true) {
             for (int j = 0; j < batch; ++j, ++level) {
                 const int cur = level & 1;
                 const int median = level >= kMedianLevel ? 1 : 0;
+                // bin (the plan's chunk list), split (plane, part offsets, node and
+                // children), partition
                 mark(s);
                 sahPlanKernel<<<1, kBlock, 0, s>>>(T, cur, median);
-                sahClearBinsKernel<<<clearGrid, kBlock, 0, s>>>(T, cur);
                 sahBinKernel<<<chunkGrid, kBlock, 0, s>>>(T, cur);
                 mark(s);
-                sahSplitKernel<<<T.maxBig, 64, 0, s>>>(T, cur, median);
+                sahSplitKernel<<<T.maxBig, kBlock, 0, s>>>(T, cur, median, n, b.nodes.p, b.parent.p, b.farEnd.p);
                 mark(s);
-                sahChunkScanKernel<<<T.maxBig, kBlock, 0, s>>>(T, cur);
                 sahPartScatterKernel<<<chunkGrid, kBlock, 0, s>>>(T, cur);
-                sahAssignKernel<<<taskGrid, kBlock, 0, s>>>(T, cur, median, n, b.nodes.p, b.parent.p, b.farEnd.p);
                 mark(s);
             }
             hipCheck(hipGetLastError(), "sah level");
