@@ -123,16 +123,26 @@ def main() -> int:
     sah_wide_ms = sah.build_wide()
     st = sah.timings
     sah_levels_ms = st.tree_ms
+    # The phase split of the level loop comes from one more build with the per-level
+    # phase events on (they cost about 0.2 ms, so the timed build runs without them).
+    A.check(A.lib().qr_set_build_phase_timing(1))
+    try:
+        probe = A.Bvh.from_device(A.PRIM_TRI, dtris.ptr.value, n, A.BUILDER_SAH)
+        pt = probe.timings
+        probe.close()
+    finally:
+        A.check(A.lib().qr_set_build_phase_timing(0))
     # levels: the large tasks' level loop (binning, plane choice, partition + child
     # routing) and the local subtrees + serial tails (scatter_ms)
     sah_parts = {"prep_ms": round(st.morton_ms, 4), "levels_ms": round(st.tree_ms, 4),
-                 "bin_ms": round(st.sah_bin_ms, 4), "split_ms": round(st.sah_split_ms, 4),
-                 "partition_ms": round(st.sah_part_ms, 4), "subtrees_ms": round(st.scatter_ms, 4),
-                 "leaves_boxes_ms": round(st.refit_ms, 4)}
-    log(f"SAH build: {sah_build_ms:.3f} ms (prep {st.morton_ms:.3f}, levels {st.tree_ms:.3f}: large tasks bin "
-        f"{st.sah_bin_ms:.3f}, split {st.sah_split_ms:.3f}, partition {st.sah_part_ms:.3f}, local subtrees "
+                 "subtrees_ms": round(st.scatter_ms, 4), "leaves_boxes_ms": round(st.refit_ms, 4),
+                 "with_phase_events": {"total_ms": round(pt.total_ms, 4), "bin_ms": round(pt.sah_bin_ms, 4),
+                                       "split_ms": round(pt.sah_split_ms, 4),
+                                       "partition_ms": round(pt.sah_part_ms, 4)}}
+    log(f"SAH build: {sah_build_ms:.3f} ms (prep {st.morton_ms:.3f}, levels {st.tree_ms:.3f} incl. local subtrees "
         f"{st.scatter_ms:.3f}; leaves+boxes {st.refit_ms:.3f}); depth {sah.info().depth}; "
-        f"wide collapse {sah_wide_ms:.3f} ms")
+        f"wide collapse {sah_wide_ms:.3f} ms; with phase events {pt.total_ms:.3f} ms: large tasks bin "
+        f"{pt.sah_bin_ms:.3f}, split {pt.sah_split_ms:.3f}, partition {pt.sah_part_ms:.3f}")
     serve = sah if args.builder == "sah" else lbvh
 
     pb1 = A.PointBuffer(queries, 1)

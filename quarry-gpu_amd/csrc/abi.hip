@@ -341,6 +341,13 @@ int qr_set_frontier_cap(int cap) {
     return guard([&] { setFrontierCap(cap); });
 }
 
+int qr_set_build_phase_timing(int on) {
+    return guard([&] {
+        need(on == 0 || on == 1, "phase timing must be 0 or 1");
+        gSahPhaseTiming = on != 0;
+    });
+}
+
 int qr_set_stream(void* s) {
     tStream = static_cast<hipStream_t>(s);
     return QR_OK;

@@ -344,7 +344,10 @@ __global__ __launch_bounds__(kBlock) void leafUpLocalKernel(
     const typename L::Item* __restrict__ src, const uint32_t* __restrict__ order, int n,
     typename L::Item* __restrict__ sorted, int* __restrict__ primIds, Node2* __restrict__ nodes,
     const int* __restrict__ parent, int* __restrict__ arrive, int* __restrict__ height,
-    const int* __restrict__ farEnd) {
+    const int* __restrict__ farEnd, const int* __restrict__ gate) {
+    // a builder whose topology failed (its error word set) leaves it unclimbed; the
+    // host reports the error after the stream completes
+    if (gate != nullptr && *gate != 0) return;
     __shared__ float sLeaf[kBlock][6];
     __shared__ float sNode[kBlock][6];
     __shared__ int sCount[kBlock];
@@ -1218,24 +1221,24 @@ void buildLbvhDevice(BvhState& b, int kind, const void* dItems, int n, hipStream
 // Leaf gather, leaf nodes and bottom-up boxes for a topology built elsewhere (the
 // SAH builder): internal nodes' left/right, parent[] (root -1) and zeroed arrival
 // counters must be in place; order[k] is the scene index of leaf slot k.
-void finishTopology(BvhState& b, const void* dSrc, const uint32_t* order, hipStream_t s) {
+void finishTopology(BvhState& b, const void* dSrc, const uint32_t* order, hipStream_t s, const int* gate) {
     const int n = b.n;
     const int grid = blocksFor(n, kBlock);
     switch (b.kind) {
         case kPrimTri:
             leafUpLocalKernel<TriLeaf, true><<<grid, kBlock, 0, s>>>(
                 static_cast<const Tri*>(dSrc), order, n, reinterpret_cast<Tri*>(b.items.p), b.primIds.p,
-                b.nodes.p, b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
+                b.nodes.p, b.parent.p, b.arrive.p, b.height.p, b.farEnd.p, gate);
             break;
         case kPrimPoint:
             leafUpLocalKernel<PointLeaf, true><<<grid, kBlock, 0, s>>>(
                 static_cast<const V3*>(dSrc), order, n, reinterpret_cast<V3*>(b.items.p), b.primIds.p,
-                b.nodes.p, b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
+                b.nodes.p, b.parent.p, b.arrive.p, b.height.p, b.farEnd.p, gate);
             break;
         default:
             leafUpLocalKernel<GaussLeaf, true><<<grid, kBlock, 0, s>>>(
                 static_cast<const Gaussian*>(dSrc), order, n, reinterpret_cast<Gaussian*>(b.items.p),
-                b.primIds.p, b.nodes.p, b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
+                b.primIds.p, b.nodes.p, b.parent.p, b.arrive.p, b.height.p, b.farEnd.p, gate);
             break;
     }
     hipCheck(hipGetLastError(), "finish topology");
@@ -1250,17 +1253,17 @@ void propagateAll(BvhState& b, hipStream_t s) {
         case kPrimTri:
             leafUpLocalKernel<TriLeaf, false><<<grid, kBlock, 0, s>>>(
                 nullptr, nullptr, n, reinterpret_cast<Tri*>(b.items.p), nullptr, b.nodes.p,
-                b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
+                b.parent.p, b.arrive.p, b.height.p, b.farEnd.p, nullptr);
             break;
         case kPrimPoint:
             leafUpLocalKernel<PointLeaf, false><<<grid, kBlock, 0, s>>>(
                 nullptr, nullptr, n, reinterpret_cast<V3*>(b.items.p), nullptr, b.nodes.p,
-                b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
+                b.parent.p, b.arrive.p, b.height.p, b.farEnd.p, nullptr);
             break;
         default:
             leafUpLocalKernel<GaussLeaf, false><<<grid, kBlock, 0, s>>>(
                 nullptr, nullptr, n, reinterpret_cast<Gaussian*>(b.items.p), nullptr, b.nodes.p,
-                b.parent.p, b.arrive.p, b.height.p, b.farEnd.p);
+                b.parent.p, b.arrive.p, b.height.p, b.farEnd.p, nullptr);
             break;
     }
     hipCheck(hipGetLastError(), "propagate");

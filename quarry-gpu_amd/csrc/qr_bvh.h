@@ -26,6 +26,8 @@ struct SahBuildStats {
     long long binnedSlots = 0, partitionedSlots = 0, serialSlots = 0;
 };
 extern thread_local SahBuildStats tSahStats;
+// SAH builds record per-level phase events (bin / split / partition timings) when set.
+extern bool gSahPhaseTiming;
 
 struct BvhState {
     int kind = kPrimTri;
@@ -133,7 +135,10 @@ void propagateAll(BvhState& b, hipStream_t s);
 // Leaf gather + leaf nodes + bottom-up boxes and heights for a topology already in
 // b.nodes (internal left/right), b.parent (root -1) and zeroed b.arrive; order[k]
 // is the scene index of leaf slot k.  The root's height (b.height[0]) is the depth.
-void finishTopology(BvhState& b, const void* dSrc, const uint32_t* order, hipStream_t s);
+// gate: a device word that, when non-zero at launch, leaves the topology unclimbed
+// (a builder's error flag), or nullptr.
+void finishTopology(BvhState& b, const void* dSrc, const uint32_t* order, hipStream_t s,
+                    const int* gate = nullptr);
 
 // Scene-order items of the new scene of a topology edit, on the device, from the
 // old tree's items (survivors) and the uploaded items: dStaged = inserted items

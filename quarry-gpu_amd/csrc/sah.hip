@@ -1447,6 +1447,7 @@ int* pinnedWords() {
 }  // namespace
 
 thread_local SahBuildStats tSahStats;
+bool gSahPhaseTiming = false;  // qr_set_build_phase_timing
 
 template <typename L>
 static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStream_t s, BuildTimingsC* t) {
@@ -1518,15 +1519,22 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     // says whether to go on.  The first batch covers the levels a balanced split
     // would need to bring every task under kChunk.
     std::vector<hipEvent_t> lev;
+    // per-level phase events only when asked for (each costs a few microseconds of
+    // device time per level)
     auto mark = [&](hipStream_t st) {
+        if (!gSahPhaseTiming) return;
         const hipEvent_t e = takeEvent();
         hipCheck(hipEventRecord(e, st), "event record");
         lev.push_back(e);
     };
     int level = 0;
     if (n > kChunk) {
+        // the first batch: the levels a balanced split needs to bring every task under
+        // kChunk, and two more (an empty level costs four short launches, a host
+        // round trip more)
         int batch = 1;
         while ((static_cast<long long>(kChunk) << batch) < n) ++batch;
+        batch += 2;
         const int chunkGrid = T.maxChunks;
         const int clearGrid = std::min(blocksFor(2LL * T.maxBig * kHist, kBlock), 1024);
         sahClearBinsKernel<<<clearGrid, kBlock, 0, s>>>(T);
@@ -1566,8 +1574,8 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     if (hw[kCtlError] != 0) failed();
     ev.record(2, s);
     // local subtrees (the host knows their number from the last read), then the
-    // serial tails; the topology is checked complete before the bottom-up pass
-    // climbs it
+    // serial tails; the bottom-up pass runs only if no kernel set the error word (a
+    // device-side gate: no host round trip before it)
     if (hw[kCtlLocal] > 0)
         sahSubtreeKernel<kBlock><<<std::min(hw[kCtlLocal], T.maxLocal), kBlock, 0, s>>>(
             T, n, b.nodes.p, b.parent.p, b.farEnd.p);
@@ -1581,17 +1589,19 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     sahSerialKernel<<<std::min(blocksFor(T.maxSerial, kBlock), 1024), kBlock, 0, s>>>(T, n, order, b.nodes.p,
                                                                                      b.parent.p, b.farEnd.p);
     hipCheck(hipGetLastError(), "sah subtrees");
-    hipCheck(hipMemcpyAsync(hw, T.ctl, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, s), "sah control");
     ev.record(4, s);
-    ev.sync(4);
-    if (hw[kCtlError] != 0) failed();
 
-    finishTopology(b, dSrc, order, s);
+    finishTopology(b, dSrc, order, s, &T.ctl[kCtlError]);
     hipCheck(hipMemcpyAsync(hw, T.ctl, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, s), "sah control");
     int depth = 0;
     hipCheck(hipMemcpyAsync(&depth, b.height.p, sizeof(int), hipMemcpyDeviceToHost, s), "depth");
     ev.record(5, s);
     ev.sync(5);
+    if (hw[kCtlError] != 0) {
+        b.n = 0;  // the topology is incomplete: the tree is not usable
+        b.nodeCount = 0;
+        failed();
+    }
     b.depth = depth;
     SahBuildStats st;
     st.levels = hw[kCtlMaxLevel] + 1;

@@ -103,6 +103,7 @@ def lib():
             "qr_dev_copy": ([_vp, _vp, ctypes.c_size_t, _dp], _ip),
             "qr_set_traversal_mode": ([_ip], _ip),
             "qr_set_frontier_cap": ([_ip], _ip),
+            "qr_set_build_phase_timing": ([_ip], _ip),
             "qr_bvh_build": ([_ip, _ip, _vp, _ip, _vp, _vp], _ip),
             "qr_bvh_build_device": ([_ip, _ip, _vp, _ip, _vp, _vp], _ip),
             "qr_bvh_destroy": ([_vp], _ip),
