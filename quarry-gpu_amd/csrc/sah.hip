@@ -194,7 +194,7 @@ __device__ __forceinline__ void binRunsFlush(uint32_t* h, const BinRun* run, con
 // Bins a contiguous range of records: each of the kBT threads takes a run of
 // consecutive records (neighbouring records mostly share a bin, so the runs flush
 // rarely).
-template <int kBT>
+template <int kBT, bool kCent = true>
 __device__ __forceinline__ void histAddRange(uint32_t* h, const PrimRec* __restrict__ rec, int count,
                                              const AxisMap& m) {
     const int per = (count + kBT - 1) / kBT;
@@ -202,8 +202,8 @@ __device__ __forceinline__ void histAddRange(uint32_t* h, const PrimRec* __restr
     const int end = beg + per < count ? beg + per : count;
     BinRun run[3];
     binRunsInit(run);
-    for (int i = beg; i < end; ++i) binRunAdd(h, run, rec[i], m);
-    binRunsFlush(h, run, m);
+    for (int i = beg; i < end; ++i) binRunAdd<kCent>(h, run, rec[i], m);
+    binRunsFlush<kCent>(h, run, m);
 }
 
 __device__ __forceinline__ float areaOf(float lx, float ly, float lz, float hx, float hy, float hz) {
@@ -456,10 +456,10 @@ __device__ __forceinline__ int blockExclusive(int v, int* lds, int* total) {
 // counts meet in LDS (one barrier), and a second pass writes every record at its
 // rank.  Called by the whole workgroup.  A destination outside the task's side
 // (counts that disagree with the histogram) is reported through err, never written.
-template <int kBT = kBlock>
+template <int kBT = kBlock, bool kBoxes = false>
 __device__ __forceinline__ void partitionRange(const PrimRec* __restrict__ src, PrimRec* __restrict__ dst, int begin, int cnt,
                                int leftDst, int rightDst, const SahTask& t, const SahResult& s, const AxisMap& m,
-                               int* ldsWave, int* err) {
+                               int* ldsWave, int* err, uint32_t* waveBox = nullptr) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ax = s.axis < 0 ? 0 : s.axis;  // the split axis (any for a split by position)
     constexpr int kWaves = kBT / 64;
@@ -482,6 +482,14 @@ __device__ __forceinline__ void partitionRange(const PrimRec* __restrict__ src, 
     int runR = rightDst + (wBeg - leftBefore);
     const int leftEnd = t.start + s.nL, rightEnd = t.start + t.count;
     bool bad = false;
+    // kBoxes: the sides' centroid boxes as ordered bits (left min xyz, max xyz, right
+    // min xyz, max xyz), reduced per wave into waveBox[w][12]
+    uint32_t bx[12];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        bx[k] = bx[6 + k] = 0xFFFFFFFFu;
+        bx[3 + k] = bx[9 + k] = 0u;
+    }
     for (int r = wBeg; r < wEnd; r += 64) {
         const int i = r + lane;
         const bool valid = i < wEnd;
@@ -498,6 +506,20 @@ __device__ __forceinline__ void partitionRange(const PrimRec* __restrict__ src, 
             w4 = sw[4];
             const float c = ax == 0 ? __uint_as_float(w0.x) : ax == 1 ? __uint_as_float(w0.y) : __uint_as_float(w1.x);
             left = goesLeft(c, begin + i, t, s, m);
+            if (kBoxes) {
+                const uint32_t u[3] = {orderedBits(__uint_as_float(w0.x)), orderedBits(__uint_as_float(w0.y)),
+                                       orderedBits(__uint_as_float(w1.x))};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int o = left ? 0 : 6;
+                    const uint32_t lo = left ? bx[k] : bx[6 + k], hi = left ? bx[3 + k] : bx[9 + k];
+                    const uint32_t nlo = u[k] < lo ? u[k] : lo, nhi = u[k] > hi ? u[k] : hi;
+                    bx[k] = o == 0 ? nlo : bx[k];
+                    bx[6 + k] = o == 6 ? nlo : bx[6 + k];
+                    bx[3 + k] = o == 0 ? nhi : bx[3 + k];
+                    bx[9 + k] = o == 6 ? nhi : bx[9 + k];
+                }
+            }
         }
         const unsigned long long lm = __ballot(valid && left);
         const unsigned long long rm = __ballot(valid && !left);
@@ -519,6 +541,17 @@ __device__ __forceinline__ void partitionRange(const PrimRec* __restrict__ src, 
         runR += __popcll(rm);
     }
     if (bad) atomicOr(err, 2);
+    if (kBoxes) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const bool isMin = k < 3 || (k >= 6 && k < 9);
+            bx[k] = isMin ? reduce64<MinU>(bx[k]) : reduce64<MaxU>(bx[k]);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) waveBox[w * 12 + k] = bx[k];
+        }
+    }
 }
 
 // Stable partition of the records [begin, begin + cnt) by one wave whose left and
@@ -1064,6 +1097,7 @@ __global__ __launch_bounds__(kBT) void sahSubtreeKernel(SahTables T, int n, Node
     __shared__ int ssp;
     __shared__ int lds[4];
     __shared__ SerialTask serBuf[kSerialBatch];
+    __shared__ uint32_t waveBox[kBT / 64][12];
     int serN = 0;  // thread 0's count of buffered serial ranges
     int binned = 0, parted = 0, deepest = 0;
     // local tasks blockIdx.x, + gridDim.x, ...
@@ -1093,17 +1127,38 @@ __global__ __launch_bounds__(kBT) void sahSubtreeKernel(SahTables T, int n, Node
         if (!median) {
             histInit(h);
             __syncthreads();
-            histAddRange<kBT>(h, src + t.start, t.count, m);
+            histAddRange<kBT, false>(h, src + t.start, t.count, m);
             __syncthreads();
             if (threadIdx.x < 64) chooseSplit(h, t, m, &sres);
         } else if (threadIdx.x == 0) {
             sres = medianResult(t);
         }
         __syncthreads();
-        const SahResult s = sres;
+        SahResult s = sres;
         if (!median) {
-            partitionRange<kBT>(src, dst, t.start, t.count, t.start, t.start + s.nL, t, s, m, lds,
-                                &T.ctl[kCtlError]);
+            // the histogram kept no centroid bounds: a split by plane takes its
+            // children's centroid boxes from the records the partition moves (the same
+            // min / max the bins would give)
+            partitionRange<kBT, true>(src, dst, t.start, t.count, t.start, t.start + s.nL, t, s, m, lds,
+                                      &T.ctl[kCtlError], &waveBox[0][0]);
+            __syncthreads();
+            if (s.axis >= 0) {
+                uint32_t bx[12];
+#pragma unroll
+                for (int k = 0; k < 12; ++k) {
+                    const bool isMin = k < 3 || (k >= 6 && k < 9);
+                    uint32_t x = waveBox[0][k];
+                    for (int v = 1; v < kBT / 64; ++v) x = isMin ? MinU::op(x, waveBox[v][k]) : MaxU::op(x, waveBox[v][k]);
+                    bx[k] = x;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    s.lLo[k] = fromOrderedBits(bx[k]);
+                    s.lHi[k] = fromOrderedBits(bx[3 + k]);
+                    s.rLo[k] = fromOrderedBits(bx[6 + k]);
+                    s.rHi[k] = fromOrderedBits(bx[9 + k]);
+                }
+            }
             binned += t.count;
             parted += t.count;
         }
@@ -1386,53 +1441,47 @@ __global__ __launch_bounds__(kBT) void sahSmallKernel(SahTables T, int n, uint32
     }
 }
 
-// One thread per final range of at most kSerial primitives (grid-stride over the
+// One wave per final range of at most kSerial primitives (grid-stride over the
 // serial list): its leaf order (the scene indices, read from the record buffer that
-// holds the range) and, for two or more primitives, a balanced subtree of count/2
-// splits.
-__global__ void sahSerialKernel(SahTables T, int n, uint32_t* __restrict__ order, Node2* __restrict__ nodes,
-                                int* __restrict__ parent, int* __restrict__ farEnd) {
+// holds the range, one lane per position) unless the small-task tier wrote it, and,
+// for two or more primitives, a balanced subtree of count/2 splits.  Each internal
+// node of that subtree splits after a different position of the range, so lane j
+// takes the node that splits after position start + j, found by descending from the
+// range's root (at most 6 steps); the nodes of a range are consecutive ids, so the
+// wave's writes are contiguous.
+__global__ __launch_bounds__(kBlock) void sahSerialKernel(SahTables T, int n, uint32_t* __restrict__ order,
+                                                          Node2* __restrict__ nodes, int* __restrict__ parent,
+                                                          int* __restrict__ farEnd) {
+    static_assert(kSerial <= 64, "one lane per position of a serial range");
     const int count = T.ctl[kCtlSerial] < T.maxSerial ? T.ctl[kCtlSerial] : T.maxSerial;
-    int slots = 0;
-    for (int ti = blockIdx.x * kBlock + threadIdx.x; ti < count; ti += kBlock * gridDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int waves = static_cast<int>(gridDim.x) * (kBlock / 64);
+    for (int ti = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); ti < count; ti += waves) {
         const SerialTask t = T.serial[ti];
-        const PrimRec* src = recOf(T, t.buf);
-        slots += t.count;
-        if (t.buf != kOrderWritten)
-            for (int i = 0; i < t.count; ++i) order[t.start + i] = static_cast<uint32_t>(src[t.start + i].id);
-        if (t.count < 2) continue;
-        int stS[8], stE[8], stId[8];
-        int sp = 0;
-        stS[0] = t.start;
-        stE[0] = t.start + t.count - 1;
-        stId[0] = t.node;
-        sp = 1;
-        while (sp > 0) {
-            --sp;
-            const int s = stS[sp], e = stE[sp], id = stId[sp];
-            const int g = s + (e - s + 1) / 2 - 1;
-            const int l = (g == s) ? n - 1 + s : g;
-            const int r = (g + 1 == e) ? n - 1 + e : g + 1;
-            nodes[id].left = l;
-            nodes[id].right = r;
-            parent[l] = id;
-            parent[r] = id;
-            farEnd[id] = id == s ? e : s;
-            if (g > s) {
-                stS[sp] = s;
-                stE[sp] = g;
-                stId[sp] = g;
-                ++sp;
+        if (t.buf != kOrderWritten && lane < t.count)
+            order[t.start + lane] = static_cast<uint32_t>(recOf(T, t.buf)[t.start + lane].id);
+        if (lane >= t.count - 1) continue;  // count - 1 internal nodes, one per lane
+        const int target = t.start + lane;  // the split position this lane's node owns
+        int lo = t.start, hi = t.start + t.count - 1, id = t.node;
+        int g = lo + (hi - lo + 1) / 2 - 1;
+        while (g != target) {
+            if (target < g) {
+                hi = g;
+                id = g;  // an internal left child is numbered by its range's last slot
+            } else {
+                lo = g + 1;
+                id = g + 1;  // an internal right child by its range's first slot
             }
-            if (e > g + 1) {
-                stS[sp] = g + 1;
-                stE[sp] = e;
-                stId[sp] = g + 1;
-                ++sp;
-            }
+            g = lo + (hi - lo + 1) / 2 - 1;
         }
+        const int l = (g == lo) ? n - 1 + lo : g;
+        const int r = (g + 1 == hi) ? n - 1 + hi : g + 1;
+        nodes[id].left = l;
+        nodes[id].right = r;
+        parent[l] = id;
+        parent[r] = id;
+        farEnd[id] = id == lo ? hi : lo;
     }
-    if (slots) atomicAdd(&T.ctl[kCtlSerialSlots], slots);
 }
 
 // Pinned host words for the loop's control reads (per host thread, kept for the
@@ -1586,8 +1635,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
         sahSmallKernel<kSmallBT><<<smallGrid, kSmallBT, 0, s>>>(T, n, order, b.nodes.p, b.parent.p, b.farEnd.p);
     }
     ev.record(3, s);
-    sahSerialKernel<<<std::min(blocksFor(T.maxSerial, kBlock), 1024), kBlock, 0, s>>>(T, n, order, b.nodes.p,
-                                                                                     b.parent.p, b.farEnd.p);
+    sahSerialKernel<<<2048, kBlock, 0, s>>>(T, n, order, b.nodes.p, b.parent.p, b.farEnd.p);
     hipCheck(hipGetLastError(), "sah subtrees");
     ev.record(4, s);
 
@@ -1607,7 +1655,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     st.levels = hw[kCtlMaxLevel] + 1;
     st.binnedSlots = static_cast<unsigned int>(hw[kCtlBinned]);
     st.partitionedSlots = static_cast<unsigned int>(hw[kCtlParted]);
-    st.serialSlots = static_cast<unsigned int>(hw[kCtlSerialSlots]);
+    st.serialSlots = static_cast<unsigned int>(n);  // every primitive ends in exactly one serial range
     tSahStats = st;
     if (t) {
         double binMs = 0.0, splitMs = 0.0, partMs = 0.0;
