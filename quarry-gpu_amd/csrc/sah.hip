@@ -98,11 +98,14 @@ __device__ __forceinline__ int binOf(float c, float lo, float scale) {
     return static_cast<int>(f);
 }
 
+// The empty value of histogram word i: count 0, box and centroid bounds inverted.
+__device__ __forceinline__ uint32_t binInit(int i) {
+    const int v = i % kBinVals;
+    return v == 0 ? 0u : ((v >= 1 && v <= 3) || (v >= 7 && v <= 9)) ? orderedBits(kInf) : orderedBits(-kInf);
+}
+
 __device__ __forceinline__ void histInit(uint32_t* h) {
-    for (int i = threadIdx.x; i < kHist; i += blockDim.x) {
-        const int v = i % kBinVals;
-        h[i] = v == 0 ? 0u : ((v >= 1 && v <= 3) || (v >= 7 && v <= 9)) ? orderedBits(kInf) : orderedBits(-kInf);
-    }
+    for (int i = threadIdx.x; i < kHist; i += blockDim.x) h[i] = binInit(i);
 }
 
 // Bins of a contiguous range of records.  Each thread takes a run of consecutive
@@ -675,13 +678,7 @@ __global__ __launch_bounds__(kBlock) void sahPrepKernel(const typename L::Item* 
     }
 }
 
-__global__ void sahResetKernel(uint32_t* __restrict__ cb, int* __restrict__ arrive, int internal) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i == 8) cb[8] = 0u;  // partition error counter
-    if (i < 3) cb[i] = orderedBits(kInf);
-    if (i >= 3 && i < 6) cb[i] = orderedBits(-kInf);
-    for (int j = i; j < internal; j += kBlock * gridDim.x) arrive[j] = 0;
-}
+
 
 // ---- device-driven split loop ------------------------------------------------------------
 // Control words of the loop (device memory): the big-task counts of the two level
@@ -822,9 +819,25 @@ __device__ __forceinline__ void routeChild(const SahTables& T, int nextBuf, int 
 }
 
 // The root: one task over [0, n) with the centroid bounds of the prep kernel.
-__global__ void sahRootKernel(SahTables T, const uint32_t* __restrict__ cb, int n) {
-    if (threadIdx.x != 0) return;
-    for (int i = 0; i < kCtlWords; ++i) T.ctl[i] = 0;
+// Build set-up: centroid bounds to empty, arrival counters to zero, and (given the
+// loop's tables) the control words to zero and both histogram buffers to empty.
+__global__ void sahResetKernel(uint32_t* __restrict__ cb, int* __restrict__ arrive, int internal, SahTables T,
+                               int clearBins) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int stride = kBlock * gridDim.x;
+    if (i == 8) cb[8] = 0u;  // partition error counter
+    if (i < 3) cb[i] = orderedBits(kInf);
+    if (i >= 3 && i < 6) cb[i] = orderedBits(-kInf);
+    if (i < kCtlWords) T.ctl[i] = 0;
+    for (int j = i; j < internal; j += stride) arrive[j] = 0;
+    if (clearBins) {
+        const long long words = 2LL * T.maxBig * kHist;
+        for (long long j = i; j < words; j += stride) T.bins[j] = binInit(static_cast<int>(j % kHist));
+    }
+}
+
+// The root task, from the scene's centroid bounds (the control words are zero).
+__device__ __forceinline__ void routeRoot(const SahTables& T, const uint32_t* __restrict__ cb, int n) {
     float lo[3], hi[3];
     for (int a = 0; a < 3; ++a) {
         lo[a] = fromOrderedBits(cb[a]);
@@ -833,14 +846,23 @@ __global__ void sahRootKernel(SahTables T, const uint32_t* __restrict__ cb, int 
     routeChild(T, 0, 0, n, 0, 0, 0, lo, hi);
 }
 
+__global__ void sahRootKernel(SahTables T, const uint32_t* __restrict__ cb, int n) {
+    if (threadIdx.x == 0) routeRoot(T, cb, n);
+}
+
 // Level plan (one workgroup): the current big tasks' chunk list (chunks of one task
 // consecutive, first[task] .. first[task + 1]); clears the count of the buffer this
 // level's children go to.  Per group of kBlock tasks, an exclusive scan of their chunk
 // counts, then every thread writes chunks, each finding its task by binary search of
 // the group's chunk offsets in LDS.
-__global__ __launch_bounds__(kBlock) void sahPlanKernel(SahTables T, int cur, int median) {
+__global__ __launch_bounds__(kBlock) void sahPlanKernel(SahTables T, int cur, int median, const uint32_t* __restrict__ cb,
+                                                        int rootN) {
     __shared__ int lds[4];
     __shared__ int sEx[kBlock], sStart[kBlock], sCount[kBlock];
+    if (rootN > 0) {  // the first level routes the root task first
+        if (threadIdx.x == 0) routeRoot(T, cb, rootN);
+        __syncthreads();
+    }
     const int nt = T.ctl[kCtlBig0 + cur];
     int carry = 0;
     for (int base = 0; base < nt && !median; base += kBlock) {
@@ -886,11 +908,6 @@ __global__ __launch_bounds__(kBlock) void sahPlanKernel(SahTables T, int cur, in
 // The histogram buffer of a level's tasks (two buffers, by level parity).
 __device__ __forceinline__ uint32_t* binsOf(const SahTables& T, int cur) {
     return T.bins + (cur ? static_cast<size_t>(T.maxBig) * kHist : size_t(0));
-}
-
-__device__ __forceinline__ uint32_t binInit(int i) {
-    const int v = i % kBinVals;
-    return v == 0 ? 0u : ((v >= 1 && v <= 3) || (v >= 7 && v <= 9)) ? orderedBits(kInf) : orderedBits(-kInf);
 }
 
 // One workgroup per chunk of a large task: LDS histogram, then one global flush.
@@ -957,14 +974,6 @@ __device__ __forceinline__ SahResult medianResult(const SahTask& t) {
     return res;
 }
 
-// Both histogram buffers to their empty state, once per build; afterwards each
-// task's split leaves its histogram empty again for the level two on.
-__global__ void sahClearBinsKernel(SahTables T) {
-    const long long words = 2LL * T.maxBig * kHist;
-    for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < words;
-         i += static_cast<long long>(gridDim.x) * kBlock)
-        T.bins[i] = binInit(static_cast<int>(i % kHist));
-}
 
 // One workgroup per large task: the plane choice (wave 0), then the exclusive
 // offsets of the task's chunk parts' left counts (part e = kParts * chunk + p, from
@@ -1555,11 +1564,12 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
 
     EventSet<6> ev;
     ev.record(0, s);
-    sahResetKernel<<<blocksFor(n - 1 > 256 ? n - 1 : 256, kBlock), kBlock, 0, s>>>(cb, b.arrive.p, n - 1);
+    sahResetKernel<<<std::min(blocksFor(n - 1 > 256 ? n - 1 : 256, kBlock), 2048), kBlock, 0, s>>>(
+        cb, b.arrive.p, n - 1, T, n > kChunk ? 1 : 0);
     int grid = blocksFor(n, kBlock * 4);
     if (grid > 512) grid = 512;  // two workgroups per CU stream the items; few bounds atomics
     sahPrepKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, T.rec[0], cb, b.parent.p);
-    sahRootKernel<<<1, 64, 0, s>>>(T, cb, n);
+    if (n <= kChunk) sahRootKernel<<<1, 64, 0, s>>>(T, cb, n);  // (else the first level's plan routes it)
     hipCheck(hipGetLastError(), "sah prep");
     ev.record(1, s);
 
@@ -1585,8 +1595,6 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
         while ((static_cast<long long>(kChunk) << batch) < n) ++batch;
         batch += 2;
         const int chunkGrid = T.maxChunks;
-        const int clearGrid = std::min(blocksFor(2LL * T.maxBig * kHist, kBlock), 1024);
-        sahClearBinsKernel<<<clearGrid, kBlock, 0, s>>>(T);
         while (true) {
             for (int j = 0; j < batch; ++j, ++level) {
                 const int cur = level & 1;
@@ -1594,7 +1602,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
                 // bin (the plan's chunk list), split (plane, part offsets, node and
                 // children), partition
                 mark(s);
-                sahPlanKernel<<<1, kBlock, 0, s>>>(T, cur, median);
+                sahPlanKernel<<<1, kBlock, 0, s>>>(T, cur, median, cb, level == 0 ? n : 0);
                 sahBinKernel<<<chunkGrid, kBlock, 0, s>>>(T, cur);
                 mark(s);
                 sahSplitKernel<<<T.maxBig, kBlock, 0, s>>>(T, cur, median, n, b.nodes.p, b.parent.p, b.farEnd.p);
@@ -1634,7 +1642,6 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
         const int smallGrid = std::min(T.maxLocal, 8192);
         sahSmallKernel<kSmallBT><<<smallGrid, kSmallBT, 0, s>>>(T, n, order, b.nodes.p, b.parent.p, b.farEnd.p);
     }
-    ev.record(3, s);
     sahSerialKernel<<<2048, kBlock, 0, s>>>(T, n, order, b.nodes.p, b.parent.p, b.farEnd.p);
     hipCheck(hipGetLastError(), "sah subtrees");
     ev.record(4, s);
