@@ -100,13 +100,14 @@ QR_API int qr_warmup(void);
 /* Shape of the 4-wide traversal, process-wide: 0 automatic (closest / kNN below
  * 65,536 queries by query count, leaf kind and k; from 65,536 on, the first call
  * for a tree, k class and query-count octave times one lane, four lanes, 16 and 32
- * lanes and one wave per query on its own queries and keeps the fastest; radius one
- * wave per query below 131,072 queries, four lanes above; rays one lane per ray), 1 one
- * lane per query, 2 four lanes per query, 3 one wave per query or ray, 4 as 0 but
- * closest / kNN and rays tuned at every count, 5 / 6 closest and kNN with 16 / 32
- * lanes per query (four / two queries per wave; kNN with k > 16 takes 32 lanes),
- * other queries as in mode 0.  Results are identical in every mode.  No reference
- * counterpart (a tuning knob of this backend). */
+ * lanes and one wave per query on its own queries and keeps the fastest; radius 32
+ * lanes per query (two queries per wave) below 131,072 queries, four lanes above;
+ * rays one lane per ray), 1 one lane per query, 2 four lanes per query, 3 one wave
+ * per query or ray, 4 as 0 but closest / kNN and rays tuned at every count, 5 / 6
+ * closest and kNN with 16 / 32 lanes per query (four / two queries per wave; kNN
+ * with k > 16 takes 32 lanes), radius as in mode 0 (5) or 32 lanes per query at
+ * every count (6), other queries as in mode 0.  Results are identical in every
+ * mode.  No reference counterpart (a tuning knob of this backend). */
 QR_API int qr_set_traversal_mode(int mode);
 /* LBVH build: how many frontier nodes a workgroup (or group of workgroups) may hand
  * up to the next level, process-wide, 0..32 (default 32).  Units with more send

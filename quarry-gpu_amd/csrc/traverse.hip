@@ -1133,31 +1133,76 @@ constexpr int kWaveStack = 1024;
 constexpr int kWaveHeadroom = 3 * 32 + 4;
 constexpr int kInsertMax = 16;  // a round's candidates inserted one by one up to this many (measured: 4 -> 0.473 ms, 16 -> 0.425, 64 -> 0.422 for C2 kNN16; the sort and merge 0.493)
 
-template <typename L, bool kStats>
+// Bitonic sort by (dist2, index) of up to 64 pairs held two per lane by a group of
+// 32 lanes (lane l holds elements l and l + 32; empty elements are (inf, INT_MAX)).
+__device__ __forceinline__ void groupSort64x2(float& ak, int& av, float& bk, int& bv, int gl) {
+    for (int size = 2; size <= 64; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride == 32) {  // (size 64) elements l and l + 32 meet in the lane
+                if (closer(bk, bv, ak, av)) {
+                    const float tk = ak;
+                    const int tv = av;
+                    ak = bk;
+                    av = bv;
+                    bk = tk;
+                    bv = tv;
+                }
+                continue;
+            }
+            const float oak = waveXor(ak, stride), obk = waveXor(bk, stride);
+            const int oav = waveXor(av, stride), obv = waveXor(bv, stride);
+            const bool low = (gl & stride) == 0;
+            const bool upA = (gl & size) == 0 || size == 64;
+            const bool upB = ((gl + 32) & size) == 0 || size == 64;
+            const bool aOther = closer(oak, oav, ak, av), bOther = closer(obk, obv, bk, bv);
+            if ((low == upA) ? aOther : !aOther) {
+                ak = oak;
+                av = oav;
+            }
+            if ((low == upB) ? bOther : !bOther) {
+                bk = obk;
+                bv = obv;
+            }
+        }
+    }
+}
+
+// G lanes per query (G = 64: one wave; G = 32: two queries per wave, their groups'
+// ballots and shuffles masked to the group, the LDS stack split between them): the
+// group's quads expand up to G / 4 nodes of the walk per round.  A captured segment of
+// at most 64 hits is left sorted (over the group's lanes, two per lane above G = 32).
+template <typename L, bool kStats, int G>
 __global__ __launch_bounds__(kBlock) void radiusWaveKernel(
     const Wide4* __restrict__ wide, const typename L::Item* __restrict__ prims,
     const int* __restrict__ primIds, const V3* __restrict__ points, const uint32_t* __restrict__ order,
     int count, float r2, int* __restrict__ counts, uint64_t* __restrict__ capture,
     unsigned long long* __restrict__ stats, int* __restrict__ maxCount, int cap) {
+    static_assert(G == 32 || G == 64, "lanes per query");
+    constexpr int kPerWave = 64 / G;
+    constexpr int kQuads = G / 4;
+    constexpr int kStack = kWaveStack / kPerWave;
     __shared__ int lds[kWaveStack * (kBlock / 64)];
     const int lane = threadIdx.x & 63;
-    const int quad = lane >> 2;
+    const int gl = lane & (G - 1);
+    const int gb = lane - gl;
+    const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull) << gb;
+    const int quad = gl >> 2;
     const int slot = lane & 3;
-    const int qi = xcdBlock() * (kBlock / 64) + (threadIdx.x >> 6);
-    const unsigned long long below = (1ull << lane) - 1ull;
+    const int qi = (xcdBlock() * (kBlock / 64) + (threadIdx.x >> 6)) * kPerWave + lane / G;
+    const unsigned long long below = ((1ull << lane) - 1ull) & gmask;
     int nodeTests = 0, leafTests = 0, overflow = 0, opened = 0;
     int queryHits = 0;
-    if (qi < count) {  // the same in the whole wave
+    if (qi < count) {  // the same in the whole group
         const int q = order ? static_cast<int>(order[qi]) : qi;
         const V3 p = points[q];
-        LdsInt* stk = (LdsInt*)(lds + (threadIdx.x >> 6) * kWaveStack);
+        LdsInt* stk = (LdsInt*)(lds + (threadIdx.x >> 6) * kWaveStack + (lane / G) * kStack);
         uint64_t* seg = capture + static_cast<long long>(q) * cap;
-        if (lane == 0) stk[0] = 0;
+        if (gl == 0) stk[0] = 0;
         int sp = 1;
         int found = 0;
         while (sp > 0) {
-            int width = (kWaveStack - kWaveHeadroom - sp) / 3;
-            width = width < 1 ? 1 : (width > 16 ? 16 : width);
+            int width = (kStack - kWaveHeadroom - sp) / 3;
+            width = width < 1 ? 1 : (width > kQuads ? kQuads : width);
             const int take = width < sp ? width : sp;
             const int node = quad < take ? stk[sp - 1 - quad] : -1;
             sp -= take;
@@ -1178,10 +1223,10 @@ __global__ __launch_bounds__(kBlock) void radiusWaveKernel(
             const bool adm = ch != kWideEmpty && boxDist2(p, entryBox(e)) <= r2;
             if (adm) ++nodeTests;
             const bool push = adm && !leaf;
-            const unsigned long long pm = __ballot(push);
+            const unsigned long long pm = __ballot(push) & gmask;
             const int pushes = __popcll(pm);
-            if (sp + pushes > kWaveStack) {  // deeper than the bound above allows
-                if (lane == 0) overflow = 1;
+            if (sp + pushes > kStack) {  // deeper than the bound above allows
+                if (gl == 0) overflow = 1;
                 break;
             }
             if (push) stk[sp + __popcll(pm & below)] = ch;
@@ -1193,29 +1238,45 @@ __global__ __launch_bounds__(kBlock) void radiusWaveKernel(
                 d2 = L::dist2(p, it);
                 hit = d2 <= r2;
             }
-            const unsigned long long hm = __ballot(hit);
+            const unsigned long long hm = __ballot(hit) & gmask;
             if (hit) {
                 const int at = found + __popcll(hm & below);
                 if (at < cap) seg[at] = (static_cast<uint64_t>(floatBits(d2)) << 32) | static_cast<uint32_t>(pid);
             }
             found += __popcll(hm);
         }
-        if (found > 1 && found <= 64 && found <= cap) {
-            // a segment of at most 64 hits leaves sorted by (dist2, index): the order
+        if (found > 1 && found <= G && found <= cap) {
+            // a segment of at most G hits leaves sorted by (dist2, index): the order
             // of the 64-bit keys, dist2 being non-negative
             int w = 2;
             while (w < found) w <<= 1;
             float sk = kInf;
             int sv = 0x7fffffff;
-            if (lane < found) {
-                const uint64_t key = seg[lane];
+            if (gl < found) {
+                const uint64_t key = seg[gl];
                 sk = bitsFloat(static_cast<uint32_t>(key >> 32));
                 sv = static_cast<int>(static_cast<uint32_t>(key));
             }
             waveSortPairs(sk, sv, lane, w);
-            if (lane < found) seg[lane] = (static_cast<uint64_t>(floatBits(sk)) << 32) | static_cast<uint32_t>(sv);
+            if (gl < found) seg[gl] = (static_cast<uint64_t>(floatBits(sk)) << 32) | static_cast<uint32_t>(sv);
+        } else if (G == 32 && found > G && found <= 64 && found <= cap) {
+            float ak = kInf, bk = kInf;
+            int av = 0x7fffffff, bv = 0x7fffffff;
+            {
+                const uint64_t key = seg[gl];
+                ak = bitsFloat(static_cast<uint32_t>(key >> 32));
+                av = static_cast<int>(static_cast<uint32_t>(key));
+            }
+            if (gl + 32 < found) {
+                const uint64_t key = seg[gl + 32];
+                bk = bitsFloat(static_cast<uint32_t>(key >> 32));
+                bv = static_cast<int>(static_cast<uint32_t>(key));
+            }
+            groupSort64x2(ak, av, bk, bv, gl);
+            seg[gl] = (static_cast<uint64_t>(floatBits(ak)) << 32) | static_cast<uint32_t>(av);
+            if (gl + 32 < found) seg[gl + 32] = (static_cast<uint64_t>(floatBits(bk)) << 32) | static_cast<uint32_t>(bv);
         }
-        if (lane == 0) counts[q] = found;
+        if (gl == 0) counts[q] = found;
         queryHits = found;
     }
     waveMaxAtomic(maxCount, queryHits);
@@ -2286,12 +2347,24 @@ static void launchCaptureT(const TraverseArgs& a, const V3* pts, int count, floa
                            uint64_t* capture, int cap, int* maxCount, hipStream_t s) {
     const int qgrid = blocksFor(4LL * count, kBlock);
     const auto* prims = static_cast<const typename L::Item*>(a.items);
-    if ((count < gRadiusWaveBelow && (gTraversalMode == 0 || gTraversalMode == 4)) || gTraversalMode == 3) {
+    // below gRadiusWaveBelow two queries per wave (measured ahead of one per wave at
+    // every count tried: terrain r = 2 at 20k-115k queries 0.045-0.127 ms against
+    // 0.049-0.161, the 1M-point cloud r = 1 at 20k / 65k 0.106 / 0.605 against
+    // 0.124 / 0.745; its segments of up to 64 hits leave sorted, so the quads' segment
+    // sort is not needed either); mode 3 forces one wave per query
+    if ((count < gRadiusWaveBelow && (gTraversalMode == 0 || gTraversalMode == 4 || gTraversalMode == 5)) ||
+        gTraversalMode == 6) {
+        const int wgrid = blocksFor(count, 2 * (kBlock / 64));
+        if (a.stats)
+            radiusWaveKernel<L, true, 32><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, capture, a.stats, maxCount, cap);
+        else
+            radiusWaveKernel<L, false, 32><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, capture, nullptr, maxCount, cap);
+    } else if (gTraversalMode == 3) {
         const int wgrid = blocksFor(count, kBlock / 64);
         if (a.stats)
-            radiusWaveKernel<L, true><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, capture, a.stats, maxCount, cap);
+            radiusWaveKernel<L, true, 64><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, capture, a.stats, maxCount, cap);
         else
-            radiusWaveKernel<L, false><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, capture, nullptr, maxCount, cap);
+            radiusWaveKernel<L, false, 64><<<wgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, capture, nullptr, maxCount, cap);
     } else if (a.stats)
         radiusQuadKernel<L, kRadiusCaptureMode, true><<<qgrid, kBlock, 0, s>>>(a.wideNodes, prims, a.primIds, pts, a.order, count, r2, counts, nullptr, capture, a.stats, maxCount, cap);
     else
@@ -2313,8 +2386,9 @@ int radiusCaptureCap(int count) {
 }
 
 bool radiusCapturePresorts(const TraverseArgs& a, int count) {
-    return radiusOnQuads(a) && ((count < gRadiusWaveBelow && (gTraversalMode == 0 || gTraversalMode == 4)) ||
-                                gTraversalMode == 3);
+    return radiusOnQuads(a) &&
+           ((count < gRadiusWaveBelow && (gTraversalMode == 0 || gTraversalMode == 4 || gTraversalMode == 5)) ||
+            gTraversalMode == 3 || gTraversalMode == 6);
 }
 
 bool launchRadiusCapture(const TraverseArgs& a, const V3* pts, int count, float r2, int* counts,

@@ -359,7 +359,7 @@ def test_radius_modes_agree(A, orc, scenes, kind):
     b.build_wide()
     want = orc.radius_brute(kind, items, q, 2.5, threads=THREADS)
     try:
-        for mode in (1, 2, 3, 0):
+        for mode in (1, 2, 3, 6, 0):
             A.check(A.lib().qr_set_traversal_mode(mode))
             o, k, _, _ = b.radius(q, 2.5, order=A.ORDER_MORTON, wide=True)
             assert eq(o, want[0]) and eq(k, want[1]), mode
