@@ -339,15 +339,14 @@ __device__ __forceinline__ void leafClimb(const typename L::Item* __restrict__ s
 // window is published with agent-scope stores and from there on the climb follows
 // the global protocol of leafClimb.  Boxes are merged in the same order with the
 // same operands, so the node array is identical to leafClimb's.
+// One workgroup's window of leaf slots [base, base + kBlock) of a tree of n leaves
+// (the whole body of leafUpLocalKernel; the batched build runs it per scene).
 template <typename L, bool kGather>
-__global__ __launch_bounds__(kBlock) void leafUpLocalKernel(
+__device__ __forceinline__ void leafUpWindow(
     const typename L::Item* __restrict__ src, const uint32_t* __restrict__ order, int n,
     typename L::Item* __restrict__ sorted, int* __restrict__ primIds, Node2* __restrict__ nodes,
     const int* __restrict__ parent, int* __restrict__ arrive, int* __restrict__ height,
-    const int* __restrict__ farEnd, const int* __restrict__ gate) {
-    // a builder whose topology failed (its error word set) leaves it unclimbed; the
-    // host reports the error after the stream completes
-    if (gate != nullptr && *gate != 0) return;
+    const int* __restrict__ farEnd, int base) {
     __shared__ float sLeaf[kBlock][6];
     __shared__ float sNode[kBlock][6];
     __shared__ int sCount[kBlock];
@@ -357,7 +356,6 @@ __global__ __launch_bounds__(kBlock) void leafUpLocalKernel(
     __shared__ int sFar[kBlock];
     __shared__ int sLeft[kBlock];
     __shared__ int sRight[kBlock];
-    const int base = blockIdx.x * kBlock;
     const int last = (base + kBlock < n ? base + kBlock : n) - 1;
     const int k = base + threadIdx.x;
     const bool live = k < n;
@@ -465,6 +463,19 @@ __global__ __launch_bounds__(kBlock) void leafUpLocalKernel(
         p = parent[p];
         if (p < 0) break;
     }
+}
+
+template <typename L, bool kGather>
+__global__ __launch_bounds__(kBlock) void leafUpLocalKernel(
+    const typename L::Item* __restrict__ src, const uint32_t* __restrict__ order, int n,
+    typename L::Item* __restrict__ sorted, int* __restrict__ primIds, Node2* __restrict__ nodes,
+    const int* __restrict__ parent, int* __restrict__ arrive, int* __restrict__ height,
+    const int* __restrict__ farEnd, const int* __restrict__ gate) {
+    // a builder whose topology failed (its error word set) leaves it unclimbed; the
+    // host reports the error after the stream completes
+    if (gate != nullptr && *gate != 0) return;
+    leafUpWindow<L, kGather>(src, order, n, sorted, primIds, nodes, parent, arrive, height, farEnd,
+                             static_cast<int>(blockIdx.x) * kBlock);
 }
 
 // ---- fused hierarchy emission + bottom-up boxes ------------------------------------------
@@ -970,47 +981,71 @@ __global__ void batchBoundsInitKernel(uint32_t* __restrict__ cb, int scenes) {
     cb[i] = (i % 6) < 3 ? orderedBits(kInf) : orderedBits(-kInf);
 }
 
-// Per-scene centroid bounds.  A wave whose 64 items all belong to one scene reduces
-// in registers and issues 6 atomics; mixed waves fall back to per-lane atomics.
+// Per-scene centroid bounds.  Each wave takes kBoundsSegs consecutive 64-item
+// segments (coalesced loads); a segment inside one scene is reduced across the wave
+// (DPP, on ordered bits: exact and order-free) into the wave's running bounds, which
+// go out with 6 atomics when the scene changes or the wave ends; a segment that
+// straddles scenes falls back to per-lane atomics.
+constexpr int kBoundsSegs = 16;
+
 template <typename L>
 __global__ __launch_bounds__(kBlock) void batchBoundsKernel(const typename L::Item* __restrict__ items, int n,
                                                             const int* __restrict__ sceneId,
                                                             uint32_t* __restrict__ cb) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    const bool live = i < n;
-    const int s0 = __shfl(live ? sceneId[i] : -1, 0, 64);
-    int sc = live ? sceneId[i] : s0;
-    V3 c{kInf, kInf, kInf};
-    V3 ch{-kInf, -kInf, -kInf};
-    if (live) {
-        c = L::centroid(items[i]);
-        ch = c;
-    }
-    if (__all(sc == s0) && s0 >= 0) {
-        float v[6] = {c.x, c.y, c.z, ch.x, ch.y, ch.z};
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
+    const int lane = threadIdx.x & 63;
+    const long long first = ((static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x) >> 6) * 64 * kBoundsSegs;
+    int cur = -1;  // the scene of the running bounds (the same in every lane)
+    uint32_t acc[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    auto flush = [&]() {
+        if (cur >= 0 && lane == 0) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                v[k] = fminf(v[k], __shfl_xor(v[k], off, 64));
-                v[3 + k] = fmaxf(v[3 + k], __shfl_xor(v[3 + k], off, 64));
+                atomicMin(&cb[cur * 6 + k], acc[k]);
+                atomicMax(&cb[cur * 6 + 3 + k], acc[3 + k]);
             }
         }
-        if ((threadIdx.x & 63) == 0) {
+    };
+    for (int j = 0; j < kBoundsSegs; ++j) {
+        const long long seg = first + static_cast<long long>(j) * 64;
+        if (seg >= n) break;
+        const long long i = seg + lane;
+        const bool live = i < n;
+        const int sc = live ? sceneId[i] : -1;
+        const int s0 = __builtin_amdgcn_readlane(sc, 0);  // lane 0 is live
+        uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+        if (live) {
+            const V3 c = L::centroid(items[i]);
+            v[0] = v[3] = orderedBits(c.x);
+            v[1] = v[4] = orderedBits(c.y);
+            v[2] = v[5] = orderedBits(c.z);
+        }
+        if (__all(!live || sc == s0)) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                atomicMin(&cb[s0 * 6 + k], orderedBits(v[k]));
-                atomicMax(&cb[s0 * 6 + 3 + k], orderedBits(v[3 + k]));
+                v[k] = reduce64<MinU>(v[k]);
+                v[3 + k] = reduce64<MaxU>(v[3 + k]);
+            }
+            if (s0 != cur) {
+                flush();
+                cur = s0;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) acc[k] = v[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    acc[k] = MinU::op(acc[k], v[k]);
+                    acc[3 + k] = MaxU::op(acc[3 + k], v[3 + k]);
+                }
+            }
+        } else if (live) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(&cb[sc * 6 + k], v[k]);
+                atomicMax(&cb[sc * 6 + 3 + k], v[3 + k]);
             }
         }
-    } else if (live) {
-        atomicMin(&cb[sc * 6 + 0], orderedBits(c.x));
-        atomicMin(&cb[sc * 6 + 1], orderedBits(c.y));
-        atomicMin(&cb[sc * 6 + 2], orderedBits(c.z));
-        atomicMax(&cb[sc * 6 + 3], orderedBits(c.x));
-        atomicMax(&cb[sc * 6 + 4], orderedBits(c.y));
-        atomicMax(&cb[sc * 6 + 5], orderedBits(c.z));
     }
+    flush();
 }
 
 template <typename L>
@@ -1064,7 +1099,7 @@ __global__ __launch_bounds__(kBlock) void batchKarrasKernel(const uint32_t* __re
                                                             const uint32_t* __restrict__ s2,
                                                             const int* __restrict__ primOff, int n,
                                                             Node2* __restrict__ nodes, int* __restrict__ parent,
-                                                            int* __restrict__ arrive) {
+                                                            int* __restrict__ arrive, int* __restrict__ farEnd) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const int sc = static_cast<int>(s2[i]);
@@ -1072,24 +1107,34 @@ __global__ __launch_bounds__(kBlock) void batchKarrasKernel(const uint32_t* __re
     const int il = i - ps;
     if (il >= ns - 1) return;
     karrasNode(codes2 + ps, idxLocal + ps, ns, il, nodes + (2 * ps - sc), parent + (2 * ps - sc),
-               arrive + (ps - sc), nullptr);
+               arrive + (ps - sc), farEnd + (ps - sc));
 }
 
+// Leaf gather, leaf nodes and bottom-up boxes of every scene: workgroup w takes a
+// window of kBlock leaf slots of one scene (winOff[sc] .. winOff[sc + 1] - 1 are scene
+// sc's windows) and runs the plain build's window pass on that scene's arrays, so the
+// merges inside a window stay in LDS and only nodes leaving it take the arrival
+// counters.
 template <typename L>
 __global__ __launch_bounds__(kBlock) void batchLeafKernel(const typename L::Item* __restrict__ items,
                                                           const uint32_t* __restrict__ idxLocal,
-                                                          const uint32_t* __restrict__ s2,
-                                                          const int* __restrict__ primOff, int n,
+                                                          const int* __restrict__ primOff,
+                                                          const int* __restrict__ winOff, int scenes,
                                                           typename L::Item* __restrict__ sorted,
                                                           int* __restrict__ primIds, Node2* __restrict__ nodes,
                                                           const int* __restrict__ parent, int* __restrict__ arrive,
-                                                          int* __restrict__ height) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= n) return;
-    const int sc = static_cast<int>(s2[k]);
+                                                          int* __restrict__ height, const int* __restrict__ farEnd) {
+    const int w = static_cast<int>(blockIdx.x);
+    int lo = 0, hi = scenes - 1;  // the scene whose windows hold w
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (winOff[mid] <= w) lo = mid; else hi = mid - 1;
+    }
+    const int sc = lo;
     const int ps = primOff[sc], ns = primOff[sc + 1] - ps;
-    leafClimb<L, true>(items + ps, idxLocal + ps, ns, k - ps, sorted + ps, primIds + ps, nodes + (2 * ps - sc),
-                       parent + (2 * ps - sc), arrive + (ps - sc), height + (ps - sc));
+    leafUpWindow<L, true>(items + ps, idxLocal + ps, ns, sorted + ps, primIds + ps, nodes + (2 * ps - sc),
+                          parent + (2 * ps - sc), arrive + (ps - sc), height + (ps - sc), farEnd + (ps - sc),
+                          (w - winOff[sc]) * kBlock);
 }
 
 __global__ void batchDepthKernel(const int* __restrict__ height, const int* __restrict__ primOff, int scenes,
@@ -1327,7 +1372,7 @@ static void batchOrderT(const typename L::Item* items, int n, const int* sceneId
                         uint32_t* cb, uint32_t* codes, uint32_t* tmpA, uint32_t* tmpB, uint32_t* s2,
                         uint32_t* order2, hipStream_t s) {
     batchBoundsInitKernel<<<blocksFor(static_cast<long long>(scenes) * 6, kBlock), kBlock, 0, s>>>(cb, scenes);
-    batchBoundsKernel<L><<<blocksFor(n, kBlock), kBlock, 0, s>>>(items, n, sceneId, cb);
+    batchBoundsKernel<L><<<blocksFor(n, kBlock * kBoundsSegs), kBlock, 0, s>>>(items, n, sceneId, cb);
     sortClear(ws, static_cast<size_t>(n), 4, s);
     batchMortonKernel<L><<<blocksFor(n, kBlock), kBlock, 0, s>>>(items, n, sceneId, cb, codes, ws.hist.p);
     hipCheck(hipGetLastError(), "batch morton");
@@ -1381,6 +1426,13 @@ static void buildBatchT(BatchState& b, const typename L::Item* dSrc, hipStream_t
     b.ws.reserve(static_cast<size_t>(n));
     DevBuf<uint32_t> cb(static_cast<size_t>(S) * 6), codes(n), tmpA(n), tmpB(n), s2(n), order2(n), codes2(n),
         idxLocal(n);
+    DevBuf<int> farEnd(static_cast<size_t>(n - S)), winOff(static_cast<size_t>(S) + 1);
+    // each scene's windows of kBlock leaf slots, numbered scene after scene
+    std::vector<int> winOffH(static_cast<size_t>(S) + 1, 0);
+    for (int sc = 0; sc < S; ++sc)
+        winOffH[sc + 1] = winOffH[sc] + blocksFor(b.primOffH[sc + 1] - b.primOffH[sc], kBlock);
+    hipCheck(hipMemcpyAsync(winOff.p, winOffH.data(), winOffH.size() * sizeof(int), hipMemcpyHostToDevice, s),
+             "batch windows");
 
     EventSet<5> ev;
     ev.record(0, s);
@@ -1390,12 +1442,12 @@ static void buildBatchT(BatchState& b, const typename L::Item* dSrc, hipStream_t
     batchLocalKernel<<<blocksFor(n, kBlock), kBlock, 0, s>>>(order2.p, s2.p, codes.p, b.primOff.p, n, codes2.p,
                                                              idxLocal.p);
     batchKarrasKernel<<<blocksFor(n, kBlock), kBlock, 0, s>>>(codes2.p, idxLocal.p, s2.p, b.primOff.p, n,
-                                                              b.nodes.p, b.parent.p, b.arrive.p);
+                                                              b.nodes.p, b.parent.p, b.arrive.p, farEnd.p);
     hipCheck(hipGetLastError(), "batch karras");
     ev.record(2, s);
-    batchLeafKernel<L><<<blocksFor(n, kBlock), kBlock, 0, s>>>(dSrc, idxLocal.p, s2.p, b.primOff.p, n,
-                                                               reinterpret_cast<Item*>(b.items.p), b.primIds.p,
-                                                               b.nodes.p, b.parent.p, b.arrive.p, b.height.p);
+    batchLeafKernel<L><<<winOffH[S], kBlock, 0, s>>>(dSrc, idxLocal.p, b.primOff.p, winOff.p, S,
+                                                     reinterpret_cast<Item*>(b.items.p), b.primIds.p, b.nodes.p,
+                                                     b.parent.p, b.arrive.p, b.height.p, farEnd.p);
     hipCheck(hipGetLastError(), "batch leaf up");
     ev.record(3, s);
     batchDepthKernel<<<blocksFor(S, kBlock), kBlock, 0, s>>>(b.height.p, b.primOff.p, S, b.depth.p);

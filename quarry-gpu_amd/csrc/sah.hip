@@ -214,99 +214,6 @@ __device__ __forceinline__ float areaOf(float lx, float ly, float lz, float hx, 
     return dx * dy + dy * dz + dz * dx;
 }
 
-// ---- cross-lane steps within each 32-lane half of a wave (DPP) -------------------------
-// Lanes 32..63 of the plane-choice wave repeat lanes 0..31, so every step works on
-// the two halves alike and a half's result can be read from lanes 0..31.
-// kCtrl is a DPP control: row_shr:k 0x110+k (lane i reads lane i-k of its 16-lane
-// row), row_shl:k 0x100+k (lane i+k), wave_shr:1 0x138 (lane i-1 across the wave),
-// quad_perm [1,0,3,2] 0xB1 and [2,3,0,1] 0x4E, row_half_mirror 0x141, row_mirror
-// 0x140.  A lane whose source lies outside its row keeps `id`.
-template <int kCtrl>
-__device__ __forceinline__ int dppI(int x, int id) {
-    return __builtin_amdgcn_update_dpp(id, x, kCtrl, 0xF, 0xF, false);
-}
-template <int kCtrl>
-__device__ __forceinline__ float dppF(float x, float id) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(id), __float_as_int(x), kCtrl, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float readlaneF(float x, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
-}
-
-struct MinF {
-    static constexpr float id = kInf;
-    __device__ static float op(float a, float b) { return fminf(a, b); }
-};
-struct MaxF {
-    static constexpr float id = -kInf;
-    __device__ static float op(float a, float b) { return fmaxf(a, b); }
-};
-
-// Inclusive prefix over the lanes of each half: four row shifts, then rows 1 and 3
-// take the total of the row below (lane 15 of the half; the halves are equal).
-template <typename Op>
-__device__ __forceinline__ float prefix32(float x, int lane) {
-    x = Op::op(x, dppF<0x111>(x, Op::id));
-    x = Op::op(x, dppF<0x112>(x, Op::id));
-    x = Op::op(x, dppF<0x114>(x, Op::id));
-    x = Op::op(x, dppF<0x118>(x, Op::id));
-    const float low = readlaneF(x, 15);
-    return (lane & 16) ? Op::op(x, low) : x;
-}
-__device__ __forceinline__ int prefixAdd32(int x, int lane) {
-    x += dppI<0x111>(x, 0);
-    x += dppI<0x112>(x, 0);
-    x += dppI<0x114>(x, 0);
-    x += dppI<0x118>(x, 0);
-    const int low = __builtin_amdgcn_readlane(x, 15);
-    return (lane & 16) ? x + low : x;
-}
-// Inclusive suffix (the lane and the lanes above it in its half).
-template <typename Op>
-__device__ __forceinline__ float suffix32(float x, int lane) {
-    x = Op::op(x, dppF<0x101>(x, Op::id));
-    x = Op::op(x, dppF<0x102>(x, Op::id));
-    x = Op::op(x, dppF<0x104>(x, Op::id));
-    x = Op::op(x, dppF<0x108>(x, Op::id));
-    const float high = readlaneF(x, 16);
-    return (lane & 16) ? x : Op::op(x, high);
-}
-// Reduction over a half, the same value returned to every lane.
-template <typename Op>
-__device__ __forceinline__ uint32_t reduce32(uint32_t x) {
-    int v = static_cast<int>(x);
-    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0xB1>(v, 0))));
-    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0x4E>(v, 0))));
-    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0x141>(v, 0))));
-    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0x140>(v, 0))));
-    return Op::op(static_cast<uint32_t>(__builtin_amdgcn_readlane(v, 0)),
-                  static_cast<uint32_t>(__builtin_amdgcn_readlane(v, 16)));
-}
-// Reduction over a whole wave (lanes 0..63 all distinct), the same value returned to
-// every lane.
-template <typename Op>
-__device__ __forceinline__ uint32_t reduce64(uint32_t x) {
-    int v = static_cast<int>(x);
-    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0xB1>(v, 0))));
-    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0x4E>(v, 0))));
-    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0x141>(v, 0))));
-    v = static_cast<int>(Op::op(static_cast<uint32_t>(v), static_cast<uint32_t>(dppI<0x140>(v, 0))));
-    const uint32_t r0 = Op::op(static_cast<uint32_t>(__builtin_amdgcn_readlane(v, 0)),
-                               static_cast<uint32_t>(__builtin_amdgcn_readlane(v, 16)));
-    const uint32_t r1 = Op::op(static_cast<uint32_t>(__builtin_amdgcn_readlane(v, 32)),
-                               static_cast<uint32_t>(__builtin_amdgcn_readlane(v, 48)));
-    return Op::op(r0, r1);
-}
-struct MinU {
-    __device__ static uint32_t op(uint32_t a, uint32_t b) { return a < b ? a : b; }
-};
-struct MaxU {
-    __device__ static uint32_t op(uint32_t a, uint32_t b) { return a > b ? a : b; }
-};
-struct AddU {
-    __device__ static uint32_t op(uint32_t a, uint32_t b) { return a + b; }
-};
-
 // Plane choice by one wave (called by all 64 lanes): lane b holds bin b (lanes
 // 32..63 repeat lanes 0..31); inclusive prefix and suffix unions of the bins' counts
 // and boxes give every plane p = 1..31 its left side (bins 0..p-1, the prefix one
