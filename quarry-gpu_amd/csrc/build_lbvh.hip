@@ -786,21 +786,67 @@ __global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
 }
 
 // Level L >= 1 of the hand-up: workgroup g merges the frontiers of units
-// [g kMergeUnits, g kMergeUnits + kMergeUnits) of level L-1 (one thread per entry)
-// and stages its own frontier as unit g of level L.
+// [g kMergeUnits, g kMergeUnits + kMergeUnits) of level L-1 and stages its own
+// frontier as unit g of level L.
+//
+// The group's entries partition its leaf range in leaf order, so the nodes above
+// them are fixed by the prefixes at the boundaries between consecutive entries
+// (slot j between entries j and j+1, plus the group's two outer boundaries): the node
+// splitting at slot j spans the entries between the nearest boundaries on either
+// side with a shorter prefix (prefixes are distinct along any range, so "nearest
+// shorter" is well defined), and it hangs below whichever of those two boundaries
+// has the longer prefix -- the attach rule of the bottom-up pass.  A node is formed
+// here when both nearest shorter boundaries exist inside the group's boundaries;
+// a formed node or entry whose parent boundary is an outer one or an unformed slot
+// is on the group's frontier.  One thread per slot finds both boundaries by a
+// scan of 8 prefixes per LDS read; the boxes then go up the formed forest by
+// arrival counters, each merge taking the left child's box as the left operand,
+// so every node is byte-identical to the one-node-at-a-time climb.
+constexpr int kMergeNodes = 2 * kMergeBlock;  // entries 0..count-1, slot j at kMergeBlock + j
+constexpr int kPrefixWords = (kMergeBlock + 8) / 8 + 1;
+
+// High bit of each byte of w whose value is below x (bytes and x in [0, 127], x >= 1).
+__device__ __forceinline__ uint64_t bytesBelow(uint64_t w, uint32_t x) {
+    constexpr uint64_t kHigh = 0x8080808080808080ull;
+    return ~((w | kHigh) - x * 0x0101010101010101ull) & kHigh;
+}
+// Largest i <= p with byte v[i] < x, or -1.
+__device__ __forceinline__ int lastBelow(const uint64_t* v, int p, uint32_t x) {
+    int w = p >> 3;
+    const int keep = (p & 7) * 8 + 7;  // bits 0..keep hold bytes 0..p&7
+    uint64_t m = bytesBelow(v[w], x) & (keep == 63 ? ~0ull : ((1ull << (keep + 1)) - 1ull));
+    while (m == 0) {
+        if (--w < 0) return -1;
+        m = bytesBelow(v[w], x);
+    }
+    return w * 8 + ((63 - __clzll(static_cast<long long>(m))) >> 3);
+}
+// Smallest i in [p, last] with byte v[i] < x, or -1.
+__device__ __forceinline__ int firstBelow(const uint64_t* v, int p, int last, uint32_t x) {
+    int w = p >> 3;
+    const int wl = last >> 3;
+    uint64_t m = bytesBelow(v[w], x) & (~0ull << ((p & 7) * 8));
+    while (m == 0) {
+        if (++w > wl) return -1;
+        m = bytesBelow(v[w], x);
+    }
+    const int i = w * 8 + ((__ffsll(static_cast<long long>(m)) - 1) >> 3);
+    return i <= last ? i : -1;
+}
+
 __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
     const FrontierEntry* __restrict__ inStage, const int* __restrict__ inCounts, int inUnits,
     FrontierEntry* __restrict__ outStage, int* __restrict__ outCounts, bool top,
     const uint32_t* __restrict__ code, const uint32_t* __restrict__ order, int n,
     Node2* __restrict__ nodes, int* __restrict__ parent, int* __restrict__ arrive,
     int* __restrict__ height, int* __restrict__ farEnd, int frontierCap) {
-    __shared__ float sBoxL[kMergeBlock][6];  // box of the node starting at entry j
-    __shared__ float sBoxR[kMergeBlock][6];  // box of the node ending at entry j
-    __shared__ int sHL[kMergeBlock], sHR[kMergeBlock];
-    __shared__ int sSlot[kMergeBlock];
-    __shared__ int sFarLeaf[2][kMergeBlock];  // per side of a slot, the arriving node's far end,
-    __shared__ int sFarEntry[2][kMergeBlock]; //   its far entry index
-    __shared__ int sFarD[2][kMergeBlock];     //   and its prefix at that end
+    __shared__ float sBox[kMergeNodes][6];
+    __shared__ int sH[kMergeNodes];
+    __shared__ int sId[kMergeNodes];
+    __shared__ int sLo[kMergeBlock], sHi[kMergeBlock];  // leaf range of entry e
+    __shared__ uint64_t sPre[kPrefixWords];  // byte i: prefix + 1 at boundary i - 1 (boundary -1 = left edge)
+    __shared__ int4 sSlot[kMergeBlock];      // formed slot: parent slot (-1: none here), child 0, child 1, formed
+    __shared__ int sArrive[kMergeBlock];
     __shared__ int sMark[kMergeBlock];
     __shared__ int sWave[kMergeBlock / 64 + 1];
     __shared__ int sOff[kMergeUnits + 1];
@@ -825,115 +871,183 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
     const int count = sOff[members];
     const bool broken = sBroken != 0;
     NodeState s{};
-    bool pend = false;
-    if (t < count) {
+    const bool entry = t < count;
+    if (entry) {
         int m = 0;
         while (sOff[m + 1] <= t) ++m;
         s = fromEntry(inStage[(first + m) * kFrontierCap + (t - sOff[m])]);
-        pend = true;
     }
     if (broken) {  // a unit below went global: so does the whole group
-        if (pend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
+        if (entry) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
         if (t == 0 && !top) outCounts[g] = -1;
         return;
     }
 
-    // climb over the group's entries; slot j lies between entries j and j + 1
-    sSlot[t] = 0;
-    if (pend) {
-        float* bl = sBoxL[t];
-        float* br = sBoxR[t];
-        bl[0] = br[0] = s.box.lo.x;
-        bl[1] = br[1] = s.box.lo.y;
-        bl[2] = br[2] = s.box.lo.z;
-        bl[3] = br[3] = s.box.hi.x;
-        bl[4] = br[4] = s.box.hi.y;
-        bl[5] = br[5] = s.box.hi.z;
-        sHL[t] = s.h;
-        sHR[t] = s.h;
+    // ---- entries and boundary prefixes into LDS ------------------------------------------
+    auto* pre = reinterpret_cast<unsigned char*>(sPre);
+    if (t < kPrefixWords) sPre[t] = 0x7F7F7F7F7F7F7F7Full;  // padding: never below any prefix
+    sArrive[t] = 0;
+    __syncthreads();
+    if (entry) {
+        float* b = sBox[t];
+        b[0] = s.box.lo.x;
+        b[1] = s.box.lo.y;
+        b[2] = s.box.lo.z;
+        b[3] = s.box.hi.x;
+        b[4] = s.box.hi.y;
+        b[5] = s.box.hi.z;
+        sH[t] = s.h;
+        sId[t] = s.id;
+        sLo[t] = s.l;
+        sHi[t] = s.r;
+        pre[t + 1] = static_cast<unsigned char>(s.dr + 1);  // boundary t (slot t, or the right edge)
+        if (t == 0) pre[0] = static_cast<unsigned char>(s.dl + 1);
     }
     __syncthreads();
-    int jl = t, jr = t, hang = -1;
-    while (pend) {
-        if (s.l == 0 && s.r == n - 1) {  // the root (never staged; kept for safety)
-            pend = false;
-            break;
+
+    // ---- the slots: nearest shorter boundaries, parent, id ---------------------------------
+    const int slots = count - 1;
+    const bool isSlot = t < slots;
+    int sa = -1, sb = -1;  // byte indices of the nearest shorter boundaries (boundary = index - 1)
+    bool formed = false, slotRoot = false;
+    int slotParent = -1, slotSide = 0;  // boundary index of the parent split (-1: the left edge)
+    if (isSlot) {
+        const uint32_t x = pre[t + 1];
+        sa = lastBelow(sPre, t, x);
+        sb = firstBelow(sPre, t + 2, count, x);
+        formed = sa >= 0 && sb >= 0;
+        if (formed) {
+            const int dl = static_cast<int>(pre[sa]) - 1, dr = static_cast<int>(pre[sb]) - 1;
+            const int lo = sLo[sa], hi = sHi[sb - 1];
+            const bool left = dr > dl;
+            slotRoot = lo == 0 && hi == n - 1;
+            sId[kMergeBlock + t] = slotRoot ? 0 : (left ? hi : lo);
+            slotParent = left ? sb - 1 : sa - 1;
+            slotSide = left ? 0 : 1;
         }
-        const Attach a2 = attachOf(s.l, s.r, s.dl, s.dr);
-        const int slot = a2.left ? jr : jl - 1;
-        if (slot < 0 || slot >= count - 1) break;  // splits outside the group: still pending
-        const int side = a2.left ? 0 : 1;
-        sFarLeaf[side][slot] = a2.left ? s.l : s.r;
-        sFarEntry[side][slot] = a2.left ? jl : jr;
-        sFarD[side][slot] = a2.left ? s.dl : s.dr;
-        const int prior = __hip_atomic_fetch_add(&sSlot[slot], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (prior == 0) {
-            hang = slot;
-            break;
+    }
+    sSlot[t] = make_int4(-1, -1, -1, formed ? 1 : 0);
+    __syncthreads();
+    // a node's parent counts only if it is an inner slot formed here
+    auto innerFormed = [&](int p) { return p >= 0 && p < slots && sSlot[p].w != 0; };
+    int entryParent = -1, entrySide = 0;
+    if (entry) {
+        const bool left = pre[t + 1] > pre[t];  // dr > dl
+        entryParent = left ? t : t - 1;
+        entrySide = left ? 0 : 1;
+        if (s.l == 0 && s.r == n - 1) entryParent = -1;  // the root (never staged)
+    }
+    const bool entryPend = entry && !innerFormed(entryParent) && !(s.l == 0 && s.r == n - 1);
+    const bool slotUp = formed && !slotRoot && innerFormed(slotParent);
+    const bool slotPend = formed && !slotRoot && !slotUp;
+    if (entry && !entryPend && entryParent >= 0) {
+        if (entrySide == 0) sSlot[entryParent].y = t;
+        else sSlot[entryParent].z = t;
+    }
+    if (slotUp) {
+        if (slotSide == 0) sSlot[slotParent].y = kMergeBlock + t;
+        else sSlot[slotParent].z = kMergeBlock + t;
+    }
+    __syncthreads();
+    if (formed) sSlot[t].x = slotUp ? slotParent : -1;
+    __syncthreads();
+
+    // ---- boxes and heights up the formed forest ---------------------------------------------
+    if (entry && !entryPend && entryParent >= 0) {
+        int p = entryParent;
+        while (p >= 0) {
+            const int prior = __hip_atomic_fetch_add(&sArrive[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (prior == 0) break;
+            const int4 tp = sSlot[p];
+            const float* a = sBox[tp.y];
+            const float* c = sBox[tp.z];
+            const Box box = mergeChildren(Box{{a[0], a[1], a[2]}, {a[3], a[4], a[5]}},
+                                          Box{{c[0], c[1], c[2]}, {c[3], c[4], c[5]}});
+            const int ha = sH[tp.y], hc = sH[tp.z];
+            float* d = sBox[kMergeBlock + p];
+            d[0] = box.lo.x;
+            d[1] = box.lo.y;
+            d[2] = box.lo.z;
+            d[3] = box.hi.x;
+            d[4] = box.hi.y;
+            d[5] = box.hi.z;
+            sH[kMergeBlock + p] = (ha > hc ? ha : hc) + 1;
+            p = tp.x;
         }
-        const int sibFar = sFarLeaf[1 - side][slot];
-        const int sibEntry = sFarEntry[1 - side][slot];
-        const int sibD = sFarD[1 - side][slot];
-        int lo, hi, lid, rid, sib;
-        mergeIds(n, s.l, s.r, s.id, a2.split, a2.left, sibFar, lo, hi, lid, rid, sib);
-        (void)sib;
-        const float* sb = a2.left ? sBoxL[jr + 1] : sBoxR[jl - 1];
-        const Box other{{sb[0], sb[1], sb[2]}, {sb[3], sb[4], sb[5]}};
-        const int sh = a2.left ? sHL[jr + 1] : sHR[jl - 1];
-        s.box = a2.left ? mergeChildren(s.box, other) : mergeChildren(other, s.box);
-        s.h = (s.h > sh ? s.h : sh) + 1;
-        if (a2.left) {
-            s.dr = sibD;
-            jr = sibEntry;
-        } else {
-            s.dl = sibD;
-            jl = sibEntry;
-        }
-        const bool root = lo == 0 && hi == n - 1;
-        const int pid = root ? 0 : attachOf(lo, hi, s.dl, s.dr).id;
-        float* bl = sBoxL[jl];
-        float* br = sBoxR[jr];
-        bl[0] = br[0] = s.box.lo.x;
-        bl[1] = br[1] = s.box.lo.y;
-        bl[2] = br[2] = s.box.lo.z;
-        bl[3] = br[3] = s.box.hi.x;
-        bl[4] = br[4] = s.box.hi.y;
-        bl[5] = br[5] = s.box.hi.z;
-        sHL[jl] = s.h;
-        sHR[jr] = s.h;
+    }
+    __syncthreads();
+
+    // ---- the formed nodes to memory -----------------------------------------------------------
+    const int4 tp = sSlot[t];
+    if (formed && tp.y >= 0 && tp.z >= 0) {  // (both children are always linked)
+        const int pid = sId[kMergeBlock + t];
+        const int lid = sId[tp.y], rid = sId[tp.z];
+        const float* b = sBox[kMergeBlock + t];
         Node2 nd;
-        nd.lo[0] = s.box.lo.x;
-        nd.lo[1] = s.box.lo.y;
-        nd.lo[2] = s.box.lo.z;
-        nd.hi[0] = s.box.hi.x;
-        nd.hi[1] = s.box.hi.y;
-        nd.hi[2] = s.box.hi.z;
+        nd.lo[0] = b[0];
+        nd.lo[1] = b[1];
+        nd.lo[2] = b[2];
+        nd.hi[0] = b[3];
+        nd.hi[1] = b[4];
+        nd.hi[2] = b[5];
         nd.left = lid;
         nd.right = rid;
         nodes[pid] = nd;
-        if (root) height[0] = s.h;  // the tree depth; other heights travel in the entries
         parent[lid] = pid;
         parent[rid] = pid;
+        const int lo = sLo[sa], hi = sHi[sb - 1];
         farEnd[pid] = pid == lo ? hi : lo;
-        s.l = lo;
-        s.r = hi;
-        s.id = pid;
-        if (root) pend = false;
+        if (pid == 0) height[0] = sH[kMergeBlock + t];  // the tree depth
     }
-    __syncthreads();
-    // pending: stopped at a split outside the group, or a first arrival left alone
-    if (hang >= 0) pend = sSlot[hang] == 1;
-    if (top) {  // the whole tree: only the root is left (kept for safety)
-        if (pend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
+
+    // ---- hand-up: stage the group's frontier in leaf order ------------------------------------
+    if (top) {  // the whole tree: every node was formed (kept for safety)
+        if (entryPend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
+        if (slotPend) {
+            const float* b = sBox[kMergeBlock + t];
+            const NodeState q{sLo[sa], sHi[sb - 1], sId[kMergeBlock + t], sH[kMergeBlock + t],
+                              static_cast<int>(pre[sa]) - 1, static_cast<int>(pre[sb]) - 1,
+                              {{b[0], b[1], b[2]}, {b[3], b[4], b[5]}}};
+            globalClimb(q, code, order, n, nodes, parent, arrive, height, farEnd);
+        }
         return;
     }
-    const int rank = frontierRank<kMergeBlock>(pend, jl, sMark, sWave);
-    const int total = sWave[kMergeBlock / 64];
+    // positions: an entry at its index, a slot node at its first entry (distinct)
+    sMark[t] = 0;
+    __syncthreads();
+    if (entryPend) sMark[t] = 1;
+    if (slotPend) sMark[sa] = 1;
+    __syncthreads();
+    const uint64_t bal = __ballot(sMark[t]);
+    const int lane = t & 63;
+    const int before = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) sWave[t >> 6] = __popcll(bal);
+    __syncthreads();
+    int prefix = 0;
+    for (int w = 0; w < (t >> 6); ++w) prefix += sWave[w];
+    sMark[t] = prefix + before;
+    int total = 0;
+    for (int w = 0; w < kMergeBlock / 64; ++w) total += sWave[w];
+    __syncthreads();
     if (total > frontierCap) {
-        if (pend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
+        if (entryPend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
+        if (slotPend) {
+            const float* b = sBox[kMergeBlock + t];
+            const NodeState q{sLo[sa], sHi[sb - 1], sId[kMergeBlock + t], sH[kMergeBlock + t],
+                              static_cast<int>(pre[sa]) - 1, static_cast<int>(pre[sb]) - 1,
+                              {{b[0], b[1], b[2]}, {b[3], b[4], b[5]}}};
+            globalClimb(q, code, order, n, nodes, parent, arrive, height, farEnd);
+        }
         if (t == 0) outCounts[g] = -1;
     } else {
-        if (pend) outStage[g * kFrontierCap + rank] = toEntry(s);
+        if (entryPend) outStage[g * kFrontierCap + sMark[t]] = toEntry(s);
+        if (slotPend) {
+            const float* b = sBox[kMergeBlock + t];
+            outStage[g * kFrontierCap + sMark[sa]] =
+                FrontierEntry{sLo[sa], sHi[sb - 1], sId[kMergeBlock + t], sH[kMergeBlock + t],
+                              static_cast<int>(pre[sa]) - 1, static_cast<int>(pre[sb]) - 1,
+                              {b[0], b[1], b[2], b[3], b[4], b[5]}};
+        }
         if (t == 0) outCounts[g] = total;
     }
 }
