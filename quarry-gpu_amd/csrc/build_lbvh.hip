@@ -804,33 +804,62 @@ __global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
 // so every node is byte-identical to the one-node-at-a-time climb.
 constexpr int kMergeNodes = 2 * kMergeBlock;  // entries 0..count-1, slot j at kMergeBlock + j
 constexpr int kPrefixWords = (kMergeBlock + 8) / 8 + 1;
+constexpr int kPrefixMinWords = (kPrefixWords + 7) / 8 + 1;
 
 // High bit of each byte of w whose value is below x (bytes and x in [0, 127], x >= 1).
 __device__ __forceinline__ uint64_t bytesBelow(uint64_t w, uint32_t x) {
     constexpr uint64_t kHigh = 0x8080808080808080ull;
     return ~((w | kHigh) - x * 0x0101010101010101ull) & kHigh;
 }
-// Largest i <= p with byte v[i] < x, or -1.
-__device__ __forceinline__ int lastBelow(const uint64_t* v, int p, uint32_t x) {
-    int w = p >> 3;
-    const int keep = (p & 7) * 8 + 7;  // bits 0..keep hold bytes 0..p&7
-    uint64_t m = bytesBelow(v[w], x) & (keep == 63 ? ~0ull : ((1ull << (keep + 1)) - 1ull));
-    while (m == 0) {
-        if (--w < 0) return -1;
-        m = bytesBelow(v[w], x);
-    }
-    return w * 8 + ((63 - __clzll(static_cast<long long>(m))) >> 3);
+// Bytes 0..k of a word (bits 0 .. 8k + 7).
+__device__ __forceinline__ uint64_t bytesUpTo(int k) {
+    return k == 7 ? ~0ull : ((1ull << (8 * k + 8)) - 1ull);
 }
-// Smallest i in [p, last] with byte v[i] < x, or -1.
-__device__ __forceinline__ int firstBelow(const uint64_t* v, int p, int last, uint32_t x) {
-    int w = p >> 3;
-    const int wl = last >> 3;
-    uint64_t m = bytesBelow(v[w], x) & (~0ull << ((p & 7) * 8));
-    while (m == 0) {
-        if (++w > wl) return -1;
-        m = bytesBelow(v[w], x);
+__device__ __forceinline__ int highByte(uint64_t m) {
+    return (63 - __clzll(static_cast<long long>(m))) >> 3;
+}
+__device__ __forceinline__ int lowByte(uint64_t m) {
+    return (__ffsll(static_cast<long long>(m)) - 1) >> 3;
+}
+// Largest i <= p with byte v[i] < x, or -1.  vm holds the minimum byte of each word
+// of v, so a scan passes 64 prefixes per read after the first word.
+__device__ __forceinline__ int lastBelow(const uint64_t* v, const uint64_t* vm, int p, uint32_t x) {
+    const int w = p >> 3;
+    const uint64_t m = bytesBelow(v[w], x) & bytesUpTo(p & 7);
+    if (m != 0) return w * 8 + highByte(m);
+    const int q = w - 1;
+    if (q < 0) return -1;
+    int mw = q >> 3;
+    uint64_t mm = bytesBelow(vm[mw], x) & bytesUpTo(q & 7);
+    while (mm == 0) {
+        if (--mw < 0) return -1;
+        mm = bytesBelow(vm[mw], x);
     }
-    const int i = w * 8 + ((__ffsll(static_cast<long long>(m)) - 1) >> 3);
+    const int fw = mw * 8 + highByte(mm);
+    return fw * 8 + highByte(bytesBelow(v[fw], x));
+}
+// Smallest i in [p, last] with byte v[i] < x, or -1 (bytes past last are padding
+// that is never below x).
+__device__ __forceinline__ int firstBelow(const uint64_t* v, const uint64_t* vm, int p, int last, uint32_t x) {
+    const int w = p >> 3;
+    const uint64_t m = bytesBelow(v[w], x) & (~0ull << ((p & 7) * 8));
+    int i = -1;
+    if (m != 0) {
+        i = w * 8 + lowByte(m);
+    } else {
+        const int q = w + 1, wl = last >> 3;
+        if (q > wl) return -1;
+        int mw = q >> 3;
+        const int mwl = wl >> 3;
+        uint64_t mm = bytesBelow(vm[mw], x) & (~0ull << ((q & 7) * 8));
+        while (mm == 0) {
+            if (++mw > mwl) return -1;
+            mm = bytesBelow(vm[mw], x);
+        }
+        const int fw = mw * 8 + lowByte(mm);
+        if (fw > wl) return -1;
+        i = fw * 8 + lowByte(bytesBelow(v[fw], x));
+    }
     return i <= last ? i : -1;
 }
 
@@ -849,23 +878,29 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
     __shared__ int sArrive[kMergeBlock];
     __shared__ int sMark[kMergeBlock];
     __shared__ int sWave[kMergeBlock / 64 + 1];
+    __shared__ uint64_t sPreMin[kPrefixMinWords];  // byte w: the smallest byte of sPre[w]
     __shared__ int sOff[kMergeUnits + 1];
+    __shared__ unsigned char sMember[kMergeBlock];  // unit of entry e
     __shared__ int sBroken;
     const int t = threadIdx.x;
     const int g = blockIdx.x;
     const int first = g * kMergeUnits;
     const int members = inUnits - first < kMergeUnits ? inUnits - first : kMergeUnits;
-    if (t == 0) sBroken = 0;
-    __syncthreads();
-    if (t < members) {
-        const int c = inCounts[first + t];
-        if (c < 0) sBroken = 1;
-        sOff[t + 1] = c < 0 ? 0 : (c > kFrontierCap ? kFrontierCap : c);
-    }
-    __syncthreads();
-    if (t == 0) {
-        sOff[0] = 0;
-        for (int m = 1; m <= members; ++m) sOff[m] += sOff[m - 1];
+    if (t < 64) {  // wave 0: unit counts, their offsets, each entry's unit, the broken flag
+        const int raw = t < members ? inCounts[first + t] : 0;
+        const int own = raw < 0 ? 0 : (raw > kFrontierCap ? kFrontierCap : raw);
+        int c = own;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(c, d);
+            if (t >= d) c += y;
+        }
+        const bool anyBroken = __any(raw < 0);
+        if (t < members) sOff[t + 1] = c;
+        if (t == 0) {
+            sOff[0] = 0;
+            sBroken = anyBroken ? 1 : 0;
+        }
+        for (int i = c - own; i < c; ++i) sMember[i] = static_cast<unsigned char>(t);
     }
     __syncthreads();
     const int count = sOff[members];
@@ -873,8 +908,7 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
     NodeState s{};
     const bool entry = t < count;
     if (entry) {
-        int m = 0;
-        while (sOff[m + 1] <= t) ++m;
+        const int m = sMember[t];
         s = fromEntry(inStage[(first + m) * kFrontierCap + (t - sOff[m])]);
     }
     if (broken) {  // a unit below went global: so does the whole group
@@ -886,6 +920,7 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
     // ---- entries and boundary prefixes into LDS ------------------------------------------
     auto* pre = reinterpret_cast<unsigned char*>(sPre);
     if (t < kPrefixWords) sPre[t] = 0x7F7F7F7F7F7F7F7Full;  // padding: never below any prefix
+    if (t < kPrefixMinWords) sPreMin[t] = 0x7F7F7F7F7F7F7F7Full;
     sArrive[t] = 0;
     __syncthreads();
     if (entry) {
@@ -904,6 +939,17 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
         if (t == 0) pre[0] = static_cast<unsigned char>(s.dl + 1);
     }
     __syncthreads();
+    if (t <= (count >> 3)) {
+        const uint64_t w = sPre[t];
+        uint32_t mn = 0x7F;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t byte = static_cast<uint32_t>(w >> (8 * k)) & 0xFFu;
+            mn = byte < mn ? byte : mn;
+        }
+        reinterpret_cast<unsigned char*>(sPreMin)[t] = static_cast<unsigned char>(mn);
+    }
+    __syncthreads();
 
     // ---- the slots: nearest shorter boundaries, parent, id ---------------------------------
     const int slots = count - 1;
@@ -913,8 +959,8 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
     int slotParent = -1, slotSide = 0;  // boundary index of the parent split (-1: the left edge)
     if (isSlot) {
         const uint32_t x = pre[t + 1];
-        sa = lastBelow(sPre, t, x);
-        sb = firstBelow(sPre, t + 2, count, x);
+        sa = lastBelow(sPre, sPreMin, t, x);
+        sb = firstBelow(sPre, sPreMin, t + 2, count, x);
         formed = sa >= 0 && sb >= 0;
         if (formed) {
             const int dl = static_cast<int>(pre[sa]) - 1, dr = static_cast<int>(pre[sb]) - 1;
@@ -954,16 +1000,19 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
 
     // ---- boxes and heights up the formed forest ---------------------------------------------
     if (entry && !entryPend && entryParent >= 0) {
-        int p = entryParent;
+        int me = t, p = entryParent, h = s.h;
+        Box box = s.box;
         while (p >= 0) {
+            const int4 tp = sSlot[p];  // fixed since the last barrier: read beside the arrival
             const int prior = __hip_atomic_fetch_add(&sArrive[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (prior == 0) break;
-            const int4 tp = sSlot[p];
-            const float* a = sBox[tp.y];
-            const float* c = sBox[tp.z];
-            const Box box = mergeChildren(Box{{a[0], a[1], a[2]}, {a[3], a[4], a[5]}},
-                                          Box{{c[0], c[1], c[2]}, {c[3], c[4], c[5]}});
-            const int ha = sH[tp.y], hc = sH[tp.z];
+            const bool mineLeft = tp.y == me;
+            const int sib = mineLeft ? tp.z : tp.y;
+            const float* c = sBox[sib];
+            const Box other{{c[0], c[1], c[2]}, {c[3], c[4], c[5]}};
+            const int hs = sH[sib];
+            box = mineLeft ? mergeChildren(box, other) : mergeChildren(other, box);
+            h = (h > hs ? h : hs) + 1;
             float* d = sBox[kMergeBlock + p];
             d[0] = box.lo.x;
             d[1] = box.lo.y;
@@ -971,7 +1020,8 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
             d[3] = box.hi.x;
             d[4] = box.hi.y;
             d[5] = box.hi.z;
-            sH[kMergeBlock + p] = (ha > hc ? ha : hc) + 1;
+            sH[kMergeBlock + p] = h;
+            me = kMergeBlock + p;
             p = tp.x;
         }
     }

@@ -22,10 +22,10 @@ namespace qr {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kStatAgg = 1u << 30;
 constexpr uint32_t kStatIncl = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
+constexpr int kSortHugeThreads = 1024;  // threads per 8,192-key tile
 constexpr int kLookBack = 8;  // predecessors read per look-back round (32 measured slower: 0.144 vs 0.120 ms for the 2M sort)
 
 __device__ __forceinline__ uint32_t loadAgent(const uint32_t* p) {
@@ -62,10 +62,12 @@ __global__ void sortHistogramKernel(const uint32_t* __restrict__ keys, long long
     flushHistogram(&h[0][0], passes * 256, hist);
 }
 
-// Exclusive scan of 256 values held one per thread; returns the prefix and the
-// total through *total.
+// Exclusive scan of the values held one per thread by the first 256 threads (the
+// other threads hold 0); returns the prefix and the total through *total.
+template <int kThreads = kBlock>
 __device__ __forceinline__ uint32_t blockScan256(uint32_t v, uint32_t* scratch,
                                                  uint32_t* total) {
+    constexpr int kW = kThreads / 64;
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int w = t >> 6;
@@ -81,7 +83,7 @@ __device__ __forceinline__ uint32_t blockScan256(uint32_t v, uint32_t* scratch,
     uint32_t waveBase = 0;
     uint32_t all = 0;
 #pragma unroll
-    for (int i = 0; i < kWaves; ++i) {
+    for (int i = 0; i < kW; ++i) {
         const uint32_t s = scratch[i];
         if (i < w) waveBase += s;
         all += s;
@@ -91,17 +93,22 @@ __device__ __forceinline__ uint32_t blockScan256(uint32_t v, uint32_t* scratch,
     return waveBase + x - v;
 }
 
-template <int kKpt>
-__global__ __launch_bounds__(kBlock) void onesweepKernel(
+// kThreads = 256 or 1024: a tile of kKpt * kThreads keys; with 1024 threads four
+// waves share each SIMD, so the per-row ranking chain is a quarter as long and its
+// latency, and the scatter's, is hidden behind the other waves.  Digit-wise steps
+// (prefixes, descriptors, look-back) run on the first 256 threads.
+template <int kKpt, int kThreads = kBlock>
+__global__ __launch_bounds__(kThreads) void onesweepKernel(
     const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
     uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, long long n, int shift,
     const uint32_t* __restrict__ histPass, uint32_t* __restrict__ desc,
     uint32_t* __restrict__ ticket) {
-    __shared__ uint32_t waveCnt[kWaves][256];
+    constexpr int kW = kThreads / 64;
+    __shared__ uint32_t waveCnt[kW][256];
     __shared__ uint32_t tileStart[256];
     __shared__ uint32_t globalBase[256];
-    __shared__ uint32_t scanScratch[kWaves];
-    constexpr int kTile = kKpt * kBlock;
+    __shared__ uint32_t scanScratch[kW];
+    constexpr int kTile = kKpt * kThreads;
     __shared__ uint32_t stageKeys[kTile];
     __shared__ uint32_t stageVals[kTile];
     __shared__ int tileShared;
@@ -110,8 +117,9 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(
     const int lane = t & 63;
     const int w = t >> 6;
 
+    const bool digitThread = t < 256;  // thread t owns digit t in the digit-wise steps
     if (t == 0) tileShared = static_cast<int>(atomicAdd(ticket, 1u));
-    for (int i = t; i < kWaves * 256; i += kBlock) (&waveCnt[0][0])[i] = 0u;
+    for (int i = t; i < kW * 256; i += kThreads) (&waveCnt[0][0])[i] = 0u;
     __syncthreads();
     const int tile = tileShared;
     const long long tileBase = static_cast<long long>(tile) * kTile;
@@ -121,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(
     uint32_t key[kKpt];
     uint32_t val[kKpt];
     uint32_t rank[kKpt];
-    const long long waveBase = tileBase + static_cast<long long>(w) * (kTile / kWaves);
+    const long long waveBase = tileBase + static_cast<long long>(w) * (kTile / kW);
 #pragma unroll
     for (int r = 0; r < kKpt; ++r) {
         const long long idx = waveBase + r * 64 + lane;
@@ -157,37 +165,39 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(
 
     // Per-digit tile totals and per-wave exclusive prefixes (thread t = digit t).
     uint32_t tileTotal = 0;
-    {
+    uint32_t* myDesc = desc + static_cast<size_t>(tile) * 256;
+    if (digitThread) {
         uint32_t run = 0;
 #pragma unroll
-        for (int i = 0; i < kWaves; ++i) {
+        for (int i = 0; i < kW; ++i) {
             const uint32_t c = waveCnt[i][t];
             waveCnt[i][t] = run;
             run += c;
         }
         tileTotal = run;
-    }
-    uint32_t* myDesc = desc + static_cast<size_t>(tile) * 256;
-    if (tile == 0) {
-        storeAgent(&myDesc[t], kStatIncl | tileTotal);
-    } else {
-        storeAgent(&myDesc[t], kStatAgg | tileTotal);
+        if (tile == 0) {
+            storeAgent(&myDesc[t], kStatIncl | tileTotal);
+        } else {
+            storeAgent(&myDesc[t], kStatAgg | tileTotal);
+        }
     }
 
     uint32_t dummy;
-    const uint32_t startInTile = blockScan256(tileTotal, scanScratch, &dummy);
-    tileStart[t] = startInTile;
+    const uint32_t startInTile = blockScan256<kThreads>(tileTotal, scanScratch, &dummy);
+    if (digitThread) tileStart[t] = startInTile;
     uint32_t digitTotal = 0;
+    if (digitThread) {
 #pragma unroll
-    for (int c = 0; c < kHistCopies; ++c) digitTotal += histPass[c * kHistCopyWords + t];
-    const uint32_t bucketStart = blockScan256(digitTotal, scanScratch, &dummy);
+        for (int c = 0; c < kHistCopies; ++c) digitTotal += histPass[c * kHistCopyWords + t];
+    }
+    const uint32_t bucketStart = blockScan256<kThreads>(digitTotal, scanScratch, &dummy);
 
     // Look-back over a window of kLookBack predecessors per round (loads in flight
     // together): aggregates are summed in order up to the first inclusive prefix; a
     // predecessor that has not published yet ends the round, which resumes there.
     // Tile 0 publishes an inclusive prefix at once, so the walk ends.
     uint32_t exclusive = 0;
-    if (tile > 0) {
+    if (digitThread && tile > 0) {
         int j = tile - 1;
         while (true) {
             uint32_t v[kLookBack];
@@ -213,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(
         }
         storeAgent(&myDesc[t], kStatIncl | (exclusive + tileTotal));
     }
-    globalBase[t] = bucketStart + exclusive;
+    if (digitThread) globalBase[t] = bucketStart + exclusive;
     __syncthreads();
 
 #pragma unroll
@@ -230,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(
 
 #pragma unroll
     for (int j = 0; j < kKpt; ++j) {
-        const int i = j * kBlock + t;
+        const int i = j * kThreads + t;
         if (i < validCount) {
             const uint32_t k = stageKeys[i];
             const uint32_t d = (k >> shift) & 255u;
@@ -393,7 +403,7 @@ void sortPairs(SortWorkspace& ws, const uint32_t* keys, const uint32_t* vals, ui
                 srcK, srcV, dstK, dstV, static_cast<long long>(n), 8 * p, ws.hist.p + p * 256, desc,
                 ws.tickets.p + p);
         else
-            onesweepKernel<kSortTileHuge / kBlock><<<tiles, kBlock, 0, s>>>(
+            onesweepKernel<kSortTileHuge / kSortHugeThreads, kSortHugeThreads><<<tiles, kSortHugeThreads, 0, s>>>(
                 srcK, srcV, dstK, dstV, static_cast<long long>(n), 8 * p, ws.hist.p + p * 256, desc,
                 ws.tickets.p + p);
         hipCheck(hipGetLastError(), "onesweep");
