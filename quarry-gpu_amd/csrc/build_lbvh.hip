@@ -86,37 +86,10 @@ __global__ void buildPrepKernel(uint32_t* __restrict__ cb, uint32_t* __restrict_
     }
 }
 
-template <typename L>
-__global__ __launch_bounds__(kBlock) void centroidBoundsKernel(
-    const typename L::Item* __restrict__ items, int n, uint32_t* __restrict__ cb) {
-    float lo0 = kInf, lo1 = kInf, lo2 = kInf;
-    float hi0 = -kInf, hi1 = -kInf, hi2 = -kInf;
-    const int stride = kBlock * gridDim.x;
-    int i = blockIdx.x * kBlock + threadIdx.x;
-    for (; i + (kUnroll - 1) * stride < n; i += kUnroll * stride) {
-        typename L::Item it[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) it[u] = items[i + u * stride];
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            const V3 c = L::centroid(it[u]);
-            lo0 = fminf(lo0, c.x);
-            lo1 = fminf(lo1, c.y);
-            lo2 = fminf(lo2, c.z);
-            hi0 = fmaxf(hi0, c.x);
-            hi1 = fmaxf(hi1, c.y);
-            hi2 = fmaxf(hi2, c.z);
-        }
-    }
-    for (; i < n; i += stride) {
-        const V3 c = L::centroid(items[i]);
-        lo0 = fminf(lo0, c.x);
-        lo1 = fminf(lo1, c.y);
-        lo2 = fminf(lo2, c.z);
-        hi0 = fmaxf(hi0, c.x);
-        hi1 = fmaxf(hi1, c.y);
-        hi2 = fmaxf(hi2, c.z);
-    }
+// Workgroup reduction of the per-thread centroid bounds and the flush into copy
+// blockIdx.x % kBoundsCopies.
+__device__ __forceinline__ void boundsFlush(float lo0, float lo1, float lo2, float hi0, float hi1, float hi2,
+                                            uint32_t* __restrict__ cb) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         lo0 = fminf(lo0, __shfl_xor(lo0, off, 64));
@@ -150,6 +123,40 @@ __global__ __launch_bounds__(kBlock) void centroidBoundsKernel(
     }
 }
 
+template <typename L>
+__global__ __launch_bounds__(kBlock) void centroidBoundsKernel(
+    const typename L::Item* __restrict__ items, int n, uint32_t* __restrict__ cb) {
+    float lo0 = kInf, lo1 = kInf, lo2 = kInf;
+    float hi0 = -kInf, hi1 = -kInf, hi2 = -kInf;
+    const int stride = kBlock * gridDim.x;
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    for (; i + (kUnroll - 1) * stride < n; i += kUnroll * stride) {
+        typename L::Item it[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) it[u] = items[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const V3 c = L::centroid(it[u]);
+            lo0 = fminf(lo0, c.x);
+            lo1 = fminf(lo1, c.y);
+            lo2 = fminf(lo2, c.z);
+            hi0 = fmaxf(hi0, c.x);
+            hi1 = fmaxf(hi1, c.y);
+            hi2 = fmaxf(hi2, c.z);
+        }
+    }
+    for (; i < n; i += stride) {
+        const V3 c = L::centroid(items[i]);
+        lo0 = fminf(lo0, c.x);
+        lo1 = fminf(lo1, c.y);
+        lo2 = fminf(lo2, c.z);
+        hi0 = fmaxf(hi0, c.x);
+        hi1 = fmaxf(hi1, c.y);
+        hi2 = fmaxf(hi2, c.z);
+    }
+    boundsFlush(lo0, lo1, lo2, hi0, hi1, hi2, cb);
+}
+
 // ---- Morton codes + radix histogram ----------------------------------------------
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
@@ -167,13 +174,15 @@ __device__ __forceinline__ uint32_t quantize1023(float v, float lo, float extent
     return static_cast<uint32_t>(static_cast<int>(fminf(1023.0f, fmaxf(0.0f, q))));
 }
 
-template <typename L>
-__global__ __launch_bounds__(kBlock) void mortonHistKernel(const typename L::Item* __restrict__ items,
-                                                           int n, const uint32_t* __restrict__ cb,
-                                                           uint32_t* __restrict__ codes,
-                                                           uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[4][256];
-    for (int i = threadIdx.x; i < 4 * 256; i += kBlock) (&h[0][0])[i] = 0u;
+// The build's quantization frame: the centroid bounds folded over their copies.
+struct MortonFrame {
+    float lo0, lo1, lo2, ex0, ex1, ex2;
+    __device__ uint32_t code(const V3& c) const {
+        return spread10(quantize1023(c.x, lo0, ex0)) | (spread10(quantize1023(c.y, lo1, ex1)) << 1) |
+               (spread10(quantize1023(c.z, lo2, ex2)) << 2);
+    }
+};
+__device__ __forceinline__ MortonFrame mortonFrame(const uint32_t* __restrict__ cb) {
     uint32_t b[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) b[c] = cb[c];
@@ -186,14 +195,20 @@ __global__ __launch_bounds__(kBlock) void mortonHistKernel(const typename L::Ite
         }
     }
     const float lo0 = fromOrderedBits(b[0]), lo1 = fromOrderedBits(b[1]), lo2 = fromOrderedBits(b[2]);
-    const float hi0 = fromOrderedBits(b[3]), hi1 = fromOrderedBits(b[4]), hi2 = fromOrderedBits(b[5]);
-    const float ex0 = hi0 - lo0, ex1 = hi1 - lo1, ex2 = hi2 - lo2;
+    return {lo0, lo1, lo2, fromOrderedBits(b[3]) - lo0, fromOrderedBits(b[4]) - lo1, fromOrderedBits(b[5]) - lo2};
+}
+
+template <typename L>
+__global__ __launch_bounds__(kBlock) void mortonHistKernel(const typename L::Item* __restrict__ items,
+                                                           int n, const uint32_t* __restrict__ cb,
+                                                           uint32_t* __restrict__ codes,
+                                                           uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[4][256];
+    for (int i = threadIdx.x; i < 4 * 256; i += kBlock) (&h[0][0])[i] = 0u;
+    const MortonFrame f = mortonFrame(cb);
     __syncthreads();
     auto emit = [&](int i, const typename L::Item& item) {
-        const V3 c = L::centroid(item);
-        const uint32_t code = spread10(quantize1023(c.x, lo0, ex0)) |
-                              (spread10(quantize1023(c.y, lo1, ex1)) << 1) |
-                              (spread10(quantize1023(c.z, lo2, ex2)) << 2);
+        const uint32_t code = f.code(L::centroid(item));
         codes[i] = code;
         atomicAdd(&h[0][code & 255u], 1u);
         atomicAdd(&h[1][(code >> 8) & 255u], 1u);
@@ -212,6 +227,62 @@ __global__ __launch_bounds__(kBlock) void mortonHistKernel(const typename L::Ite
     for (; i < n; i += stride) emit(i, items[i]);
     __syncthreads();
     flushHistogram(&h[0][0], 4 * 256, hist);
+}
+
+// ---- triangle bounds with coalesced loads ------------------------------------------------
+// A triangle is 36 bytes, so a lane-per-item load makes every lane start its own
+// 36-byte run; here a workgroup copies kStageTris consecutive triangles (a multiple of
+// 16 bytes) into LDS with 16-byte loads on consecutive lanes, then each thread reads
+// its triangles from LDS (a 9-word stride: no bank conflicts).  Used when the
+// triangle array is 16-byte aligned.
+constexpr int kStageTris = 4 * kBlock;
+constexpr int kStageVec = kStageTris * 36 / 16 / kBlock;  // 16-byte loads per thread
+static_assert(kStageTris * 36 % (16 * kBlock) == 0, "staging chunk");
+
+__device__ __forceinline__ void stageTris(const Tri* __restrict__ tris, long long base, int cnt,
+                                          uint4* __restrict__ lds) {
+    if (cnt == kStageTris) {
+        const uint4* src = reinterpret_cast<const uint4*>(tris + base);
+        uint4 v[kStageVec];
+#pragma unroll
+        for (int k = 0; k < kStageVec; ++k) v[k] = src[k * kBlock + threadIdx.x];
+#pragma unroll
+        for (int k = 0; k < kStageVec; ++k) lds[k * kBlock + threadIdx.x] = v[k];
+    } else {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(tris + base);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(lds);
+        for (int k = threadIdx.x; k < cnt * 9; k += kBlock) dst[k] = src[k];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void triBoundsStagedKernel(const Tri* __restrict__ tris, int n,
+                                                                uint32_t* __restrict__ cb) {
+    __shared__ uint4 sStage[kStageTris * 36 / 16];
+    float lo0 = kInf, lo1 = kInf, lo2 = kInf;
+    float hi0 = -kInf, hi1 = -kInf, hi2 = -kInf;
+    const int chunks = (n + kStageTris - 1) / kStageTris;
+    for (int c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const long long base = static_cast<long long>(c) * kStageTris;
+        const int cnt = n - base < kStageTris ? static_cast<int>(n - base) : kStageTris;
+        stageTris(tris, base, cnt, sStage);
+        __syncthreads();
+        const Tri* st = reinterpret_cast<const Tri*>(sStage);
+#pragma unroll
+        for (int u = 0; u < kStageTris / kBlock; ++u) {
+            const int j = u * kBlock + threadIdx.x;
+            if (j < cnt) {
+                const V3 q = triCentroid(st[j]);
+                lo0 = fminf(lo0, q.x);
+                lo1 = fminf(lo1, q.y);
+                lo2 = fminf(lo2, q.z);
+                hi0 = fmaxf(hi0, q.x);
+                hi1 = fmaxf(hi1, q.y);
+                hi2 = fmaxf(hi2, q.z);
+            }
+        }
+        __syncthreads();
+    }
+    boundsFlush(lo0, lo1, lo2, hi0, hi1, hi2, cb);
 }
 
 // ---- Karras 2012 hierarchy ---------------------------------------------------------
@@ -1352,7 +1423,18 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     // few enough workgroups that the bounds / histogram atomics stay cheap
     int grid = blocksFor(n, kBlock * 4 * kUnroll);
     if (grid > 512) grid = 512;
-    centroidBoundsKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb);
+    if (L::kKind == 0 && (reinterpret_cast<uintptr_t>(dSrc) & 15) == 0) {
+        // staged triangle bounds: four workgroups per CU (36 KB of LDS each); 2M
+        // triangles 14.4 -> 13.3 us
+        int sgrid = blocksFor(n, kStageTris);
+        if (sgrid > 1024) sgrid = 1024;
+        const Tri* tris = reinterpret_cast<const Tri*>(dSrc);
+        triBoundsStagedKernel<<<sgrid, kBlock, 0, s>>>(tris, n, cb);
+    } else {
+        centroidBoundsKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb);
+    }
+    // (a staged variant of this pass measured slower: 21.4 against 18.2 us on 2M
+    // triangles; the codes are written per item, so the lane-per-item loads suffice)
     mortonHistKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb, codes, b.sortWs.hist.p);
     hipCheck(hipGetLastError(), "morton");
     ev.record(1, s);

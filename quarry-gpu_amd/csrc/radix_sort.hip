@@ -25,7 +25,11 @@ constexpr int kBlock = 256;
 constexpr uint32_t kStatAgg = 1u << 30;
 constexpr uint32_t kStatIncl = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
-constexpr int kSortHugeThreads = 1024;  // threads per 8,192-key tile
+// Threads per 8,192-key tile: 1,024 up to kSortWideTilesMax tiles (a few per CU,
+// where each tile's latency shows: 2M keys 0.119 -> 0.093 ms), 256 above (two tiles
+// per CU hide it better: 48M keys 0.338 -> 0.382 ms per pass with 1,024).
+constexpr int kSortHugeThreads = 1024;
+constexpr int kSortWideTilesMax = 1024;
 constexpr int kLookBack = 8;  // predecessors read per look-back round (32 measured slower: 0.144 vs 0.120 ms for the 2M sort)
 
 __device__ __forceinline__ uint32_t loadAgent(const uint32_t* p) {
@@ -402,8 +406,12 @@ void sortPairs(SortWorkspace& ws, const uint32_t* keys, const uint32_t* vals, ui
             onesweepKernel<kSortTileLarge / kBlock><<<tiles, kBlock, 0, s>>>(
                 srcK, srcV, dstK, dstV, static_cast<long long>(n), 8 * p, ws.hist.p + p * 256, desc,
                 ws.tickets.p + p);
-        else
+        else if (tiles <= kSortWideTilesMax)
             onesweepKernel<kSortTileHuge / kSortHugeThreads, kSortHugeThreads><<<tiles, kSortHugeThreads, 0, s>>>(
+                srcK, srcV, dstK, dstV, static_cast<long long>(n), 8 * p, ws.hist.p + p * 256, desc,
+                ws.tickets.p + p);
+        else
+            onesweepKernel<kSortTileHuge / kBlock><<<tiles, kBlock, 0, s>>>(
                 srcK, srcV, dstK, dstV, static_cast<long long>(n), 8 * p, ws.hist.p + p * 256, desc,
                 ws.tickets.p + p);
         hipCheck(hipGetLastError(), "onesweep");
