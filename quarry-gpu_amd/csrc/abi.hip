@@ -44,6 +44,9 @@ struct qr_points {
     // scratch, [0] = largest per-query count, and the captured first hits per query
     DevBuf<int> rCounts, rScratch, rMax;
     DevBuf<uint64_t> rCapture;
+    // the last radius call's hit total plus an eighth: the size of the result keys
+    // compacted before the total reaches the host (0: none yet)
+    int rKeysGuess = 0;
 };
 
 // Result buffers handed to the caller: stream-ordered allocations from the device's
@@ -100,6 +103,13 @@ int* hostWords() {
     if (!w) hipCheck(hipHostMalloc(reinterpret_cast<void**>(&w), (1 + kMaxCopies) * sizeof(int), hipHostMallocDefault),
                      "pinned words");
     return w;
+}
+// The same words as the device addresses them (kernels store the call's figures
+// there directly).
+int* hostWordsDevice() {
+    static thread_local int* d = nullptr;
+    if (!d) hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&d), hostWords(), 0), "pinned words map");
+    return d;
 }
 
 struct qr_batch {
@@ -1452,19 +1462,30 @@ int qr_radius(const qr_bvh* bvh, qr_points* p, float radius, int collectStats, i
             kev.record(1, s);
             scanExclusive(counts, res->offsets.p, n, res->offsets.p + n, scratch, s);
             int* hw = hostWords();
-            hipCheck(hipMemcpyAsync(hw, res->offsets.p + n, sizeof(int), hipMemcpyDeviceToHost, s), "D2H total");
-            hipCheck(hipMemcpyAsync(hw + 1, p->rMax.p, kMaxCopies * sizeof(int), hipMemcpyDeviceToHost, s),
-                     "D2H max");
+            hw[0] = hw[1] = -1;
+            radiusSummary(res->offsets.p + n, p->rMax.p, hostWordsDevice(), s);
+            // Compaction before the total is known on the host, into keys sized by the
+            // last call on these queries: the common case then needs no host round
+            // trip in the middle of the call.  A short guess skips the segments past it
+            // and the compaction is redone below at the exact size.
+            const int guess = captured ? p->rKeysGuess : 0;
+            if (guess > 0) {
+                res->keys.alloc(static_cast<size_t>(guess), s);
+                radiusCompact(counts, res->offsets.p, p->rCapture.p, n, cap, res->keys.p, guess, s);
+            }
             hipCheck(hipStreamSynchronize(s), "radius total sync");
             const int total = hw[0];
-            int maxSeg = 0;
-            for (int i = 1; i <= kMaxCopies; ++i) maxSeg = hw[i] > maxSeg ? hw[i] : maxSeg;
+            const int maxSeg = hw[1];
+            if (total < 0 || maxSeg < 0) throw DeviceError("radius: summary words not written");
             res->total = total;
-            res->keys.alloc(static_cast<size_t>(total > 0 ? total : 1), s);
+            const bool compacted = guess > 0 && total <= guess;
+            if (!compacted) res->keys.alloc(static_cast<size_t>(total > 0 ? total : 1), s);
+            p->rKeysGuess = total + total / 8 + 64;
             kev.record(2, s);
             if (total > 0) {
                 a.stats = nullptr;
-                if (captured) radiusCompact(counts, res->offsets.p, p->rCapture.p, n, cap, res->keys.p, s);
+                if (captured && !compacted)
+                    radiusCompact(counts, res->offsets.p, p->rCapture.p, n, cap, res->keys.p, total, s);
                 // a second walk only for the queries the capture could not hold
                 if (!captured || maxSeg > cap)
                     launchRadiusFill(a, p->pts.p, n, r2, counts, captured ? cap : 0, res->offsets.p, res->keys.p, s);

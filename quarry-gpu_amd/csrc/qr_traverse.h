@@ -74,7 +74,10 @@ void launchRadiusFill(const TraverseArgs& a, const V3* pts, int count, float r2,
                       int cap, const int* offsets, uint64_t* keys, hipStream_t s);
 // Copies the captured segments into the CSR keys.
 void radiusCompact(const int* counts, const int* offsets, const uint64_t* capture, int count, int cap,
-                   uint64_t* keys, hipStream_t s);
+                   uint64_t* keys, long long capacity, hipStream_t s);
+// Writes the total (*total) and the largest of the kMaxCopies per-query maxima to
+// hostDev[0] and hostDev[1] (device view of pinned host words).
+void radiusSummary(const int* total, const int* maxCopies, int* hostDev, hipStream_t s);
 // scratchInts must hold 4 + 2 * count ints.
 void segmentSort(const int* offsets, uint64_t* keys, int count, int maxSeg, int* scratchInts,
                  hipStream_t s);

@@ -2426,26 +2426,53 @@ void launchRadiusFill(const TraverseArgs& a, const V3* pts, int count, float r2,
 
 namespace {
 // One wave per query copies its captured hits (queries with more than `cap` hits are
-// left to the fill walk).
+// left to the fill walk); a segment that would end past `capacity` keys is skipped
+// (the caller compacts again into a buffer of the exact size).
 __global__ void radiusCompactKernel(const int* __restrict__ counts, const int* __restrict__ offsets,
                                     const uint64_t* __restrict__ capture, int count, int cap,
-                                    uint64_t* __restrict__ keys) {
+                                    uint64_t* __restrict__ keys, long long capacity) {
     const long long q = static_cast<long long>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
     if (q >= count) return;
     const int c = counts[q];
     if (c > cap) return;
+    const long long off = offsets[q];
+    if (off + c > capacity) return;
     const uint64_t* src = capture + q * cap;
-    uint64_t* dst = keys + offsets[q];
+    uint64_t* dst = keys + off;
     for (int j = threadIdx.x & 63; j < c; j += 64) dst[j] = src[j];
+}
+
+// The radius call's two host-side figures in one step: the total hit count and the
+// largest per-query count (the maximum of the kMaxCopies words), stored straight into
+// the caller's pinned host words.
+__global__ void radiusSummaryKernel(const int* __restrict__ total, const int* __restrict__ maxCopies,
+                                    int* __restrict__ host) {
+    const int lane = threadIdx.x;
+    int m = lane < kMaxCopies ? maxCopies[lane] : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    if (lane == 0) {
+        __hip_atomic_store(&host[0], *total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&host[1], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 }  // namespace
 
 void radiusCompact(const int* counts, const int* offsets, const uint64_t* capture, int count, int cap,
-                   uint64_t* keys, hipStream_t s) {
+                   uint64_t* keys, long long capacity, hipStream_t s) {
     if (count <= 0 || cap <= 0) return;
     radiusCompactKernel<<<blocksFor(count, kBlock / 64), kBlock, 0, s>>>(counts, offsets, capture, count, cap,
-                                                                        keys);
+                                                                        keys, capacity);
     hipCheck(hipGetLastError(), "radius compact");
+}
+
+void radiusSummary(const int* total, const int* maxCopies, int* hostDev, hipStream_t s) {
+    static_assert(kMaxCopies <= 64, "one lane per copy");
+    radiusSummaryKernel<<<1, 64, 0, s>>>(total, maxCopies, hostDev);
+    hipCheck(hipGetLastError(), "radius summary");
 }
 
 void segmentSort(const int* offsets, uint64_t* keys, int count, int maxSeg, int* scratchInts,
