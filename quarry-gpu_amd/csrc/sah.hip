@@ -72,6 +72,15 @@ struct Chunk {
 };
 
 __device__ __forceinline__ int hidx(int a, int b, int v) { return (a * kBins + b) * kBinVals + v; }
+// Histograms that keep no centroid bounds (kCent = false: the subtree tiers, which
+// take their children's centroid boxes from the records) store 7 values per bin
+// (count, box lo xyz, box hi xyz) in a smaller LDS array.
+template <bool kCent>
+constexpr int binVals() { return kCent ? kBinVals : 7; }
+template <bool kCent>
+constexpr int histWords() { return 3 * kBins * binVals<kCent>(); }
+template <bool kCent>
+__device__ __forceinline__ int hidxT(int a, int b, int v) { return (a * kBins + b) * binVals<kCent>() + v; }
 
 struct AxisMap {
     float lo[3], scale[3];
@@ -104,8 +113,10 @@ __device__ __forceinline__ uint32_t binInit(int i) {
     return v == 0 ? 0u : ((v >= 1 && v <= 3) || (v >= 7 && v <= 9)) ? orderedBits(kInf) : orderedBits(-kInf);
 }
 
+template <bool kCent = true>
 __device__ __forceinline__ void histInit(uint32_t* h) {
-    for (int i = threadIdx.x; i < kHist; i += blockDim.x) h[i] = binInit(i);
+    // (word v of a bin: the same empty values for both layouts, v < 7 being shared)
+    for (int i = threadIdx.x; i < histWords<kCent>(); i += blockDim.x) h[i] = binInit(i % binVals<kCent>());
 }
 
 // Bins of a contiguous range of records.  Each thread takes a run of consecutive
@@ -128,15 +139,15 @@ struct BinRun {
 template <bool kCent = true>
 __device__ __forceinline__ void binRunFlush(uint32_t* h, int a, const BinRun& r, uint32_t* part = nullptr) {
     if (r.cnt == 0u) return;
-    atomicAdd(&h[hidx(a, r.bin, 0)], r.cnt);
+    atomicAdd(&h[hidxT<kCent>(a, r.bin, 0)], r.cnt);
     if (part) atomicAdd(&part[a * kBins + r.bin], r.cnt);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        atomicMin(&h[hidx(a, r.bin, 1 + k)], r.v[k]);
-        atomicMax(&h[hidx(a, r.bin, 4 + k)], r.v[3 + k]);
-        if (kCent) {
-            atomicMin(&h[hidx(a, r.bin, 7 + k)], r.v[6 + k]);
-            atomicMax(&h[hidx(a, r.bin, 10 + k)], r.v[9 + k]);
+        atomicMin(&h[hidxT<kCent>(a, r.bin, 1 + k)], r.v[k]);
+        atomicMax(&h[hidxT<kCent>(a, r.bin, 4 + k)], r.v[3 + k]);
+        if constexpr (kCent) {
+            atomicMin(&h[hidxT<kCent>(a, r.bin, 7 + k)], r.v[6 + k]);
+            atomicMax(&h[hidxT<kCent>(a, r.bin, 10 + k)], r.v[9 + k]);
         }
     }
 }
@@ -151,8 +162,8 @@ __device__ __forceinline__ void binRunsInit(BinRun* run) {
 
 // One record into the thread's runs (a run is flushed when its bin changes); one
 // instantiation per axis, so that the runs and the axis map stay in registers.
-template <int a, bool kCent>
-__device__ __forceinline__ void binRunAddAxis(uint32_t* h, BinRun& u, const PrimRec& r, const AxisMap& m,
+template <int a, bool kCent, typename R>
+__device__ __forceinline__ void binRunAddAxis(uint32_t* h, BinRun& u, const R& r, const AxisMap& m,
                                               uint32_t* part) {
     if (!m.use[a]) return;
     const int b = binOf(r.c[a], m.lo[a], m.scale[a]);
@@ -178,8 +189,8 @@ __device__ __forceinline__ void binRunAddAxis(uint32_t* h, BinRun& u, const Prim
 }
 
 // r is taken by value: loaded once, not re-read after the flushes' LDS atomics.
-template <bool kCent = true>
-__device__ __forceinline__ void binRunAdd(uint32_t* h, BinRun* run, const PrimRec r, const AxisMap& m,
+template <bool kCent = true, typename R = PrimRec>
+__device__ __forceinline__ void binRunAdd(uint32_t* h, BinRun* run, const R r, const AxisMap& m,
                                           uint32_t* part = nullptr) {
     binRunAddAxis<0, kCent>(h, run[0], r, m, part);
     binRunAddAxis<1, kCent>(h, run[1], r, m, part);
@@ -222,6 +233,9 @@ __device__ __forceinline__ float areaOf(float lx, float ly, float lz, float hx, 
 // bits as summing the bins plane by plane.  The (cost, axis * 32 + plane) minimum
 // is reduced across the lanes, then the chosen plane's side counts and centroid
 // boxes; lane 0 writes the result.  h is the task's histogram (LDS or global).
+// kCent = false: a histogram without centroid bounds; the chosen plane's side
+// centroid boxes are then left 0 for the caller to fill from the records.
+template <bool kCent = true>
 __device__ void chooseSplit(const uint32_t* h, const SahTask& t, const AxisMap& m, SahResult* out) {
     const int lane = threadIdx.x & 63;
     const int bin = lane & (kBins - 1);
@@ -232,12 +246,12 @@ __device__ void chooseSplit(const uint32_t* h, const SahTask& t, const AxisMap& 
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (!m.use[a]) continue;
-        const int c = static_cast<int>(h[hidx(a, bin, 0)]);
+        const int c = static_cast<int>(h[hidxT<kCent>(a, bin, 0)]);
         float v[6];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            v[k] = fromOrderedBits(h[hidx(a, bin, 1 + k)]);
-            v[3 + k] = fromOrderedBits(h[hidx(a, bin, 4 + k)]);
+            v[k] = fromOrderedBits(h[hidxT<kCent>(a, bin, 1 + k)]);
+            v[3 + k] = fromOrderedBits(h[hidxT<kCent>(a, bin, 4 + k)]);
         }
         const int pc = prefixAdd32(c, lane);
         const int total = __builtin_amdgcn_readlane(pc, 31);
@@ -298,18 +312,22 @@ __device__ void chooseSplit(const uint32_t* h, const SahTask& t, const AxisMap& 
     // right of it (a min / max of ordered bits is exact and order-free; empty bins
     // take the identities)
     const int a = bestKey / kBins, plane = bestKey % kBins;
-    const uint32_t c = h[hidx(a, bin, 0)];
+    const uint32_t c = h[hidxT<kCent>(a, bin, 0)];
     const bool isLeft = bin < plane;
     const uint32_t nL = reduce32<AddU>(isLeft ? c : 0u);
     uint32_t v[12];  // left min xyz, left max xyz, right min xyz, right max xyz
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const uint32_t lo = c ? h[hidx(a, bin, 7 + k)] : 0xFFFFFFFFu;
-        const uint32_t hi = c ? h[hidx(a, bin, 10 + k)] : 0u;
-        v[k] = reduce32<MinU>(isLeft ? lo : 0xFFFFFFFFu);
-        v[3 + k] = reduce32<MaxU>(isLeft ? hi : 0u);
-        v[6 + k] = reduce32<MinU>(isLeft ? 0xFFFFFFFFu : lo);
-        v[9 + k] = reduce32<MaxU>(isLeft ? 0u : hi);
+    for (int k = 0; k < 12; ++k) v[k] = 0u;
+    if constexpr (kCent) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t lo = c ? h[hidxT<kCent>(a, bin, 7 + k)] : 0xFFFFFFFFu;
+            const uint32_t hi = c ? h[hidxT<kCent>(a, bin, 10 + k)] : 0u;
+            v[k] = reduce32<MinU>(isLeft ? lo : 0xFFFFFFFFu);
+            v[3 + k] = reduce32<MaxU>(isLeft ? hi : 0u);
+            v[6 + k] = reduce32<MinU>(isLeft ? 0xFFFFFFFFu : lo);
+            v[9 + k] = reduce32<MaxU>(isLeft ? 0u : hi);
+        }
     }
     if (lane != 0) return;
     res.axis = a;
@@ -1006,7 +1024,7 @@ __device__ __forceinline__ void flushSerial(const SahTables& T, const SerialTask
 template <int kBT>
 __global__ __launch_bounds__(kBT) void sahSubtreeKernel(SahTables T, int n, Node2* __restrict__ nodes,
                                                         int* __restrict__ parent, int* __restrict__ farEnd) {
-    __shared__ uint32_t h[kHist];
+    __shared__ uint32_t h[histWords<false>()];
     __shared__ SahResult sres;
     __shared__ SahTask stk[kSubtreeStack];
     __shared__ SahTask curTask;
@@ -1041,11 +1059,11 @@ __global__ __launch_bounds__(kBT) void sahSubtreeKernel(SahTables T, int n, Node
         PrimRec* dst = recOf(T, 1 - buf);
         const AxisMap m = axisMap(t);
         if (!median) {
-            histInit(h);
+            histInit<false>(h);
             __syncthreads();
             histAddRange<kBT, false>(h, src + t.start, t.count, m);
             __syncthreads();
-            if (threadIdx.x < 64) chooseSplit(h, t, m, &sres);
+            if (threadIdx.x < 64) chooseSplit<false>(h, t, m, &sres);
         } else if (threadIdx.x == 0) {
             sres = medianResult(t);
         }
@@ -1136,9 +1154,21 @@ __global__ __launch_bounds__(kBT) void sahSubtreeKernel(SahTables T, int n, Node
 // for its balanced splits.
 constexpr int kOrderWritten = 2;  // SerialTask.buf: the range's leaf order is already in place
 constexpr int kSmallBT = 128;     // threads per small task
+constexpr int kSmallStack = 24;   // a chain of count/8 sides from 512 to 64 is 16 deep
+constexpr int kSmallSerialBatch = 16;
+
+// A record as the small tier keeps it in LDS: centroid and box (36 B); the scene id
+// is read from the record buffer when the task's leaf order is written at its end.
+// With the 7-value histogram and the shorter stack and serial batch, a workgroup
+// takes 25 KB of LDS, so six fit a CU instead of five.
+struct PrimBox {
+    float c[3];
+    float lo[3];
+    float hi[3];
+};
 
 template <int kBT>
-__device__ __forceinline__ void histAddLds(uint32_t* h, const PrimRec* recL, const unsigned short* idx, int count,
+__device__ __forceinline__ void histAddLds(uint32_t* h, const PrimBox* recL, const unsigned short* idx, int count,
                                            const AxisMap& m) {
     const int per = (count + kBT - 1) / kBT;
     const int beg = threadIdx.x * per;
@@ -1154,12 +1184,13 @@ __global__ __launch_bounds__(kBT) void sahSmallKernel(SahTables T, int n, uint32
                                                       Node2* __restrict__ nodes, int* __restrict__ parent,
                                                       int* __restrict__ farEnd) {
     constexpr int kWaves = kBT / 64;
-    __shared__ PrimRec recL[kSmallTask];
+    __shared__ PrimBox recL[kSmallTask];
+    __shared__ unsigned char finalBuf[kSmallTask];  // index buffer holding the final serial ranges' positions
     __shared__ unsigned short idxL[2][kSmallTask];
-    __shared__ uint32_t h[kHist];
+    __shared__ uint32_t h[histWords<false>()];
     __shared__ SahResult sres;
-    __shared__ SahTask stk[kSubtreeStack];
-    __shared__ SerialTask serBuf[kSerialBatch];
+    __shared__ SahTask stk[kSmallStack];
+    __shared__ SerialTask serBuf[kSmallSerialBatch];
     __shared__ int waveLefts[kWaves];
     __shared__ uint32_t waveBox[kWaves][12];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1177,8 +1208,10 @@ __global__ __launch_bounds__(kBT) void sahSmallKernel(SahTables T, int n, uint32
         const int base = root.start;
         const PrimRec* src = recOf(T, root.meta & 1) + base;
         for (int i = threadIdx.x; i < root.count; i += kBT) {
-            recL[i] = src[i];
+            const PrimRec q = src[i];
+            recL[i] = {{q.c[0], q.c[1], q.c[2]}, {q.lo[0], q.lo[1], q.lo[2]}, {q.hi[0], q.hi[1], q.hi[2]}};
             idxL[0][i] = static_cast<unsigned short>(i);
+            finalBuf[i] = 0xFF;
         }
         int sp = 1;  // uniform: every thread keeps the stack depth
         if (threadIdx.x == 0)
@@ -1193,11 +1226,11 @@ __global__ __launch_bounds__(kBT) void sahSmallKernel(SahTables T, int n, uint32
             const unsigned short* idxS = idxL[buf] + ls;
             const AxisMap m = axisMap(t);
             if (!median) {
-                histInit(h);
+                histInit<false>(h);
                 __syncthreads();
                 histAddLds<kBT>(h, recL, idxS, t.count, m);
                 __syncthreads();
-                if (threadIdx.x < 64) chooseSplit(h, t, m, &sres);
+                if (threadIdx.x < 64) chooseSplit<false>(h, t, m, &sres);
             } else if (threadIdx.x == 0) {
                 sres = medianResult(t);
             }
@@ -1272,7 +1305,7 @@ __global__ __launch_bounds__(kBT) void sahSmallKernel(SahTables T, int n, uint32
                 }
                 const unsigned short* idxD = idxL[childBuf] + ls;
                 for (int j = threadIdx.x; j < t.count; j += kBT) {
-                    const PrimRec r = recL[idxD[j]];
+                    const PrimBox r = recL[idxD[j]];
                     const bool isLeft = j < nL;
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
@@ -1327,18 +1360,18 @@ __global__ __launch_bounds__(kBT) void sahSmallKernel(SahTables T, int n, uint32
 #pragma unroll
             for (int side = 1; side >= 0; --side) {
                 if (cc[side] <= kSerial) {
-                    // the range's leaf order, one thread per position (at most kSerial)
+                    // which index buffer holds the range's final order (the leaf order
+                    // is written for the whole task at its end), one thread per position
                     if (threadIdx.x < cc[side])
-                        order[cs[side] + threadIdx.x] =
-                            static_cast<uint32_t>(recL[idxL[childBuf][cs[side] - base + threadIdx.x]].id);
+                        finalBuf[cs[side] - base + threadIdx.x] = static_cast<unsigned char>(childBuf);
                     if (threadIdx.x == 0) {
-                        if (serN == kSerialBatch) {
+                        if (serN == kSmallSerialBatch) {
                             flushSerial(T, serBuf, serN);
                             serN = 0;
                         }
                         serBuf[serN++] = {cs[side], cc[side], ci[side], kOrderWritten};
                     }
-                } else if (sp < kSubtreeStack) {
+                } else if (sp < kSmallStack) {
                     if (threadIdx.x == 0)
                         stk[sp] = childTask(cs[side], cc[side], ci[side], metaOf(childBuf, level + 1), sc, side);
                     ++sp;
@@ -1348,6 +1381,13 @@ __global__ __launch_bounds__(kBT) void sahSmallKernel(SahTables T, int n, uint32
             }
             __syncthreads();
         }
+        // the task's leaf order (scene ids from the record buffer), all positions at
+        // once: the reads overlap instead of one dependent read per serial range
+        for (int i = threadIdx.x; i < root.count; i += kBT) {
+            const int fb = finalBuf[i];
+            if (fb != 0xFF) order[base + i] = static_cast<uint32_t>(src[idxL[fb][i]].id);
+        }
+        __syncthreads();
     }
     if (threadIdx.x == 0) {
         flushSerial(T, serBuf, serN);
