@@ -10,9 +10,9 @@
 // by walking parent chains, depth-sorts the nodes and launches one merge kernel
 // per level; here the sort runs 4 passes and the Karras-numbered tree is built
 // agglomeratively in a single bottom-up pass where the second child to arrive at
-// a split merges it.  The two-pass form (karrasNode, then a bottom-up climb over
-// the given parent[]) stays for the batched build and for boxes over a topology
-// that is already known (SAH, refits).
+// a split merges it; the batched build runs the same window and hand-up passes per
+// scene.  A bottom-up climb over a given parent[] (leafUpLocalKernel) serves boxes
+// over a topology that is already known (SAH, refits).
 #include "qr_bvh.h"
 
 namespace qr {
@@ -297,107 +297,7 @@ __device__ __forceinline__ int prefixLen(const uint32_t* __restrict__ code,
     return x == 0u ? 64 : 32 + __clz(x);
 }
 
-// Karras node i of one tree whose sorted keys are (code[j] << 32 | idx[j]), j < n.
-// Node ids, children and parent[] are local to the tree (root 0, leaf n-1+slot).
-__device__ __forceinline__ void karrasNode(const uint32_t* __restrict__ code,
-                                           const uint32_t* __restrict__ idx, int n, int i,
-                                           Node2* __restrict__ nodes, int* __restrict__ parent,
-                                           int* __restrict__ arrive, int* __restrict__ farEnd) {
-    arrive[i] = 0;
-    if (i == 0) parent[0] = -1;
-
-    const int dir = prefixLen(code, idx, n, i, i + 1) > prefixLen(code, idx, n, i, i - 1) ? 1 : -1;
-    const int dmin = prefixLen(code, idx, n, i, i - dir);
-
-    // Exponential search for an upper bound on the range length, then binary
-    // search for the far end j.
-    int span = 2;
-    while (prefixLen(code, idx, n, i, i + span * dir) > dmin) span <<= 1;
-    int len = 0;
-    for (int step = span >> 1; step > 0; step >>= 1) {
-        if (prefixLen(code, idx, n, i, i + (len + step) * dir) > dmin) len += step;
-    }
-    const int j = i + len * dir;
-    if (farEnd) farEnd[i] = j;  // node i covers leaf slots [min(i, j), max(i, j)]
-    const int first = i < j ? i : j;
-    const int last = i < j ? j : i;
-
-    // Split: the last position whose key still shares more than the node's common
-    // prefix with the first key of the range.
-    const int dnode = prefixLen(code, idx, n, i, j);
-    int lo = first, hi = last;  // invariant: prefixLen(first, lo) > dnode
-    while (hi - lo > 1) {
-        const int mid = lo + ((hi - lo) >> 1);
-        if (prefixLen(code, idx, n, first, mid) > dnode) {
-            lo = mid;
-        } else {
-            hi = mid;
-        }
-    }
-    const int split = lo;
-
-    const int left = (split == first) ? (n - 1 + split) : split;
-    const int right = (split + 1 == last) ? (n - 1 + last) : split + 1;
-    nodes[i].left = left;
-    nodes[i].right = right;
-    parent[left] = i;
-    parent[right] = i;
-}
-
-// ---- leaf scatter + bottom-up propagation -------------------------------------------
-// One thread per leaf slot k: gather the item into leaf order, write leaf node
-// n-1+k, then climb.  At each parent the arrival counter decides: the first child
-// to arrive stops, the second merges both child boxes (left operand = left child,
-// as the reference's refit does), resets the counter and continues.  Height
-// (edges to the deepest leaf) rides along, so the root's height is the tree
-// depth.  Counters start at zero and end at zero.
-template <typename L, bool kGather>
-__device__ __forceinline__ void leafClimb(const typename L::Item* __restrict__ src,
-                                          const uint32_t* __restrict__ order, int n, int k,
-                                          typename L::Item* __restrict__ sorted,
-                                          int* __restrict__ primIds, Node2* __restrict__ nodes,
-                                          const int* __restrict__ parent, int* __restrict__ arrive,
-                                          int* __restrict__ height) {
-    typename L::Item item;
-    if (kGather) {
-        const int sceneIdx = static_cast<int>(order[k]);
-        item = src[sceneIdx];
-        sorted[k] = item;
-        primIds[k] = sceneIdx;
-    } else {
-        item = sorted[k];
-    }
-    Box box = L::bounds(item);
-    Node2* leaf = &nodes[n - 1 + k];
-    writeBoxAgent(leaf, box);
-    leaf->left = -(k + 1);
-    leaf->right = 1;
-
-    int node = n - 1 + k;
-    int h = 0;
-    while (true) {
-        const int p = parent[node];
-        if (p < 0) break;
-        storesDone();
-        const int prior = __hip_atomic_fetch_add(&arrive[p], 1, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-        if (prior == 0) break;
-        const int l = nodes[p].left;
-        const int r = nodes[p].right;
-        const int sib = (l == node) ? r : l;
-        const Box sb = readBoxAgent(&nodes[sib]);
-        const int sh = sib >= n - 1 ? 0 : ldAgentI(&height[sib]);
-        box = (l == node) ? mergeChildren(box, sb) : mergeChildren(sb, box);
-        h = (h > sh ? h : sh) + 1;
-        writeBoxAgent(&nodes[p], box);
-        stAgentI(&height[p], h);
-        // Both children have arrived: return the counter to zero so every arrival
-        // counter is zero between kernels (the dirty refit relies on it).
-        stAgentI(&arrive[p], 0);
-        node = p;
-    }
-}
-
+// ---- bottom-up boxes over a given topology --------------------------------------------
 // Bottom-up boxes with the block-local part in LDS.  Workgroup b owns leaf slots
 // [256 b, 256 b + 255] = [base, last].  An internal node p whose leaf range
 // [min(p, far[p]), max(p, far[p])] lies inside that window has both subtrees in this
@@ -407,11 +307,13 @@ __device__ __forceinline__ void leafClimb(const typename L::Item* __restrict__ s
 // can be local -- parent[], far[], children of ids base..last, parents of the
 // window's leaves -- is read into LDS up front with coalesced loads and the local
 // climb makes no dependent global loads.  The first node whose range leaves the
-// window is published with agent-scope stores and from there on the climb follows
-// the global protocol of leafClimb.  Boxes are merged in the same order with the
-// same operands, so the node array is identical to leafClimb's.
+// window is published with agent-scope stores and from there on the climb takes
+// device-wide arrival counters: the first child to arrive stops, the second merges
+// both boxes (left operand = left child, as the reference's refit does), returns the
+// counter to zero and continues, with the height riding along.  Every node is merged
+// from the same operands in the same order as by a climb from each leaf.
 // One workgroup's window of leaf slots [base, base + kBlock) of a tree of n leaves
-// (the whole body of leafUpLocalKernel; the batched build runs it per scene).
+// (the whole body of leafUpLocalKernel).
 template <typename L, bool kGather>
 __device__ __forceinline__ void leafUpWindow(
     const typename L::Item* __restrict__ src, const uint32_t* __restrict__ order, int n,
@@ -562,7 +464,7 @@ __device__ __forceinline__ int adjPrefix(const uint32_t* __restrict__ code,
 // are never equal).  As a left child its parent splits at r, as a right child at l-1.
 // Karras numbering names an internal node by the end of its range that faces its
 // parent's split: a left child is node r, a right child node l, the root node 0
-// (karrasNode above produces the same ids).
+// (Karras 2012's numbering).
 struct Attach {
     int split;  // parent's split position (keys split..split+1 straddle it)
     bool left;  // this node is the parent's left child
@@ -690,6 +592,10 @@ __device__ void globalClimb(NodeState s, const uint32_t* __restrict__ code,
 // count of -1, which sends the whole group, and every group above it, the same
 // way -- so the two protocols never meet at one split.
 constexpr int kMergeUnits = 32;
+// The batched build merges 8 units per group (256 threads, 25 KB of LDS: six groups per
+// CU, where the thousands of small scenes' groups are latency bound) at the cost of a
+// level more for its largest scenes.
+constexpr int kBatchMergeUnits = 8;
 constexpr int kFrontierCap = 32;
 constexpr int kMergeBlock = kMergeUnits * kFrontierCap;  // one thread per entry
 constexpr int kEmitMaxLevels = 10;
@@ -709,7 +615,7 @@ int gFrontierCap = kFrontierCap;  // qr_set_frontier_cap
 // parent's split from two adjacent-key prefixes, and the split slot's arrival hands
 // the second child the far end of its sibling, from which both child ids, the
 // parent range and the parent's Karras id follow.  The result is byte-identical to
-// karrasNode + leafUpLocalKernel (same ids, parent[], farEnd[], boxes merged with
+// a Karras pass (Karras 2012) + leafUpLocalKernel (same ids, parent[], farEnd[], boxes merged with
 // the left child's box as the left operand, heights).
 //
 // Workgroup b owns leaf slots [256 b, 256 b + 255] = [base, last].  Split slots
@@ -718,13 +624,15 @@ int gFrontierCap = kFrontierCap;  // qr_set_frontier_cap
 // node whose parent splits outside the window, or the first child at an LDS slot
 // whose sibling was not finished here (known after the workgroup barrier), is on the
 // window's frontier and is handed up.
+// One window (win, of kBlock leaf slots) of a tree of n leaves; its frontier goes to
+// stageUnit / *countUnit; single: the window holds the whole tree.
 template <typename L>
-__global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
+__device__ __forceinline__ void emitWindow(
     const typename L::Item* __restrict__ src, const uint32_t* __restrict__ code,
     const uint32_t* __restrict__ order, int n, typename L::Item* __restrict__ sorted,
     int* __restrict__ primIds, Node2* __restrict__ nodes, int* __restrict__ parent,
     int* __restrict__ arrive, int* __restrict__ height, int* __restrict__ farEnd,
-    FrontierEntry* __restrict__ stage, int* __restrict__ counts, int frontierCap) {
+    FrontierEntry* __restrict__ stageUnit, int* __restrict__ countUnit, int frontierCap, int win, bool single) {
     __shared__ float sLeaf[kBlock][6];
     __shared__ float sNode[kBlock][6];
     __shared__ int sSlot[kBlock];
@@ -733,7 +641,7 @@ __global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
     __shared__ int sMark[kBlock];
     __shared__ int sWave[kBlock / 64 + 1];
     const int t = threadIdx.x;
-    const int base = blockIdx.x * kBlock;
+    const int base = win * kBlock;
     const int last = (base + kBlock < n ? base + kBlock : n) - 1;
     const int k = base + t;
     const bool live = k < n;
@@ -741,7 +649,7 @@ __global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
     sDelta[t] = adjPrefix(code, order, n, base - 1 + t);
     if (t == 0) {
         sDelta[kBlock] = adjPrefix(code, order, n, base + kBlock - 1);
-        if (blockIdx.x == 0) parent[0] = -1;
+        if (win == 0) parent[0] = -1;
     }
 
     NodeState s{k, k, n - 1 + k, 0, -1, -1, {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}}};
@@ -841,7 +749,7 @@ __global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
     int pos = pend ? s.l - base : 0;  // position of the entry inside its unit
 
     // ---- hand-up: stage the window's frontier in leaf order ----------------------------
-    if (gridDim.x == 1) {  // one window holds the whole tree: the root is made
+    if (single) {  // one window holds the whole tree: the root is made
         if (pend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
         return;
     }
@@ -849,11 +757,23 @@ __global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
     const int total = sWave[kBlock / 64];
     if (total > frontierCap) {
         if (pend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
-        if (t == 0) counts[blockIdx.x] = -1;
+        if (t == 0) *countUnit = -1;
     } else {
-        if (pend) stage[blockIdx.x * kFrontierCap + rank] = toEntry(s);
-        if (t == 0) counts[blockIdx.x] = total;
+        if (pend) stageUnit[rank] = toEntry(s);
+        if (t == 0) *countUnit = total;
     }
+}
+
+template <typename L>
+__global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
+    const typename L::Item* __restrict__ src, const uint32_t* __restrict__ code,
+    const uint32_t* __restrict__ order, int n, typename L::Item* __restrict__ sorted,
+    int* __restrict__ primIds, Node2* __restrict__ nodes, int* __restrict__ parent,
+    int* __restrict__ arrive, int* __restrict__ height, int* __restrict__ farEnd,
+    FrontierEntry* __restrict__ stage, int* __restrict__ counts, int frontierCap) {
+    emitWindow<L>(src, code, order, n, sorted, primIds, nodes, parent, arrive, height, farEnd,
+                  stage + static_cast<size_t>(blockIdx.x) * kFrontierCap, counts + blockIdx.x, frontierCap,
+                  static_cast<int>(blockIdx.x), gridDim.x == 1);
 }
 
 // Level L >= 1 of the hand-up: workgroup g merges the frontiers of units
@@ -873,9 +793,6 @@ __global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
 // scan of 8 prefixes per LDS read; the boxes then go up the formed forest by
 // arrival counters, each merge taking the left child's box as the left operand,
 // so every node is byte-identical to the one-node-at-a-time climb.
-constexpr int kMergeNodes = 2 * kMergeBlock;  // entries 0..count-1, slot j at kMergeBlock + j
-constexpr int kPrefixWords = (kMergeBlock + 8) / 8 + 1;
-constexpr int kPrefixMinWords = (kPrefixWords + 7) / 8 + 1;
 
 // High bit of each byte of w whose value is below x (bytes and x in [0, 127], x >= 1).
 __device__ __forceinline__ uint64_t bytesBelow(uint64_t w, uint32_t x) {
@@ -934,31 +851,36 @@ __device__ __forceinline__ int firstBelow(const uint64_t* v, const uint64_t* vm,
     return i <= last ? i : -1;
 }
 
-__global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
-    const FrontierEntry* __restrict__ inStage, const int* __restrict__ inCounts, int inUnits,
-    FrontierEntry* __restrict__ outStage, int* __restrict__ outCounts, bool top,
+// One group: the frontiers of `members` consecutive units (their staging from inStage,
+// kFrontierCap entries each, their counts from inCounts) merged; the group's own
+// frontier goes to outStageUnit / *outCountUnit unless top (the group makes the root).
+template <int kUnits>
+__device__ __forceinline__ void mergeGroup(
+    const FrontierEntry* __restrict__ inStage, const int* __restrict__ inCounts, int members,
+    FrontierEntry* __restrict__ outStageUnit, int* __restrict__ outCountUnit, bool top,
     const uint32_t* __restrict__ code, const uint32_t* __restrict__ order, int n,
     Node2* __restrict__ nodes, int* __restrict__ parent, int* __restrict__ arrive,
     int* __restrict__ height, int* __restrict__ farEnd, int frontierCap) {
-    __shared__ float sBox[kMergeNodes][6];
-    __shared__ int sH[kMergeNodes];
-    __shared__ int sId[kMergeNodes];
-    __shared__ int sLo[kMergeBlock], sHi[kMergeBlock];  // leaf range of entry e
-    __shared__ uint64_t sPre[kPrefixWords];  // byte i: prefix + 1 at boundary i - 1 (boundary -1 = left edge)
-    __shared__ int4 sSlot[kMergeBlock];      // formed slot: parent slot (-1: none here), child 0, child 1, formed
-    __shared__ int sArrive[kMergeBlock];
-    __shared__ int sMark[kMergeBlock];
-    __shared__ int sWave[kMergeBlock / 64 + 1];
-    __shared__ uint64_t sPreMin[kPrefixMinWords];  // byte w: the smallest byte of sPre[w]
-    __shared__ int sOff[kMergeUnits + 1];
-    __shared__ unsigned char sMember[kMergeBlock];  // unit of entry e
+    constexpr int kThreads = kUnits * kFrontierCap;  // one thread per entry; LDS node j: entry j,
+                                                     // slot j at kThreads + j
+    constexpr int kPreWords = (kThreads + 8) / 8 + 1;
+    constexpr int kPreMinWords = (kPreWords + 7) / 8 + 1;
+    __shared__ float sBox[2 * kThreads][6];
+    __shared__ int sH[2 * kThreads];
+    __shared__ int sId[2 * kThreads];
+    __shared__ int sLo[kThreads], sHi[kThreads];  // leaf range of entry e
+    __shared__ uint64_t sPre[kPreWords];  // byte i: prefix + 1 at boundary i - 1 (boundary -1 = left edge)
+    __shared__ int4 sSlot[kThreads];      // formed slot: parent slot (-1: none here), child 0, child 1, formed
+    __shared__ int sArrive[kThreads];
+    __shared__ int sMark[kThreads];
+    __shared__ int sWave[kThreads / 64 + 1];
+    __shared__ uint64_t sPreMin[kPreMinWords];  // byte w: the smallest byte of sPre[w]
+    __shared__ int sOff[kUnits + 1];
+    __shared__ unsigned char sMember[kThreads];  // unit of entry e
     __shared__ int sBroken;
     const int t = threadIdx.x;
-    const int g = blockIdx.x;
-    const int first = g * kMergeUnits;
-    const int members = inUnits - first < kMergeUnits ? inUnits - first : kMergeUnits;
     if (t < 64) {  // wave 0: unit counts, their offsets, each entry's unit, the broken flag
-        const int raw = t < members ? inCounts[first + t] : 0;
+        const int raw = t < members ? inCounts[t] : 0;
         const int own = raw < 0 ? 0 : (raw > kFrontierCap ? kFrontierCap : raw);
         int c = own;
         for (int d = 1; d < 64; d <<= 1) {
@@ -980,18 +902,18 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
     const bool entry = t < count;
     if (entry) {
         const int m = sMember[t];
-        s = fromEntry(inStage[(first + m) * kFrontierCap + (t - sOff[m])]);
+        s = fromEntry(inStage[m * kFrontierCap + (t - sOff[m])]);
     }
     if (broken) {  // a unit below went global: so does the whole group
         if (entry) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
-        if (t == 0 && !top) outCounts[g] = -1;
+        if (t == 0 && !top) *outCountUnit = -1;
         return;
     }
 
     // ---- entries and boundary prefixes into LDS ------------------------------------------
     auto* pre = reinterpret_cast<unsigned char*>(sPre);
-    if (t < kPrefixWords) sPre[t] = 0x7F7F7F7F7F7F7F7Full;  // padding: never below any prefix
-    if (t < kPrefixMinWords) sPreMin[t] = 0x7F7F7F7F7F7F7F7Full;
+    if (t < kPreWords) sPre[t] = 0x7F7F7F7F7F7F7F7Full;  // padding: never below any prefix
+    if (t < kPreMinWords) sPreMin[t] = 0x7F7F7F7F7F7F7F7Full;
     sArrive[t] = 0;
     __syncthreads();
     if (entry) {
@@ -1038,7 +960,7 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
             const int lo = sLo[sa], hi = sHi[sb - 1];
             const bool left = dr > dl;
             slotRoot = lo == 0 && hi == n - 1;
-            sId[kMergeBlock + t] = slotRoot ? 0 : (left ? hi : lo);
+            sId[kThreads + t] = slotRoot ? 0 : (left ? hi : lo);
             slotParent = left ? sb - 1 : sa - 1;
             slotSide = left ? 0 : 1;
         }
@@ -1062,8 +984,8 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
         else sSlot[entryParent].z = t;
     }
     if (slotUp) {
-        if (slotSide == 0) sSlot[slotParent].y = kMergeBlock + t;
-        else sSlot[slotParent].z = kMergeBlock + t;
+        if (slotSide == 0) sSlot[slotParent].y = kThreads + t;
+        else sSlot[slotParent].z = kThreads + t;
     }
     __syncthreads();
     if (formed) sSlot[t].x = slotUp ? slotParent : -1;
@@ -1084,15 +1006,15 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
             const int hs = sH[sib];
             box = mineLeft ? mergeChildren(box, other) : mergeChildren(other, box);
             h = (h > hs ? h : hs) + 1;
-            float* d = sBox[kMergeBlock + p];
+            float* d = sBox[kThreads + p];
             d[0] = box.lo.x;
             d[1] = box.lo.y;
             d[2] = box.lo.z;
             d[3] = box.hi.x;
             d[4] = box.hi.y;
             d[5] = box.hi.z;
-            sH[kMergeBlock + p] = h;
-            me = kMergeBlock + p;
+            sH[kThreads + p] = h;
+            me = kThreads + p;
             p = tp.x;
         }
     }
@@ -1101,9 +1023,9 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
     // ---- the formed nodes to memory -----------------------------------------------------------
     const int4 tp = sSlot[t];
     if (formed && tp.y >= 0 && tp.z >= 0) {  // (both children are always linked)
-        const int pid = sId[kMergeBlock + t];
+        const int pid = sId[kThreads + t];
         const int lid = sId[tp.y], rid = sId[tp.z];
-        const float* b = sBox[kMergeBlock + t];
+        const float* b = sBox[kThreads + t];
         Node2 nd;
         nd.lo[0] = b[0];
         nd.lo[1] = b[1];
@@ -1118,15 +1040,15 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
         parent[rid] = pid;
         const int lo = sLo[sa], hi = sHi[sb - 1];
         farEnd[pid] = pid == lo ? hi : lo;
-        if (pid == 0) height[0] = sH[kMergeBlock + t];  // the tree depth
+        if (pid == 0) height[0] = sH[kThreads + t];  // the tree depth
     }
 
     // ---- hand-up: stage the group's frontier in leaf order ------------------------------------
     if (top) {  // the whole tree: every node was formed (kept for safety)
         if (entryPend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
         if (slotPend) {
-            const float* b = sBox[kMergeBlock + t];
-            const NodeState q{sLo[sa], sHi[sb - 1], sId[kMergeBlock + t], sH[kMergeBlock + t],
+            const float* b = sBox[kThreads + t];
+            const NodeState q{sLo[sa], sHi[sb - 1], sId[kThreads + t], sH[kThreads + t],
                               static_cast<int>(pre[sa]) - 1, static_cast<int>(pre[sb]) - 1,
                               {{b[0], b[1], b[2]}, {b[3], b[4], b[5]}}};
             globalClimb(q, code, order, n, nodes, parent, arrive, height, farEnd);
@@ -1148,29 +1070,45 @@ __global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
     for (int w = 0; w < (t >> 6); ++w) prefix += sWave[w];
     sMark[t] = prefix + before;
     int total = 0;
-    for (int w = 0; w < kMergeBlock / 64; ++w) total += sWave[w];
+    for (int w = 0; w < kThreads / 64; ++w) total += sWave[w];
     __syncthreads();
     if (total > frontierCap) {
         if (entryPend) globalClimb(s, code, order, n, nodes, parent, arrive, height, farEnd);
         if (slotPend) {
-            const float* b = sBox[kMergeBlock + t];
-            const NodeState q{sLo[sa], sHi[sb - 1], sId[kMergeBlock + t], sH[kMergeBlock + t],
+            const float* b = sBox[kThreads + t];
+            const NodeState q{sLo[sa], sHi[sb - 1], sId[kThreads + t], sH[kThreads + t],
                               static_cast<int>(pre[sa]) - 1, static_cast<int>(pre[sb]) - 1,
                               {{b[0], b[1], b[2]}, {b[3], b[4], b[5]}}};
             globalClimb(q, code, order, n, nodes, parent, arrive, height, farEnd);
         }
-        if (t == 0) outCounts[g] = -1;
+        if (t == 0) *outCountUnit = -1;
     } else {
-        if (entryPend) outStage[g * kFrontierCap + sMark[t]] = toEntry(s);
+        if (entryPend) outStageUnit[sMark[t]] = toEntry(s);
         if (slotPend) {
-            const float* b = sBox[kMergeBlock + t];
-            outStage[g * kFrontierCap + sMark[sa]] =
-                FrontierEntry{sLo[sa], sHi[sb - 1], sId[kMergeBlock + t], sH[kMergeBlock + t],
+            const float* b = sBox[kThreads + t];
+            outStageUnit[sMark[sa]] =
+                FrontierEntry{sLo[sa], sHi[sb - 1], sId[kThreads + t], sH[kThreads + t],
                               static_cast<int>(pre[sa]) - 1, static_cast<int>(pre[sb]) - 1,
                               {b[0], b[1], b[2], b[3], b[4], b[5]}};
         }
-        if (t == 0) outCounts[g] = total;
+        if (t == 0) *outCountUnit = total;
     }
+}
+
+// Level L >= 1 of the hand-up of one tree: workgroup g merges units
+// [g kMergeUnits, g kMergeUnits + kMergeUnits) of level L-1.
+__global__ __launch_bounds__(kMergeBlock) void lbvhMergeKernel(
+    const FrontierEntry* __restrict__ inStage, const int* __restrict__ inCounts, int inUnits,
+    FrontierEntry* __restrict__ outStage, int* __restrict__ outCounts, bool top,
+    const uint32_t* __restrict__ code, const uint32_t* __restrict__ order, int n,
+    Node2* __restrict__ nodes, int* __restrict__ parent, int* __restrict__ arrive,
+    int* __restrict__ height, int* __restrict__ farEnd, int frontierCap) {
+    const int g = static_cast<int>(blockIdx.x);
+    const int first = g * kMergeUnits;
+    const int members = inUnits - first < kMergeUnits ? inUnits - first : kMergeUnits;
+    mergeGroup<kMergeUnits>(inStage + static_cast<size_t>(first) * kFrontierCap, inCounts + first, members,
+               top ? nullptr : outStage + static_cast<size_t>(g) * kFrontierCap, top ? nullptr : outCounts + g, top,
+               code, order, n, nodes, parent, arrive, height, farEnd, frontierCap);
 }
 
 // Hand-up levels for n leaves: windows of kBlock leaves, then groups of kMergeUnits
@@ -1319,57 +1257,80 @@ __global__ void batchGatherKernel(const uint32_t* __restrict__ idx, const int* _
     if (i < n) dst[i] = static_cast<uint32_t>(src[idx[i]]);
 }
 
+// Scene-local codes and indices of the sorted order; also returns every scene's
+// arrival counters to zero (the hand-up's device-wide path expects them so).
 __global__ void batchLocalKernel(const uint32_t* __restrict__ order2, const uint32_t* __restrict__ s2,
                                  const uint32_t* __restrict__ codes, const int* __restrict__ primOff, int n,
-                                 uint32_t* __restrict__ codes2, uint32_t* __restrict__ idxLocal) {
+                                 uint32_t* __restrict__ codes2, uint32_t* __restrict__ idxLocal,
+                                 int* __restrict__ arrive) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t o = order2[i];
-    codes2[i] = codes[o];
-    idxLocal[i] = o - static_cast<uint32_t>(primOff[s2[i]]);
-}
-
-__global__ __launch_bounds__(kBlock) void batchKarrasKernel(const uint32_t* __restrict__ codes2,
-                                                            const uint32_t* __restrict__ idxLocal,
-                                                            const uint32_t* __restrict__ s2,
-                                                            const int* __restrict__ primOff, int n,
-                                                            Node2* __restrict__ nodes, int* __restrict__ parent,
-                                                            int* __restrict__ arrive, int* __restrict__ farEnd) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
     const int sc = static_cast<int>(s2[i]);
-    const int ps = primOff[sc], ns = primOff[sc + 1] - ps;
-    const int il = i - ps;
-    if (il >= ns - 1) return;
-    karrasNode(codes2 + ps, idxLocal + ps, ns, il, nodes + (2 * ps - sc), parent + (2 * ps - sc),
-               arrive + (ps - sc), farEnd + (ps - sc));
+    const int ps = primOff[sc];
+    codes2[i] = codes[o];
+    idxLocal[i] = o - static_cast<uint32_t>(ps);
+    if (i - ps < primOff[sc + 1] - ps - 1) arrive[i - sc] = 0;  // scene slot i - ps of arrive + (ps - sc)
 }
 
-// Leaf gather, leaf nodes and bottom-up boxes of every scene: workgroup w takes a
-// window of kBlock leaf slots of one scene (winOff[sc] .. winOff[sc + 1] - 1 are scene
-// sc's windows) and runs the plain build's window pass on that scene's arrays, so the
-// merges inside a window stay in LDS and only nodes leaving it take the arrival
-// counters.
+// The last scene sc with off[sc] <= w (off: S + 1 ascending offsets; a scene with an
+// empty range is passed over, so the scene found holds w).
+__device__ __forceinline__ int sceneOf(const int* __restrict__ off, int scenes, int w) {
+    int lo = 0, hi = scenes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= w) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Hierarchy, leaf gather and boxes of every scene: workgroup w runs the plain build's
+// window pass (emitWindow) on window w - winOff[sc] of scene sc, on that scene's
+// arrays; a scene of one window makes its root there, the others hand their frontiers
+// up through batchMergeKernel.
 template <typename L>
-__global__ __launch_bounds__(kBlock) void batchLeafKernel(const typename L::Item* __restrict__ items,
+__global__ __launch_bounds__(kBlock) void batchEmitKernel(const typename L::Item* __restrict__ items,
+                                                          const uint32_t* __restrict__ codes2,
                                                           const uint32_t* __restrict__ idxLocal,
                                                           const int* __restrict__ primOff,
                                                           const int* __restrict__ winOff, int scenes,
                                                           typename L::Item* __restrict__ sorted,
                                                           int* __restrict__ primIds, Node2* __restrict__ nodes,
-                                                          const int* __restrict__ parent, int* __restrict__ arrive,
-                                                          int* __restrict__ height, const int* __restrict__ farEnd) {
+                                                          int* __restrict__ parent, int* __restrict__ arrive,
+                                                          int* __restrict__ height, int* __restrict__ farEnd,
+                                                          FrontierEntry* __restrict__ stage, int* __restrict__ counts,
+                                                          int frontierCap) {
     const int w = static_cast<int>(blockIdx.x);
-    int lo = 0, hi = scenes - 1;  // the scene whose windows hold w
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (winOff[mid] <= w) lo = mid; else hi = mid - 1;
-    }
-    const int sc = lo;
+    const int sc = sceneOf(winOff, scenes, w);
     const int ps = primOff[sc], ns = primOff[sc + 1] - ps;
-    leafUpWindow<L, true>(items + ps, idxLocal + ps, ns, sorted + ps, primIds + ps, nodes + (2 * ps - sc),
-                          parent + (2 * ps - sc), arrive + (ps - sc), height + (ps - sc), farEnd + (ps - sc),
-                          (w - winOff[sc]) * kBlock);
+    emitWindow<L>(items + ps, codes2 + ps, idxLocal + ps, ns, sorted + ps, primIds + ps, nodes + (2 * ps - sc),
+                  parent + (2 * ps - sc), arrive + (ps - sc), height + (ps - sc), farEnd + (ps - sc),
+                  stage + static_cast<size_t>(w) * kFrontierCap, counts + w, frontierCap, w - winOff[sc],
+                  winOff[sc + 1] - winOff[sc] == 1);
+}
+
+// One hand-up level of every scene that still has more than one unit: workgroup g is
+// group g - grpOff[sc] of scene sc (grpOff: this level's groups, scene after scene);
+// its members are the scene's units from unitOff[sc] on, in the level below's
+// numbering; a scene's only group makes its root.
+template <int kUnits>
+__global__ __launch_bounds__(kUnits * kFrontierCap) void batchMergeKernel(
+    const FrontierEntry* __restrict__ inStage, const int* __restrict__ inCounts, const int* __restrict__ unitOff,
+    const int* __restrict__ grpOff, int scenes, FrontierEntry* __restrict__ outStage, int* __restrict__ outCounts,
+    const uint32_t* __restrict__ codes2, const uint32_t* __restrict__ idxLocal, const int* __restrict__ primOff,
+    Node2* __restrict__ nodes, int* __restrict__ parent, int* __restrict__ arrive, int* __restrict__ height,
+    int* __restrict__ farEnd, int frontierCap) {
+    const int g = static_cast<int>(blockIdx.x);
+    const int sc = sceneOf(grpOff, scenes, g);
+    const int first = (g - grpOff[sc]) * kUnits;
+    const int unitsIn = unitOff[sc + 1] - unitOff[sc];
+    const int members = unitsIn - first < kUnits ? unitsIn - first : kUnits;
+    const bool top = grpOff[sc + 1] - grpOff[sc] == 1;
+    const int ps = primOff[sc], ns = primOff[sc + 1] - ps;
+    mergeGroup<kUnits>(inStage + static_cast<size_t>(unitOff[sc] + first) * kFrontierCap, inCounts + unitOff[sc] + first,
+               members, top ? nullptr : outStage + static_cast<size_t>(g) * kFrontierCap,
+               top ? nullptr : outCounts + g, top, codes2 + ps, idxLocal + ps, ns, nodes + (2 * ps - sc),
+               parent + (2 * ps - sc), arrive + (ps - sc), height + (ps - sc), farEnd + (ps - sc), frontierCap);
 }
 
 __global__ void batchDepthKernel(const int* __restrict__ height, const int* __restrict__ primOff, int scenes,
@@ -1679,6 +1640,38 @@ static void buildBatchT(BatchState& b, const typename L::Item* dSrc, hipStream_t
         winOffH[sc + 1] = winOffH[sc] + blocksFor(b.primOffH[sc + 1] - b.primOffH[sc], kBlock);
     hipCheck(hipMemcpyAsync(winOff.p, winOffH.data(), winOffH.size() * sizeof(int), hipMemcpyHostToDevice, s),
              "batch windows");
+    // hand-up levels per scene: level 0 = the windows; at level L a scene with u > 1
+    // units of level L-1 has ceil(u / kBatchMergeUnits) groups (one: it makes the root).
+    // Offsets of every level, scene after scene, in one array; unit numbering per level.
+    std::vector<int> lvOffH;             // (S + 1) ints per level
+    std::vector<size_t> lvOff;           // where level L's offsets start in lvOffH
+    std::vector<int> lvTotal, lvUnitBase;  // units (groups) per level and their first staging unit
+    lvOff.push_back(0);
+    lvOffH.insert(lvOffH.end(), winOffH.begin(), winOffH.end());
+    lvTotal.push_back(winOffH[S]);
+    lvUnitBase.push_back(0);
+    for (int level = 1;; ++level) {
+        const int* prev = lvOffH.data() + lvOff[level - 1];
+        std::vector<int> off(static_cast<size_t>(S) + 1, 0);
+        for (int sc = 0; sc < S; ++sc) {
+            const int u = prev[sc + 1] - prev[sc];
+            off[sc + 1] = off[sc] + (u > 1 ? (u + kBatchMergeUnits - 1) / kBatchMergeUnits : 0);
+        }
+        if (off[S] == 0) break;
+        if (level >= kEmitMaxLevels) throw ArgError("batch build: too many primitives for the hand-up levels");
+        // (a scene whose only group makes its root counts one unit at this level and
+        // so has no group at the next)
+        lvOff.push_back(lvOffH.size());
+        lvOffH.insert(lvOffH.end(), off.begin(), off.end());
+        lvTotal.push_back(off[S]);
+        lvUnitBase.push_back(lvUnitBase.back() + lvTotal[lvTotal.size() - 2]);
+    }
+    DevBuf<int> lvOffDev(lvOffH.size());
+    hipCheck(hipMemcpyAsync(lvOffDev.p, lvOffH.data(), lvOffH.size() * sizeof(int), hipMemcpyHostToDevice, s),
+             "batch hand-up levels");
+    const int stageUnits = lvUnitBase.back() + lvTotal.back();
+    DevBuf<int> bstage(static_cast<size_t>(stageUnits) * kFrontierCap * (sizeof(FrontierEntry) / sizeof(int)));
+    DevBuf<int> bcounts(static_cast<size_t>(stageUnits));
 
     EventSet<5> ev;
     ev.record(0, s);
@@ -1686,15 +1679,26 @@ static void buildBatchT(BatchState& b, const typename L::Item* dSrc, hipStream_t
     batchOrderT<L>(dSrc, n, b.sceneId.p, S, b.ws, cb.p, codes.p, tmpA.p, tmpB.p, s2.p, order2.p, s);
     ev.record(1, s);
     batchLocalKernel<<<blocksFor(n, kBlock), kBlock, 0, s>>>(order2.p, s2.p, codes.p, b.primOff.p, n, codes2.p,
-                                                             idxLocal.p);
-    batchKarrasKernel<<<blocksFor(n, kBlock), kBlock, 0, s>>>(codes2.p, idxLocal.p, s2.p, b.primOff.p, n,
-                                                              b.nodes.p, b.parent.p, b.arrive.p, farEnd.p);
-    hipCheck(hipGetLastError(), "batch karras");
+                                                             idxLocal.p, b.arrive.p);
+    hipCheck(hipGetLastError(), "batch local");
     ev.record(2, s);
-    batchLeafKernel<L><<<winOffH[S], kBlock, 0, s>>>(dSrc, idxLocal.p, b.primOff.p, winOff.p, S,
+    // hierarchy, leaves and boxes: every scene's windows, then the hand-up levels
+    auto* stage = reinterpret_cast<FrontierEntry*>(bstage.p);
+    batchEmitKernel<L><<<winOffH[S], kBlock, 0, s>>>(dSrc, codes2.p, idxLocal.p, b.primOff.p, winOff.p, S,
                                                      reinterpret_cast<Item*>(b.items.p), b.primIds.p, b.nodes.p,
-                                                     b.parent.p, b.arrive.p, b.height.p, farEnd.p);
-    hipCheck(hipGetLastError(), "batch leaf up");
+                                                     b.parent.p, b.arrive.p, b.height.p, farEnd.p, stage, bcounts.p,
+                                                     gFrontierCap);
+    hipCheck(hipGetLastError(), "batch emit");
+    for (size_t level = 1; level < lvOff.size(); ++level) {
+        const int groups = lvTotal[level];
+        if (groups == 0) break;
+        batchMergeKernel<kBatchMergeUnits><<<groups, kBatchMergeUnits * kFrontierCap, 0, s>>>(
+            stage + static_cast<size_t>(lvUnitBase[level - 1]) * kFrontierCap, bcounts.p + lvUnitBase[level - 1],
+            lvOffDev.p + lvOff[level - 1], lvOffDev.p + lvOff[level], S,
+            stage + static_cast<size_t>(lvUnitBase[level]) * kFrontierCap, bcounts.p + lvUnitBase[level], codes2.p,
+            idxLocal.p, b.primOff.p, b.nodes.p, b.parent.p, b.arrive.p, b.height.p, farEnd.p, gFrontierCap);
+        hipCheck(hipGetLastError(), "batch merge");
+    }
     ev.record(3, s);
     batchDepthKernel<<<blocksFor(S, kBlock), kBlock, 0, s>>>(b.height.p, b.primOff.p, S, b.depth.p);
     std::vector<int> d(static_cast<size_t>(S));
