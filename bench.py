@@ -68,7 +68,7 @@ def main() -> int:
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus above 1 is not "
-                                   "supported (sharding is described in DESIGN.md)",
+                                   "supported (the path is measured on one GPU's share)",
                           "n_gpus": args.gpus}))
         return 2
 
