@@ -216,7 +216,14 @@ __device__ __forceinline__ void histAddRange(uint32_t* h, const PrimRec* __restr
     const int end = beg + per < count ? beg + per : count;
     BinRun run[3];
     binRunsInit(run);
-    for (int i = beg; i < end; ++i) binRunAdd<kCent>(h, run, rec[i], m);
+    if (beg < end) {  // the next record is requested before the current one is binned
+        PrimRec cur = rec[beg];
+        for (int i = beg; i < end; ++i) {
+            const PrimRec nxt = rec[i + 1 < end ? i + 1 : i];
+            binRunAdd<kCent>(h, run, cur, m);
+            cur = nxt;
+        }
+    }
     binRunsFlush<kCent>(h, run, m);
 }
 
@@ -862,7 +869,14 @@ __global__ __launch_bounds__(kBlock) void sahBinKernel(SahTables T, int cur) {
         const int end = beg + per < pCnt ? beg + per : pCnt;
         BinRun run[3];
         binRunsInit(run);
-        for (int i = beg; i < end; ++i) binRunAdd(h, run, src[i], m, part[w]);
+        if (beg < end) {  // the next record is requested before the current one is binned
+            PrimRec cur = src[beg];
+            for (int i = beg; i < end; ++i) {
+                const PrimRec nxt = src[i + 1 < end ? i + 1 : i];
+                binRunAdd(h, run, cur, m, part[w]);
+                cur = nxt;
+            }
+        }
         binRunsFlush(h, run, m, part[w]);
     }
     __syncthreads();
