@@ -776,9 +776,9 @@ __global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
                   static_cast<int>(blockIdx.x), gridDim.x == 1);
 }
 
-// Level L >= 1 of the hand-up: workgroup g merges the frontiers of units
-// [g kMergeUnits, g kMergeUnits + kMergeUnits) of level L-1 and stages its own
-// frontier as unit g of level L.
+// Level L >= 1 of the hand-up: a group (one workgroup) merges the frontiers of
+// consecutive units of level L-1 (32 for one tree, 8 in the batched build) and stages
+// its own frontier as one unit of level L.
 //
 // The group's entries partition its leaf range in leaf order, so the nodes above
 // them are fixed by the prefixes at the boundaries between consecutive entries
@@ -787,10 +787,11 @@ __global__ __launch_bounds__(kBlock) void lbvhEmitKernel(
 // side with a shorter prefix (prefixes are distinct along any range, so "nearest
 // shorter" is well defined), and it hangs below whichever of those two boundaries
 // has the longer prefix -- the attach rule of the bottom-up pass.  A node is formed
-// here when both nearest shorter boundaries exist inside the group's boundaries;
-// a formed node or entry whose parent boundary is an outer one or an unformed slot
-// is on the group's frontier.  One thread per slot finds both boundaries by a
-// scan of 8 prefixes per LDS read; the boxes then go up the formed forest by
+// here when both nearest shorter boundaries are among the group's boundaries (the
+// outer two included); a formed node or entry whose parent boundary is an outer one
+// or an unformed slot is on the group's frontier.  One thread per slot finds both
+// boundaries by scanning the prefix bytes, 8 per LDS read within its own word and 64
+// per read beyond it through a table of word minima; the boxes then go up the formed forest by
 // arrival counters, each merge taking the left child's box as the left operand,
 // so every node is byte-identical to the one-node-at-a-time climb.
 
