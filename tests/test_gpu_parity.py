@@ -367,6 +367,28 @@ def test_radius_modes_agree(A, orc, scenes, kind):
         A.check(A.lib().qr_set_traversal_mode(0))
 
 
+def test_radius_repeated_calls_on_one_query_set(A, orc, scenes):
+    """Radius calls on one resident query set compact their hits into keys sized by
+    the previous call's total before the host reads the new total: a larger radius
+    (more hits than that size, so the compaction is redone at the exact size), a
+    smaller one (fits), and one whose densest queries pass the 64-hit capture (the
+    fill walk writes into the same keys) must each give the oracle's CSR."""
+    rng = np.random.default_rng(77)
+    cloud = scenes.point_cloud(20000, 6)
+    q = rng.uniform(0.0, 64.0, (3000, 3)).astype(np.float32)
+    b = A.Bvh(1, cloud)
+    b.build_wide()
+    pts = A.PointBuffer(q)
+    try:
+        for r in (1.0, 3.0, 1.0, 6.0, 2.0):
+            log(f"radius {r} on the resident query set")
+            _, offs, keys = pts.radius(b, r, keep=True)
+            want = orc.radius_brute(1, cloud, q, r, threads=THREADS)
+            assert eq(offs, want[0]) and eq(keys, want[1]), r
+    finally:
+        pts.close()
+
+
 def test_radius_wave_walk_wide_frontier(A, orc):
     """The one-wave-per-query capture on walks whose frontier is the whole tree: a
     radius holding every point (2,048-hit capture overflows, the fill walk takes
