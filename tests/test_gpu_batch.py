@@ -42,6 +42,30 @@ def test_batch_trees_equal_plain(A, work):
         assert b.tree_hash(s) == plain.tree_hash()
 
 
+@pytest.mark.parametrize("cap", [32, 5, 0])
+def test_batch_trees_equal_plain_large_scenes(A, scenes, cap):
+    """Scenes from 3 to 70,001 primitives in one batch (one to three hand-up levels of
+    the per-scene merge, and windows of one leaf), at the default frontier capacity and
+    with units sent through the device-wide arrival path (cap 5 and 0): every tree
+    equals the plain per-scene build."""
+    rng = np.random.default_rng(23 + cap)
+    sizes = [70001, 3, 20000, 257, 9000, 513]
+    parts = [rng.uniform(-9, 9, (n, 3)).astype(np.float32) for n in sizes]
+    poff = np.zeros(len(parts) + 1, np.int32)
+    np.cumsum(sizes, out=poff[1:])
+    try:
+        A.check(A.lib().qr_set_frontier_cap(cap))
+        b = A.Batch(1, np.concatenate(parts), poff)
+        for s, part in enumerate(parts):
+            plain = A.Bvh(1, part)
+            nodes, ids = b.read_scene(s)
+            pn, pid = plain.read()
+            assert np.array_equal(nodes, pn), (cap, s)
+            assert np.array_equal(ids, pid), (cap, s)
+    finally:
+        A.check(A.lib().qr_set_frontier_cap(32))
+
+
 def _plain_results(A, items, poff, pts, qoff, k):
     outs = []
     for s in range(poff.shape[0] - 1):
