@@ -61,33 +61,34 @@ __device__ __forceinline__ Box readBoxAgent(const Node2* nd) {
 }
 
 // ---- centroid bounds -----------------------------------------------------------
-// Per copy c < kBoundsCopies: cb[c * kBoundsStride + 0..2] = ordered(min),
-// [3..5] = ordered(max) (src/gpu/lbvh.cu:233-275).  Workgroup w reduces into copy
-// w % kBoundsCopies (one 128 B line each), which spreads the same-address atomics;
-// the Morton pass folds the copies.
-constexpr int kBoundsCopies = 8;
-constexpr int kBoundsStride = 32;
-constexpr int kBoundsWords = kBoundsCopies * kBoundsStride;
+// Each bounds workgroup w writes its partial bounds to part[w * kPartStride + 0..5]
+// (ordered bits: min xyz, max xyz; src/gpu/lbvh.cu:233-275 takes the same min / max
+// with atomics); the Morton pass reduces the partials, so nothing needs
+// initialising and no same-address atomics serialize.  The bounds pass also zeroes
+// what the rest of the build expects zero (sort histograms, look-back descriptors and
+// tickets, arrival counters), so the build needs no preparation launch.
+constexpr int kPartStride = 8;
+constexpr int kMaxBoundsBlocks = 1024;
 constexpr int kUnroll = 4;  // items in flight per thread in the streaming passes
 
-__global__ void buildPrepKernel(uint32_t* __restrict__ cb, uint32_t* __restrict__ hist,
-                                uint32_t* __restrict__ desc, long long descWords,
-                                uint32_t* __restrict__ tickets, int* __restrict__ arrive,
-                                int slots) {
-    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
-    for (long long j = i; j < descWords; j += stride) desc[j] = 0u;
-    for (long long j = i; j < slots; j += stride) arrive[j] = 0;
-    for (long long j = i; j < kHistCopies * kHistCopyWords; j += stride) hist[j] = 0u;
-    if (i < 4) tickets[i] = 0u;
-    if (i < kBoundsWords) {
-        const int c = static_cast<int>(i % kBoundsStride);
-        cb[i] = c < 3 ? orderedBits(kInf) : orderedBits(-kInf);
-    }
+struct BuildZero {
+    uint32_t* hist;
+    uint32_t* desc;
+    long long descWords;
+    uint32_t* tickets;
+    int* arrive;
+    int slots;
+};
+__device__ __forceinline__ void zeroForBuild(const BuildZero& z) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long j = i; j < z.descWords; j += stride) z.desc[j] = 0u;
+    for (long long j = i; j < z.slots; j += stride) z.arrive[j] = 0;
+    for (long long j = i; j < kHistCopies * kHistCopyWords; j += stride) z.hist[j] = 0u;
+    if (i < 4) z.tickets[i] = 0u;
 }
 
-// Workgroup reduction of the per-thread centroid bounds and the flush into copy
-// blockIdx.x % kBoundsCopies.
+// Workgroup reduction of the per-thread centroid bounds into this workgroup's partial.
 __device__ __forceinline__ void boundsFlush(float lo0, float lo1, float lo2, float hi0, float hi1, float hi2,
                                             uint32_t* __restrict__ cb) {
 #pragma unroll
@@ -114,18 +115,14 @@ __device__ __forceinline__ void boundsFlush(float lo0, float lo1, float lo2, flo
         const int c = threadIdx.x;
         float v = s[c][0];
         for (int k = 1; k < kBlock / 64; ++k) v = c < 3 ? fminf(v, s[c][k]) : fmaxf(v, s[c][k]);
-        uint32_t* dst = cb + (blockIdx.x % kBoundsCopies) * kBoundsStride;
-        if (c < 3) {
-            atomicMin(&dst[c], orderedBits(v));
-        } else {
-            atomicMax(&dst[c], orderedBits(v));
-        }
+        cb[blockIdx.x * kPartStride + c] = orderedBits(v);
     }
 }
 
 template <typename L>
 __global__ __launch_bounds__(kBlock) void centroidBoundsKernel(
-    const typename L::Item* __restrict__ items, int n, uint32_t* __restrict__ cb) {
+    const typename L::Item* __restrict__ items, int n, uint32_t* __restrict__ cb, BuildZero z) {
+    zeroForBuild(z);
     float lo0 = kInf, lo1 = kInf, lo2 = kInf;
     float hi0 = -kInf, hi1 = -kInf, hi2 = -kInf;
     const int stride = kBlock * gridDim.x;
@@ -182,17 +179,36 @@ struct MortonFrame {
                (spread10(quantize1023(c.z, lo2, ex2)) << 2);
     }
 };
-__device__ __forceinline__ MortonFrame mortonFrame(const uint32_t* __restrict__ cb) {
-    uint32_t b[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) b[c] = cb[c];
-#pragma unroll
-    for (int k = 1; k < kBoundsCopies; ++k) {
+// The frame from the bounds partials of `parts` workgroups, reduced by the whole
+// workgroup (every thread returns it).
+__device__ __forceinline__ MortonFrame mortonFrame(const uint32_t* __restrict__ cb, int parts) {
+    __shared__ uint32_t sb[kBlock / 64][6];
+    uint32_t b[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    for (int w = threadIdx.x; w < parts; w += blockDim.x) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            const uint32_t v = cb[k * kBoundsStride + c];
+            const uint32_t v = cb[w * kPartStride + c];
             b[c] = c < 3 ? (v < b[c] ? v : b[c]) : (v > b[c] ? v : b[c]);
         }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        b[c] = reduce64<MinU>(b[c]);
+        b[3 + c] = reduce64<MaxU>(b[3 + c]);
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) sb[threadIdx.x >> 6][c] = b[c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        uint32_t x = sb[0][c];
+        for (int w = 1; w < static_cast<int>(blockDim.x) / 64; ++w) {
+            const uint32_t v = sb[w][c];
+            x = c < 3 ? (v < x ? v : x) : (v > x ? v : x);
+        }
+        b[c] = x;
     }
     const float lo0 = fromOrderedBits(b[0]), lo1 = fromOrderedBits(b[1]), lo2 = fromOrderedBits(b[2]);
     return {lo0, lo1, lo2, fromOrderedBits(b[3]) - lo0, fromOrderedBits(b[4]) - lo1, fromOrderedBits(b[5]) - lo2};
@@ -200,12 +216,12 @@ __device__ __forceinline__ MortonFrame mortonFrame(const uint32_t* __restrict__ 
 
 template <typename L>
 __global__ __launch_bounds__(kBlock) void mortonHistKernel(const typename L::Item* __restrict__ items,
-                                                           int n, const uint32_t* __restrict__ cb,
+                                                           int n, const uint32_t* __restrict__ cb, int parts,
                                                            uint32_t* __restrict__ codes,
                                                            uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[4][256];
     for (int i = threadIdx.x; i < 4 * 256; i += kBlock) (&h[0][0])[i] = 0u;
-    const MortonFrame f = mortonFrame(cb);
+    const MortonFrame f = mortonFrame(cb, parts);
     __syncthreads();
     auto emit = [&](int i, const typename L::Item& item) {
         const uint32_t code = f.code(L::centroid(item));
@@ -256,7 +272,8 @@ __device__ __forceinline__ void stageTris(const Tri* __restrict__ tris, long lon
 }
 
 __global__ __launch_bounds__(kBlock) void triBoundsStagedKernel(const Tri* __restrict__ tris, int n,
-                                                                uint32_t* __restrict__ cb) {
+                                                                uint32_t* __restrict__ cb, BuildZero z) {
+    zeroForBuild(z);
     __shared__ uint4 sStage[kStageTris * 36 / 16];
     float lo0 = kInf, lo1 = kInf, lo2 = kInf;
     float hi0 = -kInf, hi1 = -kInf, hi2 = -kInf;
@@ -1357,7 +1374,7 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
     b.height.reserve(static_cast<size_t>(n - 1));
     b.farEnd.reserve(static_cast<size_t>(n - 1));
     b.sortWs.reserve(static_cast<size_t>(n));
-    b.counters.reserve(kBoundsWords);
+    b.counters.reserve(static_cast<size_t>(kMaxBoundsBlocks) * kPartStride);
     int stageEntries = 0, countInts = 0;
     const EmitLevels lv = emitLevels(n, &stageEntries, &countInts);
     b.emitStage.reserve(static_cast<size_t>(stageEntries) * (sizeof(FrontierEntry) / sizeof(int)));
@@ -1376,28 +1393,25 @@ static void buildLbvhT(BvhState& b, const typename L::Item* dSrc, int n, hipStre
 
     const int tiles = sortTiles(static_cast<size_t>(n));
     const long long descWords = 4LL * tiles * 256;
-    const long long prepWords = descWords > n ? descWords : n;
-    int prepGrid = blocksFor(prepWords > 1024 ? prepWords : 1024, kBlock);
-    if (prepGrid > 1024) prepGrid = 1024;
-    buildPrepKernel<<<prepGrid, kBlock, 0, s>>>(cb, b.sortWs.hist.p, b.sortWs.desc.p, descWords,
-                                                b.sortWs.tickets.p, b.arrive.p, n - 1);
+    const BuildZero zero{b.sortWs.hist.p, b.sortWs.desc.p, descWords, b.sortWs.tickets.p, b.arrive.p, n - 1};
     // two workgroups per CU, 16 items per thread: enough loads in flight to stream,
-    // few enough workgroups that the bounds / histogram atomics stay cheap
+    // few enough workgroups that the histogram atomics stay cheap
     int grid = blocksFor(n, kBlock * 4 * kUnroll);
     if (grid > 512) grid = 512;
+    int parts = grid;
     if (L::kKind == 0 && (reinterpret_cast<uintptr_t>(dSrc) & 15) == 0) {
         // staged triangle bounds: four workgroups per CU (36 KB of LDS each); 2M
         // triangles 14.4 -> 13.3 us
-        int sgrid = blocksFor(n, kStageTris);
-        if (sgrid > 1024) sgrid = 1024;
+        parts = blocksFor(n, kStageTris);
+        if (parts > kMaxBoundsBlocks) parts = kMaxBoundsBlocks;
         const Tri* tris = reinterpret_cast<const Tri*>(dSrc);
-        triBoundsStagedKernel<<<sgrid, kBlock, 0, s>>>(tris, n, cb);
+        triBoundsStagedKernel<<<parts, kBlock, 0, s>>>(tris, n, cb, zero);
     } else {
-        centroidBoundsKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb);
+        centroidBoundsKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb, zero);
     }
     // (a staged variant of this pass measured slower: 21.4 against 18.2 us on 2M
     // triangles; the codes are written per item, so the lane-per-item loads suffice)
-    mortonHistKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb, codes, b.sortWs.hist.p);
+    mortonHistKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, cb, parts, codes, b.sortWs.hist.p);
     hipCheck(hipGetLastError(), "morton");
     ev.record(1, s);
 
