@@ -93,19 +93,27 @@ def main() -> int:
     dtris = A.DeviceArray(tris)
     builds, wides = [], []
     bvh = None
-    for i in range(5 if not args.profile else 2):
-        if bvh is not None:
-            bvh.close()
-        bvh = A.Bvh.from_device(A.PRIM_TRI, dtris.ptr.value, n)
-        builds.append(bvh.timings)
-        # the collapse of each build: the first one in the process also loads its
-        # kernels, so the median is the steady-state figure, as for the build
-        wides.append(bvh.build_wide())
+    # The timed builds record no phase events (each event between two kernels idles
+    # the device for ~12 us); one more build with them gives the phase split.
+    A.check(A.lib().qr_set_build_phase_timing(0))
+    try:
+        for i in range(5 if not args.profile else 2):
+            if bvh is not None:
+                bvh.close()
+            bvh = A.Bvh.from_device(A.PRIM_TRI, dtris.ptr.value, n)
+            builds.append(bvh.timings)
+            # the collapse of each build: the first one in the process also loads its
+            # kernels, so the median is the steady-state figure, as for the build
+            wides.append(bvh.build_wide())
+    finally:
+        A.check(A.lib().qr_set_build_phase_timing(1))
     build_ms = statistics.median(t.total_ms for t in builds)
-    bt = sorted(builds, key=lambda t: t.total_ms)[len(builds) // 2]
+    probe = A.Bvh.from_device(A.PRIM_TRI, dtris.ptr.value, n)
+    bt = probe.timings
+    probe.close()
     wide_ms = statistics.median(wides)
     info = bvh.info()
-    log(f"build: {build_ms:.3f} ms (morton {bt.morton_ms:.3f}, sort {bt.sort_ms:.3f}, "
+    log(f"build: {build_ms:.3f} ms (with phase events {bt.total_ms:.3f}: morton {bt.morton_ms:.3f}, sort {bt.sort_ms:.3f}, "
         f"tree {bt.tree_ms:.3f}, leaves+boxes {bt.refit_ms:.3f}); depth {info.depth}; "
         f"wide collapse {wide_ms:.3f} ms ({info.wide_count} nodes)")
 
@@ -119,34 +127,41 @@ def main() -> int:
     # the first SAH build also sizes its workspaces; the median of the next five is
     # the figure (as for the LBVH build)
     sah_builds = []
-    for _ in range(5 if not args.profile else 1):
-        sah.close()
-        sah = A.Bvh.from_device(A.PRIM_TRI, dtris.ptr.value, n, A.BUILDER_SAH)
-        sah_builds.append(sah.timings)
+    A.check(A.lib().qr_set_build_phase_timing(0))
+    try:
+        for _ in range(5 if not args.profile else 1):
+            sah.close()
+            sah = A.Bvh.from_device(A.PRIM_TRI, dtris.ptr.value, n, A.BUILDER_SAH)
+            sah_builds.append(sah.timings)
+    finally:
+        A.check(A.lib().qr_set_build_phase_timing(1))
     sah_build_ms = statistics.median(t.total_ms for t in sah_builds)
     sah_work = A.last_sah_stats()
     sah_wide_ms = sah.build_wide()
-    st = sorted(sah_builds, key=lambda t: t.total_ms)[len(sah_builds) // 2]
-    sah_levels_ms = st.tree_ms
-    # The phase split of the level loop comes from one more build with the per-level
-    # phase events on (they cost about 0.2 ms, so the timed build runs without them).
-    A.check(A.lib().qr_set_build_phase_timing(1))
+    # the phase split from one build with the phase events, and the split of the level
+    # loop from one more with the per-level events too (those cost about 0.2 ms)
+    probe = A.Bvh.from_device(A.PRIM_TRI, dtris.ptr.value, n, A.BUILDER_SAH)
+    sst = probe.timings
+    probe.close()
+    sah_levels_ms = sst.tree_ms
+    sah_split_total_ms = sst.total_ms
+    A.check(A.lib().qr_set_build_phase_timing(2))
     try:
         probe = A.Bvh.from_device(A.PRIM_TRI, dtris.ptr.value, n, A.BUILDER_SAH)
         pt = probe.timings
         probe.close()
     finally:
-        A.check(A.lib().qr_set_build_phase_timing(0))
+        A.check(A.lib().qr_set_build_phase_timing(1))
     # levels: the large tasks' level loop (binning, plane choice, partition + child
     # routing) and the local subtrees + serial tails (scatter_ms)
-    sah_parts = {"prep_ms": round(st.morton_ms, 4), "levels_ms": round(st.tree_ms, 4),
-                 "subtrees_ms": round(st.scatter_ms, 4), "leaves_boxes_ms": round(st.refit_ms, 4),
-                 "with_phase_events": {"total_ms": round(pt.total_ms, 4), "bin_ms": round(pt.sah_bin_ms, 4),
+    sah_parts = {"prep_ms": round(sst.morton_ms, 4), "levels_ms": round(sst.tree_ms, 4),
+                 "subtrees_ms": round(sst.scatter_ms, 4), "leaves_boxes_ms": round(sst.refit_ms, 4),
+                 "with_level_events": {"total_ms": round(pt.total_ms, 4), "bin_ms": round(pt.sah_bin_ms, 4),
                                        "split_ms": round(pt.sah_split_ms, 4),
                                        "partition_ms": round(pt.sah_part_ms, 4)}}
-    log(f"SAH build: {sah_build_ms:.3f} ms (prep {st.morton_ms:.3f}, levels {st.tree_ms:.3f} incl. local subtrees "
-        f"{st.scatter_ms:.3f}; leaves+boxes {st.refit_ms:.3f}); depth {sah.info().depth}; "
-        f"wide collapse {sah_wide_ms:.3f} ms; with phase events {pt.total_ms:.3f} ms: large tasks bin "
+    log(f"SAH build: {sah_build_ms:.3f} ms (with phase events {sst.total_ms:.3f}: prep {sst.morton_ms:.3f}, "
+        f"levels {sst.tree_ms:.3f} incl. local subtrees {sst.scatter_ms:.3f}; leaves+boxes {sst.refit_ms:.3f}); depth {sah.info().depth}; "
+        f"wide collapse {sah_wide_ms:.3f} ms; with per-level events {pt.total_ms:.3f} ms: large tasks bin "
         f"{pt.sah_bin_ms:.3f}, split {pt.sah_split_ms:.3f}, partition {pt.sah_part_ms:.3f}")
     serve = sah if args.builder == "sah" else lbvh
 
@@ -281,10 +296,12 @@ def main() -> int:
                 "serial_slots": int(sah_work.serial_slots),
                 "record_bytes": int(sah_work.binned_slots * 40 + sah_work.partitioned_slots * 80),
                 "levels_ms": round(sah_levels_ms, 4)}
-    build = {"ms": round(build_ms, 4), "morton_ms": round(bt.morton_ms, 4),
+    build = {"ms": round(build_ms, 4), "ms_with_phase_events": round(bt.total_ms, 4),
+             "morton_ms": round(bt.morton_ms, 4),
              "sort_ms": round(bt.sort_ms, 4), "tree_ms": round(bt.tree_ms, 4),
              "leaves_boxes_ms": round(bt.refit_ms, 4), "wide_ms": round(wide_ms, 4),
-             "depth": info.depth, "sah_ms": round(sah_build_ms, 4), "sah_phases": sah_parts,
+             "depth": info.depth, "sah_ms": round(sah_build_ms, 4),
+             "sah_ms_with_phase_events": round(sah_split_total_ms, 4), "sah_phases": sah_parts,
              "sah_roofline": sah_roof, "sah_wide_ms": round(sah_wide_ms, 4),
              "roofline": {"bound": "hbm", "achieved": round(build_gbs, 2), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
@@ -695,11 +712,15 @@ def leg_c5(A, S, O, threads):
     S_ = poff.shape[0] - 1
     di = A.DeviceArray(items)
     builds = []
-    for _ in range(3):
-        bt = A.Batch.from_device(0, di.ptr.value, poff)
-        builds.append(bt.timings.total_ms)
-        if len(builds) < 3:
-            bt.close()
+    A.check(L.qr_set_build_phase_timing(0))  # (as for the plain builds: no phase events)
+    try:
+        for _ in range(3):
+            bt = A.Batch.from_device(0, di.ptr.value, poff)
+            builds.append(bt.timings.total_ms)
+            if len(builds) < 3:
+                bt.close()
+    finally:
+        A.check(L.qr_set_build_phase_timing(1))
     plan_c = A.BatchPlan(bt, cp, coff, 0, tune=True)
     plan_k = A.BatchPlan(bt, kp, koff, 8, tune=True)
     ms_c = _median_ms(lambda: plan_c.run())

@@ -115,11 +115,13 @@ QR_API int qr_set_traversal_mode(int mode);
  * that way.  Trees are identical byte for byte at every setting.  No reference
  * counterpart (a tuning knob of this backend; tests use it to cover both paths). */
 QR_API int qr_set_frontier_cap(int cap);
-/* SAH builds: 1 records the per-level phase events behind qr_build_timings'
- * sah_bin_ms, sah_split_ms and sah_part_ms, process-wide; 0 (the default) leaves
- * them 0.  Each event costs a few microseconds of device time per level (about
- * 0.2 ms of a 2M-triangle build).  No reference counterpart (a profiling knob of
- * this backend). */
+/* Which events a build records for its qr_build_timings, process-wide: 1 (the
+ * default, as the reference fills them) the phase fields and total_ms; 0 total_ms
+ * only, the phase fields left 0 (each event between two kernels costs ~12 us of idle
+ * device time: a 2M-triangle LBVH build runs ~24 us faster); 2 also the SAH
+ * per-level events behind sah_bin_ms, sah_split_ms and sah_part_ms (about 0.2 ms of
+ * a 2M-triangle SAH build).  No reference counterpart (a profiling knob of this
+ * backend). */
 QR_API int qr_set_build_phase_timing(int on);
 /* Stream for subsequent calls on this host thread (a hipStream_t; NULL = default). */
 QR_API int qr_set_stream(void* hip_stream);

@@ -276,10 +276,12 @@ void buildInto(BvhState& b, int prim, int builder, const void* dItems, int n, qr
     need(builder == QR_BUILDER_LBVH || builder == QR_BUILDER_SAH, "unknown builder");
     need(n >= 2, "at least 2 primitives are required");
     BuildTimingsC bt{};
+    // no timings asked for: the builders record no phase events
+    BuildTimingsC* want = t ? &bt : nullptr;
     if (builder == QR_BUILDER_LBVH) {
-        buildLbvhDevice(b, prim, dItems, n, currentStream(), &bt);
+        buildLbvhDevice(b, prim, dItems, n, currentStream(), want);
     } else {
-        buildSahDevice(b, prim, dItems, n, currentStream(), &bt);
+        buildSahDevice(b, prim, dItems, n, currentStream(), want);
     }
     if (t) {
         t->morton_ms = bt.mortonMs;
@@ -353,8 +355,8 @@ int qr_set_frontier_cap(int cap) {
 
 int qr_set_build_phase_timing(int on) {
     return guard([&] {
-        need(on == 0 || on == 1, "phase timing must be 0 or 1");
-        gSahPhaseTiming = on != 0;
+        need(on >= 0 && on <= 2, "phase timing must be 0, 1 or 2");
+        gBuildPhaseTiming = on;
     });
 }
 
@@ -1001,7 +1003,7 @@ int qr_batch_build_device(int prim, const void* items, const int* offsets, int s
             hipCheck(hipMemcpy(b.primOff.p, offsets, sizeof(int) * (static_cast<size_t>(scenes) + 1),
                                hipMemcpyHostToDevice), "batch offsets");
             BuildTimingsC bt{};
-            buildBatchDevice(b, prim, items, currentStream(), &bt);
+            buildBatchDevice(b, prim, items, currentStream(), t ? &bt : nullptr);
             if (t) {
                 *t = qr_build_timings{};
                 t->morton_ms = bt.mortonMs;

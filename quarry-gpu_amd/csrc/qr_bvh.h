@@ -27,7 +27,7 @@ struct SahBuildStats {
 };
 extern thread_local SahBuildStats tSahStats;
 // SAH builds record per-level phase events (bin / split / partition timings) when set.
-extern bool gSahPhaseTiming;
+extern int gBuildPhaseTiming;
 
 struct BvhState {
     int kind = kPrimTri;

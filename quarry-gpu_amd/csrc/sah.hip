@@ -1466,7 +1466,7 @@ int* pinnedWords() {
 }  // namespace
 
 thread_local SahBuildStats tSahStats;
-bool gSahPhaseTiming = false;  // qr_set_build_phase_timing
+int gBuildPhaseTiming = 1;  // qr_set_build_phase_timing: 0 total only, 1 phases, 2 + SAH levels
 
 template <typename L>
 static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStream_t s, BuildTimingsC* t) {
@@ -1532,7 +1532,9 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     sahPrepKernel<L><<<grid, kBlock, 0, s>>>(dSrc, n, T.rec[0], cb, b.parent.p);
     if (n <= kChunk) sahRootKernel<<<1, 64, 0, s>>>(T, cb, n);  // (else the first level's plan routes it)
     hipCheck(hipGetLastError(), "sah prep");
-    ev.record(1, s);
+    // phase events only for a caller that asks for the timings with phases on
+    const bool phases = t && gBuildPhaseTiming >= 1;
+    if (phases) ev.record(1, s);
 
     // Large tasks, level by level, enqueued in batches with no host work between the
     // levels of a batch; after each batch one read of the next level's task count
@@ -1542,7 +1544,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     // per-level phase events only when asked for (each costs a few microseconds of
     // device time per level)
     auto mark = [&](hipStream_t st) {
-        if (!gSahPhaseTiming) return;
+        if (gBuildPhaseTiming < 2) return;
         const hipEvent_t e = takeEvent();
         hipCheck(hipEventRecord(e, st), "event record");
         lev.push_back(e);
@@ -1590,7 +1592,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
         throw DeviceError("sah: empty side in a split");
     };
     if (hw[kCtlError] != 0) failed();
-    ev.record(2, s);
+    if (phases) ev.record(2, s);
     // local subtrees (the host knows their number from the last read), then the
     // serial tails; the bottom-up pass runs only if no kernel set the error word (a
     // device-side gate: no host round trip before it)
@@ -1605,7 +1607,7 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
     }
     sahSerialKernel<<<2048, kBlock, 0, s>>>(T, n, order, b.nodes.p, b.parent.p, b.farEnd.p);
     hipCheck(hipGetLastError(), "sah subtrees");
-    ev.record(4, s);
+    if (phases) ev.record(4, s);
 
     finishTopology(b, dSrc, order, s, &T.ctl[kCtlError]);
     hipCheck(hipMemcpyAsync(hw, T.ctl, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, s), "sah control");
@@ -1636,11 +1638,11 @@ static void buildSahT(BvhState& b, const typename L::Item* dSrc, int n, hipStrea
             hipCheck(hipEventElapsedTime(&v, lev[i + 2], lev[i + 3]), "event elapsed");
             partMs += v;
         }
-        t->mortonMs = ev.ms(0, 1);
+        t->mortonMs = phases ? ev.ms(0, 1) : 0.0;
         t->sortMs = 0.0;
-        t->treeMs = ev.ms(1, 4);
-        t->scatterMs = ev.ms(2, 4);  // local subtrees and serial tails
-        t->refitMs = ev.ms(4, 5);
+        t->treeMs = phases ? ev.ms(1, 4) : 0.0;
+        t->scatterMs = phases ? ev.ms(2, 4) : 0.0;  // local subtrees and serial tails
+        t->refitMs = phases ? ev.ms(4, 5) : 0.0;
         t->sahBinMs = binMs;
         t->sahSplitMs = splitMs;
         t->sahPartMs = partMs;

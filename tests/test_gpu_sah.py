@@ -131,3 +131,32 @@ def test_sah_city_fewer_node_tests(A, orc, scenes):
         tests[builder] = st.node_tests / rays.shape[0]
     print("node tests per ray:", tests, flush=True)
     assert tests[A.BUILDER_SAH] < tests[A.BUILDER_LBVH]
+
+
+def test_build_phase_timing_levels(A, scenes):
+    """qr_set_build_phase_timing: 0 fills total_ms only, 1 (the default) the phase
+    fields too, 2 also the SAH per-level split; the trees are the same at every level."""
+    L = A.lib()
+    items = _items(0, 300000, 11, scenes)
+    hashes = {A.BUILDER_LBVH: set(), A.BUILDER_SAH: set()}
+    try:
+        for level in (0, 1, 2):
+            A.check(L.qr_set_build_phase_timing(level))
+            for builder in (A.BUILDER_LBVH, A.BUILDER_SAH):
+                print(f"phase timing level={level} builder={builder}", flush=True)
+                b = A.Bvh(0, items, builder)
+                t = b.timings
+                assert t.total_ms > 0.0
+                if level == 0:
+                    assert t.morton_ms == 0.0 and t.refit_ms == 0.0
+                else:
+                    assert t.morton_ms > 0.0 and t.refit_ms > 0.0
+                if builder == A.BUILDER_SAH:
+                    assert (t.sah_bin_ms > 0.0) == (level == 2)
+                hashes[builder].add(b.tree_hash())
+                b.close()
+        with pytest.raises(Exception):
+            A.check(L.qr_set_build_phase_timing(3))
+    finally:
+        A.check(L.qr_set_build_phase_timing(1))
+    assert all(len(h) == 1 for h in hashes.values())
