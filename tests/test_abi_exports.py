@@ -38,3 +38,21 @@ def test_no_device_reports_cleanly(qabi):
     # Without a GPU (this container) describe returns None rather than crashing.
     d = qabi.describe()
     assert d is None or "gfx" in d
+
+
+def test_process_wide_knobs_check_their_range(qabi):
+    # Host-only setters: in-range values are taken, out-of-range ones are refused
+    # with the invalid-argument code; the defaults are restored afterwards.
+    lib = qabi.lib()
+    try:
+        for v in (0, 1, 2):
+            assert lib.qr_set_build_phase_timing(v) == 0
+        for v in (-1, 3):
+            assert lib.qr_set_build_phase_timing(v) != 0
+        for v in (0, 5, 32):
+            assert lib.qr_set_frontier_cap(v) == 0
+        for v in (-1, 33):
+            assert lib.qr_set_frontier_cap(v) != 0
+    finally:
+        assert lib.qr_set_build_phase_timing(1) == 0
+        assert lib.qr_set_frontier_cap(32) == 0
