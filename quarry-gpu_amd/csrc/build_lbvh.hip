@@ -180,7 +180,7 @@ struct MortonFrame {
     }
 };
 // The frame from the bounds partials of `parts` workgroups, reduced by the whole
-// workgroup (every thread returns it).
+// workgroup of kBlock threads (every thread returns it).
 __device__ __forceinline__ MortonFrame mortonFrame(const uint32_t* __restrict__ cb, int parts) {
     __shared__ uint32_t sb[kBlock / 64][6];
     uint32_t b[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
@@ -222,7 +222,7 @@ __device__ __forceinline__ MortonFrame mortonFrame(const uint32_t* __restrict__ 
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
         uint32_t x = sb[0][c];
-        for (int w = 1; w < static_cast<int>(blockDim.x) / 64; ++w) {
+        for (int w = 1; w < kBlock / 64; ++w) {
             const uint32_t v = sb[w][c];
             x = c < 3 ? (v < x ? v : x) : (v > x ? v : x);
         }
